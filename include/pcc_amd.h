@@ -1,0 +1,266 @@
+/*
+ * pcc_amd.h -- C ABI of the MI355X-native ParallelCorrelationClusterer /
+ * ParallelModularityClusterer hot path.
+ *
+ * The library keeps the graph CSR and the clustering state resident in HBM for
+ * the whole Cluster call and runs the Louvain best-move rounds, the cluster
+ * contraction and the multilevel refinement as hand-written gfx950 kernels.
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ *
+ * Conventions:
+ *   - Return values are absl::StatusCode numbers (PCC_OK ... PCC_INTERNAL);
+ *     pcc_last_error() gives the thread-local message of the last failure.
+ *   - The caller owns every host array.  A pcc_graph owns its device buffers
+ *     and one HIP stream on the device that was current when it was created.
+ *   - Handles are independent; a single handle is not reentrant.
+ *   - Vertex ids and cluster ids are uint32 (gbbs::uintE); n < 2^32 - 1.
+ */
+#ifndef PCC_AMD_H_
+#define PCC_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* absl::StatusCode values used by the reference (clustering/status_macros.h). */
+#define PCC_OK 0
+#define PCC_INVALID_ARGUMENT 3
+#define PCC_NOT_FOUND 5
+#define PCC_RESOURCE_EXHAUSTED 8
+#define PCC_UNIMPLEMENTED 12
+#define PCC_INTERNAL 13
+
+/* Sentinel for "no move" in per-vertex move arrays. */
+#define PCC_NO_MOVE 0xFFFFFFFFu
+
+/* CorrelationClustererConfig enums, clustering/config.proto:115-160. */
+#define PCC_DEFAULT_CLUSTER_MOVES 0
+#define PCC_LOUVAIN 1
+#define PCC_NBHR_CLUSTER_MOVE 0
+#define PCC_ALL_MOVE 1
+#define PCC_NBHR_MOVE 2
+
+/* Objective selector: ParallelCorrelationClusterer or
+ * ParallelModularityClusterer (clustering/cluster-in-memory_main.cc:270-275). */
+#define PCC_OBJECTIVE_CORRELATION 0
+#define PCC_OBJECTIVE_MODULARITY 1
+
+/*
+ * CorrelationClustererConfig (clustering/config.proto:94-183) with its proto2
+ * defaults.  Fields the path does not read are kept so that every text-format
+ * config the reference accepts parses here too.
+ */
+typedef struct pcc_config {
+  double resolution;                     /* 1 */
+  double edge_weight_offset;             /* 2 */
+  uint32_t random_seed;                  /* 3 (not read by the parallel path) */
+  int64_t num_iterations;                /* 4 */
+  int64_t louvain_num_iterations;        /* 8.1 LouvainConfig.num_iterations */
+  int64_t louvain_num_inner_iterations;  /* 8.2 LouvainConfig.num_inner_iterations */
+  int32_t clustering_moves_method;       /* 9, default PCC_LOUVAIN */
+  int32_t subclustering_method;          /* 10, only NONE_SUBCLUSTERING (0) */
+  int32_t preclustering_method;          /* 11, only NONE_PRECLUSTERING (0) */
+  int32_t move_method;                   /* 16, default PCC_NBHR_CLUSTER_MOVE */
+  uint8_t refine;                        /* 13 */
+  uint8_t async;                         /* 14 */
+  uint8_t connect_stats;                 /* 15 (never read by the reference) */
+  uint8_t all_iter;                      /* 17 (sequential clusterers only) */
+  uint8_t permute;                       /* 18 */
+  uint8_t _pad[3];
+  uint64_t permute_seed;                 /* processing-order seed (async + permute) */
+} pcc_config;
+
+/* Proto2 defaults of CorrelationClustererConfig. */
+void pcc_config_default(pcc_config* cfg);
+
+/* Parses a text-format ClustererConfig holding a correlation_clusterer_config
+ * (e.g. "correlation_clusterer_config { resolution: 0.01 move_method: ALL_MOVE }")
+ * or the bare CorrelationClustererConfig fields.  Replaces
+ * google::protobuf::TextFormat::ParseFromString at
+ * clustering/cluster-in-memory_main.cc:257-265: PCC_INVALID_ARGUMENT on
+ * malformed text or unknown fields. */
+int pcc_config_parse_text(const char* text, pcc_config* out);
+
+/* ---------------------------------------------------------------- graph -- */
+
+typedef struct pcc_graph pcc_graph;
+
+/* New empty graph under construction; replaces GbbsGraph's constructor
+ * (clustering/gbbs-graph.h:39-56). */
+int pcc_graph_new(pcc_graph** out);
+
+/* InMemoryClusterer::Graph::Import (clustering/in-memory-clusterer.h:60-65,
+ * clustering/gbbs-graph.cc:32-49).  Safe to call concurrently from several
+ * threads on one graph.  node_weight is accepted and ignored, as GbbsGraph
+ * ignores it (clustering/gbbs-graph.h:34).  weights may be NULL (1.0). */
+int pcc_graph_import(pcc_graph* g, uint32_t id, double node_weight,
+                     const uint32_t* nbrs, const double* weights, uint64_t degree);
+
+/* GbbsGraph::FinishImport (clustering/gbbs-graph.cc:51-76): builds the CSR in
+ * import order, rounds weights to float32, and uploads it to HBM. */
+int pcc_graph_finish_import(pcc_graph* g);
+
+/* Direct construction from a host CSR, the path the reference's CLI takes
+ * (clustering/cluster-in-memory_main.cc:231-250 CopyGraph + :306).  offsets
+ * has n+1 entries; weights may be NULL (unit weights).  The graph must be
+ * symmetric (clustering/clusterers/parallel-correlation-clusterer.h:43). */
+int pcc_graph_from_csr(int64_t n, int64_t m, const int64_t* offsets,
+                       const uint32_t* nbrs, const float* weights, pcc_graph** out);
+
+int64_t pcc_graph_num_nodes(const pcc_graph* g);
+int64_t pcc_graph_num_edges(const pcc_graph* g);
+
+/* Copies the device CSR back to host arrays (n+1, m, m entries). */
+int pcc_graph_copy_csr(const pcc_graph* g, int64_t* offsets, uint32_t* nbrs, float* weights);
+
+void pcc_graph_destroy(pcc_graph* g);
+
+/* ---------------------------------------------------------------- cluster -- */
+
+typedef struct pcc_stats {
+  int64_t levels;              /* levels run (final iter + 1) */
+  int64_t rounds;              /* best-move rounds, including refinement */
+  int64_t edges_scanned;       /* sum over rounds of sum_{v in frontier} deg(v) */
+  int64_t frontier_vertices;   /* sum over rounds of |frontier| */
+  int64_t num_clusters;        /* clusters in the output */
+  int64_t fallback_vertices;   /* vertices whose sums took the ordered path */
+  double objective;            /* ComputeObjective on the input graph */
+  double modularity;           /* ComputeModularity (modularity runs) */
+  double cluster_seconds;      /* span of "Cluster Time:", device-side */
+  double bestmove_seconds;     /* sum of best-move kernel time (timing on) */
+  double bestmove_bytes;       /* algorithmic bytes 12*TE + 40*|F| summed */
+  double contract_seconds;     /* contraction time (timing on) */
+  double apply_seconds;        /* apply + frontier time (timing on) */
+  int64_t bestmove_launches;   /* best-move kernel launches (timing on) */
+  double ordered_seconds;      /* ordered fallback kernel time (timing on) */
+  /* Per degree bin (0..4: deg <= 4, 8, 16, 32, 64; 5: one-wave LDS hash;
+   * 6: one-workgroup LDS hash; 7: global-memory hash). */
+  double bin_seconds[8];       /* kernel time (timing on) */
+  int64_t bin_edges[8];        /* edges scanned */
+  int64_t bin_vertices[8];     /* vertices processed */
+  int64_t bin_launches[8];     /* launches (timing on) */
+} pcc_stats;
+
+/* ParallelCorrelationClusterer::Cluster
+ * (clustering/clusterers/parallel-correlation-clusterer.cc:641-653) or, with
+ * objective == PCC_OBJECTIVE_MODULARITY, ParallelModularityClusterer::Cluster
+ * (clustering/clusterers/parallel-modularity-clusterer.cc:98-111).
+ * out_cluster_ids[n] receives the reference's final cluster_ids vector
+ * (before OutputIndicesById, parallel-correlation-clusterer.cc:606-616).
+ * stats may be NULL. */
+int pcc_cluster(pcc_graph* g, const pcc_config* cfg, int objective,
+                uint32_t* out_cluster_ids, pcc_stats* stats);
+
+/* RefineClusters(config, clustering, node_weights)
+ * (parallel-correlation-clusterer.cc:399-412, 631-639;
+ * parallel-modularity-clusterer.cc:68-96).  inout_cluster_ids[n] holds the
+ * initial clustering as per-vertex cluster indices in [0, n) (cluster i of the
+ * reference's Clustering vector); it is overwritten with the result.
+ * node_weights may be NULL (1.0, or degrees for modularity). */
+int pcc_refine_clusters(pcc_graph* g, const pcc_config* cfg, int objective,
+                        const double* node_weights, uint32_t* inout_cluster_ids,
+                        pcc_stats* stats);
+
+/* ClusteringHelper::ComputeObjective
+ * (parallel-correlation-clusterer-internal.cc:118-149) on device. */
+int pcc_compute_objective(pcc_graph* g, const pcc_config* cfg,
+                          const uint32_t* cluster_ids, const double* node_weights,
+                          double* out_objective);
+
+/* ComputeModularity (parallel-modularity-clusterer.cc:36-66) on device. */
+int pcc_compute_modularity(pcc_graph* g, double resolution,
+                           const uint32_t* cluster_ids, double* out_modularity);
+
+/* ------------------------------------------------------------ one phase -- */
+/* Phase-level entry points used by parity tests and by callers that drive the
+ * Louvain loop themselves.  State arrays are the ClusteringHelper SoA
+ * (parallel-correlation-clusterer-internal.h:158-164), n entries each. */
+
+/* The sync vertex map of BestMovesForVertexSubset
+ * (parallel-correlation-clusterer.cc:132-160): EfficientBestMove
+ * (parallel-correlation-clusterer-internal.cc:448-583), the singleton rule and
+ * the delta > 0 filter, for every vertex with frontier[v] != 0 (frontier NULL:
+ * all).  out_moves[v] = target or PCC_NO_MOVE, out_moved[v] in {0,1};
+ * out_target / out_delta (may be NULL) hold EfficientBestMove's own return. */
+int pcc_best_moves(pcc_graph* g, const pcc_config* cfg, const uint32_t* cluster_ids,
+                   const uint32_t* cluster_sizes, const double* node_weights,
+                   const double* cluster_weights, const uint8_t* frontier,
+                   uint32_t* out_moves, uint8_t* out_moved, uint32_t* out_target,
+                   double* out_delta);
+
+/* One BestMovesForVertexSubset round (parallel-correlation-clusterer.cc:115-338)
+ * in place: state updated as MoveNodesToCluster / AsyncMove leave it, frontier
+ * (n bytes) replaced by the next frontier, whose size is returned in
+ * out_frontier_size. */
+int pcc_round(pcc_graph* g, const pcc_config* cfg, uint32_t* cluster_ids,
+              uint32_t* cluster_sizes, const double* node_weights,
+              double* cluster_weights, uint8_t* frontier, int64_t* out_frontier_size);
+
+/* CompressGraph (parallel-correlation-clusterer-internal.cc:709-782):
+ * creates *out_graph = the contracted graph and writes its aggregated node
+ * weights to out_node_weights (capacity n; n' = pcc_graph_num_nodes(*out_graph)
+ * entries are written). */
+int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_weights,
+                 pcc_graph** out_graph, double* out_node_weights);
+
+/* --------------------------------------------------------------- tuning -- */
+
+/* Runtime options of a graph handle.  Keys: */
+#define PCC_OPT_TIMING 1         /* 1: time kernels with HIP events into pcc_stats */
+#define PCC_OPT_FORCE_ORDERED 2  /* 1: route every hashed vertex to the ordered path */
+#define PCC_OPT_WAVE_MAX_DEG 3   /* upper degree of the one-wave LDS-hash bin */
+#define PCC_OPT_BLOCK_MAX_DEG 4  /* upper degree of the one-workgroup LDS-hash bin */
+int pcc_set_option(pcc_graph* g, int key, int64_t value);
+
+/* ------------------------------------------------------------ host data -- */
+
+/* Host CSR returned by the generators and the GBBS adjacency reader; release
+ * with pcc_host_csr_free. */
+typedef struct pcc_host_csr {
+  int64_t n;
+  int64_t m;
+  int64_t* offsets;   /* n + 1 */
+  uint32_t* nbrs;     /* m */
+  float* weights;     /* m */
+} pcc_host_csr;
+
+void pcc_host_csr_free(pcc_host_csr* csr);
+
+/* Planted-partition SBM: num_blocks equal blocks (the last takes the
+ * remainder), undirected, no self-loops, pair probabilities p_in / p_out,
+ * weights uniform in [w_lo, w_hi] (w_lo == w_hi: constant), float32-rounded.
+ * Deterministic in seed. */
+int pcc_gen_sbm(int64_t n, int64_t num_blocks, double p_in, double p_out,
+                double w_lo, double w_hi, uint64_t seed, pcc_host_csr* out);
+
+/* SBM with explicit block sizes (sum = n). */
+int pcc_gen_sbm_blocks(int64_t num_blocks, const int64_t* block_sizes, double p_in,
+                       double p_out, double w_lo, double w_hi, uint64_t seed,
+                       pcc_host_csr* out);
+
+/* RMAT: 2^scale vertices, edge_factor * 2^scale draws with quadrant
+ * probabilities a, b, c (d = 1 - a - b - c), symmetrized, deduplicated,
+ * self-loops removed; weights uniform in [w_lo, w_hi] per undirected pair. */
+int pcc_gen_rmat(int scale, double edge_factor, double a, double b, double c,
+                 double w_lo, double w_hi, uint64_t seed, pcc_host_csr* out);
+
+/* GBBS AdjacencyGraph / WeightedAdjacencyGraph text format, the reference's
+ * input (read_unweighted_symmetric_graph / read_weighted_symmetric_graph at
+ * clustering/cluster-in-memory_main.cc:292-305). */
+int pcc_read_adjacency_graph(const char* path, int float_weighted, pcc_host_csr* out);
+int pcc_write_adjacency_graph(const char* path, const pcc_host_csr* csr, int float_weighted);
+
+/* ---------------------------------------------------------------- misc -- */
+
+const char* pcc_last_error(void);
+const char* pcc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCC_AMD_H_ */

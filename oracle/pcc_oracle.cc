@@ -1,0 +1,759 @@
+// pcc_oracle.cc -- CPU restatement of the ParallelCorrelationClusterer /
+// ParallelModularityClusterer hot path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path.  Nothing in
+// the product library links, loads or calls this file.
+//
+// Reference files restated (paths relative to the reference repository root):
+//   clustering/clusterers/parallel-correlation-clusterer.cc        (PCC.cc)
+//   clustering/clusterers/parallel-correlation-clusterer-internal.cc (PCCI.cc)
+//   clustering/clusterers/correlation-clusterer-util.h             (UTIL.h)
+//   clustering/clusterers/parallel-modularity-clusterer.cc         (PMC.cc)
+//   parallel/parallel-graph-utils.cc                                (PGU.cc)
+//   parallel/parallel-sequence-ops.h                                (PSO.h)
+//
+// Floating-point policy (pinned identically in the HIP kernels):
+//   * EdgeSum::NetWeight (UTIL.h:36-39) computes weight_ - res * x.  The
+//     reference is built by GCC with -O2 -march=native and GCC's default
+//     -ffp-contract=fast for C++, which fuses that expression into one
+//     fnmadd; it is written here as std::fma(-res, x, weight_).  Everything
+//     else is compiled with -ffp-contract=off.
+//   * Block-structured sums follow pbbslib's reduce (GBBS d385adac,
+//     pbbslib/sequence_ops.h): blocks of max(1024, 4*ceil(sqrt(n))) elements,
+//     each folded left starting from its first element, then the block sums
+//     reduced the same way.  This blocking is restated from the published
+//     pbbslib source, which is not vendored in the reference: parity with the
+//     reference for sums longer than 1024 terms is unpinned.
+//   * Per-cluster sums of hub vertices (degree > 1000) go through
+//     sparse_additive_map atomics in the reference (PCCI.cc:518-536), whose
+//     order is nondeterministic there.  The oracle uses adjacency order, one
+//     of the reference's possible interleavings.
+
+#include "pcc_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kNone = ORACLE_NO_MOVE;
+constexpr int64_t kHubDegree = 1000;  // PCCI.cc:456
+
+int g_threads = 1;
+int64_t g_edges_scanned = 0;  // sum over rounds of sum_{v in frontier} deg(v)
+
+// pbbslib reduce: block size and left folds (see the header comment).
+template <class T, class F>
+T PbbsReduce(const T* a, size_t n, T identity, F f) {
+  if (n == 0) return identity;
+  size_t bs = std::max<size_t>(1024, 4 * (size_t)std::ceil(std::sqrt((double)n)));
+  size_t l = (n + bs - 1) / bs;
+  if (l <= 1) {
+    T r = a[0];
+    for (size_t i = 1; i < n; ++i) r = f(r, a[i]);
+    return r;
+  }
+  std::vector<T> sums(l);
+  for (size_t b = 0; b < l; ++b) {
+    size_t s = b * bs, e = std::min(n, s + bs);
+    T r = a[s];
+    for (size_t i = s + 1; i < e; ++i) r = f(r, a[i]);
+    sums[b] = r;
+  }
+  return PbbsReduce(sums.data(), l, identity, f);
+}
+
+// EdgeSum::NetWeight, UTIL.h:36-39, fused as the reference binary computes it.
+inline double NetWeight(double weight, double sum_prod_node_weights, double res) {
+  return std::fma(-res, sum_prod_node_weights, weight);
+}
+
+// Per-thread open-addressing map ClusterId -> double (absl::flat_hash_map /
+// sparse_additive_map stand-in; iteration order never matters because the
+// best-move selection is a total order, PCCI.cc:493-503).
+struct ClusterMap {
+  std::vector<uint32_t> keys;
+  std::vector<double> vals;
+  std::vector<uint32_t> used;
+  uint32_t mask = 0;
+  void Reset(size_t expected) {
+    size_t cap = 16;
+    while (cap < 2 * expected + 2) cap <<= 1;
+    if (keys.size() < cap) {
+      keys.assign(cap, kNone);
+      vals.assign(cap, 0.0);
+    } else {
+      for (uint32_t s : used) keys[s] = kNone, vals[s] = 0.0;
+      cap = keys.size();
+    }
+    used.clear();
+    mask = (uint32_t)(cap - 1);
+  }
+  double& At(uint32_t key) {
+    uint32_t h = (key * 0x9E3779B1u) & mask;
+    while (true) {
+      if (keys[h] == key) return vals[h];
+      if (keys[h] == kNone) {
+        keys[h] = key;
+        vals[h] = 0.0;
+        used.push_back(h);
+        return vals[h];
+      }
+      h = (h + 1) & mask;
+    }
+  }
+};
+
+thread_local ClusterMap t_map;
+
+// ClusteringHelper::EfficientBestMove, PCCI.cc:448-583.
+void BestMove(const oracle_graph& g, const oracle_state& s, double res,
+              double offset, uint32_t v, uint32_t* out_t, double* out_d) {
+  const int64_t b = g.offsets[v], e = g.offsets[(int64_t)v + 1];
+  const int64_t deg = e - b;
+  const uint32_t n = (uint32_t)g.n;
+  const uint32_t node_cluster = s.cluster_ids[v];
+  const double k = s.node_weights[v];
+  ClusterMap& after = t_map;
+  after.Reset((size_t)deg);
+
+  if (deg <= kHubDegree) {
+    // PCCI.cc:457-511
+    double together = 0.0;  // class_1_currently_together (EdgeSum starts at 0)
+    double moving_nodes_weight = 0;
+    moving_nodes_weight += k;
+    for (int64_t p = b; p < e; ++p) {
+      double weight = (double)g.weights[p];
+      weight -= offset;
+      const uint32_t u = g.nbrs[p];
+      const uint32_t neighbor_cluster = s.cluster_ids[u];
+      if (v != u) {
+        if (node_cluster == neighbor_cluster) together += weight;
+        after.At(neighbor_cluster) += weight;
+      }
+    }
+    double change = 0;
+    double max_edges = k * (s.cluster_weights[node_cluster] - k);
+    change -= NetWeight(together, max_edges, res);
+    bool has = false;
+    uint32_t best = 0;
+    double best_delta = change;
+    for (uint32_t slot : after.used) {
+      const uint32_t c = after.keys[slot];
+      max_edges = moving_nodes_weight * s.cluster_weights[c];
+      const double overall = change + NetWeight(after.vals[slot], max_edges, res);
+      if (overall > best_delta || (overall == best_delta && has && c < best)) {
+        best = c;
+        best_delta = overall;
+        has = true;
+      }
+    }
+    *out_t = has ? best : n;
+    *out_d = best_delta;
+    return;
+  }
+
+  // Hub path, PCCI.cc:516-582.
+  std::vector<double> curr_together_seq((size_t)deg, 0.0);
+  double moving_nodes_weight = 0;
+  moving_nodes_weight += k;
+  for (int64_t j = 0; j < deg; ++j) {
+    float weight = g.weights[b + j];
+    weight -= offset;  // float arithmetic, PCCI.cc:529
+    const uint32_t u = g.nbrs[b + j];
+    const uint32_t neighbor_cluster = s.cluster_ids[u];
+    if (v != u) {
+      if (node_cluster == neighbor_cluster) curr_together_seq[(size_t)j] = weight;
+      after.At(neighbor_cluster) += (double)weight;
+    }
+  }
+  const double max_edges = k * (s.cluster_weights[node_cluster] - k);
+  const double curr_together = NetWeight(
+      PbbsReduce(curr_together_seq.data(), curr_together_seq.size(), 0.0,
+                 [](double x, double y) { return x + y; }),
+      max_edges, res);
+  double change = 0;
+  change -= curr_together;
+  // best_move_seq + max reduce with identity (0, 0.0), PCCI.cc:551-575.
+  std::vector<std::pair<uint32_t, double>> seq;
+  seq.reserve(after.used.size());
+  for (uint32_t slot : after.used) {
+    const uint32_t c = after.keys[slot];
+    double data = after.vals[slot];
+    const double me = moving_nodes_weight * s.cluster_weights[c];
+    data = NetWeight(data, me, res);  // data -= res * max_edges
+    seq.emplace_back(c, change + data);
+  }
+  auto f_max = [](const std::pair<uint32_t, double>& a,
+                  const std::pair<uint32_t, double>& bb) {
+    if (a.second == bb.second) return a.first < bb.first ? a : bb;
+    if (a.second > bb.second) return a;
+    return bb;
+  };
+  const auto max_move = PbbsReduce(seq.data(), seq.size(),
+                                   std::make_pair(uint32_t{0}, 0.0), f_max);
+  if (change >= max_move.second) {  // PCCI.cc:577-581
+    *out_t = n;
+    *out_d = change;
+    return;
+  }
+  *out_t = max_move.first;
+  *out_d = max_move.second;
+}
+
+// Sync decision for one vertex, PCC.cc:140-156.
+inline void SyncDecision(const oracle_graph& g, const oracle_state& s,
+                         const oracle_config& cfg, uint32_t v, uint32_t* move,
+                         uint8_t* moved, uint32_t* tgt, double* dlt) {
+  uint32_t t;
+  double d;
+  BestMove(g, s, cfg.resolution, cfg.edge_weight_offset, v, &t, &d);
+  if (tgt) *tgt = t;
+  if (dlt) *dlt = d;
+  const uint32_t cur = s.cluster_ids[v];
+  if (t < (uint32_t)g.n && s.cluster_sizes[t] == 1 && s.cluster_sizes[cur] == 1 &&
+      cur >= t) {
+    t = cur;
+    d = 0;
+  }
+  if (d > 0) {
+    *move = t;
+    *moved = 1;
+  } else {
+    *move = kNone;
+    *moved = 0;
+  }
+}
+
+// ClusteringHelper::AsyncMove(node), PCCI.cc:371-403 (sequential execution).
+bool AsyncMove(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+               uint32_t v) {
+  uint32_t t;
+  double d;
+  BestMove(g, s, cfg.resolution, cfg.edge_weight_offset, v, &t, &d);
+  const uint32_t cur = s.cluster_ids[v];
+  if (d <= 0) return false;
+  const uint32_t n = (uint32_t)g.n;
+  s.cluster_sizes[cur] -= 1;
+  s.cluster_weights[cur] += -1 * s.node_weights[v];
+  if (t != n) {
+    s.cluster_sizes[t] += 1;
+    s.cluster_weights[t] += s.node_weights[v];
+    s.cluster_ids[v] = t;
+    return true;
+  }
+  size_t i = v;
+  while (true) {
+    if (s.cluster_sizes[i] == 0) {
+      s.cluster_sizes[i] = 1;
+      s.cluster_weights[i] += s.node_weights[v];
+      s.cluster_ids[v] = (uint32_t)i;
+      return true;
+    }
+    i = (i + 1) % n;
+  }
+}
+
+uint64_t SplitMix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Next frontier = neighbours of the seed set (gbbs::edgeMap with an
+// always-true map, PCC.cc:46-55, 329-331).
+int64_t ExpandFrontier(const oracle_graph& g, const std::vector<uint8_t>& seed,
+                       uint8_t* frontier) {
+  std::memset(frontier, 0, (size_t)g.n);
+  for (int64_t v = 0; v < g.n; ++v) {
+    if (!seed[(size_t)v]) continue;
+    for (int64_t p = g.offsets[v]; p < g.offsets[v + 1]; ++p) frontier[g.nbrs[p]] = 1;
+  }
+  int64_t cnt = 0;
+  for (int64_t v = 0; v < g.n; ++v) cnt += frontier[v];
+  return cnt;
+}
+
+struct OwnedCsr {
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> offsets;
+  std::vector<uint32_t> nbrs;
+  std::vector<float> weights;
+  std::vector<double> node_weights;
+  oracle_graph View() const {
+    return oracle_graph{n, m, offsets.data(), nbrs.data(), weights.data()};
+  }
+};
+
+// CompressGraph, PCCI.cc:709-782, with ComputeInterClusterEdgesSort,
+// PGU.cc:156-231, RetrieveInterClusterEdges, PGU.cc:70-122, GetOffsets,
+// PGU.cc:33-62.
+void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
+              OwnedCsr* out) {
+  uint32_t max_id = 0;
+  for (int64_t i = 0; i < g.n; ++i) max_id = std::max(max_id, ids[i]);
+  const int64_t n2 = 1 + (int64_t)max_id;  // PCCI.cc:713-717
+  struct E {
+    uint32_t cu, cv;
+    float w;
+  };
+  std::vector<E> edges;
+  edges.reserve((size_t)g.m);
+  for (int64_t u = 0; u < g.n; ++u) {
+    for (int64_t p = g.offsets[u]; p < g.offsets[u + 1]; ++p) {
+      const uint32_t v = g.nbrs[p];
+      // PGU.cc:93-103: keep iff v <= u or the clusters differ.
+      if ((int64_t)v <= u || ids[v] != ids[u]) edges.push_back(E{ids[u], ids[v], g.weights[p]});
+    }
+  }
+  std::stable_sort(edges.begin(), edges.end(), [](const E& a, const E& b) {
+    return a.cu != b.cu ? a.cu < b.cu : a.cv < b.cv;
+  });
+  out->n = n2;
+  out->offsets.assign((size_t)n2 + 1, 0);
+  out->nbrs.clear();
+  out->weights.clear();
+  std::vector<float> seg;
+  std::vector<int64_t> counts((size_t)n2, 0);
+  for (size_t i = 0; i < edges.size();) {
+    size_t j = i;
+    seg.clear();
+    while (j < edges.size() && edges[j].cu == edges[i].cu && edges[j].cv == edges[i].cv) {
+      seg.push_back(edges[j].w);
+      ++j;
+    }
+    // PGU.cc:206-217: float aggregate through a (double, double) -> double lambda.
+    const float w = PbbsReduce(seg.data(), seg.size(), 0.0f, [](float a, float b) {
+      return (float)((double)a + (double)b);
+    });
+    out->nbrs.push_back(edges[i].cv);
+    out->weights.push_back(w);
+    counts[edges[i].cu] += 1;
+    i = j;
+  }
+  out->m = (int64_t)out->nbrs.size();
+  for (int64_t c = 0; c < n2; ++c) out->offsets[c + 1] = out->offsets[c] + counts[c];
+  // Node weights, PCCI.cc:733-777: stable sort of (id, weight) by id, then a
+  // pbbs reduce per group.
+  std::vector<uint32_t> order((size_t)g.n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return ids[a] < ids[b]; });
+  out->node_weights.assign((size_t)n2, 0.0);
+  std::vector<double> grp;
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    grp.clear();
+    while (j < order.size() && ids[order[j]] == ids[order[i]]) {
+      grp.push_back(nw[order[j]]);
+      ++j;
+    }
+    out->node_weights[ids[order[i]]] =
+        PbbsReduce(grp.data(), grp.size(), 0.0, [](double a, double b) { return a + b; });
+    i = j;
+  }
+}
+
+// ClusteringHelper::SetClustering from per-vertex ids whose groups are
+// numbered by ascending id (PCCI.cc:60-77 applied to OutputIndicesById's
+// output, PSO.h:183-209).  ids are replaced by the group rank.
+void ResetFromIds(int64_t n, oracle_state& s, const std::vector<uint32_t>& flat) {
+  std::vector<uint32_t> rank;
+  std::vector<uint32_t> sorted(flat);
+  std::sort(sorted.begin(), sorted.end());
+  sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+  for (int64_t i = 0; i < n; ++i) {  // ResetClustering, PCCI.cc:50-58
+    s.cluster_weights[i] = 0;
+    s.cluster_sizes[i] = 0;
+  }
+  for (int64_t j = 0; j < n; ++j) {  // ascending member order within a group
+    const uint32_t r =
+        (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), flat[j]) - sorted.begin());
+    s.cluster_ids[j] = r;
+    s.cluster_sizes[r] += 1;
+    s.cluster_weights[r] += s.node_weights[j];
+  }
+}
+
+struct Level {
+  OwnedCsr csr;  // unused for level 0 (the input graph)
+  std::vector<uint32_t> ids, sizes;
+  std::vector<double> nw, K;
+  oracle_state State() {
+    return oracle_state{ids.data(), sizes.data(), nw.data(), K.data()};
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int threads) {
+  g_threads = threads < 1 ? 1 : threads;
+}
+int oracle_get_threads(void) { return g_threads; }
+
+void oracle_best_move(const oracle_graph* g, const oracle_state* s,
+                      const oracle_config* cfg, uint32_t v, uint32_t* out_target,
+                      double* out_delta) {
+  BestMove(*g, *s, cfg->resolution, cfg->edge_weight_offset, v, out_target, out_delta);
+}
+
+void oracle_sync_decisions(const oracle_graph* g, const oracle_state* s,
+                           const oracle_config* cfg, const uint8_t* frontier,
+                           uint32_t* out_moves, uint8_t* out_moved,
+                           uint32_t* out_target, double* out_delta) {
+  const int64_t n = g->n;
+#pragma omp parallel for schedule(dynamic, 512) num_threads(g_threads)
+  for (int64_t v = 0; v < n; ++v) {
+    if (frontier && !frontier[v]) {
+      out_moves[v] = kNone;
+      out_moved[v] = 0;
+      if (out_target) out_target[v] = kNone;
+      if (out_delta) out_delta[v] = 0.0;
+      continue;
+    }
+    SyncDecision(*g, *s, *cfg, (uint32_t)v, &out_moves[v], &out_moved[v],
+                 out_target ? &out_target[v] : nullptr, out_delta ? &out_delta[v] : nullptr);
+  }
+}
+
+void oracle_apply_moves(int64_t n, oracle_state* s, const uint32_t* moves,
+                        uint8_t* out_modified) {
+  // ClusteringHelper::MoveNodesToCluster(moves), PCCI.cc:256-369.
+  std::memset(out_modified, 0, (size_t)n);
+  std::vector<uint32_t> moving;
+  for (int64_t v = 0; v < n; ++v)
+    if (moves[v] != kNone && moves[v] != s->cluster_ids[v]) moving.push_back((uint32_t)v);
+  if (moving.empty()) return;
+  std::vector<uint32_t> by_old(moving);
+  std::stable_sort(by_old.begin(), by_old.end(), [&](uint32_t a, uint32_t b) {
+    return s->cluster_ids[a] < s->cluster_ids[b];
+  });
+  for (size_t i = 0; i < by_old.size();) {  // PCCI.cc:294-303
+    size_t j = i;
+    const uint32_t prev = s->cluster_ids[by_old[i]];
+    while (j < by_old.size() && s->cluster_ids[by_old[j]] == prev) ++j;
+    s->cluster_sizes[prev] -= (uint32_t)(j - i);
+    out_modified[prev] = 1;
+    for (size_t q = i; q < j; ++q) s->cluster_weights[prev] -= s->node_weights[by_old[q]];
+    i = j;
+  }
+  std::vector<uint32_t> by_new(moving);
+  std::stable_sort(by_new.begin(), by_new.end(),
+                   [&](uint32_t a, uint32_t b) { return moves[a] < moves[b]; });
+  size_t new_start = by_new.size();
+  for (size_t i = 0; i < by_new.size();) {  // PCCI.cc:325-337
+    size_t j = i;
+    const uint32_t move_id = moves[by_new[i]];
+    while (j < by_new.size() && moves[by_new[j]] == move_id) ++j;
+    if (move_id != (uint32_t)n) {
+      s->cluster_sizes[move_id] += (uint32_t)(j - i);
+      out_modified[move_id] = 1;
+      for (size_t q = i; q < j; ++q) {
+        s->cluster_ids[by_new[q]] = move_id;
+        s->cluster_weights[move_id] += s->node_weights[by_new[q]];
+      }
+    } else {
+      new_start = i;
+    }
+    i = j;
+  }
+  if (moves[by_new.back()] == (uint32_t)n) {  // PCCI.cc:340-366
+    std::vector<uint32_t> zero;
+    for (int64_t id = 0; id < n; ++id)
+      if (s->cluster_sizes[id] == 0) zero.push_back((uint32_t)id);
+    for (size_t i = new_start; i < by_new.size(); ++i) {
+      const uint32_t c = zero[i - new_start];
+      const uint32_t v = by_new[i];
+      s->cluster_ids[v] = c;
+      s->cluster_sizes[c] = 1;
+      out_modified[c] = 1;
+      s->cluster_weights[c] = s->node_weights[v];
+    }
+  }
+}
+
+int64_t oracle_round(const oracle_graph* g, oracle_state* s,
+                     const oracle_config* cfg, uint8_t* frontier,
+                     uint64_t round_seed) {
+  // BestMovesForVertexSubset, PCC.cc:115-338.
+  const int64_t n = g->n;
+  for (int64_t v = 0; v < n; ++v)
+    if (frontier[v]) g_edges_scanned += g->offsets[v + 1] - g->offsets[v];
+  std::vector<uint8_t> moved_vertex((size_t)n, 0);
+  std::vector<uint8_t> moved_clusters((size_t)n, 0);
+  if (cfg->async) {
+    std::vector<uint32_t> order;
+    for (int64_t v = 0; v < n; ++v)
+      if (frontier[v]) order.push_back((uint32_t)v);
+    if (cfg->permute) {  // vertexMapPermute, PCC.cc:69-109 (own seeded shuffle)
+      uint64_t st = SplitMix64(cfg->permute_seed ^ SplitMix64(round_seed));
+      for (size_t i = order.size(); i > 1; --i) {
+        st = SplitMix64(st);
+        std::swap(order[i - 1], order[(size_t)(st % i)]);
+      }
+    }
+    for (uint32_t v : order) {  // PCC.cc:133-138
+      if (AsyncMove(*g, *s, *cfg, v)) {
+        moved_clusters[s->cluster_ids[v]] = 1;
+        moved_vertex[v] = 1;
+      }
+    }
+  } else {
+    std::vector<uint32_t> moves((size_t)n);
+    oracle_sync_decisions(g, s, cfg, frontier, moves.data(), moved_vertex.data(), nullptr,
+                          nullptr);
+    oracle_apply_moves(n, s, moves.data(), moved_clusters.data());  // PCC.cc:163-165
+  }
+  if (cfg->move_method == 1) {  // ALL_MOVE, PCC.cc:299-307
+    std::memset(frontier, 1, (size_t)n);
+    return n;
+  }
+  std::vector<uint8_t> seed((size_t)n);
+  for (int64_t v = 0; v < n; ++v)  // PCC.cc:312-324
+    seed[(size_t)v] = cfg->move_method == 0 ? moved_clusters[s->cluster_ids[v]] : moved_vertex[v];
+  return ExpandFrontier(*g, seed, frontier);
+}
+
+int oracle_iterate_best_moves(const oracle_graph* g, oracle_state* s,
+                              const oracle_config* cfg, int64_t inner, uint64_t seed,
+                              int64_t* out_rounds) {
+  // IterateBestMoves, PCC.cc:340-365.
+  std::vector<uint8_t> frontier((size_t)g->n, 1);
+  bool moved = false, local_moved = true;
+  int64_t it = 0;
+  for (it = 0; it < inner && local_moved; ++it) {
+    const int64_t sz = oracle_round(g, s, cfg, frontier.data(), seed * 1315423911ull + (uint64_t)it);
+    local_moved = sz != 0;
+    moved |= local_moved;
+  }
+  if (out_rounds) *out_rounds += it;
+  return moved ? 1 : 0;
+}
+
+oracle_csr* oracle_compress(const oracle_graph* g, const uint32_t* cluster_ids,
+                            const double* node_weights) {
+  auto* c = new OwnedCsr();
+  Compress(*g, cluster_ids, node_weights, c);
+  return reinterpret_cast<oracle_csr*>(c);
+}
+
+void oracle_csr_sizes(const oracle_csr* c, int64_t* n, int64_t* m) {
+  const auto* o = reinterpret_cast<const OwnedCsr*>(c);
+  *n = o->n;
+  *m = o->m;
+}
+
+void oracle_csr_copy(const oracle_csr* c, int64_t* offsets, uint32_t* nbrs, float* weights,
+                     double* node_weights) {
+  const auto* o = reinterpret_cast<const OwnedCsr*>(c);
+  if (offsets) std::copy(o->offsets.begin(), o->offsets.end(), offsets);
+  if (nbrs) std::copy(o->nbrs.begin(), o->nbrs.end(), nbrs);
+  if (weights) std::copy(o->weights.begin(), o->weights.end(), weights);
+  if (node_weights) std::copy(o->node_weights.begin(), o->node_weights.end(), node_weights);
+}
+
+void oracle_csr_free(oracle_csr* c) { delete reinterpret_cast<OwnedCsr*>(c); }
+
+double oracle_objective(const oracle_graph* g, const uint32_t* ids, const double* nw,
+                        double resolution, double offset) {
+  // ClusteringHelper::ComputeObjective, PCCI.cc:118-149.
+  const int64_t n = g->n;
+  std::vector<double> K((size_t)n, 0.0);
+  for (int64_t v = 0; v < n; ++v) K[ids[v]] += nw[v];
+  std::vector<double> shifted((size_t)n), tmp;
+  for (int64_t i = 0; i < n; ++i) {
+    tmp.clear();
+    for (int64_t p = g->offsets[i]; p < g->offsets[i + 1]; ++p) {
+      const float weight = g->weights[p];
+      tmp.push_back(ids[i] == ids[g->nbrs[p]] ? (weight - offset) / 2 : 0.0);
+    }
+    shifted[(size_t)i] =
+        PbbsReduce(tmp.data(), tmp.size(), 0.0, [](double a, double b) { return a + b; });
+  }
+  auto add = [](double a, double b) { return a + b; };
+  double objective = PbbsReduce(shifted.data(), shifted.size(), 0.0, add);
+  std::vector<double> rs((size_t)n);
+  for (int64_t i = 0; i < n; ++i) rs[(size_t)i] = nw[i] * (K[ids[i]] - nw[i]);
+  objective -= resolution * PbbsReduce(rs.data(), rs.size(), 0.0, add) / 2;
+  return objective;
+}
+
+double oracle_modularity(const oracle_graph* g, const uint32_t* ids, double resolution) {
+  // ComputeModularity, PMC.cc:36-66.  Clusters are visited in ascending id
+  // order, as OutputIndicesById lists them.
+  double total_edge_weight = 0;
+  double modularity = 0;
+  for (int64_t i = 0; i < g->n; ++i) {
+    for (int64_t p = g->offsets[i]; p < g->offsets[i + 1]; ++p) {
+      total_edge_weight++;
+      if (ids[i] == ids[g->nbrs[p]]) modularity++;
+    }
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> order((size_t)g->n);
+  for (int64_t i = 0; i < g->n; ++i) order[(size_t)i] = {ids[i], (uint32_t)i};
+  std::sort(order.begin(), order.end());
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    double degree = 0;
+    while (j < order.size() && order[j].first == order[i].first) {
+      const uint32_t v = order[j].second;
+      degree += (double)(g->offsets[v + 1] - g->offsets[v]);
+      ++j;
+    }
+    modularity -= (resolution * degree * degree) / (total_edge_weight);
+    i = j;
+  }
+  modularity = modularity / (total_edge_weight);
+  return modularity;
+}
+
+void oracle_flatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out) {
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = ids[i] == kNone ? kNone : compressed[ids[i]];  // PGU.cc:237-241
+}
+
+void oracle_rank_relabel(int64_t n, const uint32_t* ids, uint32_t* out) {
+  std::vector<uint32_t> sorted(ids, ids + n);
+  std::sort(sorted.begin(), sorted.end());
+  sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), ids[i]) - sorted.begin());
+}
+
+int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modularity,
+                   const double* node_weights, const uint32_t* initial_ids, uint32_t* out_ids,
+                   oracle_result* out) {
+  oracle_config cfg = *cfg_in;
+  const int64_t n = g->n;
+  std::vector<double> nw0((size_t)n, 1.0);
+  if (node_weights) {
+    std::copy(node_weights, node_weights + n, nw0.begin());
+  } else if (modularity) {
+    // ComputeModularityConfig, PGU.cc:141-153.
+    for (int64_t v = 0; v < n; ++v) nw0[(size_t)v] = (double)(g->offsets[v + 1] - g->offsets[v]);
+  }
+  const double gamma = cfg.resolution;
+  if (modularity) {  // PMC.cc:74-79
+    const double total = PbbsReduce(nw0.data(), nw0.size(), 0.0,
+                                    [](double a, double b) { return a + b; });
+    cfg.resolution = gamma / total;
+    cfg.edge_weight_offset = 0;
+  }
+  // RefineClusters_subroutine, PCC.cc:476-629.
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t num_iterations = 0, num_inner = 0;
+  if (cfg.clustering_moves_method == 1) {  // PCC.cc:493-500
+    num_iterations = cfg.louvain_num_iterations > 0 ? cfg.louvain_num_iterations : 10;
+    num_inner = cfg.louvain_num_inner_iterations > 0 ? cfg.louvain_num_inner_iterations : 10;
+  } else {
+    return 12;  // DEFAULT_CLUSTER_MOVES is not on this path
+  }
+  int64_t rounds = 0;
+  g_edges_scanned = 0;
+  std::vector<Level> levels;
+  levels.reserve((size_t)num_iterations + 1);
+  levels.emplace_back();
+  {
+    Level& L = levels.back();
+    L.nw = nw0;
+    L.ids.assign((size_t)n, 0);
+    L.sizes.assign((size_t)n, 0);
+    L.K.assign((size_t)n, 0.0);
+    if (initial_ids) {  // SetClustering with a non-empty clustering, PCCI.cc:68-75
+      for (int64_t j = 0; j < n; ++j) {
+        L.ids[(size_t)j] = initial_ids[j];
+        L.sizes[initial_ids[j]] += 1;
+        L.K[initial_ids[j]] += L.nw[(size_t)j];
+      }
+    } else {
+      for (int64_t j = 0; j < n; ++j) {
+        L.ids[(size_t)j] = (uint32_t)j;
+        L.sizes[(size_t)j] = 1;
+        L.K[(size_t)j] = 0 + L.nw[(size_t)j];
+      }
+    }
+  }
+  std::vector<uint32_t> cluster_ids((size_t)n);
+  std::iota(cluster_ids.begin(), cluster_ids.end(), 0u);
+  auto graph_of = [&](size_t lvl) { return lvl == 0 ? *g : levels[lvl].csr.View(); };
+  int64_t iter = 0;
+  for (iter = 0; iter < num_iterations; ++iter) {
+    oracle_graph cg = graph_of((size_t)iter);
+    oracle_state st = levels[(size_t)iter].State();
+    const int moved = oracle_iterate_best_moves(&cg, &st, &cfg, num_inner,
+                                                cfg.permute_seed + (uint64_t)iter, &rounds);
+    if (!moved) {  // PCC.cc:543-547
+      iter--;
+      break;
+    }
+    if (!cfg.refine) {  // PCC.cc:550
+      std::vector<uint32_t> flat((size_t)n);
+      oracle_flatten(n, cluster_ids.data(), levels[(size_t)iter].ids.data(), flat.data());
+      cluster_ids.swap(flat);
+    }
+    if (iter == num_iterations - 1) break;  // PCC.cc:556
+    levels.emplace_back();                  // CompressGraph, PCC.cc:569-571
+    Level& cur = levels[(size_t)iter];
+    Level& nxt = levels.back();
+    Compress(cg, cur.ids.data(), cur.nw.data(), &nxt.csr);
+    const int64_t n2 = nxt.csr.n;
+    nxt.nw = nxt.csr.node_weights;  // singletons with aggregated weights, PCC.cc:583-585
+    nxt.ids.resize((size_t)n2);
+    nxt.sizes.assign((size_t)n2, 1);
+    nxt.K.resize((size_t)n2);
+    for (int64_t i = 0; i < n2; ++i) {
+      nxt.ids[(size_t)i] = (uint32_t)i;
+      nxt.K[(size_t)i] = nxt.nw[(size_t)i];
+    }
+    if (!cfg.refine) {
+      // Without refinement only the current level is needed.
+      cur.ids.clear();
+      cur.ids.shrink_to_fit();
+    }
+  }
+  if (cfg.refine && iter > 0) {  // PCC.cc:589-607
+    for (int64_t i = iter - 1; i >= 0; --i) {
+      oracle_graph cg = graph_of((size_t)i);
+      Level& L = levels[(size_t)i];
+      const int64_t ni = cg.n;
+      std::vector<uint32_t> flat((size_t)ni);
+      oracle_flatten(ni, L.ids.data(), levels[(size_t)i + 1].ids.data(), flat.data());
+      oracle_state st = L.State();
+      ResetFromIds(ni, st, flat);
+      oracle_iterate_best_moves(&cg, &st, &cfg, num_inner, cfg.permute_seed + 7919ull * (uint64_t)(i + 1),
+                                &rounds);
+    }
+    cluster_ids = levels[0].ids;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  std::copy(cluster_ids.begin(), cluster_ids.end(), out_ids);
+  if (out) {
+    out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->levels = iter + 1;
+    out->rounds = rounds;
+    out->edges_scanned = g_edges_scanned;
+    std::vector<uint32_t> ranks((size_t)n);
+    oracle_rank_relabel(n, cluster_ids.data(), ranks.data());
+    uint32_t k = 0;
+    for (int64_t v = 0; v < n; ++v) k = std::max(k, ranks[(size_t)v] + 1);
+    out->num_clusters = n ? k : 0;
+    out->objective = oracle_objective(g, ranks.data(), nw0.data(), cfg.resolution,
+                                      cfg.edge_weight_offset);
+    out->modularity = modularity ? oracle_modularity(g, ranks.data(), gamma) : 0.0;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// k_round.hip -- the rest of one best-move round: the sync apply
+// (ClusteringHelper::MoveNodesToCluster,
+// clustering/clusterers/parallel-correlation-clusterer-internal.cc:256-369),
+// the empty-id reuse for vertices forming new clusters (same function,
+// :339-366) and the next-frontier expansion
+// (clustering/clusterers/parallel-correlation-clusterer.cc:299-337).
+//
+// Cluster sizes and weights are updated with device atomics.  The reference
+// subtracts/adds node weights group by group; with integer node weights (unit
+// weights, degrees, and their aggregates on contracted levels) every partial
+// sum is an exact integer, so the atomic order cannot change a result.
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace pcc {
+
+namespace {
+
+struct Bins {
+  int64_t base[kBins];
+  int64_t count[kBins];
+};
+
+__global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __restrict__ list,
+                                               StateView st, uint32_t n,
+                                               const uint32_t* __restrict__ moves,
+                                               uint8_t* __restrict__ modified,
+                                               uint8_t* __restrict__ new_flag,
+                                               RoundCounters* __restrict__ rc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long new_movers = 0;
+  for (int b = 0; b < kBins; ++b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bins.count[b];
+         i += stride) {
+      const uint32_t v = list[bins.base[b] + i];
+      const uint32_t t = moves[v];
+      if (t == kNoneId) continue;
+      const uint32_t cur = st.ids[v];
+      if (t == cur) continue;
+      const double k = st.nw[v];
+      atomicSub(&st.sizes[cur], 1u);
+      atomicAdd(&st.K[cur], -k);
+      modified[cur] = 1;
+      if (t != n) {
+        atomicAdd(&st.sizes[t], 1u);
+        atomicAdd(&st.K[t], k);
+        modified[t] = 1;
+        st.ids[v] = t;
+      } else {
+        new_flag[v] = 1;
+        ++new_movers;
+      }
+    }
+  }
+  for (int s = 32; s > 0; s >>= 1) new_movers += __shfl_xor(new_movers, s);
+  if (LaneId() == 0 && new_movers) atomicAdd(&rc->new_count, new_movers);
+}
+
+__global__ __launch_bounds__(256) void k_zero_flags(uint32_t n, const uint32_t* __restrict__ sizes,
+                                                    uint8_t* __restrict__ z,
+                                                    const unsigned long long* __restrict__ gate) {
+  if (*gate == 0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    z[i] = sizes[i] == 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_compact(uint32_t n, const uint8_t* __restrict__ z,
+                                                      const uint32_t* __restrict__ zrank,
+                                                      uint32_t* __restrict__ zero_list,
+                                                      const unsigned long long* __restrict__ gate) {
+  if (*gate == 0) return;
+  const unsigned long long need = *gate;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (z[i] && zrank[i] < need) zero_list[zrank[i]] = (uint32_t)i;
+}
+
+// i-th new-cluster mover in ascending vertex order takes the i-th empty id in
+// ascending order (PCCI.cc:346-365).
+__global__ __launch_bounds__(256) void k_new_assign(uint32_t n, StateView st,
+                                                    uint8_t* __restrict__ new_flag,
+                                                    const uint32_t* __restrict__ nrank,
+                                                    const uint32_t* __restrict__ zero_list,
+                                                    uint8_t* __restrict__ modified,
+                                                    const unsigned long long* __restrict__ gate) {
+  if (*gate == 0) return;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (!new_flag[v]) continue;
+    const uint32_t c = zero_list[nrank[v]];
+    new_flag[v] = 0;
+    // Sizes summing to n guarantee enough empty ids; inconsistent caller state
+    // must not turn into an out-of-range write.
+    if (c >= n) continue;
+    st.ids[v] = c;
+    st.sizes[c] = 1;
+    modified[c] = 1;
+    st.K[c] = st.nw[v];
+  }
+}
+
+// 16 lanes per seed vertex mark its neighbours.
+__global__ __launch_bounds__(256) void k_expand_moved(Bins bins, const uint32_t* __restrict__ list,
+                                                      const int64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint8_t* __restrict__ moved,
+                                                      uint8_t* __restrict__ next) {
+  const int sub = threadIdx.x & 15;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
+  for (int b = 0; b < kBins; ++b) {
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; i < bins.count[b];
+         i += stride) {
+      const uint32_t v = list[bins.base[b] + i];
+      if (!moved[v]) continue;
+      for (int64_t p = off[v] + sub; p < off[v + 1]; p += 16) next[nbr[p]] = 1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_expand_clusters(int64_t n, const int64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ ids,
+                                                         const uint8_t* __restrict__ modified,
+                                                         uint8_t* __restrict__ next) {
+  const int sub = threadIdx.x & 15;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
+  for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; v < n; v += stride) {
+    if (!modified[ids[v]]) continue;
+    for (int64_t p = off[v] + sub; p < off[v + 1]; p += 16) next[nbr[p]] = 1;
+  }
+}
+
+int Grid(int64_t items, int per_block, int max_blocks) {
+  int64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+Bins MakeBins(const int64_t counts[kBins], const BinLayout& layout, int64_t* total) {
+  Bins b;
+  *total = 0;
+  for (int i = 0; i < kBins; ++i) {
+    b.base[i] = layout.base[i];
+    b.count[i] = counts[i];
+    *total += counts[i];
+  }
+  return b;
+}
+
+}  // namespace
+
+void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
+                 const StateView& st, uint32_t n, const uint32_t* moves, uint8_t* modified,
+                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s) {
+  int64_t total;
+  const Bins b = MakeBins(counts, layout, &total);
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_apply, dim3(Grid(total, 256, 4096)), dim3(256), 0, s, b, list, st, n, moves,
+                     modified, new_flag, rc);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n, uint8_t* new_flag,
+                         uint8_t* modified, RoundCounters* rc, hipStream_t s) {
+  const unsigned long long* gate = &rc->new_count;
+  DevBuf<uint8_t> z(al, n);
+  DevBuf<uint32_t> zrank(al, n), zero_list(al, n), nrank(al, n);
+  const int grid = Grid(n, 256, 4096);
+  PCC_HIP_CHECK(hipMemsetAsync(zero_list.get(), 0xFF, sizeof(uint32_t) * (size_t)n, s));
+  hipLaunchKernelGGL(k_zero_flags, dim3(grid), dim3(256), 0, s, n, st.sizes, z.get(), gate);
+  ExclusiveScanU8toU32(al, z.get(), zrank.get(), n, nullptr, s, gate);
+  hipLaunchKernelGGL(k_zero_compact, dim3(grid), dim3(256), 0, s, n, z.get(), zrank.get(),
+                     zero_list.get(), gate);
+  ExclusiveScanU8toU32(al, new_flag, nrank.get(), n, nullptr, s, gate);
+  hipLaunchKernelGGL(k_new_assign, dim3(grid), dim3(256), 0, s, n, st, new_flag, nrank.get(),
+                     zero_list.get(), modified, gate);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchExpandMoved(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
+                       const CsrView& g, const uint8_t* moved, uint8_t* next, hipStream_t s) {
+  int64_t total;
+  const Bins b = MakeBins(counts, layout, &total);
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_expand_moved, dim3(Grid(total, 16, 8192)), dim3(256), 0, s, b, list, g.off,
+                     g.nbr, moved, next);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                          uint8_t* next, hipStream_t s) {
+  if (g.n == 0) return;
+  hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 16, 8192)), dim3(256), 0, s, g.n, g.off,
+                     g.nbr, ids, modified, next);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pcc

@@ -1,0 +1,277 @@
+// kernels.h -- host-side launchers of the gfx950 kernels and the small device
+// allocator they draw temporaries from.  Everything runs on one HIP stream
+// per graph handle.
+#ifndef PCC_AMD_KERNELS_H_
+#define PCC_AMD_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace pcc {
+
+// ------------------------------------------------------------- allocator --
+
+// Caching device allocator: freed blocks go to per-size free lists and are
+// reused without hipFree, so a clustering call never synchronizes on memory
+// management after its first level.
+class DeviceAllocator {
+ public:
+  ~DeviceAllocator();
+  void* Alloc(size_t bytes);  // throws std::bad_alloc-like AllocError
+  void Free(void* p);
+  void ReleaseAll();
+  size_t BytesInUse() const { return in_use_; }
+
+ private:
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> live_;
+  size_t in_use_ = 0;
+};
+
+struct AllocError {
+  size_t bytes;
+};
+
+// Owning typed device buffer tied to an allocator.
+template <typename T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(DeviceAllocator* a, int64_t n) { Reset(a, n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : a_(o.a_), p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      Release();
+      a_ = o.a_;
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { Release(); }
+  void Reset(DeviceAllocator* a, int64_t n) {
+    Release();
+    a_ = a;
+    n_ = n;
+    p_ = n > 0 ? static_cast<T*>(a->Alloc(sizeof(T) * (size_t)n)) : nullptr;
+  }
+  void Release() {
+    if (p_ && a_) a_->Free(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  int64_t size() const { return n_; }
+
+ private:
+  DeviceAllocator* a_ = nullptr;
+  T* p_ = nullptr;
+  int64_t n_ = 0;
+};
+
+#define PCC_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) throw ::pcc::HipError{_e, #expr, __FILE__, __LINE__}; \
+  } while (0)
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+  const char* file;
+  int line;
+};
+
+// ------------------------------------------------------------- structures --
+
+struct CsrView {
+  int64_t n = 0;
+  int64_t m = 0;
+  const int64_t* off = nullptr;
+  const uint32_t* nbr = nullptr;
+  const float* w = nullptr;
+};
+
+// ClusteringHelper SoA (parallel-correlation-clusterer-internal.h:158-164).
+struct StateView {
+  uint32_t* ids = nullptr;
+  uint32_t* sizes = nullptr;
+  const double* nw = nullptr;
+  double* K = nullptr;
+};
+
+// Degree bins of the best-move kernels.
+constexpr int kBins = 8;
+// 0..4: deg <= 4, 8, 16, 32, 64 -> 4..64 lanes per vertex, ordered shuffles
+// 5: deg <= wave_max   -> one wave, LDS hash
+// 6: deg <= block_max  -> one workgroup, LDS hash
+// 7: larger            -> one workgroup, global-memory hash
+struct BinThresholds {
+  int64_t wave_max = 512;
+  int64_t block_max = 4096;
+};
+
+// Device-resident per-round counters.
+struct RoundCounters {
+  unsigned long long cnt[kBins];
+  unsigned long long te[kBins];
+  unsigned long long fb_count;
+  unsigned long long new_count;
+  unsigned long long moved_count;
+  unsigned long long pad;
+};
+
+struct BinLayout {
+  int64_t base[kBins];
+};
+
+struct MoveParams {
+  double res = 0;
+  double offset = 0;
+  uint32_t n = 0;
+  int force_ordered = 0;
+};
+
+struct MoveOut {
+  uint32_t* moves = nullptr;      // sync: target or kNoneId
+  uint8_t* moved = nullptr;       // moved_vertex
+  uint8_t* moved_clusters = nullptr;  // async: destination clusters
+  uint32_t* dbg_target = nullptr;  // EfficientBestMove's return (may be null)
+  double* dbg_delta = nullptr;
+};
+
+// Scratch for the global-memory hash bin and the ordered fallback path.
+struct HashScratch {
+  uint32_t* keys = nullptr;
+  double* vals = nullptr;
+  uint32_t cap = 0;   // entries per workgroup region (power of 2)
+  int regions = 0;    // number of workgroup regions
+};
+
+// ---------------------------------------------------------------- launchers --
+
+// Bins the frontier (frontier == nullptr: every vertex) into list regions
+// list[layout.base[b] + i], counting into rc->cnt / rc->te.
+void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThresholds& th,
+                       const BinLayout& layout, uint32_t* list, RoundCounters* rc,
+                       hipStream_t s);
+
+// Histogram of bins over all vertices (capacity of each list region).
+void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
+                       hipStream_t s);
+
+// One best-move kernel per bin.  count = number of listed vertices in the bin.
+void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, const StateView& st,
+                       const MoveParams& mp, const uint32_t* list, const MoveOut& out,
+                       uint32_t* fb_list, RoundCounters* rc, const HashScratch& hs,
+                       uint64_t permute_salt, hipStream_t s);
+
+// Ordered (sequential-order-exact) best moves for the fallback list.
+void LaunchBestMoveOrdered(const CsrView& g, const StateView& st, const MoveParams& mp,
+                           const uint32_t* fb_list, const RoundCounters* rc,
+                           const MoveOut& out, const HashScratch& hs, hipStream_t s);
+
+// Sync apply, MoveNodesToCluster (parallel-correlation-clusterer-internal.cc:256-369).
+void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
+                 const StateView& st, uint32_t n, const uint32_t* moves, uint8_t* modified,
+                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s);
+
+// Empty-id reuse for vertices forming new clusters; gated on rc->new_count.
+void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
+                         uint8_t* new_flag, uint8_t* modified, RoundCounters* rc,
+                         hipStream_t s);
+
+// Next frontier = neighbours of the seeds.
+void LaunchExpandMoved(const int64_t counts[kBins], const BinLayout& layout,
+                       const uint32_t* list, const CsrView& g, const uint8_t* moved,
+                       uint8_t* next, hipStream_t s);
+void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                          uint8_t* next, hipStream_t s);
+
+// ------------------------------------------------------------- primitives --
+
+// Exclusive scan; total (device pointer, may be null) receives the sum.  If
+// gate is non-null and *gate == 0 the kernels return immediately.
+void ExclusiveScanU8toU32(DeviceAllocator* al, const uint8_t* in, uint32_t* out, int64_t n,
+                          uint32_t* total, hipStream_t s,
+                          const unsigned long long* gate = nullptr);
+void ExclusiveScanU32toI64(DeviceAllocator* al, const uint32_t* in, int64_t* out, int64_t n,
+                           int64_t* total, hipStream_t s);
+void ExclusiveScanU32toU32(DeviceAllocator* al, const uint32_t* in, uint32_t* out, int64_t n,
+                           uint32_t* total, hipStream_t s);
+
+void ReduceMaxU32(DeviceAllocator* al, const uint32_t* in, int64_t n, uint32_t* out,
+                  hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on key bits [begin_bit, end_bit).
+void RadixSortPairsF32(DeviceAllocator* al, uint64_t* keys, float* vals, int64_t n, int begin_bit,
+                       int end_bit, hipStream_t s);
+void RadixSortPairsF64(DeviceAllocator* al, uint64_t* keys, double* vals, int64_t n,
+                       int begin_bit, int end_bit, hipStream_t s);
+
+// ------------------------------------------------------------ contraction --
+
+struct DevCsr {
+  int64_t n = 0;
+  int64_t m = 0;
+  DevBuf<int64_t> off;
+  DevBuf<uint32_t> nbr;
+  DevBuf<float> w;
+  CsrView View() const {
+    CsrView v;
+    v.n = n;
+    v.m = m;
+    v.off = off.get();
+    v.nbr = nbr.get();
+    v.w = w.get();
+    return v;
+  }
+};
+
+// CompressGraph (parallel-correlation-clusterer-internal.cc:709-782).
+void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
+                   DevCsr* out, DevBuf<double>* out_nw, hipStream_t s);
+
+// FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
+void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,
+                   hipStream_t s);
+
+// Compact rank relabel + ResetClustering/SetClustering
+// (parallel-correlation-clusterer.cc:594-600): ids := rank of id among the
+// distinct ids; sizes/K rebuilt.
+void ResetClusteringFromIds(DeviceAllocator* al, int64_t n, const uint32_t* flat,
+                            const StateView& st, hipStream_t s);
+
+// Singleton state (ids = i, sizes = 1, K = nw).
+void LaunchSingletons(int64_t n, const StateView& st, hipStream_t s);
+
+// SetClustering from per-vertex cluster indices (initial clustering).
+void LaunchSetClustering(int64_t n, const uint32_t* init, const StateView& st, hipStream_t s);
+
+// Node weights = degrees (ComputeModularityConfig,
+// parallel/parallel-graph-utils.cc:141-153).
+void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s);
+
+// Fixed-order double sum.
+void SumF64(DeviceAllocator* al, const double* in, int64_t n, double* out, hipStream_t s);
+
+// ComputeObjective (parallel-correlation-clusterer-internal.cc:118-149).
+void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
+                      const double* nw, double res, double offset, double* out_dev,
+                      hipStream_t s);
+
+// ComputeModularity (parallel-modularity-clusterer.cc:36-66).
+void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, double gamma,
+                       double* out_dev, hipStream_t s);
+
+}  // namespace pcc
+
+#endif  // PCC_AMD_KERNELS_H_

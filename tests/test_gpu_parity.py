@@ -1,0 +1,174 @@
+"""HIP path vs the CPU oracle, phase by phase and end to end.
+
+Sync mode must be bit-exact: per-vertex best-move targets and FP64 deltas,
+cluster state after a round, the next frontier, the contracted CSR (float
+weights compared bit for bit) and the final cluster ids.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _graphs(pcc):
+    """Seeded inputs covering every degree bin, signed weights and self-loops."""
+    out = {
+        "sbm_unit": pcc.gen_sbm(600, 12, 0.15, 0.01, seed=11),
+        "rmat_signed": pcc.gen_rmat(12, 8, w_lo=-1.0, w_hi=1.0, seed=12),
+        "rmat_hubs": pcc.gen_rmat(15, 16, w_lo=-1.0, w_hi=1.0, seed=13),
+        "rmat_unit_hubs": pcc.gen_rmat(15, 16, seed=14),
+    }
+    return out
+
+
+def _dev(pcc, csr):
+    return pcc.DeviceGraph.from_csr(csr)
+
+
+def _compressed(pcc, orc, csr, rng):
+    """A contracted graph (self-loops, aggregated weights) from the oracle."""
+    g = orc.G.from_csr(csr)
+    ids = rng.integers(0, csr.n // 3 + 1, size=csr.n).astype(np.uint32)
+    cg, nw2 = orc.compress(g, ids, np.ones(csr.n))
+    return pcc.Csr(cg.n, cg.offsets, cg.nbrs, cg.weights), nw2
+
+
+@pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs", "rmat_unit_hubs"])
+@pytest.mark.parametrize("state_kind", ["singletons", "random"])
+def test_best_moves_bit_exact(pcc, orc, name, state_kind):
+    csr = _graphs(pcc)[name]
+    print(f"best_moves {name} {state_kind} n={csr.n} m={csr.m}", flush=True)
+    rng = np.random.default_rng(7)
+    n = csr.n
+    if state_kind == "singletons":
+        ids, sizes, nw, K = random_state(rng, n, n)
+        ids = np.arange(n, dtype=np.uint32)
+        sizes = np.ones(n, np.uint32)
+        K = nw.copy()
+    else:
+        ids, sizes, nw, K = random_state(rng, n, max(2, n // 5))
+    for res, off in [(0.01, 0.0), (0.05, 0.0), (0.3, 0.25)]:
+        cfg = pcc.make_config(resolution=res, edge_weight_offset=off)
+        ocfg = orc.config_from(cfg)
+        dg = _dev(pcc, csr)
+        g_moves, g_moved, g_t, g_d = dg.best_moves(cfg, ids, sizes, nw, K)
+        o_moves, o_moved, o_t, o_d = orc.sync_decisions(orc.G.from_csr(csr),
+                                                        orc.S(ids, sizes, nw, K), ocfg)
+        assert np.array_equal(g_t, o_t), f"target mismatch res={res}"
+        assert np.array_equal(g_d.view(np.uint64), o_d.view(np.uint64)), f"delta mismatch res={res}"
+        assert np.array_equal(g_moves, o_moves)
+        assert np.array_equal(g_moved, o_moved)
+
+
+@pytest.mark.parametrize("name", ["rmat_signed", "rmat_hubs"])
+def test_ordered_path_matches(pcc, orc, name):
+    """The ordered fallback kernel alone reproduces the oracle (forced)."""
+    csr = _graphs(pcc)[name]
+    print(f"ordered {name} n={csr.n}", flush=True)
+    rng = np.random.default_rng(3)
+    ids, sizes, nw, K = random_state(rng, csr.n, max(2, csr.n // 7))
+    cfg = pcc.make_config(resolution=0.05)
+    dg = _dev(pcc, csr)
+    dg.set_option(pcc.OPT_FORCE_ORDERED, 1)
+    g = dg.best_moves(cfg, ids, sizes, nw, K)
+    o = orc.sync_decisions(orc.G.from_csr(csr), orc.S(ids, sizes, nw, K), orc.config_from(cfg))
+    for a, b in zip(g, o):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+
+
+def test_best_moves_compressed_graph(pcc, orc):
+    rng = np.random.default_rng(5)
+    csr, nw2 = _compressed(pcc, orc, _graphs(pcc)["rmat_signed"], rng)
+    print(f"compressed best_moves n={csr.n} m={csr.m}", flush=True)
+    n = csr.n
+    ids = np.arange(n, dtype=np.uint32)
+    sizes = np.ones(n, np.uint32)
+    cfg = pcc.make_config(resolution=0.05)
+    g = _dev(pcc, csr).best_moves(cfg, ids, sizes, nw2, nw2.copy())
+    o = orc.sync_decisions(orc.G.from_csr(csr), orc.S(ids, sizes, nw2, nw2.copy()),
+                           orc.config_from(cfg))
+    for a, b in zip(g, o):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+
+
+@pytest.mark.parametrize("move_method", [0, 1, 2])
+@pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs"])
+def test_round_bit_exact(pcc, orc, name, move_method):
+    csr = _graphs(pcc)[name]
+    print(f"round {name} move_method={move_method}", flush=True)
+    rng = np.random.default_rng(9)
+    n = csr.n
+    ids = np.arange(n, dtype=np.uint32)
+    sizes = np.ones(n, np.uint32)
+    nw = np.ones(n)
+    K = nw.copy()
+    frontier = np.ones(n, np.uint8)
+    cfg = pcc.make_config(resolution=0.05, move_method=move_method)
+    ocfg = orc.config_from(cfg)
+    dg = _dev(pcc, csr)
+    og = orc.G.from_csr(csr)
+    ostate = orc.S(ids, sizes, nw, K)
+    gstate = (ids, sizes, nw, K)
+    ofr = frontier.copy()
+    gfr = frontier.copy()
+    for r in range(4):
+        gi, gs, gk, gfr, gsize = dg.round(cfg, gstate[0], gstate[1], nw, gstate[3], gfr)
+        ofr, osize = orc.round_(og, ostate, ocfg, ofr)
+        assert np.array_equal(gi, ostate.ids), f"ids differ after round {r}"
+        assert np.array_equal(gs, ostate.sizes), f"sizes differ after round {r}"
+        assert np.array_equal(gk, ostate.K), f"cluster weights differ after round {r}"
+        assert np.array_equal(gfr, ofr), f"frontier differs after round {r}"
+        assert gsize == osize
+        gstate = (gi, gs, nw, gk)
+        if osize == 0:
+            break
+
+
+@pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs"])
+def test_compress_bit_exact(pcc, orc, name):
+    csr = _graphs(pcc)[name]
+    print(f"compress {name}", flush=True)
+    rng = np.random.default_rng(2)
+    for k in (1, 7, csr.n // 10 + 1, csr.n):
+        ids = rng.integers(0, k, size=csr.n).astype(np.uint32)
+        nw = rng.integers(1, 5, size=csr.n).astype(np.float64)
+        dg = _dev(pcc, csr)
+        cg, gnw = dg.compress(ids, nw)
+        c = cg.to_csr()
+        og, onw = orc.compress(orc.G.from_csr(csr), ids, nw)
+        assert c.n == og.n and c.m == og.m
+        assert np.array_equal(c.offsets, og.offsets)
+        assert np.array_equal(c.nbrs, og.nbrs)
+        assert np.array_equal(c.weights.view(np.uint32), og.weights.view(np.uint32))
+        assert np.array_equal(gnw, onw)
+
+
+CLUSTER_CASES = [
+    # (graph, config kwargs, modularity)
+    ("sbm_unit", dict(resolution=0.01, move_method=1, louvain_num_iterations=10,
+                      louvain_num_inner_iterations=10), False),
+    ("rmat_signed", dict(resolution=0.05, move_method=2, refine=1), False),
+    ("rmat_signed", dict(resolution=0.05, move_method=0, refine=0), False),
+    ("rmat_hubs", dict(resolution=0.05, move_method=2, refine=1), False),
+    ("rmat_unit_hubs", dict(resolution=1.0, move_method=0, refine=1), True),
+    ("sbm_unit", dict(resolution=0.01, move_method=1, refine=1, louvain_num_iterations=1), False),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CLUSTER_CASES)))
+def test_cluster_bit_exact(pcc, orc, case):
+    name, kw, modularity = CLUSTER_CASES[case]
+    csr = _graphs(pcc)[name]
+    print(f"cluster {name} {kw} modularity={modularity}", flush=True)
+    cfg = pcc.make_config(**kw)
+    dg = _dev(pcc, csr)
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), modularity)
+    assert np.array_equal(ids, r.ids)
+    assert st.num_clusters == r.num_clusters
+    assert st.objective == pytest.approx(r.objective, rel=1e-9, abs=1e-6)
+    if modularity:
+        assert st.modularity == pytest.approx(r.modularity, rel=1e-9, abs=1e-12)

@@ -1,0 +1,160 @@
+"""Host-side pieces that run without a GPU: the C ABI surface, the text-format
+config parser, the seeded generators and the GBBS adjacency reader/writer."""
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pcc_amd.h")
+LIB = os.path.join(ROOT, "parallel-correlation-clustering_amd", "libpcc_amd.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)
+    return sorted({n for n in names if n.startswith("pcc_")})
+
+
+def test_library_exports_every_declared_symbol(pcc):
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = declared_functions()
+    assert len(declared) >= 25
+    missing = [d for d in declared if d not in exported]
+    assert not missing, missing
+    lib = pcc.lib()
+    for d in declared:
+        getattr(lib, d)
+    assert set(pcc.exported_symbols()) <= set(declared)
+
+
+def test_config_defaults(pcc):
+    c = pcc.default_config()
+    assert c.resolution == 0.0 and c.edge_weight_offset == 0.0
+    assert c.clustering_moves_method == pcc.LOUVAIN            # config.proto:127-128
+    assert c.move_method == pcc.NBHR_CLUSTER_MOVE              # config.proto:159-160
+    assert not c.refine and not c.async_ and not c.permute
+    assert c.louvain_num_iterations == 0 and c.louvain_num_inner_iterations == 0
+
+
+def test_config_parse_readme_example(pcc):
+    # README.md:86 of the reference.
+    c = pcc.parse_config("correlation_clusterer_config {resolution: 0.01, refine: true, "
+                         "async: true, move_method: NBHR_MOVE}")
+    assert c.resolution == 0.01 and c.refine == 1 and c.async_ == 1
+    assert c.move_method == pcc.NBHR_MOVE
+
+
+def test_config_parse_full(pcc):
+    c = pcc.parse_config("""
+        correlation_clusterer_config {
+          resolution: 1e-3  edge_weight_offset: -0.5
+          louvain_config { num_iterations: 7 num_inner_iterations: 3 }
+          clustering_moves_method: LOUVAIN
+          move_method: 1  # ALL_MOVE by number
+          permute: true all_iter: false random_seed: 5
+          kcore_config { kcore_cutoff: 3 }
+        }""")
+    assert c.resolution == 1e-3 and c.edge_weight_offset == -0.5
+    assert c.louvain_num_iterations == 7 and c.louvain_num_inner_iterations == 3
+    assert c.move_method == pcc.ALL_MOVE and c.permute == 1 and c.random_seed == 5
+    bare = pcc.parse_config("resolution: 0.25 move_method: NBHR_CLUSTER_MOVE")
+    assert bare.resolution == 0.25
+
+
+@pytest.mark.parametrize("bad", [
+    "correlation_clusterer_config { resolutio: 1 }",
+    "correlation_clusterer_config { resolution: abc }",
+    "correlation_clusterer_config { move_method: SIDEWAYS }",
+    "correlation_clusterer_config { resolution: 1 ",
+    "unknown_config { }",
+])
+def test_config_parse_errors(pcc, bad):
+    with pytest.raises(pcc.StatusError) as e:
+        pcc.parse_config(bad)
+    assert e.value.code == pcc.INVALID_ARGUMENT
+
+
+def _check_simple_symmetric(csr):
+    n = csr.n
+    src = np.repeat(np.arange(n), np.diff(csr.offsets))
+    assert np.all(src != csr.nbrs)                           # no self-loops
+    key = src.astype(np.int64) * n + csr.nbrs
+    assert np.all(np.diff(key) > 0)                          # sorted, deduplicated
+    rev = csr.nbrs.astype(np.int64) * n + src
+    order = np.argsort(rev)
+    assert np.array_equal(rev[order], key)                   # symmetric
+    assert np.array_equal(csr.weights[order], csr.weights)   # symmetric weights
+
+
+def test_sbm_generator(pcc):
+    a = pcc.gen_sbm(10000, 100, 0.1, 0.001, seed=1)
+    b = pcc.gen_sbm(10000, 100, 0.1, 0.001, seed=1)
+    assert np.array_equal(a.nbrs, b.nbrs) and np.array_equal(a.offsets, b.offsets)
+    _check_simple_symmetric(a)
+    # Expected directed edges: 2 * (100 * C(100,2) * 0.1 + C(10000,2)-blocks * 0.001).
+    intra = 100 * 100 * 99 / 2
+    inter = 10000 * 9999 / 2 - intra
+    expect = 2 * (intra * 0.1 + inter * 0.001)
+    assert abs(a.m - expect) < 6 * np.sqrt(expect)
+    src = np.repeat(np.arange(a.n), np.diff(a.offsets))
+    same = (src // 100) == (a.nbrs // 100)
+    assert abs(same.sum() - 2 * intra * 0.1) < 6 * np.sqrt(2 * intra * 0.1)
+    assert np.all(a.weights == 1.0)
+    c = pcc.gen_sbm(10000, 100, 0.1, 0.001, seed=2)
+    assert not np.array_equal(a.nbrs, c.nbrs)
+
+
+def test_sbm_blocks_and_weights(pcc):
+    g = pcc.gen_sbm_blocks([5, 50, 500], 0.3, 0.01, 0.0, 1.0, seed=3)
+    _check_simple_symmetric(g)
+    assert g.n == 555
+    assert g.weights.min() >= 0.0 and g.weights.max() < 1.0
+
+
+def test_rmat_generator(pcc):
+    g = pcc.gen_rmat(12, 16, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
+    _check_simple_symmetric(g)
+    assert g.n == 4096
+    assert g.m <= 2 * 16 * 4096
+    assert g.weights.min() >= -1.0 and g.weights.max() < 1.0
+    deg = np.diff(g.offsets)
+    assert deg.max() > 20 * deg.mean()  # skewed
+    h = pcc.gen_rmat(12, 16, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
+    assert np.array_equal(g.weights, h.weights)
+
+
+def test_adjacency_roundtrip(pcc, tmp_path):
+    g = pcc.gen_rmat(9, 4, w_lo=-1, w_hi=1, seed=5)
+    p = str(tmp_path / "g.adj")
+    pcc.write_adjacency_graph(p, g, float_weighted=True)
+    h = pcc.read_adjacency_graph(p, float_weighted=True)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.nbrs, h.nbrs)
+    assert np.array_equal(g.weights, h.weights)
+    u = pcc.read_adjacency_graph(p, float_weighted=False)
+    assert np.all(u.weights == 1.0)
+    q = str(tmp_path / "u.adj")
+    pcc.write_adjacency_graph(q, g, float_weighted=False)
+    assert open(q).readline().strip() == "AdjacencyGraph"
+    with pytest.raises(pcc.StatusError) as e:
+        pcc.read_adjacency_graph(str(tmp_path / "missing.adj"))
+    assert e.value.code == pcc.NOT_FOUND
+    bad = tmp_path / "bad.adj"
+    bad.write_text("AdjacencyGraph\n2\n2\n0\n1\n1\n5\n")
+    with pytest.raises(pcc.StatusError) as e:
+        pcc.read_adjacency_graph(str(bad))
+    assert e.value.code == pcc.INVALID_ARGUMENT
+
+
+def test_output_indices_by_id(pcc):
+    ids = np.array([5, 2, 5, 0, 2, 2])
+    assert pcc.output_indices_by_id(ids) == [[3], [1, 4, 5], [0, 2]]
+    assert pcc.clustering_to_ids([[3], [1, 4, 5], [0, 2]], 6).tolist() == [2, 1, 2, 0, 1, 1]
+    with pytest.raises(pcc.StatusError):
+        pcc.clustering_to_ids([[0, 1]], 3)
