@@ -175,6 +175,32 @@ int pcc_compute_objective(pcc_graph* g, const pcc_config* cfg,
 int pcc_compute_modularity(pcc_graph* g, double resolution,
                            const uint32_t* cluster_ids, double* out_modularity);
 
+/* ---------------------------------------------------------------- batch -- */
+/* Many independent graphs clustered together: every phase of every graph runs
+ * in the same kernel launches (one launch per phase and bin for the whole
+ * batch instead of one per graph), while each graph keeps the control flow,
+ * id numbering and results of its own pcc_cluster run.  No counterpart in the
+ * reference, whose Cluster takes one graph; the per-graph semantics are those
+ * of ParallelCorrelationClusterer::Cluster / ParallelModularityClusterer::Cluster.
+ *
+ * Graph g has vertex_counts[g] >= 1 vertices; the CSR rows of all graphs are
+ * concatenated (offsets has sum(vertex_counts) + 1 entries, offsets[0] = 0)
+ * and each graph's neighbour ids are local to that graph. */
+int pcc_batch_from_csr(int64_t num_graphs, const int64_t* vertex_counts,
+                       const int64_t* offsets, const uint32_t* nbrs,
+                       const float* weights, pcc_graph** out);
+int64_t pcc_batch_num_graphs(const pcc_graph* g);
+
+/* Clusters every graph of a batch.  resolutions[g] (NULL: cfg->resolution)
+ * is graph g's resolution (its gamma for modularity); the other config fields
+ * apply to all graphs.  out_cluster_ids receives, for every vertex, its final
+ * cluster id local to its graph (what pcc_cluster returns for that graph
+ * alone).  out_objectives / out_modularities (may be NULL) receive one value
+ * per graph. */
+int pcc_batch_cluster(pcc_graph* batch, const pcc_config* cfg, int objective,
+                      const double* resolutions, uint32_t* out_cluster_ids,
+                      double* out_objectives, double* out_modularities, pcc_stats* stats);
+
 /* ------------------------------------------------------------ one phase -- */
 /* Phase-level entry points used by parity tests and by callers that drive the
  * Louvain loop themselves.  State arrays are the ClusteringHelper SoA

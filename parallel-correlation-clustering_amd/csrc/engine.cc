@@ -1,0 +1,736 @@
+// engine.cc -- host orchestration of the multilevel Louvain path.
+//
+// Restates ParallelCorrelationClusterer::RefineClusters_subroutine
+// (clustering/clusterers/parallel-correlation-clusterer.cc:476-629),
+// IterateBestMoves (:340-365) and BestMovesForVertexSubset (:115-338) over
+// device-resident CSR and cluster state, for one graph or a batch of graphs.
+// One HIP stream per handle; the host synchronizes once per best-move round to
+// read the next frontier's size (the loop condition of IterateBestMoves) and
+// the per-bin counts that size the next launches.
+
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <new>
+
+namespace pcc {
+
+// ------------------------------------------------------------- allocator --
+
+namespace {
+size_t RoundSize(size_t b) {
+  size_t r = 256;
+  while (r < b) r <<= 1;
+  return r;
+}
+}  // namespace
+
+DeviceAllocator::~DeviceAllocator() { ReleaseAll(); }
+
+void* DeviceAllocator::Alloc(size_t bytes) {
+  const size_t sz = RoundSize(bytes);
+  auto it = free_.find(sz);
+  void* p = nullptr;
+  if (it != free_.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+  } else {
+    if (hipMalloc(&p, sz) != hipSuccess) {
+      (void)hipGetLastError();
+      // Return cached blocks to the device and try once more.
+      for (auto& kv : free_)
+        for (void* q : kv.second) (void)hipFree(q);
+      free_.clear();
+      if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        throw AllocError{bytes};
+      }
+    }
+  }
+  live_[p] = sz;
+  in_use_ += sz;
+  return p;
+}
+
+void DeviceAllocator::Free(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  in_use_ -= it->second;
+  free_[it->second].push_back(p);
+  live_.erase(it);
+}
+
+void DeviceAllocator::ReleaseAll() {
+  for (auto& kv : free_)
+    for (void* q : kv.second) (void)hipFree(q);
+  free_.clear();
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  live_.clear();
+  in_use_ = 0;
+}
+
+// --------------------------------------------------------------- handles --
+
+int Guard(const char* where, const std::function<int()>& body) {
+  try {
+    return body();
+  } catch (const HipError& e) {
+    return SetError(PCC_INTERNAL, std::string(where) + ": HIP error " + hipGetErrorString(e.err) +
+                                      " at " + e.file + ":" + std::to_string(e.line) + " (" +
+                                      e.what + ")");
+  } catch (const AllocError& e) {
+    return SetError(PCC_RESOURCE_EXHAUSTED,
+                    std::string(where) + ": device allocation of " + std::to_string(e.bytes) +
+                        " bytes failed (free HBM exhausted)");
+  } catch (const std::bad_alloc&) {
+    return SetError(PCC_RESOURCE_EXHAUSTED, std::string(where) + ": host allocation failed");
+  }
+}
+
+int NewHandle(pcc_graph** out) {
+  auto h = std::make_unique<pcc_graph>();
+  PCC_HIP_CHECK(hipGetDevice(&h->device));
+  PCC_HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  *out = h.release();
+  return PCC_OK;
+}
+
+namespace {
+
+int Grid(int64_t items, int per_block, int max_blocks) {
+  int64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+__global__ void k_validate(int64_t n, int64_t m, const int64_t* off, const uint32_t* nbr,
+                           unsigned long long* bad) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if ((int64_t)nbr[i] >= n) atomicAdd(bad, 1ull);
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (off[v + 1] < off[v]) atomicAdd(bad, 1ull);
+}
+
+__global__ void k_fill_ones(float* w, int64_t m) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = 1.0f;
+}
+
+__global__ void k_iota(uint32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+__global__ void k_max_degree(int64_t n, const int64_t* off, unsigned long long* out) {
+  unsigned long long m = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, (unsigned long long)(off[v + 1] - off[v]));
+  for (int s = 32; s > 0; s >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, s));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// vgraph[v] = g with base[g] <= v < base[g + 1].
+__global__ void k_fill_vgraph(int64_t n, const int64_t* __restrict__ base, int G,
+                              uint32_t* __restrict__ vgraph) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (base[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    vgraph[v] = (uint32_t)lo;
+  }
+}
+
+__global__ void k_sel_frontier(int64_t n, const uint32_t* __restrict__ vgraph,
+                               const uint8_t* __restrict__ sel, uint8_t* __restrict__ fr) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    fr[v] = sel[vgraph ? vgraph[v] : 0u];
+}
+
+// Final per-vertex ids, local to each graph.
+__global__ void k_output_ids(int64_t n, const uint32_t* __restrict__ vgraph0,
+                             const uint8_t* __restrict__ use_ids0,
+                             const uint32_t* __restrict__ ids0,
+                             const uint32_t* __restrict__ chain,
+                             const int64_t* __restrict__ sub, uint32_t* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = vgraph0 ? vgraph0[v] : 0u;
+    const int64_t x = use_ids0[o] ? (int64_t)ids0[v] : (int64_t)chain[v];
+    out[v] = (uint32_t)(x - sub[o]);
+  }
+}
+
+template <typename T>
+DevBuf<T> Upload(DeviceAllocator* al, const std::vector<T>& h, hipStream_t s) {
+  DevBuf<T> d(al, (int64_t)std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    PCC_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+}  // namespace
+
+int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint32_t* nbrs,
+              const float* w) {
+  if (n < 0 || m < 0 || n >= (int64_t)0xFFFFFFF0ll)
+    return SetError(PCC_INVALID_ARGUMENT, "vertex count must lie in [0, 2^32 - 16)");
+  if (!off || (m > 0 && !nbrs)) return SetError(PCC_INVALID_ARGUMENT, "null CSR array");
+  if (off[0] != 0 || off[n] != m)
+    return SetError(PCC_INVALID_ARGUMENT, "offsets must start at 0 and end at m");
+  hipStream_t s = h->stream;
+  h->csr.n = n;
+  h->csr.m = m;
+  h->csr.off.Reset(&h->alloc, n + 1);
+  h->csr.nbr.Reset(&h->alloc, m > 0 ? m : 1);
+  h->csr.w.Reset(&h->alloc, m > 0 ? m : 1);
+  PCC_HIP_CHECK(hipMemcpyAsync(h->csr.off.get(), off, sizeof(int64_t) * (size_t)(n + 1),
+                               hipMemcpyHostToDevice, s));
+  if (m > 0) {
+    PCC_HIP_CHECK(hipMemcpyAsync(h->csr.nbr.get(), nbrs, sizeof(uint32_t) * (size_t)m,
+                                 hipMemcpyHostToDevice, s));
+    if (w) {
+      PCC_HIP_CHECK(hipMemcpyAsync(h->csr.w.get(), w, sizeof(float) * (size_t)m,
+                                   hipMemcpyHostToDevice, s));
+    } else {
+      hipLaunchKernelGGL(k_fill_ones, dim3(1024), dim3(256), 0, s, h->csr.w.get(), m);
+    }
+  }
+  DevBuf<unsigned long long> bad(&h->alloc, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(bad.get(), 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_validate, dim3(2048), dim3(256), 0, s, n, m, h->csr.off.get(),
+                     h->csr.nbr.get(), bad.get());
+  unsigned long long nbad = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&nbad, bad.get(), sizeof(nbad), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  if (nbad) {
+    h->csr = DevCsr();
+    return SetError(PCC_INVALID_ARGUMENT,
+                    "CSR has out-of-range neighbour ids or decreasing offsets");
+  }
+  if (h->gbase.empty()) h->gbase = {0, n};
+  h->finished = true;
+  return PCC_OK;
+}
+
+void LaunchIota(uint32_t* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256),
+                     0, s, p, n);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- engine --
+
+Engine::Engine(pcc_graph* h, const pcc_config& cfg, pcc_stats* stats)
+    : h_(h), cfg_(cfg), s_(h->stream), al_(&h->alloc), stats_(stats) {}
+
+Engine::~Engine() {
+  for (auto& t : pending_) {
+    (void)hipEventDestroy(t.first.a);
+    (void)hipEventDestroy(t.first.b);
+  }
+}
+
+Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) {
+  const int64_t nn = n > 0 ? n : 1;
+  list.Reset(al, nn);
+  fb_list.Reset(al, nn);
+  moves.Reset(al, nn);
+  moved.Reset(al, nn);
+  flags.Reset(al, nn);
+  next.Reset(al, nn);
+  new_flag.Reset(al, nn);
+  rc.Reset(al, 1);
+  // new_flag is kept all-zero between rounds by the new-cluster assignment.
+  PCC_HIP_CHECK(hipMemsetAsync(new_flag.get(), 0, (size_t)nn, s));
+  PCC_HIP_CHECK(hipMemsetAsync(rc.get(), 0, sizeof(RoundCounters), s));
+}
+
+void Engine::ResetCounters(RoundCounters* rc, bool all) {
+  if (all) {
+    PCC_HIP_CHECK(hipMemsetAsync(rc, 0, sizeof(RoundCounters), s_));
+  } else {
+    PCC_HIP_CHECK(hipMemsetAsync(rc, 0, offsetof(RoundCounters, fb_count), s_));
+  }
+}
+
+void Engine::ReadCounters(RoundCounters* rc, int64_t counts[kBins], int64_t te[kBins],
+                          RoundCounters* full) {
+  RoundCounters hrc;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hrc, rc, sizeof(hrc), hipMemcpyDeviceToHost, s_));
+  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  for (int b = 0; b < kBins; ++b) {
+    counts[b] = (int64_t)hrc.cnt[b];
+    te[b] = (int64_t)hrc.te[b];
+  }
+  if (full) *full = hrc;
+  CollectTimers();
+}
+
+Timer Engine::StartTimer() {
+  Timer t;
+  if (!h_->timing) return t;
+  PCC_HIP_CHECK(hipEventCreate(&t.a));
+  PCC_HIP_CHECK(hipEventCreate(&t.b));
+  PCC_HIP_CHECK(hipEventRecord(t.a, s_));
+  return t;
+}
+
+void Engine::StopTimer(Timer t, int kind) {
+  if (!t.a) return;
+  PCC_HIP_CHECK(hipEventRecord(t.b, s_));
+  pending_.emplace_back(t, kind);
+}
+
+// kind: 0..7 best-move bin, 8 ordered fallback, 9 apply + frontier,
+// 10 contraction, 11 all best-move kernels of a round.
+void Engine::CollectTimers() {
+  for (auto& p : pending_) {
+    float ms = 0;
+    PCC_HIP_CHECK(hipEventSynchronize(p.first.b));
+    PCC_HIP_CHECK(hipEventElapsedTime(&ms, p.first.a, p.first.b));
+    kind_seconds_[p.second] += ms * 1e-3;
+    kind_launches_[p.second] += 1;
+    (void)hipEventDestroy(p.first.a);
+    (void)hipEventDestroy(p.first.b);
+  }
+  pending_.clear();
+}
+
+int64_t Engine::MaxDegree(const CsrView& g) {
+  DevBuf<unsigned long long> d(al_, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(d.get(), 0, sizeof(unsigned long long), s_));
+  if (g.n > 0)
+    hipLaunchKernelGGL(k_max_degree, dim3((unsigned)std::min<int64_t>((g.n + 255) / 256, 4096)),
+                       dim3(256), 0, s_, g.n, g.off, d.get());
+  unsigned long long hd = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hd, d.get(), sizeof(hd), hipMemcpyDeviceToHost, s_));
+  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  return (int64_t)hd;
+}
+
+void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay,
+                          LevelCtx* c) {
+  c->g = g;
+  c->st = st;
+  DevBuf<unsigned long long> caps(al_, kBins);
+  PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * kBins, s_));
+  LaunchBinCapacity(g, h_->th, caps.get(), s_);
+  unsigned long long hcaps[kBins];
+  PCC_HIP_CHECK(hipMemcpyAsync(hcaps, caps.get(), sizeof(hcaps), hipMemcpyDeviceToHost, s_));
+  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  int64_t run = 0;
+  for (int b = 0; b < kBins; ++b) {
+    c->layout.base[b] = run;
+    c->caps[b] = (int64_t)hcaps[b];
+    run += (int64_t)hcaps[b];
+  }
+  c->hs = HashScratch();
+  if (hcaps[5] + hcaps[6] + hcaps[7] > 0) {
+    // Regions sized for the largest hashed degree on this level.
+    const int64_t maxdeg = MaxDegree(g);
+    uint32_t cap = 64;
+    while (cap < (uint64_t)(2 * maxdeg + 2)) cap <<= 1;
+    c->hs.cap = cap;
+    c->hs.regions = cap >= (1u << 22) ? 16 : (cap >= (1u << 20) ? 32 : 64);
+    c->hkeys.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
+    c->hvals.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
+    c->hs.keys = c->hkeys.get();
+    c->hs.vals = c->hvals.get();
+  }
+  MoveParams& mp = c->mp;
+  mp = MoveParams();
+  mp.res = cfg_.resolution;
+  mp.offset = cfg_.edge_weight_offset;
+  mp.n = (uint32_t)g.n;
+  mp.force_ordered = h_->force_ordered ? 1 : 0;
+  if (lay.batch) {
+    // Batches use a sentinel outside every graph's id range.
+    mp.new_id = 0xFFFFFFFEu;
+    mp.vgraph = lay.vgraph.get();
+    mp.gres = lay.gres.get();
+    mp.gbase = lay.dbase.get();
+  } else {
+    mp.new_id = (uint32_t)g.n;
+  }
+}
+
+void Engine::BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],
+                         int64_t te[kBins]) {
+  ResetCounters(ws.rc.get(), true);
+  LaunchBinFrontier(c.g, frontier, h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
+  ReadCounters(ws.rc.get(), counts, te, nullptr);
+}
+
+int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, int G,
+                          uint8_t* gmoved, int64_t counts[kBins], int64_t te[kBins]) {
+  (void)G;
+  const CsrView& g = c.g;
+  const StateView& st = c.st;
+  const int64_t n = g.n;
+  const bool async = cfg_.async != 0;
+  rounds_ += 1;
+  int64_t F = 0, TE = 0;
+  for (int b = 0; b < kBins; ++b) {
+    F += counts[b];
+    TE += te[b];
+    bin_edges_[b] += te[b];
+    bin_vertices_[b] += counts[b];
+  }
+  edges_scanned_ += TE;
+  frontier_vertices_ += F;
+  bestmove_bytes_ += 12.0 * (double)TE + 40.0 * (double)F;
+
+  MoveOut out;
+  out.moves = ws.moves.get();
+  out.moved = ws.moved.get();
+  out.moved_clusters = ws.flags.get();
+  if (async) PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
+  PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 2 * sizeof(unsigned long long), s_));
+  const uint64_t salt =
+      (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
+  Timer all = StartTimer();
+  for (int b = 0; b < kBins; ++b) {
+    if (counts[b] == 0) continue;
+    Timer tb = StartTimer();
+    LaunchBestMoveBin(b, counts[b], async, g, st, c.mp, ws.list.get() + c.layout.base[b], out,
+                      ws.fb_list.get(), ws.rc.get(), c.hs, salt, s_);
+    StopTimer(tb, b);
+  }
+  if (!async && counts[5] + counts[6] + counts[7] > 0) {
+    Timer tf = StartTimer();
+    LaunchBestMoveOrdered(g, st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s_);
+    StopTimer(tf, 8);
+  }
+  StopTimer(all, 11);
+
+  Timer ta = StartTimer();
+  if (!async) {
+    // MoveNodesToCluster (PCCI.cc:256-369).
+    PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
+    LaunchApply(counts, c.layout, ws.list.get(), st, c.mp.new_id, ws.moves.get(), ws.flags.get(),
+                ws.new_flag.get(), ws.rc.get(), s_);
+    RunNewClusterAssign(al_, st, (uint32_t)n, c.mp.vgraph, c.mp.gbase, ws.new_flag.get(),
+                        ws.flags.get(), ws.rc.get(), s_);
+  }
+  if (cfg_.move_method == PCC_ALL_MOVE) {  // PCC.cc:299-307: frontier unchanged
+    StopTimer(ta, 9);
+    RoundCounters hrc;
+    int64_t c2[kBins], t2[kBins];
+    ReadCounters(ws.rc.get(), c2, t2, &hrc);
+    fallback_ += (int64_t)hrc.fb_count;
+    return F;
+  }
+  // Next frontier = neighbours of the seeds (PCC.cc:309-331).
+  PCC_HIP_CHECK(hipMemsetAsync(ws.next.get(), 0, (size_t)n, s_));
+  if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
+    LaunchExpandClusters(g, st.ids, ws.flags.get(), ws.next.get(), s_);
+  } else {
+    LaunchExpandMoved(counts, c.layout, ws.list.get(), g, ws.moved.get(), ws.next.get(), s_);
+  }
+  LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
+  ResetCounters(ws.rc.get(), false);
+  LaunchBinFrontier(g, ws.next.get(), h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
+  StopTimer(ta, 9);
+  RoundCounters hrc;
+  ReadCounters(ws.rc.get(), counts, te, &hrc);
+  fallback_ += (int64_t)hrc.fb_count;
+  int64_t f = 0;
+  for (int b = 0; b < kBins; ++b) f += counts[b];
+  return f;
+}
+
+std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<uint8_t>* sel,
+                                              int64_t inner) {
+  const int64_t n = L.g.n;
+  const int G = L.lay.G;
+  LevelCtx c;
+  PrepareLevel(L.g, L.State(), L.lay, &c);
+  Workspace ws(al_, n, s_);
+  DevBuf<uint8_t> gmoved(al_, G);
+  PCC_HIP_CHECK(hipMemsetAsync(gmoved.get(), 0, (size_t)G, s_));
+  // Round 0 frontier: every vertex of the selected graphs (PCC.cc:346-352).
+  bool all = true;
+  if (sel)
+    for (uint8_t x : *sel) all = all && x;
+  DevBuf<uint8_t> fr0, dsel;
+  if (!all) {
+    dsel = Upload(al_, *sel, s_);
+    fr0.Reset(al_, n > 0 ? n : 1);
+    if (n > 0)
+      hipLaunchKernelGGL(k_sel_frontier, dim3(Grid(n, 256, 8192)), dim3(256), 0, s_, n,
+                         L.lay.vgraph.get(), dsel.get(), fr0.get());
+  }
+  int64_t counts[kBins], te[kBins];
+  BinFrontier(c, ws, all ? nullptr : fr0.get(), counts, te);
+  bool local_moved = true;
+  int64_t rounds = 0;
+  for (int64_t it = 0; it < inner && local_moved; ++it) {
+    const int64_t fsize = RoundStep(c, ws, L.lay.vgraph.get(), G, gmoved.get(), counts, te);
+    local_moved = fsize != 0;
+    ++rounds;
+  }
+  std::vector<uint8_t> moved((size_t)G, 0);
+  if (cfg_.move_method == PCC_ALL_MOVE) {
+    // The frontier stays every vertex, so each selected non-empty graph reports
+    // "moved" (PCC.cc:299-307, 361-362).
+    for (int p = 0; p < G; ++p) {
+      const bool selected = !sel || (*sel)[(size_t)p];
+      moved[(size_t)p] = (rounds > 0 && selected && L.lay.base[(size_t)p + 1] > L.lay.base[(size_t)p]) ? 1 : 0;
+    }
+  } else {
+    PCC_HIP_CHECK(hipMemcpyAsync(moved.data(), gmoved.get(), (size_t)G, hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  }
+  return moved;
+}
+
+int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, uint32_t* host_out,
+                std::vector<int64_t>* out_iter) {
+  const CsrView g0 = h_->csr.View();
+  const int64_t n = g0.n;
+  const int G0 = h_->num_graphs();
+  int64_t num_iterations = 0, num_inner = 0;
+  if (cfg_.clustering_moves_method == PCC_LOUVAIN) {  // PCC.cc:493-500
+    num_iterations = cfg_.louvain_num_iterations > 0 ? cfg_.louvain_num_iterations : 10;
+    num_inner = cfg_.louvain_num_inner_iterations > 0 ? cfg_.louvain_num_inner_iterations : 10;
+  } else if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
+    return SetError(PCC_UNIMPLEMENTED,
+                    "DEFAULT_CLUSTER_MOVES (whole-cluster moves) is not on this path; use LOUVAIN");
+  } else {
+    return SetError(PCC_UNIMPLEMENTED,
+                    "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
+  }
+  if (cfg_.subclustering_method != 0)
+    return SetError(PCC_UNIMPLEMENTED, "subclustering_method other than NONE is not on this path");
+
+  hipEvent_t e0, e1;
+  PCC_HIP_CHECK(hipEventCreate(&e0));
+  PCC_HIP_CHECK(hipEventCreate(&e1));
+  PCC_HIP_CHECK(hipEventRecord(e0, s_));
+
+  const bool batch = G0 > 1;
+  const std::vector<int64_t>& base0 = h_->gbase;
+  std::vector<double> res0((size_t)G0, cfg_.resolution);
+  if (gres) res0.assign(gres, gres + G0);
+
+  std::vector<std::unique_ptr<Level>> levels;
+  levels.emplace_back(new Level());
+  {
+    Level& L = *levels.back();
+    L.g = g0;
+    L.nw = std::move(nw0);
+    L.ids.Reset(al_, n > 0 ? n : 1);
+    L.sizes.Reset(al_, n > 0 ? n : 1);
+    L.K.Reset(al_, n > 0 ? n : 1);
+    L.lay.G = G0;
+    L.lay.batch = batch;
+    L.lay.base = base0;
+    for (int o = 0; o < G0; ++o) L.lay.orig.push_back(o);
+    if (batch) {
+      L.lay.dbase = Upload(al_, base0, s_);
+      L.lay.vgraph.Reset(al_, n);
+      PCC_HIP_CHECK(hipMemcpyAsync(L.lay.vgraph.get(), h_->vgraph.get(), 4 * (size_t)n,
+                                   hipMemcpyDeviceToDevice, s_));
+      L.lay.gres = Upload(al_, res0, s_);
+    }
+    if (init)
+      LaunchSetClustering(n, init, L.State(), s_);
+    else
+      LaunchSingletons(n, L.State(), s_);
+  }
+  const uint32_t* vgraph0 = batch ? h_->vgraph.get() : nullptr;
+  DevBuf<uint32_t> cluster_ids(al_, n > 0 ? n : 1);
+  LaunchIota(cluster_ids.get(), n, s_);
+  std::vector<int64_t> iter_o((size_t)G0, 0);      // final level of each graph
+  std::vector<int64_t> cur_base(base0.begin(), base0.end() - 1);  // space of cluster_ids
+  std::vector<uint8_t> live((size_t)G0, 1);
+
+  int64_t iter = 0;
+  for (iter = 0; iter < num_iterations; ++iter) {
+    Level& L = *levels[(size_t)iter];
+    const int Gp = L.lay.G;
+    const std::vector<uint8_t> moved = IterateBestMoves(L, nullptr, num_inner);
+    std::vector<uint8_t> sel_o((size_t)G0, 0);
+    bool any = false;
+    for (int p = 0; p < Gp; ++p) {
+      const int o = L.lay.orig[(size_t)p];
+      if (!moved[(size_t)p]) {  // PCC.cc:543-547
+        iter_o[(size_t)o] = iter - 1;
+        live[(size_t)o] = 0;
+      } else {
+        sel_o[(size_t)o] = 1;
+        iter_o[(size_t)o] = iter;
+        any = true;
+      }
+    }
+    if (!cfg_.refine && any) {  // PCC.cc:550
+      DevBuf<uint8_t> dsel = Upload(al_, sel_o, s_);
+      LaunchFlattenShift(n, cluster_ids.get(), vgraph0, dsel.get(), nullptr, L.ids.get(), s_);
+    }
+    if (iter == num_iterations - 1 || !any) break;  // PCC.cc:556
+    // CompressGraph (PCC.cc:569-571) of the graphs that moved, and singleton
+    // helpers with aggregated node weights (PCC.cc:583-585).
+    Timer tc = StartTimer();
+    DevBuf<uint32_t> gmax(al_, Gp);
+    PCC_HIP_CHECK(hipMemsetAsync(gmax.get(), 0, 4 * (size_t)Gp, s_));
+    LaunchGraphMaxIds(L.g.n, L.ids.get(), L.lay.vgraph.get(), gmax.get(), s_);
+    std::vector<uint32_t> hmax((size_t)Gp);
+    PCC_HIP_CHECK(hipMemcpyAsync(hmax.data(), gmax.get(), 4 * (size_t)Gp, hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    std::vector<int64_t> newbase((size_t)Gp, -1);
+    auto next = std::make_unique<Level>();
+    next->lay.batch = batch;
+    int64_t n2 = 0;
+    next->lay.base.push_back(0);
+    std::vector<double> res_next;
+    std::vector<int64_t> shift_o((size_t)G0, 0);
+    for (int p = 0; p < Gp; ++p) {
+      if (!moved[(size_t)p]) continue;
+      const int o = L.lay.orig[(size_t)p];
+      newbase[(size_t)p] = n2;
+      shift_o[(size_t)o] = n2 - L.lay.base[(size_t)p];
+      n2 += (int64_t)hmax[(size_t)p] - L.lay.base[(size_t)p] + 1;  // 1 + max id (PCCI.cc:713-717)
+      next->lay.orig.push_back(o);
+      next->lay.base.push_back(n2);
+      res_next.push_back(res0[(size_t)o]);
+    }
+    next->lay.G = (int)next->lay.orig.size();
+    if (batch) {
+      DevBuf<int64_t> dnb = Upload(al_, newbase, s_);
+      CompressGraph(al_, L.g, L.ids.get(), L.nw.get(), L.lay.vgraph.get(), L.lay.dbase.get(),
+                    dnb.get(), n2, &next->own, &next->nw, s_);
+      next->lay.dbase = Upload(al_, next->lay.base, s_);
+      next->lay.vgraph.Reset(al_, n2 > 0 ? n2 : 1);
+      if (n2 > 0)
+        hipLaunchKernelGGL(k_fill_vgraph, dim3(Grid(n2, 256, 8192)), dim3(256), 0, s_, n2,
+                           next->lay.dbase.get(), next->lay.G, next->lay.vgraph.get());
+      next->lay.gres = Upload(al_, res_next, s_);
+    } else {
+      CompressGraph(al_, L.g, L.ids.get(), L.nw.get(), nullptr, nullptr, nullptr, n2, &next->own,
+                    &next->nw, s_);
+    }
+    next->g = next->own.View();
+    next->ids.Reset(al_, n2);
+    next->sizes.Reset(al_, n2);
+    next->K.Reset(al_, n2);
+    LaunchSingletons(n2, next->State(), s_);
+    StopTimer(tc, 10);
+    if (!cfg_.refine) {
+      // cluster_ids of the continuing graphs move to the next level's numbering.
+      if (batch) {
+        DevBuf<uint8_t> dsel = Upload(al_, sel_o, s_);
+        DevBuf<int64_t> dshift = Upload(al_, shift_o, s_);
+        LaunchFlattenShift(n, cluster_ids.get(), vgraph0, dsel.get(), dshift.get(), nullptr, s_);
+      }
+      for (int p = 0; p < Gp; ++p)
+        if (moved[(size_t)p]) cur_base[(size_t)L.lay.orig[(size_t)p]] = newbase[(size_t)p];
+      if (iter > 0) {  // only the current level is needed without refinement
+        L.own = DevCsr();
+        L.ids.Release();
+        L.sizes.Release();
+        L.K.Release();
+        L.nw.Release();
+      }
+    }
+    levels.push_back(std::move(next));
+  }
+
+  std::vector<uint8_t> use_ids0((size_t)G0, 0);
+  std::vector<int64_t> sub(base0.begin(), base0.end() - 1);
+  if (cfg_.refine) {  // PCC.cc:589-607
+    int64_t top = -1;
+    for (int o = 0; o < G0; ++o) top = std::max(top, iter_o[(size_t)o] - 1);
+    for (int64_t i = top; i >= 0; --i) {
+      Level& Li = *levels[(size_t)i];
+      Level& Ln = *levels[(size_t)i + 1];
+      std::vector<int> pos_next((size_t)G0, -1);
+      for (int p = 0; p < Ln.lay.G; ++p) pos_next[(size_t)Ln.lay.orig[(size_t)p]] = p;
+      std::vector<uint8_t> sel((size_t)Li.lay.G, 0);
+      std::vector<int64_t> nbase((size_t)Li.lay.G, 0);
+      bool anysel = false;
+      for (int p = 0; p < Li.lay.G; ++p) {
+        const int o = Li.lay.orig[(size_t)p];
+        if (iter_o[(size_t)o] - 1 >= i && pos_next[(size_t)o] >= 0) {
+          sel[(size_t)p] = 1;
+          nbase[(size_t)p] = Ln.lay.base[(size_t)pos_next[(size_t)o]];
+          anysel = true;
+        }
+      }
+      if (!anysel) continue;
+      if (batch) {
+        DevBuf<uint8_t> dsel = Upload(al_, sel, s_);
+        DevBuf<int64_t> dnb = Upload(al_, nbase, s_);
+        ResetClusteringBatch(al_, Li.g.n, Ln.g.n, Ln.ids.get(), Li.lay.vgraph.get(),
+                             Li.lay.dbase.get(), dnb.get(), dsel.get(), Li.State(), s_);
+      } else {
+        ResetClusteringBatch(al_, Li.g.n, Ln.g.n, Ln.ids.get(), nullptr, nullptr, nullptr,
+                             nullptr, Li.State(), s_);
+      }
+      IterateBestMoves(Li, &sel, num_inner);
+    }
+    for (int o = 0; o < G0; ++o) use_ids0[(size_t)o] = iter_o[(size_t)o] > 0 ? 1 : 0;
+    // refine with iter == 0 keeps the identity cluster_ids (PCC.cc:526-530, 589).
+  } else {
+    sub = cur_base;
+  }
+  PCC_HIP_CHECK(hipEventRecord(e1, s_));
+  PCC_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  PCC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CollectTimers();
+  if (n > 0) {
+    DevBuf<uint8_t> duse = Upload(al_, use_ids0, s_);
+    DevBuf<int64_t> dsub = Upload(al_, sub, s_);
+    DevBuf<uint32_t> out(al_, n);
+    hipLaunchKernelGGL(k_output_ids, dim3(Grid(n, 256, 8192)), dim3(256), 0, s_, n, vgraph0,
+                       duse.get(), levels[0]->ids.get(), cluster_ids.get(), dsub.get(), out.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    PCC_HIP_CHECK(hipMemcpyAsync(host_out, out.get(), sizeof(uint32_t) * (size_t)n,
+                                 hipMemcpyDeviceToHost, s_));
+  }
+  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  if (out_iter) *out_iter = iter_o;
+  if (stats_) {
+    int64_t lv = 0;
+    for (int o = 0; o < G0; ++o) lv = std::max(lv, iter_o[(size_t)o] + 1);
+    stats_->levels = lv;
+    stats_->rounds = rounds_;
+    stats_->edges_scanned = edges_scanned_;
+    stats_->frontier_vertices = frontier_vertices_;
+    stats_->fallback_vertices = fallback_;
+    stats_->cluster_seconds = (double)ms * 1e-3;
+    stats_->bestmove_bytes = bestmove_bytes_;
+    stats_->bestmove_seconds = kind_seconds_[11];
+    stats_->bestmove_launches = 0;
+    for (int b = 0; b <= 8; ++b) stats_->bestmove_launches += kind_launches_[b];
+    stats_->apply_seconds = kind_seconds_[9];
+    stats_->contract_seconds = kind_seconds_[10];
+    for (int b = 0; b < kBins; ++b) {
+      stats_->bin_seconds[b] = kind_seconds_[b];
+      stats_->bin_edges[b] = bin_edges_[b];
+      stats_->bin_vertices[b] = bin_vertices_[b];
+      stats_->bin_launches[b] = kind_launches_[b];
+    }
+    stats_->ordered_seconds = kind_seconds_[8];
+  }
+  return PCC_OK;
+}
+
+}  // namespace pcc

@@ -1,0 +1,164 @@
+// engine.h -- the graph handle and the multilevel Louvain engine.
+//
+// One engine drives one graph or a batch of independent graphs.  A batch is
+// laid out as one disjoint union: graph g owns the vertex range
+// [base[g], base[g + 1]) of every level, cluster ids are vertex ids of the
+// level, and per-graph quantities (resolution, level progress, termination)
+// are kept per graph, so each graph follows exactly the control flow of its
+// own ParallelCorrelationClusterer run.
+#ifndef PCC_AMD_ENGINE_H_
+#define PCC_AMD_ENGINE_H_
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.h"
+#include "pcc_amd.h"
+#include "status.h"
+
+struct pcc_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pcc::DeviceAllocator alloc;
+  // Import staging (GbbsGraph, clustering/gbbs-graph.cc:32-76).
+  std::mutex mu;
+  std::vector<std::vector<std::pair<uint32_t, float>>> adj;
+  bool finished = false;
+  pcc::DevCsr csr;
+  // Batch layout: graph g owns vertices [gbase[g], gbase[g + 1]); one graph
+  // has gbase = {0, n} and no vgraph.
+  std::vector<int64_t> gbase;
+  pcc::DevBuf<uint32_t> vgraph;
+  // Options.
+  bool timing = false;
+  bool force_ordered = false;
+  pcc::BinThresholds th;
+  int num_graphs() const { return gbase.empty() ? 1 : (int)gbase.size() - 1; }
+  ~pcc_graph() {
+    vgraph.Release();
+    csr = pcc::DevCsr();
+    alloc.ReleaseAll();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace pcc {
+
+int Guard(const char* where, const std::function<int()>& body);
+int NewHandle(pcc_graph** out);
+// Uploads a CSR (neighbour ids global) and validates it on device.
+int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint32_t* nbrs,
+              const float* w);
+
+// Per-level layout of a batch.
+struct Layout {
+  int G = 1;                   // graphs present at this level
+  std::vector<int> orig;       // original graph index of each present graph
+  std::vector<int64_t> base;   // G + 1 vertex bases
+  DevBuf<int64_t> dbase;       // device copy of base (batches only)
+  DevBuf<uint32_t> vgraph;     // present-graph index of each vertex (batches only)
+  DevBuf<double> gres;         // resolution of each present graph (batches only)
+  bool batch = false;
+};
+
+struct Level {
+  DevCsr own;   // contracted graphs (level > 0)
+  CsrView g;    // this level's graph
+  DevBuf<uint32_t> ids, sizes;
+  DevBuf<double> nw, K;
+  Layout lay;
+  StateView State() const {
+    StateView s;
+    s.ids = ids.get();
+    s.sizes = sizes.get();
+    s.nw = nw.get();
+    s.K = K.get();
+    return s;
+  }
+};
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+class Engine {
+ public:
+  Engine(pcc_graph* h, const pcc_config& cfg, pcc_stats* stats);
+  ~Engine();
+
+  // RefineClusters_subroutine (parallel-correlation-clusterer.cc:476-629) for
+  // every graph of the handle.  nw0: level-0 node weights (taken over);
+  // gres: per-graph resolutions (null: cfg.resolution for all); init: device
+  // initial cluster ids (global vertex ids of the level) or null.
+  // host_out receives each vertex's final cluster id local to its graph;
+  // out_iter (may be null) the final level index of each graph.
+  int Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, uint32_t* host_out,
+          std::vector<int64_t>* out_iter);
+
+  struct Workspace {
+    DevBuf<uint32_t> list, fb_list, moves;
+    DevBuf<uint8_t> moved, flags, next, new_flag;
+    DevBuf<RoundCounters> rc;
+    Workspace(DeviceAllocator* al, int64_t n, hipStream_t s);
+  };
+
+  // One level's best-move machinery, shared by Run and the one-phase C ABI.
+  struct LevelCtx {
+    CsrView g;
+    StateView st;
+    MoveParams mp;
+    BinLayout layout;
+    HashScratch hs;
+    DevBuf<uint32_t> hkeys;
+    DevBuf<double> hvals;
+    int64_t caps[kBins];
+  };
+  void PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay, LevelCtx* ctx);
+  // IterateBestMoves (parallel-correlation-clusterer.cc:340-365) over the
+  // graphs selected by sel (present-graph flags; null = all).  Returns the
+  // per-present-graph "moved" flags.
+  std::vector<uint8_t> IterateBestMoves(const Level& L, const std::vector<uint8_t>* sel,
+                                        int64_t inner);
+  // One BestMovesForVertexSubset round; counts / te describe the listed
+  // frontier and receive the next one.  Returns the next frontier size.
+  int64_t RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, int G, uint8_t* gmoved,
+                    int64_t counts[kBins], int64_t te[kBins]);
+  void BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],
+                   int64_t te[kBins]);
+  int64_t MaxDegree(const CsrView& g);
+
+ private:
+  void ResetCounters(RoundCounters* rc, bool all);
+  void ReadCounters(RoundCounters* rc, int64_t counts[kBins], int64_t te[kBins],
+                    RoundCounters* full);
+  Timer StartTimer();
+  void StopTimer(Timer t, int kind);
+  void CollectTimers();
+
+  pcc_graph* h_;
+  pcc_config cfg_;
+  hipStream_t s_;
+  DeviceAllocator* al_;
+  pcc_stats* stats_;
+  std::vector<std::pair<Timer, int>> pending_;
+  double kind_seconds_[16] = {0};
+  int64_t kind_launches_[16] = {0};
+  int64_t bin_edges_[kBins] = {0};
+  int64_t bin_vertices_[kBins] = {0};
+  int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
+  double bestmove_bytes_ = 0;
+};
+
+// Resolution used for the objective: for modularity runs res' = gamma / m_g
+// (ComputeModularityConfig, parallel/parallel-graph-utils.cc:141-153).
+void LaunchIota(uint32_t* p, int64_t n, hipStream_t s);
+
+}  // namespace pcc
+
+#endif  // PCC_AMD_ENGINE_H_

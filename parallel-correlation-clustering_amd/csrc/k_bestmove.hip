@@ -99,14 +99,14 @@ __global__ __launch_bounds__(256) void k_bin_capacity(int64_t n, const int64_t* 
 // Decision of EfficientBestMove from the best candidate (low-degree path,
 // PCCI.cc:490-510; hub path PCCI.cc:565-582, whose only difference is the
 // empty-candidate case).
-__device__ __forceinline__ void Decide(Cand best, double change, bool hub, uint32_t n,
+__device__ __forceinline__ void Decide(Cand best, double change, bool hub, uint32_t new_id,
                                        uint32_t* t, double* d) {
   if (best.c == kNoneId) {
     if (hub && !(change >= 0.0)) {
       *t = 0;
       *d = 0.0;
     } else {
-      *t = n;
+      *t = new_id;
       *d = change;
     }
     return;
@@ -115,18 +115,24 @@ __device__ __forceinline__ void Decide(Cand best, double change, bool hub, uint3
     *t = best.c;
     *d = best.d;
   } else {
-    *t = n;
+    *t = new_id;
     *d = change;
   }
 }
 
-// Sync epilogue: singleton rule then delta > 0 (PCC.cc:145-156).
-__device__ __forceinline__ void SyncEpilogue(uint32_t v, uint32_t t, double d, uint32_t n,
+// Resolution of the vertex's graph.
+__device__ __forceinline__ double ResOf(const MoveParams& mp, uint32_t v) {
+  return mp.vgraph ? mp.gres[mp.vgraph[v]] : mp.res;
+}
+
+// Sync epilogue: singleton rule then delta > 0 (PCC.cc:145-156).  Targets are
+// cluster ids or new_id, so "target < n" of the reference is target != new_id.
+__device__ __forceinline__ void SyncEpilogue(uint32_t v, uint32_t t, double d, uint32_t new_id,
                                              const StateView& st, const MoveOut& out) {
   if (out.dbg_target) out.dbg_target[v] = t;
   if (out.dbg_delta) out.dbg_delta[v] = d;
   const uint32_t cur = st.ids[v];
-  if (t < n && st.sizes[t] == 1 && st.sizes[cur] == 1 && cur >= t) {
+  if (t != new_id && st.sizes[t] == 1 && st.sizes[cur] == 1 && cur >= t) {
     t = cur;
     d = 0.0;
   }
@@ -142,15 +148,16 @@ __device__ __forceinline__ void SyncEpilogue(uint32_t v, uint32_t t, double d, u
 // Async apply (AsyncMove, PCCI.cc:371-403).  Moves to a new cluster claim an
 // empty id first (probing v, v+1, ... like the reference), so a failed probe
 // leaves the state untouched.
-__device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d, uint32_t n,
-                                              const StateView& st, const MoveOut& out) {
+__device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d,
+                                              const MoveParams& mp, const StateView& st,
+                                              const MoveOut& out) {
   if (!(d > 0)) {
     out.moved[v] = 0;
     return;
   }
   const uint32_t cur = st.ids[v];
   const double k = st.nw[v];
-  if (t != n) {
+  if (t != mp.new_id) {
     atomicSub(&st.sizes[cur], 1u);
     atomicAdd(&st.K[cur], -1 * k);
     atomicAdd(&st.sizes[t], 1u);
@@ -160,8 +167,15 @@ __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d, 
     out.moved[v] = 1;
     return;
   }
+  // Empty ids are searched within the vertex's own graph.
+  uint32_t lo = 0, hi = mp.n;
+  if (mp.vgraph) {
+    const uint32_t g = mp.vgraph[v];
+    lo = (uint32_t)mp.gbase[g];
+    hi = (uint32_t)mp.gbase[g + 1];
+  }
   uint32_t i = v;
-  for (uint32_t probe = 0; probe < n; ++probe) {
+  for (uint32_t probe = lo; probe < hi; ++probe) {
     if (__hip_atomic_load(&st.sizes[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
         atomicCAS(&st.sizes[i], 0u, 1u) == 0u) {
       atomicSub(&st.sizes[cur], 1u);
@@ -172,7 +186,7 @@ __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d, 
       out.moved[v] = 1;
       return;
     }
-    i = (i + 1 == n) ? 0 : i + 1;
+    i = (i + 1 == hi) ? lo : i + 1;
   }
   out.moved[v] = 0;
 }
@@ -239,24 +253,25 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
     const double T = tm ? Ts : 0.0;
     const double k = active ? st.nw[v] : 0.0;
     const double Kcv = active ? st.K[cv] : 0.0;
+    const double res = active ? ResOf(mp, v) : 0.0;
     double change = 0;
-    change -= NetWeight(T, k * (Kcv - k), mp.res);
+    change -= NetWeight(T, k * (Kcv - k), res);
     double mw = 0;
     mw += k;
     Cand cand{0.0, kNoneId};
     if (first) {
       cand.c = c;
-      cand.d = change + NetWeight(S, mw * st.K[c], mp.res);
+      cand.d = change + NetWeight(S, mw * st.K[c], res);
     }
     cand = GroupBest(cand, G);
     if (active && lane == 0) {
       uint32_t t;
       double d;
-      Decide(cand, change, false, mp.n, &t, &d);
+      Decide(cand, change, false, mp.new_id, &t, &d);
       if (kAsync)
-        AsyncEpilogue(v, t, d, mp.n, st, out);
+        AsyncEpilogue(v, t, d, mp, st, out);
       else
-        SyncEpilogue(v, t, d, mp.n, st, out);
+        SyncEpilogue(v, t, d, mp.new_id, st, out);
     }
   }
 }
@@ -361,7 +376,10 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     const uint32_t v = list[PermuteIndex(i, count, salt)];
     const int64_t start = g.off[v];
     const int64_t deg = g.off[v + 1] - start;
-    const uint32_t cap = TABLE > 0 ? (uint32_t)TABLE : min(hs.cap, NextPow2((uint64_t)(2 * deg + 2)));
+    // Table sized per vertex (load factor <= 1/2) so that clearing and
+    // scanning it costs O(deg), not O(TABLE).
+    const uint32_t cap = min(TABLE > 0 ? (uint32_t)TABLE : hs.cap,
+                             max(64u, NextPow2((uint64_t)(2 * deg + 2))));
     const uint32_t mask = cap - 1;
     for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
       keys[s] = kNoneId;
@@ -411,15 +429,16 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     }
     const double T = HashGet(keys, vals, mask, cv);
     const double k = st.nw[v];
+    const double res = ResOf(mp, v);
     double change = 0;
-    change -= NetWeight(T, k * (st.K[cv] - k), mp.res);
+    change -= NetWeight(T, k * (st.K[cv] - k), res);
     double mw = 0;
     mw += k;
     Cand best{0.0, kNoneId};
     for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
       const uint32_t c = keys[s];
       if (c != kNoneId) {
-        const Cand x{change + NetWeight(vals[s], mw * st.K[c], mp.res), c};
+        const Cand x{change + NetWeight(vals[s], mw * st.K[c], res), c};
         best = PickBetter(best, x);
       }
     }
@@ -427,11 +446,11 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     if (threadIdx.x == 0) {
       uint32_t t;
       double d;
-      Decide(best, change, hub, mp.n, &t, &d);
+      Decide(best, change, hub, mp.new_id, &t, &d);
       if (kAsync)
-        AsyncEpilogue(v, t, d, mp.n, st, out);
+        AsyncEpilogue(v, t, d, mp, st, out);
       else
-        SyncEpilogue(v, t, d, mp.n, st, out);
+        SyncEpilogue(v, t, d, mp.new_id, st, out);
     }
     __syncthreads();
   }
@@ -527,15 +546,16 @@ __global__ __launch_bounds__(64) void k_bm_ordered(CsrView g, StateView st, Move
       T = HashGet(keys, vals, mask, cv);
     }
     const double k = st.nw[v];
+    const double res = ResOf(mp, v);
     double change = 0;
-    change -= NetWeight(T, k * (st.K[cv] - k), mp.res);
+    change -= NetWeight(T, k * (st.K[cv] - k), res);
     double mw = 0;
     mw += k;
     Cand best{0.0, kNoneId};
     for (uint32_t s = lane; s < cap; s += 64) {
       const uint32_t c = keys[s];
       if (c != kNoneId) {
-        const Cand x{change + NetWeight(vals[s], mw * st.K[c], mp.res), c};
+        const Cand x{change + NetWeight(vals[s], mw * st.K[c], res), c};
         best = PickBetter(best, x);
       }
     }
@@ -543,8 +563,8 @@ __global__ __launch_bounds__(64) void k_bm_ordered(CsrView g, StateView st, Move
     if (lane == 0) {
       uint32_t t;
       double d;
-      Decide(best, change, hub, mp.n, &t, &d);
-      SyncEpilogue(v, t, d, mp.n, st, out);
+      Decide(best, change, hub, mp.new_id, &t, &d);
+      SyncEpilogue(v, t, d, mp.new_id, st, out);
     }
     __syncthreads();
   }

@@ -36,15 +36,28 @@ int BitsFor(uint64_t maxval) {
   return b < 1 ? 1 : b;
 }
 
+// Renumbering of a batch level: cluster c of graph p -> newbase[p] + c - gbase[p].
+struct GraphMap {
+  const uint32_t* vgraph;
+  const int64_t* gbase;
+  const int64_t* newbase;
+  __device__ __forceinline__ int64_t Shift(int64_t v) const {
+    if (!vgraph) return 0;
+    const uint32_t p = vgraph[v];
+    return newbase[p] < 0 ? INT64_MIN : newbase[p] - gbase[p];
+  }
+};
+
 // 16 lanes per vertex count the kept edges (PGU.cc:93-103).
 __global__ __launch_bounds__(256) void k_count_kept(CsrView g, const uint32_t* __restrict__ ids,
-                                                    uint32_t* __restrict__ kept) {
+                                                    GraphMap gm, uint32_t* __restrict__ kept) {
   const int sub = threadIdx.x & 15;
   const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
   for (int64_t u = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; u < g.n; u += stride) {
     const uint32_t cu = ids[u];
     uint32_t c = 0;
-    for (int64_t p = g.off[u] + sub; p < g.off[u + 1]; p += 16) {
+    const int64_t lim = gm.Shift(u) == INT64_MIN ? g.off[u] : g.off[u + 1];
+    for (int64_t p = g.off[u] + sub; p < lim; p += 16) {
       const uint32_t v = g.nbr[p];
       if ((int64_t)v <= u || ids[v] != cu) ++c;
     }
@@ -55,6 +68,7 @@ __global__ __launch_bounds__(256) void k_count_kept(CsrView g, const uint32_t* _
 
 // Writes the kept edges of each vertex, in CSR order, as (key, weight).
 __global__ __launch_bounds__(256) void k_emit_kept(CsrView g, const uint32_t* __restrict__ ids,
+                                                   GraphMap gm,
                                                    const int64_t* __restrict__ kept_off, int vbits,
                                                    uint64_t* __restrict__ keys,
                                                    float* __restrict__ vals) {
@@ -66,7 +80,8 @@ __global__ __launch_bounds__(256) void k_emit_kept(CsrView g, const uint32_t* __
   for (int64_t ub = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) / 16; ub < g.n;
        ub += stride) {
     const int64_t u = ub + (wl >> 4);
-    const bool active = u < g.n;
+    const int64_t shift = u < g.n ? gm.Shift(u) : INT64_MIN;
+    const bool active = u < g.n && shift != INT64_MIN;
     const int64_t b = active ? g.off[u] : 0, e = active ? g.off[u + 1] : 0;
     const uint32_t cu = active ? ids[u] : 0u;
     int64_t pos = active ? kept_off[u] : 0;
@@ -87,7 +102,7 @@ __global__ __launch_bounds__(256) void k_emit_kept(CsrView g, const uint32_t* __
       const uint64_t bal = __ballot(keep) & gmask;
       if (keep) {
         const int64_t q = pos + __popcll(bal & lt);
-        keys[q] = ((uint64_t)cu << vbits) | (uint64_t)cv;
+        keys[q] = ((uint64_t)(cu + shift) << vbits) | (uint64_t)(cv + shift);
         vals[q] = w;
       }
       pos += __popcll(bal);
@@ -193,12 +208,15 @@ __global__ __launch_bounds__(256) void k_emit_compressed(const uint64_t* __restr
 }
 
 __global__ __launch_bounds__(256) void k_node_keys(int64_t n, const uint32_t* __restrict__ ids,
+                                                   GraphMap gm, int64_t n2,
                                                    const double* __restrict__ nw,
                                                    uint64_t* __restrict__ keys,
                                                    double* __restrict__ vals) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
-    keys[v] = ids[v];
+    const int64_t shift = gm.Shift(v);
+    // Vertices of graphs that leave the batch sort last under key n2.
+    keys[v] = shift == INT64_MIN ? (uint64_t)n2 : (uint64_t)(ids[v] + shift);
     vals[v] = nw[v];
   }
 }
@@ -206,11 +224,83 @@ __global__ __launch_bounds__(256) void k_node_keys(int64_t n, const uint32_t* __
 __global__ __launch_bounds__(256) void k_scatter_node_weights(const uint64_t* __restrict__ keys,
                                                               const int64_t* __restrict__ start,
                                                               const double* __restrict__ sums,
-                                                              int64_t nseg,
+                                                              int64_t nseg, int64_t n2,
                                                               double* __restrict__ nw2) {
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
-       s += (int64_t)gridDim.x * blockDim.x)
-    nw2[keys[start[s]]] = sums[s];
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[start[s]];
+    if ((int64_t)k < n2) nw2[k] = sums[s];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_graph_max(int64_t n, const uint32_t* __restrict__ ids,
+                                                   const uint32_t* __restrict__ vgraph,
+                                                   uint32_t* __restrict__ gmax) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    atomicMax(&gmax[vgraph ? vgraph[v] : 0], ids[v]);
+}
+
+__global__ __launch_bounds__(256) void k_flatten_shift(int64_t n, uint32_t* __restrict__ ids,
+                                                       const uint32_t* __restrict__ vgraph0,
+                                                       const uint8_t* __restrict__ sel,
+                                                       const int64_t* __restrict__ shift,
+                                                       const uint32_t* __restrict__ next_ids) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = vgraph0 ? vgraph0[v] : 0u;
+    if (sel && !sel[o]) continue;
+    const int64_t x = (int64_t)ids[v] + (shift ? shift[o] : 0);
+    ids[v] = next_ids ? next_ids[x] : (uint32_t)x;
+  }
+}
+
+// Batch refinement relabel (see ResetClusteringBatch).
+__global__ __launch_bounds__(256) void k_batch_flat(int64_t n, const uint32_t* __restrict__ ids,
+                                                    const uint32_t* __restrict__ next_ids,
+                                                    const uint32_t* __restrict__ vgraph,
+                                                    const int64_t* __restrict__ gbase,
+                                                    const int64_t* __restrict__ nbase,
+                                                    const uint8_t* __restrict__ sel,
+                                                    uint32_t* __restrict__ flat,
+                                                    uint8_t* __restrict__ present) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = vgraph ? vgraph[v] : 0u;
+    if (sel && !sel[p]) continue;
+    const int64_t gb = gbase ? gbase[p] : 0, nb = nbase ? nbase[p] : 0;
+    const uint32_t y = next_ids[nb + (int64_t)ids[v] - gb];
+    flat[v] = y;
+    present[y] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_clear(int64_t n, const uint32_t* __restrict__ vgraph,
+                                                     const uint8_t* __restrict__ sel, StateView st) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (sel && !sel[vgraph ? vgraph[i] : 0u]) continue;
+    st.sizes[i] = 0;
+    st.K[i] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_assign(int64_t n, const uint32_t* __restrict__ flat,
+                                                      const uint32_t* __restrict__ prank,
+                                                      const uint32_t* __restrict__ vgraph,
+                                                      const int64_t* __restrict__ gbase,
+                                                      const int64_t* __restrict__ nbase,
+                                                      const uint8_t* __restrict__ sel, StateView st) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = vgraph ? vgraph[v] : 0u;
+    if (sel && !sel[p]) continue;
+    const int64_t gb = gbase ? gbase[p] : 0, nb = nbase ? nbase[p] : 0;
+    const uint32_t r = (uint32_t)(gb + (int64_t)(prank[flat[v]] - prank[nb]));
+    st.ids[v] = r;
+    atomicAdd(&st.sizes[r], 1u);
+    atomicAdd(&st.K[r], st.nw[v]);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_flatten(int64_t n, const uint32_t* __restrict__ ids,
@@ -315,43 +405,84 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
   }
 }
 
+// Per-vertex intra-cluster edge count and per-cluster degree sums
+// (PMC.cc:43-61).
 __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint32_t* __restrict__ ids,
-                                                          unsigned long long* __restrict__ intra,
+                                                          double* __restrict__ intra,
                                                           double* __restrict__ dsum) {
-  unsigned long long cnt = 0;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
        v += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t cv = ids[v];
+    int64_t cnt = 0;
     for (int64_t p = g.off[v]; p < g.off[v + 1]; ++p) cnt += ids[g.nbr[p]] == cv ? 1 : 0;
+    intra[v] = (double)cnt;
     atomicAdd(&dsum[cv], (double)(g.off[v + 1] - g.off[v]));
   }
-  for (int s = 32; s > 0; s >>= 1) cnt += __shfl_xor(cnt, s);
-  if (LaneId() == 0 && cnt) atomicAdd(intra, cnt);
 }
 
-__global__ __launch_bounds__(256) void k_modularity_penalty(int64_t n, const double* __restrict__ dsum,
-                                                            double gamma, double total,
+__device__ __forceinline__ int GraphOf(const int64_t* gbase, int G, int64_t v) {
+  int lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gbase[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// (gamma * d_c * d_c) / total of the graph owning cluster id c.
+__global__ __launch_bounds__(256) void k_modularity_penalty(CsrView g, const double* __restrict__ dsum,
+                                                            int G, const int64_t* __restrict__ gbase,
+                                                            const double* __restrict__ gamma,
                                                             double* __restrict__ t) {
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.n;
        c += (int64_t)gridDim.x * blockDim.x) {
+    const int gi = GraphOf(gbase, G, c);
+    const double total = (double)(g.off[gbase[gi + 1]] - g.off[gbase[gi]]);
     const double d = dsum[c];
-    t[c] = (gamma * d * d) / total;
+    t[c] = (gamma[gi] * d * d) / total;
   }
 }
 
-__global__ void k_objective_final(const double* __restrict__ esum, const double* __restrict__ rsum,
-                                  double res, double* __restrict__ out) {
-  double objective = *esum;
-  objective -= res * *rsum / 2;
-  *out = objective;
+// One workgroup per graph sums a[] and b[] over the graph's vertex range in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_graph_sums(const double* __restrict__ a,
+                                                    const double* __restrict__ b,
+                                                    const int64_t* __restrict__ gbase,
+                                                    double* __restrict__ sa, double* __restrict__ sb) {
+  __shared__ double sm[4];
+  const int gi = blockIdx.x;
+  double x = 0.0, y = 0.0;
+  for (int64_t v = gbase[gi] + threadIdx.x; v < gbase[gi + 1]; v += blockDim.x) {
+    x += a[v];
+    y += b[v];
+  }
+  x = BlockReduce(x, sm, [](double p, double q) { return p + q; });
+  y = BlockReduce(y, sm, [](double p, double q) { return p + q; });
+  if (threadIdx.x == 0) {
+    sa[gi] = x;
+    sb[gi] = y;
+  }
 }
 
-__global__ void k_modularity_final(const unsigned long long* __restrict__ intra,
-                                   const double* __restrict__ tsum, double total,
+__global__ void k_objective_final(int G, const double* __restrict__ esum,
+                                  const double* __restrict__ rsum, const double* __restrict__ res,
+                                  double* __restrict__ out) {
+  for (int gi = threadIdx.x; gi < G; gi += blockDim.x) {
+    double objective = esum[gi];
+    objective -= res[gi] * rsum[gi] / 2;
+    out[gi] = objective;
+  }
+}
+
+__global__ void k_modularity_final(CsrView g, int G, const int64_t* __restrict__ gbase,
+                                   const double* __restrict__ intra, const double* __restrict__ tsum,
                                    double* __restrict__ out) {
-  double q = (double)*intra;
-  q -= *tsum;
-  *out = q / total;
+  for (int gi = threadIdx.x; gi < G; gi += blockDim.x) {
+    const double total = (double)(g.off[gbase[gi + 1]] - g.off[gbase[gi]]);
+    double q = intra[gi];
+    q -= tsum[gi];
+    out[gi] = q / total;
+  }
 }
 
 template <typename V>
@@ -401,22 +532,24 @@ int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<in
 
 }  // namespace
 
+void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, uint32_t* gmax,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_graph_max, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids, vgraph, gmax);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
-                   DevCsr* out, DevBuf<double>* out_nw, hipStream_t s) {
+                   const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
+                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s) {
   const int64_t n = g.n;
-  // n' = 1 + max id (PCCI.cc:713-717).
-  DevBuf<uint32_t> dmax(al, 1);
-  ReduceMaxU32(al, ids, n, dmax.get(), s);
-  uint32_t max_id = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&max_id, dmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
-  const int64_t n2 = (int64_t)max_id + 1;
-  const int vbits = BitsFor((uint64_t)(n2 - 1));
+  const GraphMap gm{vgraph, gbase, newbase};
+  const int vbits = BitsFor((uint64_t)(n2 > 1 ? n2 - 1 : 1));
 
   // Kept edges in CSR order.
   DevBuf<uint32_t> kept(al, n);
   DevBuf<int64_t> kept_off(al, n + 1);
-  hipLaunchKernelGGL(k_count_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids, kept.get());
+  hipLaunchKernelGGL(k_count_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids, gm, kept.get());
   ExclusiveScanU32toI64(al, kept.get(), kept_off.get(), n, kept_off.get() + n, s);
   int64_t mk = 0;
   PCC_HIP_CHECK(hipMemcpyAsync(&mk, kept_off.get() + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -424,7 +557,7 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
   DevBuf<uint64_t> keys(al, mk);
   DevBuf<float> vals(al, mk);
   if (mk > 0) {
-    hipLaunchKernelGGL(k_emit_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids,
+    hipLaunchKernelGGL(k_emit_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids, gm,
                        kept_off.get(), vbits, keys.get(), vals.get());
     PCC_HIP_CHECK(hipGetLastError());
     RadixSortPairsF32(al, keys.get(), vals.get(), mk, 0, 2 * vbits, s);
@@ -456,10 +589,10 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
   // Node weights: stable sort of (id, weight) by id, ordered sums.
   DevBuf<uint64_t> nkeys(al, n);
   DevBuf<double> nvals(al, n);
-  hipLaunchKernelGGL(k_node_keys, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids, nw,
+  hipLaunchKernelGGL(k_node_keys, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids, gm, n2, nw,
                      nkeys.get(), nvals.get());
   PCC_HIP_CHECK(hipGetLastError());
-  RadixSortPairsF64(al, nkeys.get(), nvals.get(), n, 0, vbits, s);
+  RadixSortPairsF64(al, nkeys.get(), nvals.get(), n, 0, BitsFor((uint64_t)n2), s);
   DevBuf<int64_t> nstart;
   const int64_t ng = Segments(al, nkeys.get(), n, &nstart, s);
   DevBuf<double> gsum(al, ng > 0 ? ng : 1);
@@ -468,9 +601,34 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
   PCC_HIP_CHECK(hipMemsetAsync(out_nw->get(), 0, sizeof(double) * (size_t)n2, s));
   if (ng > 0) {
     hipLaunchKernelGGL(k_scatter_node_weights, dim3(Grid(ng, 256, 8192)), dim3(256), 0, s,
-                       nkeys.get(), nstart.get(), gsum.get(), ng, out_nw->get());
+                       nkeys.get(), nstart.get(), gsum.get(), ng, n2, out_nw->get());
     PCC_HIP_CHECK(hipGetLastError());
   }
+}
+
+void LaunchFlattenShift(int64_t n, uint32_t* ids, const uint32_t* vgraph0, const uint8_t* sel,
+                        const int64_t* shift, const uint32_t* next_ids, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_flatten_shift, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids, vgraph0,
+                     sel, shift, next_ids);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void ResetClusteringBatch(DeviceAllocator* al, int64_t n, int64_t n_next, const uint32_t* next_ids,
+                          const uint32_t* vgraph, const int64_t* gbase, const int64_t* nbase,
+                          const uint8_t* sel, const StateView& st, hipStream_t s) {
+  if (n <= 0) return;
+  DevBuf<uint8_t> present(al, n_next > 0 ? n_next : 1);
+  DevBuf<uint32_t> prank(al, n_next > 0 ? n_next : 1), flat(al, n);
+  PCC_HIP_CHECK(hipMemsetAsync(present.get(), 0, (size_t)(n_next > 0 ? n_next : 1), s));
+  const int grid = Grid(n, 256, 8192);
+  hipLaunchKernelGGL(k_batch_flat, dim3(grid), dim3(256), 0, s, n, st.ids, next_ids, vgraph, gbase,
+                     nbase, sel, flat.get(), present.get());
+  ExclusiveScanU8toU32(al, present.get(), prank.get(), n_next, nullptr, s);
+  hipLaunchKernelGGL(k_batch_clear, dim3(grid), dim3(256), 0, s, n, vgraph, sel, st);
+  hipLaunchKernelGGL(k_batch_assign, dim3(grid), dim3(256), 0, s, n, flat.get(), prank.get(),
+                     vgraph, gbase, nbase, sel, st);
+  PCC_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,
@@ -515,10 +673,14 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s) {
 }
 
 void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
-                      double res, double offset, double* out_dev, hipStream_t s) {
+                      int G, const int64_t* gbase, const double* res, double offset,
+                      double* out_dev, hipStream_t s) {
   const int64_t n = g.n;
-  DevBuf<double> K(al, n > 0 ? n : 1), e(al, n > 0 ? n : 1), r(al, n > 0 ? n : 1), sums(al, 2);
-  PCC_HIP_CHECK(hipMemsetAsync(K.get(), 0, sizeof(double) * (size_t)(n > 0 ? n : 1), s));
+  const int64_t nn = n > 0 ? n : 1;
+  DevBuf<double> K(al, nn), e(al, nn), r(al, nn), se(al, G), sr(al, G);
+  PCC_HIP_CHECK(hipMemsetAsync(K.get(), 0, sizeof(double) * (size_t)nn, s));
+  PCC_HIP_CHECK(hipMemsetAsync(e.get(), 0, sizeof(double) * (size_t)nn, s));
+  PCC_HIP_CHECK(hipMemsetAsync(r.get(), 0, sizeof(double) * (size_t)nn, s));
   if (n > 0) {
     const int grid = Grid(n, 256, 8192);
     hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, K.get());
@@ -526,31 +688,32 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
                        e.get(), r.get());
     PCC_HIP_CHECK(hipGetLastError());
   }
-  SumF64(al, e.get(), n, sums.get(), s);
-  SumF64(al, r.get(), n, sums.get() + 1, s);
-  hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(1), 0, s, sums.get(), sums.get() + 1, res,
+  hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, e.get(), r.get(), gbase, se.get(),
+                     sr.get());
+  hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(256), 0, s, G, se.get(), sr.get(), res,
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, double gamma,
-                       double* out_dev, hipStream_t s) {
+void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int G,
+                       const int64_t* gbase, const double* gamma, double* out_dev, hipStream_t s) {
   const int64_t n = g.n;
-  DevBuf<double> dsum(al, n > 0 ? n : 1), t(al, n > 0 ? n : 1), tsum(al, 1);
-  DevBuf<unsigned long long> intra(al, 1);
-  PCC_HIP_CHECK(hipMemsetAsync(dsum.get(), 0, sizeof(double) * (size_t)(n > 0 ? n : 1), s));
-  PCC_HIP_CHECK(hipMemsetAsync(intra.get(), 0, sizeof(unsigned long long), s));
-  const double total = (double)g.m;
+  const int64_t nn = n > 0 ? n : 1;
+  DevBuf<double> dsum(al, nn), t(al, nn), intra(al, nn), si(al, G), st(al, G);
+  PCC_HIP_CHECK(hipMemsetAsync(dsum.get(), 0, sizeof(double) * (size_t)nn, s));
+  PCC_HIP_CHECK(hipMemsetAsync(t.get(), 0, sizeof(double) * (size_t)nn, s));
+  PCC_HIP_CHECK(hipMemsetAsync(intra.get(), 0, sizeof(double) * (size_t)nn, s));
   if (n > 0) {
     const int grid = Grid(n, 256, 8192);
     hipLaunchKernelGGL(k_modularity_terms, dim3(grid), dim3(256), 0, s, g, ids, intra.get(),
                        dsum.get());
-    hipLaunchKernelGGL(k_modularity_penalty, dim3(grid), dim3(256), 0, s, n, dsum.get(), gamma,
-                       total, t.get());
+    hipLaunchKernelGGL(k_modularity_penalty, dim3(grid), dim3(256), 0, s, g, dsum.get(), G, gbase,
+                       gamma, t.get());
     PCC_HIP_CHECK(hipGetLastError());
   }
-  SumF64(al, t.get(), n, tsum.get(), s);
-  hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(1), 0, s, intra.get(), tsum.get(), total,
+  hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, intra.get(), t.get(), gbase, si.get(),
+                     st.get());
+  hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, g, G, gbase, si.get(), st.get(),
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());
 }

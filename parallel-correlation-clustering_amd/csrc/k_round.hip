@@ -23,7 +23,7 @@ struct Bins {
 };
 
 __global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __restrict__ list,
-                                               StateView st, uint32_t n,
+                                               StateView st, uint32_t new_id,
                                                const uint32_t* __restrict__ moves,
                                                uint8_t* __restrict__ modified,
                                                uint8_t* __restrict__ new_flag,
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __rest
       atomicSub(&st.sizes[cur], 1u);
       atomicAdd(&st.K[cur], -k);
       modified[cur] = 1;
-      if (t != n) {
+      if (t != new_id) {
         atomicAdd(&st.sizes[t], 1u);
         atomicAdd(&st.K[t], k);
         modified[t] = 1;
@@ -71,17 +71,21 @@ __global__ __launch_bounds__(256) void k_zero_compact(uint32_t n, const uint8_t*
                                                       uint32_t* __restrict__ zero_list,
                                                       const unsigned long long* __restrict__ gate) {
   if (*gate == 0) return;
-  const unsigned long long need = *gate;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    if (z[i] && zrank[i] < need) zero_list[zrank[i]] = (uint32_t)i;
+    if (z[i]) zero_list[zrank[i]] = (uint32_t)i;
 }
 
-// i-th new-cluster mover in ascending vertex order takes the i-th empty id in
-// ascending order (PCCI.cc:346-365).
+// i-th new-cluster mover of a graph in ascending vertex order takes the i-th
+// empty id of that graph in ascending order (PCCI.cc:346-365).  Exclusive
+// ranks over the whole level minus the rank at the graph's first vertex give
+// ranks within the graph.
 __global__ __launch_bounds__(256) void k_new_assign(uint32_t n, StateView st,
+                                                    const uint32_t* __restrict__ vgraph,
+                                                    const int64_t* __restrict__ gbase,
                                                     uint8_t* __restrict__ new_flag,
                                                     const uint32_t* __restrict__ nrank,
+                                                    const uint32_t* __restrict__ zrank,
                                                     const uint32_t* __restrict__ zero_list,
                                                     uint8_t* __restrict__ modified,
                                                     const unsigned long long* __restrict__ gate) {
@@ -89,11 +93,18 @@ __global__ __launch_bounds__(256) void k_new_assign(uint32_t n, StateView st,
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
     if (!new_flag[v]) continue;
-    const uint32_t c = zero_list[nrank[v]];
+    int64_t base = 0, end = n;
+    if (vgraph) {
+      const uint32_t g = vgraph[v];
+      base = gbase[g];
+      end = gbase[g + 1];
+    }
+    const uint32_t r = nrank[v] - nrank[base];
+    const uint32_t c = zero_list[zrank[base] + r];
     new_flag[v] = 0;
-    // Sizes summing to n guarantee enough empty ids; inconsistent caller state
-    // must not turn into an out-of-range write.
-    if (c >= n) continue;
+    // Sizes summing to the graph's vertex count guarantee enough empty ids;
+    // inconsistent caller state must not turn into an out-of-range write.
+    if ((int64_t)c < base || (int64_t)c >= end) continue;
     st.ids[v] = c;
     st.sizes[c] = 1;
     modified[c] = 1;
@@ -153,17 +164,34 @@ Bins MakeBins(const int64_t counts[kBins], const BinLayout& layout, int64_t* tot
 }  // namespace
 
 void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
-                 const StateView& st, uint32_t n, const uint32_t* moves, uint8_t* modified,
+                 const StateView& st, uint32_t new_id, const uint32_t* moves, uint8_t* modified,
                  uint8_t* new_flag, RoundCounters* rc, hipStream_t s) {
   int64_t total;
   const Bins b = MakeBins(counts, layout, &total);
   if (total == 0) return;
-  hipLaunchKernelGGL(k_apply, dim3(Grid(total, 256, 4096)), dim3(256), 0, s, b, list, st, n, moves,
-                     modified, new_flag, rc);
+  hipLaunchKernelGGL(k_apply, dim3(Grid(total, 256, 4096)), dim3(256), 0, s, b, list, st, new_id,
+                     moves, modified, new_flag, rc);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n, uint8_t* new_flag,
+__global__ __launch_bounds__(256) void k_graph_flags(int64_t n, const uint8_t* __restrict__ next,
+                                                     const uint32_t* __restrict__ vgraph,
+                                                     uint8_t* __restrict__ flags) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (next[v]) flags[vgraph ? vgraph[v] : 0] = 1;
+}
+
+void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_graph_flags, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, next, vgraph,
+                     flags);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
+                         const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
                          uint8_t* modified, RoundCounters* rc, hipStream_t s) {
   const unsigned long long* gate = &rc->new_count;
   DevBuf<uint8_t> z(al, n);
@@ -175,8 +203,8 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n, u
   hipLaunchKernelGGL(k_zero_compact, dim3(grid), dim3(256), 0, s, n, z.get(), zrank.get(),
                      zero_list.get(), gate);
   ExclusiveScanU8toU32(al, new_flag, nrank.get(), n, nullptr, s, gate);
-  hipLaunchKernelGGL(k_new_assign, dim3(grid), dim3(256), 0, s, n, st, new_flag, nrank.get(),
-                     zero_list.get(), modified, gate);
+  hipLaunchKernelGGL(k_new_assign, dim3(grid), dim3(256), 0, s, n, st, vgraph, gbase, new_flag,
+                     nrank.get(), zrank.get(), zero_list.get(), modified, gate);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

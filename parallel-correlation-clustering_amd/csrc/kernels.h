@@ -133,11 +133,21 @@ struct BinLayout {
   int64_t base[kBins];
 };
 
+// Parameters of the best-move kernels.  Cluster ids are global vertex ids of
+// the level (a batch of graphs occupies consecutive vertex ranges), so ties on
+// the smaller id and the singleton rule compare ids within one graph exactly
+// as the reference compares them.
 struct MoveParams {
-  double res = 0;
-  double offset = 0;
-  uint32_t n = 0;
+  double res = 0;           // resolution when every graph shares one
+  double offset = 0;        // edge_weight_offset
+  uint32_t new_id = 0;      // "form a new cluster" target (n for one graph)
   int force_ordered = 0;
+  // Batches (null for a single graph): graph of each vertex, per-graph
+  // resolution and vertex ranges [gbase[g], gbase[g + 1]).
+  const uint32_t* vgraph = nullptr;
+  const double* gres = nullptr;
+  const int64_t* gbase = nullptr;
+  uint32_t n = 0;           // vertices of the level
 };
 
 struct MoveOut {
@@ -181,13 +191,18 @@ void LaunchBestMoveOrdered(const CsrView& g, const StateView& st, const MovePara
 
 // Sync apply, MoveNodesToCluster (parallel-correlation-clusterer-internal.cc:256-369).
 void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
-                 const StateView& st, uint32_t n, const uint32_t* moves, uint8_t* modified,
+                 const StateView& st, uint32_t new_id, const uint32_t* moves, uint8_t* modified,
                  uint8_t* new_flag, RoundCounters* rc, hipStream_t s);
 
-// Empty-id reuse for vertices forming new clusters; gated on rc->new_count.
+// Empty-id reuse for vertices forming new clusters, per graph of a batch
+// (vgraph / gbase null for one graph); gated on rc->new_count.
 void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
-                         uint8_t* new_flag, uint8_t* modified, RoundCounters* rc,
-                         hipStream_t s);
+                         const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
+                         uint8_t* modified, RoundCounters* rc, hipStream_t s);
+
+// flags[graph of v] = 1 for every v with next[v] != 0.
+void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
+                      hipStream_t s);
 
 // Next frontier = neighbours of the seeds.
 void LaunchExpandMoved(const int64_t counts[kBins], const BinLayout& layout,
@@ -236,19 +251,44 @@ struct DevCsr {
   }
 };
 
-// CompressGraph (parallel-correlation-clusterer-internal.cc:709-782).
+// Per-graph maximum cluster id (gmax[graph] = max ids[v]); gmax zeroed by the
+// caller.  vgraph null: one graph.
+void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, uint32_t* gmax,
+                       hipStream_t s);
+
+// CompressGraph (parallel-correlation-clusterer-internal.cc:709-782) of every
+// graph g of the level with newbase[g] >= 0: cluster id c of graph g becomes
+// vertex newbase[g] + (c - gbase[g]) of the contracted level, whose size is n2.
+// Vertices of graphs with newbase < 0 are dropped.  vgraph / gbase / newbase
+// null: one graph, identity numbering (n2 = 1 + max id).
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
-                   DevCsr* out, DevBuf<double>* out_nw, hipStream_t s);
+                   const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
+                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s);
 
 // FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,
                    hipStream_t s);
+
+// Flatten of the level-0 assignment of batch graphs: for vertex v of original
+// graph o = vgraph0[v] with sel[o] != 0, ids[v] = next_ids[ids[v] + shift[o]]
+// (next_ids null: only the shift is applied).
+void LaunchFlattenShift(int64_t n, uint32_t* ids, const uint32_t* vgraph0, const uint8_t* sel,
+                        const int64_t* shift, const uint32_t* next_ids, hipStream_t s);
 
 // Compact rank relabel + ResetClustering/SetClustering
 // (parallel-correlation-clusterer.cc:594-600): ids := rank of id among the
 // distinct ids; sizes/K rebuilt.
 void ResetClusteringFromIds(DeviceAllocator* al, int64_t n, const uint32_t* flat,
                             const StateView& st, hipStream_t s);
+
+// Batch form of the refinement step of one level: for every vertex v of a
+// graph p with sel[p] != 0 (vgraph / gbase / nbase describe the level and the
+// next level), flat = next_ids[nbase[p] + ids[v] - gbase[p]] is relabelled to
+// gbase[p] + (rank of flat among the graph's distinct flat ids) and that
+// graph's sizes / cluster weights are rebuilt.
+void ResetClusteringBatch(DeviceAllocator* al, int64_t n, int64_t n_next, const uint32_t* next_ids,
+                          const uint32_t* vgraph, const int64_t* gbase, const int64_t* nbase,
+                          const uint8_t* sel, const StateView& st, hipStream_t s);
 
 // Singleton state (ids = i, sizes = 1, K = nw).
 void LaunchSingletons(int64_t n, const StateView& st, hipStream_t s);
@@ -263,14 +303,17 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s);
 // Fixed-order double sum.
 void SumF64(DeviceAllocator* al, const double* in, int64_t n, double* out, hipStream_t s);
 
-// ComputeObjective (parallel-correlation-clusterer-internal.cc:118-149).
+// ComputeObjective (parallel-correlation-clusterer-internal.cc:118-149) of
+// each of the G graphs [gbase[g], gbase[g + 1]) with resolution res[g]; ids
+// are global cluster ids.  out_dev[g] receives graph g's objective.
 void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
-                      const double* nw, double res, double offset, double* out_dev,
-                      hipStream_t s);
+                      const double* nw, int G, const int64_t* gbase, const double* res,
+                      double offset, double* out_dev, hipStream_t s);
 
-// ComputeModularity (parallel-modularity-clusterer.cc:36-66).
-void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, double gamma,
-                       double* out_dev, hipStream_t s);
+// ComputeModularity (parallel-modularity-clusterer.cc:36-66) of each graph
+// with resolution gamma[g].
+void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int G,
+                       const int64_t* gbase, const double* gamma, double* out_dev, hipStream_t s);
 
 }  // namespace pcc
 

@@ -146,6 +146,10 @@ _SIGS = {
     "pcc_round": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, _P, _P, _P, ctypes.POINTER(_I64)]),
     "pcc_compress": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(_P), _P]),
     "pcc_set_option": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
+    "pcc_batch_from_csr": (ctypes.c_int, [_I64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "pcc_batch_num_graphs": (_I64, [_P]),
+    "pcc_batch_cluster": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, _P, _P, _P,
+                                         ctypes.POINTER(Stats)]),
     "pcc_host_csr_free": (None, [ctypes.POINTER(HostCsr)]),
     "pcc_gen_sbm": (ctypes.c_int, [_I64, _I64, _D, _D, _D, _D, _U64, ctypes.POINTER(HostCsr)]),
     "pcc_gen_sbm_blocks": (ctypes.c_int, [_I64, _P, _D, _D, _D, _D, _U64, ctypes.POINTER(HostCsr)]),
@@ -391,6 +395,47 @@ class DeviceGraph:
         _check(lib().pcc_compress(self._h, _ptr(ids), _ptr(nw), ctypes.byref(h), _ptr(out_nw)))
         g = DeviceGraph(h)
         return g, out_nw[: g.n]
+
+
+class DeviceBatch(DeviceGraph):
+    """Many independent graphs in one device structure (pcc_batch_*)."""
+
+    @classmethod
+    def from_csrs(cls, csrs: Sequence[Csr]) -> "DeviceBatch":
+        counts = np.array([c.n for c in csrs], np.int64)
+        off = [np.zeros(1, np.int64)]
+        run = 0
+        for c in csrs:
+            off.append(c.offsets[1:] + run)
+            run += c.m
+        offsets = np.ascontiguousarray(np.concatenate(off), dtype=np.int64)
+        nbrs = np.ascontiguousarray(np.concatenate([c.nbrs for c in csrs]) if run else
+                                    np.zeros(0, np.uint32), dtype=np.uint32)
+        w = np.ascontiguousarray(np.concatenate([c.weights for c in csrs]) if run else
+                                 np.zeros(0, np.float32), dtype=np.float32)
+        h = ctypes.c_void_p()
+        _check(lib().pcc_batch_from_csr(len(csrs), _ptr(counts), _ptr(offsets), _ptr(nbrs), _ptr(w),
+                                        ctypes.byref(h)))
+        b = cls(h)
+        b.counts = counts
+        return b
+
+    @property
+    def num_graphs(self) -> int:
+        return int(lib().pcc_batch_num_graphs(self._h))
+
+    def cluster_batch(self, cfg: Config, resolutions=None, objective: int = OBJECTIVE_CORRELATION):
+        """Returns (list of per-graph local id arrays, objectives, modularities, stats)."""
+        G = self.num_graphs
+        ids = np.zeros(max(self.n, 1), np.uint32)
+        obj = np.zeros(G)
+        mod = np.zeros(G)
+        res = None if resolutions is None else np.ascontiguousarray(resolutions, dtype=np.float64)
+        st = Stats()
+        _check(lib().pcc_batch_cluster(self._h, ctypes.byref(cfg), objective, _ptr(res), _ptr(ids),
+                                       _ptr(obj), _ptr(mod), ctypes.byref(st)))
+        bounds = np.concatenate([[0], np.cumsum(self.counts)])
+        return [ids[bounds[i]:bounds[i + 1]] for i in range(G)], obj, mod, st
 
 
 # ------------------------------------------- InMemoryClusterer mirror API --

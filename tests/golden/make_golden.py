@@ -1,0 +1,61 @@
+"""Writes the golden vectors in this directory from the CPU oracle.
+
+The reference ships no fixtures (its tests were not published), so these
+vectors pin the HIP path and future versions of the oracle to the oracle as
+validated by tests/test_oracle.py.  Each .npz holds the input CSR (as the
+seeded generators produce it), the config, and the oracle's final cluster ids,
+objective and modularity.
+
+Run: python tests/golden/make_golden.py
+"""
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "parallel-correlation-clustering_amd"), os.path.join(ROOT, "oracle")]
+
+import oracle  # noqa: E402
+import pcc_amd  # noqa: E402
+
+CASES = {
+    # SBM planted partition, sync ALL_MOVE, res 0.01, no refine, 10 x 10 (the
+    # small-graph configuration, at 1/5 of its size).
+    "sbm2000_allmove": (dict(kind="sbm", n=2000, blocks=20, p_in=0.1, p_out=0.001, seed=1),
+                        "correlation_clusterer_config { resolution: 0.01 move_method: ALL_MOVE "
+                        "louvain_config { num_iterations: 10 num_inner_iterations: 10 } }", 0),
+    # RMAT, signed FP weights, sync NBHR_MOVE, res 0.05, refine.
+    "rmat10_nbhr_refine": (dict(kind="rmat", scale=10, ef=16, lo=-1.0, hi=1.0, seed=2),
+                           "correlation_clusterer_config { resolution: 0.05 move_method: NBHR_MOVE "
+                           "refine: true }", 0),
+    # RMAT, unit weights, modularity, sync NBHR_CLUSTER_MOVE, refine.
+    "rmat11_modularity": (dict(kind="rmat", scale=11, ef=8, lo=1.0, hi=1.0, seed=3),
+                          "correlation_clusterer_config { resolution: 1.0 refine: true }", 1),
+}
+
+
+def make_graph(spec):
+    if spec["kind"] == "sbm":
+        return pcc_amd.gen_sbm(spec["n"], spec["blocks"], spec["p_in"], spec["p_out"], seed=spec["seed"])
+    return pcc_amd.gen_rmat(spec["scale"], spec["ef"], 0.57, 0.19, 0.19, spec["lo"], spec["hi"],
+                            seed=spec["seed"])
+
+
+def main():
+    for name, (spec, text, modularity) in CASES.items():
+        csr = make_graph(spec)
+        cfg = pcc_amd.parse_config(text)
+        r = oracle.cluster(oracle.G.from_csr(csr), oracle.config_from(cfg), bool(modularity))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), offsets=csr.offsets, nbrs=csr.nbrs,
+                            weights=csr.weights, ids=r.ids, objective=r.objective,
+                            modularity=r.modularity, spec=json.dumps(spec), config=text,
+                            modularity_objective=modularity)
+        print(name, csr.n, csr.m, r.num_clusters, r.objective)
+
+
+if __name__ == "__main__":
+    main()
