@@ -42,11 +42,25 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads for the CPU baseline (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="rmat22",
+                    choices=["rmat22", "batched", "modularity-async", "serving"],
+                    help="rmat22: the headline line; the others are secondary measurements")
+    ap.add_argument("--graphs", type=int, default=4096, help="batched: number of SBM graphs")
+    ap.add_argument("--plain-graphs", type=int, default=0,
+                    help="batched: per-graph runs timed for the comparison (0: all)")
+    ap.add_argument("--requests", type=int, default=96, help="serving: requests in the stream")
+    ap.add_argument("--qps", type=float, default=8.0, help="serving: Poisson arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the clustering path is single-GPU in this build)", file=sys.stderr)
         return 2
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "modularity-async":
+        return bench_modularity_async(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import numpy as np
     import pcc_amd
@@ -143,6 +157,164 @@ def main() -> int:
             "traffic": None,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def c4_graphs(pcc_amd, count, seed=4):
+    """Batched workload: SBM graphs, n ~ logU[200, 2e5], mean degree ~ U[8, 64],
+    power-law (exponent 2) community sizes, p_out / p_in = 0.01, weights
+    U[0, 1] rounded to float32, resolution ~ U[0.001, 0.1]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    graphs, res = [], []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(200000))))
+        d = rng.uniform(8, 64)
+        sizes = []
+        left = n
+        while left > 0:
+            # discrete power law P(s) ~ s^-2 on [8, n], by inverse transform
+            u = rng.random()
+            s_ = int(8 / (1 - u * (1 - 8 / max(n, 9))))
+            s_ = max(1, min(s_, left))
+            sizes.append(s_)
+            left -= s_
+        sizes = np.array(sizes, np.int64)
+        # p_in so that the expected degree is d, with p_out = 0.01 p_in.
+        denom = float((sizes * (sizes - 1)).sum() + 0.01 * (sizes * (n - sizes)).sum()) / n
+        p_in = min(1.0, d / max(denom, 1e-9))
+        graphs.append(pcc_amd.gen_sbm_blocks(sizes, p_in, 0.01 * p_in, 0.0, 1.0, seed=seed * 100003 + i))
+        res.append(rng.uniform(0.001, 0.1))
+    return graphs, np.array(res)
+
+
+def bench_batched(args):
+    import numpy as np
+    import pcc_amd
+
+    t0 = time.time()
+    graphs, res = c4_graphs(pcc_amd, args.graphs)
+    print(f"# {len(graphs)} graphs, {sum(g.n for g in graphs)} vertices, "
+          f"{sum(g.m for g in graphs)} edges, generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    cfg = pcc_amd.make_config(move_method=pcc_amd.NBHR_MOVE, refine=1)
+    b = pcc_amd.DeviceBatch.from_csrs(graphs)
+    for _ in range(args.warmup):
+        b.cluster_batch(cfg, res)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ids_b, obj_b, _, st = b.cluster_batch(cfg, res)
+    batch_s = (time.perf_counter() - t0) / max(args.steps, 1)
+    print(f"# batched: {batch_s*1e3:.1f} ms per batch", file=sys.stderr, flush=True)
+    del b
+    plain_n = args.plain_graphs if args.plain_graphs > 0 else len(graphs)
+    plain_s = 0.0
+    same = 0
+    for i in range(plain_n):
+        g = pcc_amd.DeviceGraph.from_csr(graphs[i])
+        c1 = pcc_amd.make_config(resolution=float(res[i]), move_method=pcc_amd.NBHR_MOVE, refine=1)
+        t1 = time.perf_counter()
+        ids, sst = g.cluster(c1)
+        plain_s += time.perf_counter() - t1
+        same += int(np.array_equal(ids, ids_b[i]) and sst.objective == obj_b[i])
+        g.close()
+    plain_gps = plain_n / plain_s
+    batch_gps = len(graphs) / batch_s
+    line = {
+        "metric": "batched graphs/s vs plain per-graph GPU runs (identical ids and objective)",
+        "value": round(batch_gps, 3), "unit": "graphs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (move deltas) / f32 (edge weights)",
+        "data": "synthetic (seeded SBM batch)",
+        "config": {"workload": f"{len(graphs)} SBM graphs (n logU[200,2e5], mean degree U[8,64], "
+                               "power-law blocks, p_out/p_in 0.01, U[0,1] weights, res U[.001,.1]), "
+                               "sync NBHR_MOVE refine", "parallelism": "single GPU"},
+        "plain_graphs_per_s": round(plain_gps, 3), "plain_graphs_timed": plain_n,
+        "speedup_vs_plain": round(batch_gps / plain_gps, 2),
+        "identical_results": f"{same}/{plain_n}",
+        "batch_rounds": st.rounds, "batch_edges_scanned": st.edges_scanned,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_modularity_async(args):
+    import pcc_amd
+
+    csr = pcc_amd.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, 1.0, 1.0, seed=3)
+    cfg = pcc_amd.make_config(resolution=1.0, move_method=pcc_amd.NBHR_CLUSTER_MOVE, refine=1,
+                              permute=1)
+    cfg.async_ = 1
+    g = pcc_amd.DeviceGraph.from_csr(csr)
+    g.set_option(pcc_amd.OPT_TIMING, 1)
+    for _ in range(args.warmup):
+        g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        ids, st = g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
+        stats.append(st)
+    wall = (time.perf_counter() - t0) / max(args.steps, 1)
+    te = sum(s.edges_scanned for s in stats)
+    bm = sum(s.bestmove_seconds for s in stats)
+    line = {
+        "metric": "best-move GTEPS (ParallelModularityClusterer async NBHR_CLUSTER_MOVE refine permute)",
+        "value": round(te / bm / 1e9, 4) if bm else 0.0, "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (move deltas)", "data": "synthetic (seeded RMAT, unit weights)",
+        "config": {"workload": f"RMAT scale {args.scale} edge factor {args.edge_factor:g}, unit weights, "
+                               "gamma 1.0", "n": csr.n, "m_directed": csr.m, "parallelism": "single GPU"},
+        "modularity": stats[0].modularity, "num_clusters": stats[0].num_clusters,
+        "end_to_end_ms": round(sum(s.cluster_seconds for s in stats) / len(stats) * 1e3, 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_serving(args):
+    import numpy as np
+    import pcc_amd
+
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(args.requests):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+        if i % 2 == 0:
+            g = pcc_amd.gen_sbm(n, max(1, n // 50), min(1.0, 16 / 50.0 * 0.9), 16 * 0.1 / n, 0.0, 1.0,
+                                seed=5000 + i)
+        else:
+            scale = max(10, int(round(np.log2(n))))
+            g = pcc_amd.gen_rmat(scale, 8.0, 0.57, 0.19, 0.19, 0.0, 1.0, seed=5000 + i)
+        reqs.append((g, float(rng.uniform(0.001, 0.1))))
+    cfg = pcc_amd.make_config(move_method=pcc_amd.NBHR_MOVE, refine=1)
+    srv = pcc_amd.Server(cfg, workers=2, max_batch_graphs=64)
+    gaps = rng.exponential(1.0 / args.qps, size=len(reqs))
+    t0 = time.perf_counter()
+    ids = []
+    for (g, r), gap in zip(reqs, gaps):
+        time.sleep(gap)
+        ids.append(srv.submit(g, r))
+    for rid in ids:
+        srv.wait(rid)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    line = {
+        "metric": "serving graphs/s with p50/p99 latency (continuous batching)",
+        "value": round(st.graphs_per_second, 3), "unit": "graphs/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 0, "ms_per_step": round(wall / len(reqs) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (move deltas) / f32 (edge weights)", "data": "synthetic (seeded SBM/RMAT stream)",
+        "config": {"workload": f"{len(reqs)} requests, Poisson {args.qps} QPS, 50/50 SBM/RMAT, "
+                               "n logU[1e3, 2e6], mean degree 16, res U[.001,.1]",
+                   "parallelism": "single GPU, 2 workers"},
+        "p50_latency_s": round(st.p50_latency_seconds, 4), "p99_latency_s": round(st.p99_latency_seconds, 4),
+        "batches": st.batches, "mean_batch_graphs": round(st.mean_batch_graphs, 2),
+        "max_graphs_in_flight": st.max_graphs_in_flight, "failed": st.failed,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -201,6 +201,51 @@ int pcc_batch_cluster(pcc_graph* batch, const pcc_config* cfg, int objective,
                       const double* resolutions, uint32_t* out_cluster_ids,
                       double* out_objectives, double* out_modularities, pcc_stats* stats);
 
+/* -------------------------------------------------------------- serving -- */
+/* A request loop over the batch path (no counterpart in the reference, which
+ * clusters one graph per call; for callers that serve clustering requests).
+ * Submitted graphs queue on the host; worker threads (one HIP stream each)
+ * repeatedly form a batch from the queue, admitting requests while the batch's
+ * estimated HBM footprint fits the budget left by the batches already on the
+ * device, so requests that arrive while others run are picked up as soon as a
+ * worker and memory free up, and batches on different workers overlap on the
+ * GPU.  Each result equals that request's pcc_cluster run. */
+typedef struct pcc_server pcc_server;
+
+typedef struct pcc_server_options {
+  int32_t workers;              /* batches in flight at once (default 2) */
+  int32_t _pad;
+  int64_t max_batch_graphs;     /* default 512 */
+  int64_t max_batch_edges;      /* default 2^31 directed edges */
+  double hbm_budget_bytes;      /* admission budget; 0: 90% of free HBM at creation */
+  double batch_window_seconds;  /* wait this long for more requests before a batch (default 0) */
+} pcc_server_options;
+
+typedef struct pcc_server_stats {
+  int64_t completed;
+  int64_t failed;
+  int64_t batches;
+  int64_t max_graphs_in_flight;
+  double p50_latency_seconds;
+  double p99_latency_seconds;
+  double mean_batch_graphs;
+  double graphs_per_second;     /* completed / (last completion - first submission) */
+} pcc_server_stats;
+
+void pcc_server_options_default(pcc_server_options* opt);
+int pcc_server_create(const pcc_config* cfg, int objective, const pcc_server_options* opt,
+                      pcc_server** out);
+/* Copies the graph (local ids, n >= 1) and queues it; returns its request id. */
+int pcc_server_submit(pcc_server* s, int64_t n, int64_t m, const int64_t* offsets,
+                      const uint32_t* nbrs, const float* weights, double resolution,
+                      uint64_t* out_request_id);
+/* Blocks until the request is done; fills its n cluster ids (local), objective
+ * and submit-to-completion latency.  Returns the request's status. */
+int pcc_server_wait(pcc_server* s, uint64_t request_id, uint32_t* out_cluster_ids,
+                    double* out_objective, double* out_latency_seconds);
+int pcc_server_get_stats(pcc_server* s, pcc_server_stats* out);
+void pcc_server_destroy(pcc_server* s);
+
 /* ------------------------------------------------------------ one phase -- */
 /* Phase-level entry points used by parity tests and by callers that drive the
  * Louvain loop themselves.  State arrays are the ClusteringHelper SoA
@@ -240,6 +285,8 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_FORCE_ORDERED 2  /* 1: route every hashed vertex to the ordered path */
 #define PCC_OPT_WAVE_MAX_DEG 3   /* upper degree of the one-wave LDS-hash bin */
 #define PCC_OPT_BLOCK_MAX_DEG 4  /* upper degree of the one-workgroup LDS-hash bin */
+#define PCC_OPT_ASYNC_SUBROUNDS 5 /* async: launches per round, each on 1/k of the frontier */
+#define PCC_OPT_ASYNC_SINGLETON_RULE 6 /* async: 1 applies the sync singleton rule */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* ------------------------------------------------------------ host data -- */

@@ -306,6 +306,14 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
     case PCC_OPT_FORCE_ORDERED:
       g->force_ordered = value != 0;
       return PCC_OK;
+    case PCC_OPT_ASYNC_SUBROUNDS:
+      if (value < 1 || value > 1024)
+        return SetError(PCC_INVALID_ARGUMENT, "async sub-rounds must lie in [1, 1024]");
+      g->async_subrounds = (int)value;
+      return PCC_OK;
+    case PCC_OPT_ASYNC_SINGLETON_RULE:
+      g->async_singleton_rule = value != 0;
+      return PCC_OK;
     case PCC_OPT_WAVE_MAX_DEG:
       if (value < 64 || value > 512)
         return SetError(PCC_INVALID_ARGUMENT, "wave bin upper degree must lie in [64, 512]");

@@ -404,12 +404,23 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   const uint64_t salt =
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
   Timer all = StartTimer();
-  for (int b = 0; b < kBins; ++b) {
-    if (counts[b] == 0) continue;
-    Timer tb = StartTimer();
-    LaunchBestMoveBin(b, counts[b], async, g, st, c.mp, ws.list.get() + c.layout.base[b], out,
-                      ws.fb_list.get(), ws.rc.get(), c.hs, salt, s_);
-    StopTimer(tb, b);
+  // Async rounds run as sub-rounds over disjoint hashed subsets of the
+  // frontier, each launch seeing the moves of the previous ones, which keeps
+  // the number of simultaneously deciding vertices closer to a multicore
+  // run's.
+  const int subs = async ? std::max(1, h_->async_subrounds) : 1;
+  for (int sub = 0; sub < subs; ++sub) {
+    MoveParams mp = c.mp;
+    mp.sub_count = (uint32_t)subs;
+    mp.sub_index = (uint32_t)sub;
+    mp.async_singleton_rule = async ? h_->async_singleton_rule : 0;
+    for (int b = 0; b < kBins; ++b) {
+      if (counts[b] == 0) continue;
+      Timer tb = StartTimer();
+      LaunchBestMoveBin(b, counts[b], async, g, st, mp, ws.list.get() + c.layout.base[b], out,
+                        ws.fb_list.get(), ws.rc.get(), c.hs, salt, s_);
+      StopTimer(tb, b);
+    }
   }
   if (!async && counts[5] + counts[6] + counts[7] > 0) {
     Timer tf = StartTimer();

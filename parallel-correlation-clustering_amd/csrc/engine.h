@@ -38,6 +38,8 @@ struct pcc_graph {
   // Options.
   bool timing = false;
   bool force_ordered = false;
+  int async_subrounds = 8;
+  int async_singleton_rule = 1;
   pcc::BinThresholds th;
   int num_graphs() const { return gbase.empty() ? 1 : (int)gbase.size() - 1; }
   ~pcc_graph() {

@@ -151,11 +151,14 @@ __device__ __forceinline__ void SyncEpilogue(uint32_t v, uint32_t t, double d, u
 __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d,
                                               const MoveParams& mp, const StateView& st,
                                               const MoveOut& out) {
+  const uint32_t cur = st.ids[v];
+  if (mp.async_singleton_rule && t != mp.new_id && st.sizes[t] == 1 && st.sizes[cur] == 1 &&
+      cur >= t)
+    d = 0.0;
   if (!(d > 0)) {
     out.moved[v] = 0;
     return;
   }
-  const uint32_t cur = st.ids[v];
   const double k = st.nw[v];
   if (t != mp.new_id) {
     atomicSub(&st.sizes[cur], 1u);
@@ -191,6 +194,16 @@ __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d,
   out.moved[v] = 0;
 }
 
+// Sub-round of a vertex in async mode (a fixed hash of the id).
+__device__ __forceinline__ bool InSubRound(const MoveParams& mp, uint32_t v) {
+  if (mp.sub_count <= 1) return true;
+  uint32_t h = v * 0x9E3779B1u;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  return h % mp.sub_count == mp.sub_index;
+}
+
 // Item -> list position; with a salt, a bijection of [0, count) that changes
 // the processing order (vertexMapPermute, PCC.cc:69-109).
 __device__ __forceinline__ int64_t PermuteIndex(int64_t i, int64_t count, uint64_t salt) {
@@ -217,8 +230,8 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
   for (int64_t gb = (int64_t)blockIdx.x * kGroups; gb < count;
        gb += (int64_t)gridDim.x * kGroups) {
     const int64_t gi = gb + threadIdx.x / G;
-    const bool active = gi < count;
-    const uint32_t v = active ? list[PermuteIndex(gi, count, salt)] : 0u;
+    const uint32_t v = gi < count ? list[PermuteIndex(gi, count, salt)] : 0u;
+    const bool active = gi < count && (!kAsync || InSubRound(mp, v));
     int64_t start = 0;
     int deg = 0;
     if (active) {
@@ -374,6 +387,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
   }
   for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t v = list[PermuteIndex(i, count, salt)];
+    if (kAsync && !InSubRound(mp, v)) continue;  // uniform across the workgroup
     const int64_t start = g.off[v];
     const int64_t deg = g.off[v + 1] - start;
     // Table sized per vertex (load factor <= 1/2) so that clearing and

@@ -148,6 +148,12 @@ struct MoveParams {
   const double* gres = nullptr;
   const int64_t* gbase = nullptr;
   uint32_t n = 0;           // vertices of the level
+  // Async sub-rounds: only vertices with SubRoundOf(v, sub_count) == sub_index
+  // are processed by a launch (sub_count <= 1: all).
+  uint32_t sub_count = 1;
+  uint32_t sub_index = 0;
+  // Async: apply the sync singleton rule before moving (0/1).
+  int async_singleton_rule = 0;
 };
 
 struct MoveOut {

@@ -46,6 +46,8 @@ OPT_TIMING = 1
 OPT_FORCE_ORDERED = 2
 OPT_WAVE_MAX_DEG = 3
 OPT_BLOCK_MAX_DEG = 4
+OPT_ASYNC_SUBROUNDS = 5
+OPT_ASYNC_SINGLETON_RULE = 6
 
 
 class StatusError(RuntimeError):
@@ -112,6 +114,30 @@ class Stats(ctypes.Structure):
         return d
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("workers", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+        ("max_batch_graphs", ctypes.c_int64),
+        ("max_batch_edges", ctypes.c_int64),
+        ("hbm_budget_bytes", ctypes.c_double),
+        ("batch_window_seconds", ctypes.c_double),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_int64),
+        ("failed", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("max_graphs_in_flight", ctypes.c_int64),
+        ("p50_latency_seconds", ctypes.c_double),
+        ("p99_latency_seconds", ctypes.c_double),
+        ("mean_batch_graphs", ctypes.c_double),
+        ("graphs_per_second", ctypes.c_double),
+    ]
+
+
 class HostCsr(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int64),
@@ -150,6 +176,13 @@ _SIGS = {
     "pcc_batch_num_graphs": (_I64, [_P]),
     "pcc_batch_cluster": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, _P, _P, _P,
                                          ctypes.POINTER(Stats)]),
+    "pcc_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),
+    "pcc_server_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int,
+                                         ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
+    "pcc_server_submit": (ctypes.c_int, [_P, _I64, _I64, _P, _P, _P, _D, ctypes.POINTER(_U64)]),
+    "pcc_server_wait": (ctypes.c_int, [_P, _U64, _P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "pcc_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    "pcc_server_destroy": (None, [_P]),
     "pcc_host_csr_free": (None, [ctypes.POINTER(HostCsr)]),
     "pcc_gen_sbm": (ctypes.c_int, [_I64, _I64, _D, _D, _D, _D, _U64, ctypes.POINTER(HostCsr)]),
     "pcc_gen_sbm_blocks": (ctypes.c_int, [_I64, _P, _D, _D, _D, _D, _U64, ctypes.POINTER(HostCsr)]),
@@ -436,6 +469,57 @@ class DeviceBatch(DeviceGraph):
                                        _ptr(obj), _ptr(mod), ctypes.byref(st)))
         bounds = np.concatenate([[0], np.cumsum(self.counts)])
         return [ids[bounds[i]:bounds[i + 1]] for i in range(G)], obj, mod, st
+
+
+class Server:
+    """Continuous-batching request loop (pcc_server_*)."""
+
+    def __init__(self, cfg: Config, objective: int = OBJECTIVE_CORRELATION, workers: int = 2,
+                 max_batch_graphs: int = 512, hbm_budget_bytes: float = 0.0,
+                 batch_window_seconds: float = 0.0):
+        o = ServerOptions()
+        lib().pcc_server_options_default(ctypes.byref(o))
+        o.workers = workers
+        o.max_batch_graphs = max_batch_graphs
+        o.hbm_budget_bytes = hbm_budget_bytes
+        o.batch_window_seconds = batch_window_seconds
+        h = ctypes.c_void_p()
+        _check(lib().pcc_server_create(ctypes.byref(cfg), objective, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def submit(self, csr: Csr, resolution: float) -> int:
+        off = np.ascontiguousarray(csr.offsets, dtype=np.int64)
+        nb = np.ascontiguousarray(csr.nbrs, dtype=np.uint32)
+        w = np.ascontiguousarray(csr.weights, dtype=np.float32)
+        rid = ctypes.c_uint64()
+        _check(lib().pcc_server_submit(self._h, csr.n, csr.m, _ptr(off), _ptr(nb), _ptr(w),
+                                       resolution, ctypes.byref(rid)))
+        self._n[rid.value] = csr.n
+        return rid.value
+
+    def wait(self, rid: int):
+        ids = np.zeros(max(self._n.pop(rid), 1), np.uint32)
+        obj = ctypes.c_double()
+        lat = ctypes.c_double()
+        _check(lib().pcc_server_wait(self._h, rid, _ptr(ids), ctypes.byref(obj), ctypes.byref(lat)))
+        return ids, obj.value, lat.value
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().pcc_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self._h:
+            lib().pcc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------- InMemoryClusterer mirror API --
