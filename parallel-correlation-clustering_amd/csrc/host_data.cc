@@ -204,11 +204,36 @@ int GenSbm(const std::vector<int64_t>& sizes, double p_in, double p_out, double 
     return SetError(PCC_INVALID_ARGUMENT, "SBM vertex count out of range");
   if (!(p_in >= 0 && p_in <= 1 && p_out >= 0 && p_out <= 1))
     return SetError(PCC_INVALID_ARGUMENT, "SBM probabilities must lie in [0, 1]");
-  // Block pairs (a <= b), each with its own stream; geometric skips over the
-  // pair index space (Batagelj & Brandes 2005).
-  std::vector<std::pair<int64_t, int64_t>> bp;
-  for (int64_t a = 0; a < B; ++a)
-    for (int64_t b = a; b < B; ++b) bp.emplace_back(a, b);
+  std::vector<uint32_t> block_of((size_t)n);
+  for (int64_t b = 0; b < B; ++b)
+    std::fill(block_of.begin() + start[(size_t)b], block_of.begin() + start[(size_t)b + 1], (uint32_t)b);
+  // Pairs are drawn by geometric skips over a pair index space (Batagelj &
+  // Brandes 2005): each diagonal block with p_in on its own stream, and the
+  // whole upper triangle with p_out split into a fixed number of chunks (so
+  // the output does not depend on the thread count), rejecting pairs that
+  // fall inside a block.
+  constexpr int64_t kChunks = 256;
+  const uint64_t tri = (uint64_t)n * (uint64_t)(n - 1) / 2;
+  const int64_t jobs = B + kChunks;
+  auto skip_stream = [](uint64_t st, double p, uint64_t count, auto&& emit) {
+    if (p <= 0 || count == 0) return;
+    const double lq = (p < 1) ? std::log1p(-p) : 0.0;
+    uint64_t idx = 0;
+    bool first = true;
+    while (true) {
+      uint64_t skip = 0;
+      if (p < 1) {
+        st = Mix(st);
+        const double gsk = std::floor(std::log(Unit(st)) / lq);
+        if (gsk >= (double)count) break;
+        skip = (uint64_t)gsk;
+      }
+      idx = first ? skip : idx + 1 + skip;
+      first = false;
+      if (idx >= count) break;
+      emit(idx);
+    }
+  };
   const int T = HostThreads();
   std::vector<std::vector<uint64_t>> parts((size_t)T);
   std::atomic<int64_t> next{0};
@@ -218,38 +243,30 @@ int GenSbm(const std::vector<int64_t>& sizes, double p_in, double p_out, double 
       auto& out_keys = parts[(size_t)t];
       while (true) {
         const int64_t q = next.fetch_add(1);
-        if (q >= (int64_t)bp.size()) break;
-        const int64_t a = bp[(size_t)q].first, b = bp[(size_t)q].second;
-        const double p = (a == b) ? p_in : p_out;
-        const uint64_t sa = (uint64_t)sizes[(size_t)a], sb = (uint64_t)sizes[(size_t)b];
-        const uint64_t count = (a == b) ? sa * (sa - (sa ? 1 : 0)) / 2 : sa * sb;
-        if (p <= 0 || count == 0) continue;
-        uint64_t st = Hash3(seed, (uint64_t)a, (uint64_t)b);
-        const double lq = (p < 1) ? std::log1p(-p) : 0.0;
-        uint64_t idx = 0;
-        bool first = true;
-        while (true) {
-          uint64_t skip = 0;
-          if (p < 1) {
-            st = Mix(st);
-            const double g = std::floor(std::log(Unit(st)) / lq);
-            if (g >= (double)count) break;
-            skip = (uint64_t)g;
-          }
-          idx = first ? skip : idx + 1 + skip;
-          first = false;
-          if (idx >= count) break;
-          uint64_t i, j;
-          if (a == b) {
-            TriPair(idx, &i, &j);
-            i += (uint64_t)start[(size_t)a];
-            j += (uint64_t)start[(size_t)a];
-          } else {
-            i = (uint64_t)start[(size_t)a] + idx / sb;
-            j = (uint64_t)start[(size_t)b] + idx % sb;
-          }
-          out_keys.push_back((i << 32) | j);
-          out_keys.push_back((j << 32) | i);
+        if (q >= jobs) break;
+        if (q < B) {
+          const uint64_t sa = (uint64_t)sizes[(size_t)q];
+          const uint64_t base = (uint64_t)start[(size_t)q];
+          skip_stream(Hash3(seed, (uint64_t)q, (uint64_t)q), p_in, sa * (sa - (sa ? 1 : 0)) / 2,
+                      [&](uint64_t idx) {
+                        uint64_t i, j;
+                        TriPair(idx, &i, &j);
+                        i += base;
+                        j += base;
+                        out_keys.push_back((i << 32) | j);
+                        out_keys.push_back((j << 32) | i);
+                      });
+        } else {
+          const uint64_t c = (uint64_t)(q - B);
+          const uint64_t lo_i = tri * c / kChunks, hi_i = tri * (c + 1) / kChunks;
+          skip_stream(Hash3(seed ^ 0x4F555445ull, c, (uint64_t)kChunks), p_out, hi_i - lo_i,
+                      [&](uint64_t idx) {
+                        uint64_t i, j;
+                        TriPair(lo_i + idx, &i, &j);
+                        if (block_of[(size_t)i] == block_of[(size_t)j]) return;
+                        out_keys.push_back((i << 32) | j);
+                        out_keys.push_back((j << 32) | i);
+                      });
         }
       }
     });
