@@ -287,6 +287,7 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_BLOCK_MAX_DEG 4  /* upper degree of the one-workgroup LDS-hash bin */
 #define PCC_OPT_ASYNC_SUBROUNDS 5 /* async: launches per round, each on 1/k of the frontier */
 #define PCC_OPT_ASYNC_SINGLETON_RULE 6 /* async: 1 applies the sync singleton rule */
+#define PCC_OPT_HUB_PART_EDGES 7 /* edges per hash partition of the largest-degree bin */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* ------------------------------------------------------------ host data -- */

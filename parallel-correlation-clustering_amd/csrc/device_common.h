@@ -36,19 +36,52 @@ __device__ __forceinline__ int GranExp(double x) {
   return e - 1075 + __builtin_ctzll(mant | (1ull << 52));
 }
 
-// A set of doubles whose magnitudes sum to sabs and whose granularities are
-// all >= 2^gmin has every partial sum exactly representable when
-// sabs < 2^(gmin + 53); the check keeps one bit of margin for the rounding of
-// sabs itself.  Then every summation order gives the same result.
-__device__ __forceinline__ bool SumIsOrderFree(int gmin, double sabs) {
-  if (sabs == 0.0) return true;
-  if (gmin >= (1 << 19)) return true;
-  return sabs < ldexp(1.0, gmin + 52);
+// Order-free summation test of the hashed bins.  Let B bound the sum of the
+// magnitudes of every value a vertex adds into its per-cluster totals.  A
+// value x with GranExp(x) >= ThresholdFor(B) is a multiple of 2^thr, and every
+// partial sum of such values is a multiple of 2^thr of magnitude below
+// 2^(thr + 51), hence exactly representable: such totals come out the same in
+// every summation order.  A total with a finer value is order-free only if it
+// has at most two terms (floating-point addition commutes).
+__device__ __forceinline__ int ThresholdFor(double b) {
+  if (!(b < INFINITY)) return 32767;  // NaN / inf: no value counts as coarse
+  if (b == 0.0) return -32768;
+  int e = 0;
+  (void)frexp(b * (1.0 + 0x1p-20), &e);  // b' = f * 2^e, f in [0.5, 1)
+  return (e - 1) - 50;                    // floor(log2 b') - 50
+}
+
+// Per-slot term counter of the hashed bins: bits 0..30 count the terms,
+// bit 31 marks a term finer than the vertex's threshold.
+constexpr uint32_t kFineBit = 0x80000000u;
+__device__ __forceinline__ bool SlotInexact(uint32_t cnt) {
+  return (cnt & kFineBit) != 0 && (cnt & ~kFineBit) >= 3;
 }
 
 __device__ __forceinline__ uint32_t HashId(uint32_t c) {
   uint32_t h = c * 0x9E3779B1u;
   return h ^ (h >> 15);
+}
+
+// Partition of cluster c among `parts` partitions (a hash independent of
+// HashId's low bits, range-reduced by multiply-shift).
+__device__ __forceinline__ uint32_t PartOf(uint32_t c, uint32_t parts) {
+  uint32_t h = c ^ (c >> 16);
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return (uint32_t)(((uint64_t)h * parts) >> 32);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip.  This
+// bijection of [0, grid) gives each XCD a contiguous range of work items, so
+// consecutive items (the partitions of one vertex) share an L2.
+__device__ __forceinline__ int64_t XcdItem(int64_t block, int64_t grid) {
+  constexpr int64_t kXcds = 8;
+  const int64_t x = block % kXcds, j = block / kXcds;
+  const int64_t q = grid / kXcds, r = grid % kXcds;
+  return x * q + (x < r ? x : r) + j;
 }
 
 // Selection order of EfficientBestMove among candidate clusters
@@ -88,6 +121,10 @@ __device__ __forceinline__ double WaveSum(double x) {
   for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
   return x;
 }
+__device__ __forceinline__ uint32_t WaveOr(uint32_t x) {
+  for (int s = 32; s > 0; s >>= 1) x |= __shfl_xor(x, s);
+  return x;
+}
 __device__ __forceinline__ int WaveMaxInt(int x) {
   for (int s = 32; s > 0; s >>= 1) x = max(x, __shfl_xor(x, s));
   return x;
@@ -106,6 +143,47 @@ __device__ __forceinline__ T BlockReduce(T x, T* smem, Op op) {
   for (int i = 1; i < nw; ++i) r = op(r, smem[i]);
   __syncthreads();
   return r;
+}
+
+// List item of a bin-7 partition work item t: i with part_off[i] <= t <
+// part_off[i + 1].
+__device__ __forceinline__ int64_t HubItemOf(const int64_t* __restrict__ part_off, int64_t count,
+                                             int64_t t) {
+  int64_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (part_off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Exclusive block scan; `smem` needs blockDim.x / 64 slots.  *total receives
+// the block sum.
+template <typename T>
+__device__ __forceinline__ T BlockExclusiveScan(T x, T* smem, T* total) {
+  const int lane = LaneId();
+  const int w = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  T inc = x;
+  for (int s = 1; s < kWave; s <<= 1) {
+    const T y = __shfl_up(inc, s);
+    if (lane >= s) inc += y;
+  }
+  __syncthreads();
+  if (lane == kWave - 1) smem[w] = inc;
+  __syncthreads();
+  T before = 0, tot = 0;
+  for (int i = 0; i < nw; ++i) {
+    const T t = smem[i];
+    if (i < w) before += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + inc - x;
 }
 
 }  // namespace pcc

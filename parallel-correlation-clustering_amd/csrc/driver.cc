@@ -324,6 +324,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
         return SetError(PCC_INVALID_ARGUMENT, "block bin upper degree must lie in [wave_max, 4096]");
       g->th.block_max = value;
       return PCC_OK;
+    case PCC_OPT_HUB_PART_EDGES:
+      if (value < 16 || value > 4096)
+        return SetError(PCC_INVALID_ARGUMENT, "hub partition size must lie in [16, 4096]");
+      g->hub_part_edges = value;
+      return PCC_OK;
     default:
       return SetError(PCC_INVALID_ARGUMENT, "unknown option");
   }
@@ -438,24 +443,13 @@ int pcc_best_moves(pcc_graph* g, const pcc_config* cfg, const uint32_t* cluster_
     PCC_HIP_CHECK(hipMemsetAsync(dlt.get(), 0, 8 * (size_t)n, s));
     int64_t counts[kBins], te[kBins];
     e.BinFrontier(c, ws, frontier ? fr.get() : nullptr, counts, te);
-    // The ordered kernel needs scratch even when no vertex is hashed.
-    if (c.hs.regions == 0) {
-      c.hs.cap = 64;
-      c.hs.regions = 1;
-      c.hkeys.Reset(&g->alloc, 64);
-      c.hvals.Reset(&g->alloc, 64);
-      c.hs.keys = c.hkeys.get();
-      c.hs.vals = c.hvals.get();
-    }
     MoveOut out;
     out.moves = ws.moves.get();
     out.moved = ws.moved.get();
     out.moved_clusters = ws.flags.get();
     out.dbg_target = tgt.get();
     out.dbg_delta = dlt.get();
-    for (int b = 0; b < kBins; ++b)
-      LaunchBestMoveBin(b, counts[b], false, c.g, c.st, c.mp, ws.list.get() + c.layout.base[b], out,
-                        ws.fb_list.get(), ws.rc.get(), c.hs, 0, s);
+    e.LaunchMoves(c, ws, c.mp, false, counts, te, out, 0, true);
     LaunchBestMoveOrdered(c.g, c.st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s);
     PCC_HIP_CHECK(hipMemcpyAsync(out_moves, ws.moves.get(), 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     PCC_HIP_CHECK(hipMemcpyAsync(out_moved, ws.moved.get(), (size_t)n, hipMemcpyDeviceToHost, s));

@@ -327,30 +327,70 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
                           LevelCtx* c) {
   c->g = g;
   c->st = st;
-  DevBuf<unsigned long long> caps(al_, kBins);
-  PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * kBins, s_));
+  DevBuf<unsigned long long> caps(al_, 2 * kBins);
+  PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * 2 * kBins, s_));
   LaunchBinCapacity(g, h_->th, caps.get(), s_);
-  unsigned long long hcaps[kBins];
+  unsigned long long hcaps[2 * kBins];
   PCC_HIP_CHECK(hipMemcpyAsync(hcaps, caps.get(), sizeof(hcaps), hipMemcpyDeviceToHost, s_));
   PCC_HIP_CHECK(hipStreamSynchronize(s_));
   int64_t run = 0;
   for (int b = 0; b < kBins; ++b) {
     c->layout.base[b] = run;
     c->caps[b] = (int64_t)hcaps[b];
+    c->caps_te[b] = (int64_t)hcaps[kBins + b];
     run += (int64_t)hcaps[b];
   }
+  const bool hashed = c->caps[5] + c->caps[6] + c->caps[7] > 0;
+  // Ordered fallback scratch, regions sized for the largest degree.
+  const int64_t maxdeg = hashed ? MaxDegree(g) : 0;
+  if (maxdeg >= (int64_t)1 << 30)
+    throw HipError{hipErrorInvalidValue, "vertex degree >= 2^30 is not supported", __FILE__,
+                   __LINE__};
+  uint32_t cap = 64;
+  while (cap < (uint64_t)(2 * maxdeg + 2)) cap <<= 1;
   c->hs = HashScratch();
-  if (hcaps[5] + hcaps[6] + hcaps[7] > 0) {
-    // Regions sized for the largest hashed degree on this level.
-    const int64_t maxdeg = MaxDegree(g);
-    uint32_t cap = 64;
-    while (cap < (uint64_t)(2 * maxdeg + 2)) cap <<= 1;
-    c->hs.cap = cap;
-    c->hs.regions = cap >= (1u << 22) ? 16 : (cap >= (1u << 20) ? 32 : 64);
-    c->hkeys.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
-    c->hvals.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
-    c->hs.keys = c->hkeys.get();
-    c->hs.vals = c->hvals.get();
+  c->hs.cap = cap;
+  c->hs.regions = !hashed ? 1 : (cap >= (1u << 22) ? 16 : (cap >= (1u << 20) ? 32 : 64));
+  c->hkeys.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
+  c->hvals.Reset(al_, (int64_t)c->hs.cap * c->hs.regions);
+  c->hs.keys = c->hkeys.get();
+  c->hs.vals = c->hvals.get();
+  if (hashed) {
+    c->gthr.Reset(al_, g.n);
+    LaunchGranThreshold(g, cfg_.edge_weight_offset, c->gthr.get(), s_);
+  }
+  c->hb = HubScratch();
+  const int64_t c7 = c->caps[kBins - 1], t7 = c->caps_te[kBins - 1];
+  if (c7 > 0) {
+    c->hb_part.Reset(al_, c7 + 1);
+    c->hb_ent.Reset(al_, c7 + 1);
+    c->hb_ekeys.Reset(al_, t7);
+    c->hb_evals.Reset(al_, t7);
+    c->hb_ecur.Reset(al_, c7);
+    c->hb_flag.Reset(al_, c7);
+    c->hb_tval.Reset(al_, c7);
+    PCC_HIP_CHECK(hipMemsetAsync(c->hb_ecur.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
+    PCC_HIP_CHECK(hipMemsetAsync(c->hb_flag.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
+    c->hb.part_edges = h_->hub_part_edges;
+    c->hb.part_off = c->hb_part.get();
+    c->hb.ent_off = c->hb_ent.get();
+    c->hb.ekeys = c->hb_ekeys.get();
+    c->hb.evals = c->hb_evals.get();
+    c->hb.ecur = c->hb_ecur.get();
+    c->hb.hflag = c->hb_flag.get();
+    c->hb.tval = c->hb_tval.get();
+    if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
+      c->hubs.Reset(al_, c7);
+      c->hubs_part.Reset(al_, c7 + 1);
+      c->hubs_ent.Reset(al_, c7 + 1);
+      DevBuf<unsigned long long> cnt(al_, 1);
+      PCC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s_));
+      LaunchCollectHubs(g, h_->th, c->hubs.get(), cnt.get(), s_);
+      LaunchHubScan(g, c->hubs.get(), c7, h_->hub_part_edges, c->hubs_part.get(),
+                    c->hubs_ent.get(), s_);
+      c->hub_count = c7;
+      c->hub_te = t7;
+    }
   }
   MoveParams& mp = c->mp;
   mp = MoveParams();
@@ -358,6 +398,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   mp.offset = cfg_.edge_weight_offset;
   mp.n = (uint32_t)g.n;
   mp.force_ordered = h_->force_ordered ? 1 : 0;
+  mp.gthr = c->gthr.get();
   if (lay.batch) {
     // Batches use a sentinel outside every graph's id range.
     mp.new_id = 0xFFFFFFFEu;
@@ -374,6 +415,27 @@ void Engine::BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, in
   ResetCounters(ws.rc.get(), true);
   LaunchBinFrontier(c.g, frontier, h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
   ReadCounters(ws.rc.get(), counts, te, nullptr);
+}
+
+void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
+                         const int64_t counts[kBins], const int64_t te[kBins], const MoveOut& out,
+                         uint64_t salt, bool scan_hubs) {
+  for (int b = 0; b + 1 < kBins; ++b) {
+    if (counts[b] == 0) continue;
+    Timer tb = StartTimer();
+    LaunchBestMoveBin(b, counts[b], async, c.g, c.st, mp, ws.list.get() + c.layout.base[b], out,
+                      ws.fb_list.get(), ws.rc.get(), salt, s_);
+    StopTimer(tb, b);
+  }
+  const int64_t c7 = counts[kBins - 1];
+  if (c7 > 0) {
+    const uint32_t* list7 = ws.list.get() + c.layout.base[kBins - 1];
+    Timer tb = StartTimer();
+    if (scan_hubs) LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, s_);
+    LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
+                      ws.fb_list.get(), ws.rc.get(), s_);
+    StopTimer(tb, kBins - 1);
+  }
 }
 
 int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, int G,
@@ -400,7 +462,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   out.moved = ws.moved.get();
   out.moved_clusters = ws.flags.get();
   if (async) PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
-  PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 2 * sizeof(unsigned long long), s_));
+  PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 3 * sizeof(unsigned long long), s_));
   const uint64_t salt =
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
   Timer all = StartTimer();
@@ -414,13 +476,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     mp.sub_count = (uint32_t)subs;
     mp.sub_index = (uint32_t)sub;
     mp.async_singleton_rule = async ? h_->async_singleton_rule : 0;
-    for (int b = 0; b < kBins; ++b) {
-      if (counts[b] == 0) continue;
-      Timer tb = StartTimer();
-      LaunchBestMoveBin(b, counts[b], async, g, st, mp, ws.list.get() + c.layout.base[b], out,
-                        ws.fb_list.get(), ws.rc.get(), c.hs, salt, s_);
-      StopTimer(tb, b);
-    }
+    LaunchMoves(c, ws, mp, async, counts, te, out, salt, sub == 0);
   }
   if (!async && counts[5] + counts[6] + counts[7] > 0) {
     Timer tf = StartTimer();
@@ -443,15 +499,17 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     RoundCounters hrc;
     int64_t c2[kBins], t2[kBins];
     ReadCounters(ws.rc.get(), c2, t2, &hrc);
-    fallback_ += (int64_t)hrc.fb_count;
+    fallback_ += (int64_t)(hrc.fb_count + hrc.repair_count);
     return F;
   }
   // Next frontier = neighbours of the seeds (PCC.cc:309-331).
   PCC_HIP_CHECK(hipMemsetAsync(ws.next.get(), 0, (size_t)n, s_));
   if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
-    LaunchExpandClusters(g, st.ids, ws.flags.get(), ws.next.get(), s_);
+    LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.hubs.get(), c.hub_count,
+                         c.hub_te, c.hubs_part.get(), h_->hub_part_edges, ws.next.get(), s_);
   } else {
-    LaunchExpandMoved(counts, c.layout, ws.list.get(), g, ws.moved.get(), ws.next.get(), s_);
+    LaunchExpandMoved(counts, te, c.layout, ws.list.get(), c.hb.part_off, c.hb.part_edges, g,
+                      ws.moved.get(), ws.next.get(), s_);
   }
   LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
   ResetCounters(ws.rc.get(), false);
@@ -459,7 +517,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   StopTimer(ta, 9);
   RoundCounters hrc;
   ReadCounters(ws.rc.get(), counts, te, &hrc);
-  fallback_ += (int64_t)hrc.fb_count;
+  fallback_ += (int64_t)(hrc.fb_count + hrc.repair_count);
   int64_t f = 0;
   for (int b = 0; b < kBins; ++b) f += counts[b];
   return f;

@@ -40,6 +40,7 @@ struct pcc_graph {
   bool force_ordered = false;
   int async_subrounds = 8;
   int async_singleton_rule = 1;
+  int64_t hub_part_edges = pcc::kHubPartEdges;
   pcc::BinThresholds th;
   int num_graphs() const { return gbase.empty() ? 1 : (int)gbase.size() - 1; }
   ~pcc_graph() {
@@ -116,12 +117,30 @@ class Engine {
     StateView st;
     MoveParams mp;
     BinLayout layout;
+    int64_t caps[kBins];     // vertices of each bin on this level
+    int64_t caps_te[kBins];  // edges of each bin on this level
+    DevBuf<int16_t> gthr;    // MoveParams::gthr
+    // Ordered fallback scratch.
     HashScratch hs;
     DevBuf<uint32_t> hkeys;
     DevBuf<double> hvals;
-    int64_t caps[kBins];
+    // Bin-7 scratch, sized for every bin-7 vertex of the level.
+    HubScratch hb;
+    DevBuf<int64_t> hb_part, hb_ent;
+    DevBuf<uint32_t> hb_ekeys, hb_ecur, hb_flag;
+    DevBuf<double> hb_evals, hb_tval;
+    // Every bin-7 vertex with its partition prefix (NBHR_CLUSTER_MOVE
+    // expansion).
+    DevBuf<uint32_t> hubs;
+    DevBuf<int64_t> hubs_part, hubs_ent;
+    int64_t hub_count = 0, hub_te = 0;
   };
   void PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay, LevelCtx* ctx);
+  // Best-move kernels of every bin for the binned list (counts / te per bin);
+  // scan_hubs: refresh the bin-7 prefix sums first.
+  void LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
+                   const int64_t counts[kBins], const int64_t te[kBins], const MoveOut& out,
+                   uint64_t salt, bool scan_hubs);
   // IterateBestMoves (parallel-correlation-clusterer.cc:340-365) over the
   // graphs selected by sel (present-graph flags; null = all).  Returns the
   // per-present-graph "moved" flags.

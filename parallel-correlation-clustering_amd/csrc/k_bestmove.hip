@@ -17,12 +17,22 @@
 //   bins 0-4 (deg <= 64): G lanes per vertex; every lane sums the weights of
 //     its cluster by walking the group's lanes in adjacency order with
 //     shuffles, so each S_c is accumulated in exactly the reference's order.
-//   bins 5-7: LDS (or global) open-addressing hash with FP64 atomics.  The
-//     kernel also checks that every partial sum is exactly representable; if
-//     not, the vertex goes to the ordered fallback kernel, which accumulates
-//     in adjacency order (and the hub-vertex block order of
-//     curr_together_seq).  Results are therefore bit-identical to a
-//     sequential evaluation in every case.
+//   bins 5-6: one wave / one workgroup per vertex, LDS open-addressing hash
+//     with FP64 atomics.
+//   bin 7: each vertex is split into hash partitions of its neighbour
+//     clusters, one workgroup and one LDS table per partition; the
+//     partitions' (cluster, sum) entries are then reduced by a selection
+//     kernel, one workgroup per vertex.
+//   In bins 5-7 every hash slot also counts its terms and flags terms finer
+//   than the vertex's granularity threshold (ThresholdFor).  The sum of a
+//   slot with three or more terms including a fine one may depend on the
+//   summation order; those slots alone are recomputed in adjacency order (and
+//   the hub-vertex block order of curr_together_seq for the own cluster) by
+//   RepairInexactSlots.  Every other slot's sum is exact in any order, so
+//   results are bit-identical to a sequential evaluation in every case.  The
+//   ordered kernel k_bm_ordered evaluates whole vertices in adjacency order; it
+//   serves the forced-ordered test mode and the (load-factor-excluded) case of
+//   a full partition table.
 
 #include "device_common.h"
 #include "kernels.h"
@@ -47,54 +57,204 @@ __device__ __forceinline__ uint64_t LaneMaskLt() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-__global__ __launch_bounds__(256) void k_bin_frontier(int64_t n, const int64_t* __restrict__ off,
-                                                      const uint8_t* __restrict__ frontier,
-                                                      BinThresholds th, BinLayout layout,
-                                                      uint32_t* __restrict__ list,
-                                                      RoundCounters* __restrict__ rc) {
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = base + threadIdx.x;
-    const bool in = v < n && (frontier == nullptr || frontier[v] != 0);
-    int b = -1;
-    int64_t deg = 0;
-    if (in) {
-      deg = off[v + 1] - off[v];
-      b = BinOf(deg, th.wave_max, th.block_max);
+__device__ __forceinline__ unsigned long long WaveSumU64(unsigned long long x) {
+  for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
+  return x;
+}
+
+// ------------------------------------------------------------- binning --
+
+// Each workgroup bins a chunk of kBinChunk vertices: counts per bin in LDS,
+// one global reservation per bin and chunk, then the list writes.
+constexpr int kBinItems = 16;
+constexpr int kBinThreads = 256;
+constexpr int64_t kBinChunk = (int64_t)kBinItems * kBinThreads;
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
+    int64_t n, const int64_t* __restrict__ off, const uint8_t* __restrict__ frontier,
+    BinThresholds th, BinLayout layout, uint32_t* __restrict__ list,
+    RoundCounters* __restrict__ rc) {
+  __shared__ unsigned long long s_cnt[kBins], s_te[kBins], s_base[kBins];
+  __shared__ uint32_t s_pos[kBins];
+  for (int64_t c0 = (int64_t)blockIdx.x * kBinChunk; c0 < n;
+       c0 += (int64_t)gridDim.x * kBinChunk) {
+    if (threadIdx.x < kBins) {
+      s_cnt[threadIdx.x] = 0;
+      s_te[threadIdx.x] = 0;
+      s_pos[threadIdx.x] = 0;
     }
-    for (int bb = 0; bb < kBins; ++bb) {
-      const uint64_t mask = __ballot(b == bb);
-      if (mask == 0) continue;
-      const int leader = __ffsll((unsigned long long)mask) - 1;
-      unsigned long long te = (b == bb) ? (unsigned long long)deg : 0ull;
-      for (int s = 32; s > 0; s >>= 1) te += __shfl_xor(te, s);
-      unsigned long long pos = 0;
-      if (LaneId() == leader) {
-        pos = atomicAdd(&rc->cnt[bb], (unsigned long long)__popcll(mask));
-        atomicAdd(&rc->te[bb], te);
+    __syncthreads();
+    int8_t bin[kBinItems];
+#pragma unroll
+    for (int k = 0; k < kBinItems; ++k) {
+      const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
+      int b = -1;
+      unsigned long long deg = 0;
+      if (v < n && (frontier == nullptr || frontier[v] != 0)) {
+        deg = (unsigned long long)(off[v + 1] - off[v]);
+        b = BinOf((int64_t)deg, th.wave_max, th.block_max);
       }
-      pos = __shfl(pos, leader);
-      if (b == bb) {
-        const uint32_t r = (uint32_t)__popcll(mask & LaneMaskLt());
-        list[layout.base[bb] + (int64_t)pos + r] = (uint32_t)v;
+      bin[k] = (int8_t)b;
+      uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
+      while (present) {
+        const int bb = __ffs(present) - 1;
+        present &= present - 1;
+        const uint64_t mask = __ballot(b == bb);
+        const unsigned long long te = WaveSumU64(b == bb ? deg : 0ull);
+        if (LaneId() == 0) {
+          atomicAdd(&s_cnt[bb], (unsigned long long)__popcll(mask));
+          atomicAdd(&s_te[bb], te);
+        }
       }
     }
+    __syncthreads();
+    if (threadIdx.x < kBins && s_cnt[threadIdx.x] > 0) {
+      s_base[threadIdx.x] = atomicAdd(&rc->cnt[threadIdx.x], s_cnt[threadIdx.x]);
+      atomicAdd(&rc->te[threadIdx.x], s_te[threadIdx.x]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBinItems; ++k) {
+      const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
+      const int b = bin[k];
+      uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
+      while (present) {
+        const int bb = __ffs(present) - 1;
+        present &= present - 1;
+        const uint64_t mask = __ballot(b == bb);
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t pos = 0;
+        if (LaneId() == leader) pos = atomicAdd(&s_pos[bb], (uint32_t)__popcll(mask));
+        pos = __shfl(pos, leader);
+        if (b == bb) {
+          const uint32_t r = (uint32_t)__popcll(mask & LaneMaskLt());
+          list[layout.base[bb] + (int64_t)s_base[bb] + pos + r] = (uint32_t)v;
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
 __global__ __launch_bounds__(256) void k_bin_capacity(int64_t n, const int64_t* __restrict__ off,
                                                       BinThresholds th,
                                                       unsigned long long* __restrict__ caps) {
-  __shared__ unsigned long long h[kBins];
-  if (threadIdx.x < kBins) h[threadIdx.x] = 0;
+  __shared__ unsigned long long h[2 * kBins];
+  if (threadIdx.x < 2 * kBins) h[threadIdx.x] = 0;
   __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    atomicAdd(&h[BinOf(off[v + 1] - off[v], th.wave_max, th.block_max)], 1ull);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    int b = -1;
+    unsigned long long deg = 0;
+    if (v < n) {
+      deg = (unsigned long long)(off[v + 1] - off[v]);
+      b = BinOf((int64_t)deg, th.wave_max, th.block_max);
+    }
+    uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
+    while (present) {
+      const int bb = __ffs(present) - 1;
+      present &= present - 1;
+      const uint64_t mask = __ballot(b == bb);
+      const unsigned long long te = WaveSumU64(b == bb ? deg : 0ull);
+      if (LaneId() == 0) {
+        atomicAdd(&h[bb], (unsigned long long)__popcll(mask));
+        atomicAdd(&h[kBins + bb], te);
+      }
+    }
   }
   __syncthreads();
-  if (threadIdx.x < kBins && h[threadIdx.x]) atomicAdd(&caps[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x < 2 * kBins && h[threadIdx.x]) atomicAdd(&caps[threadIdx.x], h[threadIdx.x]);
 }
+
+// Every bin-7 vertex appended to list (order unspecified).
+__global__ __launch_bounds__(256) void k_collect_hubs(int64_t n, const int64_t* __restrict__ off,
+                                                      int64_t block_max,
+                                                      uint32_t* __restrict__ list,
+                                                      unsigned long long* __restrict__ count) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    const bool hub = v < n && off[v + 1] - off[v] > block_max;
+    const uint64_t mask = __ballot(hub);
+    if (mask == 0) continue;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long pos = 0;
+    if (LaneId() == leader) pos = atomicAdd(count, (unsigned long long)__popcll(mask));
+    pos = __shfl(pos, leader);
+    if (hub) list[pos + __popcll(mask & LaneMaskLt())] = (uint32_t)v;
+  }
+}
+
+// gthr[v] = ThresholdFor(sum over v's edges of |w| + |offset|) for deg > 64;
+// one wave per such vertex.
+__global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t* __restrict__ off,
+                                                        const float* __restrict__ w, double offset,
+                                                        int16_t* __restrict__ gthr) {
+  const double ao = fabs(offset);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    int64_t st = 0, deg = 0;
+    if (v < n) {
+      st = off[v];
+      deg = off[v + 1] - st;
+    }
+    uint64_t big = __ballot(v < n && deg > 64);
+    const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
+    while (big) {
+      const int l = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int64_t vs = __shfl(st, l);
+      const int64_t vd = __shfl(deg, l);
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      int64_t p = LaneId();
+      for (; p + 3 * kWave < vd; p += 4 * kWave) {
+        s0 += fabs((double)w[vs + p]) + ao;
+        s1 += fabs((double)w[vs + p + kWave]) + ao;
+        s2 += fabs((double)w[vs + p + 2 * kWave]) + ao;
+        s3 += fabs((double)w[vs + p + 3 * kWave]) + ao;
+      }
+      for (; p < vd; p += kWave) s0 += fabs((double)w[vs + p]) + ao;
+      const double s = WaveSum((s0 + s1) + (s2 + s3));
+      if (LaneId() == 0) gthr[wave_v0 + l] = (int16_t)ThresholdFor(s);
+    }
+  }
+}
+
+// Bin-7 prefix sums of a list, one workgroup.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ list,
+                                                           int64_t count, int64_t part_edges,
+                                                           int64_t* __restrict__ part_off,
+                                                           int64_t* __restrict__ ent_off) {
+  __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
+  int64_t carry_p = 0, carry_e = 0;
+  for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
+    const int64_t i = b0 + threadIdx.x;
+    int64_t parts = 0, deg = 0;
+    if (i < count) {
+      const uint32_t v = list[i];
+      deg = off[v + 1] - off[v];
+      parts = deg > 0 ? (deg + part_edges - 1) / part_edges : 1;
+    }
+    int64_t tp = 0, tdeg = 0;
+    const int64_t xp = BlockExclusiveScan(parts, sp, &tp);
+    const int64_t xe = BlockExclusiveScan(deg, se, &tdeg);
+    if (i < count) {
+      part_off[i] = carry_p + xp;
+      ent_off[i] = carry_e + xe;
+    }
+    carry_p += tp;
+    carry_e += tdeg;
+  }
+  if (threadIdx.x == 0) {
+    part_off[count] = carry_p;
+    ent_off[count] = carry_e;
+  }
+}
+
 
 // Decision of EfficientBestMove from the best candidate (low-degree path,
 // PCCI.cc:490-510; hub path PCCI.cc:565-582, whose only difference is the
@@ -291,27 +451,6 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
 
 // ------------------------------------------------------------ hash bins --
 
-template <typename KeyPtr, typename ValPtr>
-__device__ __forceinline__ void HashAdd(KeyPtr keys, ValPtr vals, uint32_t mask, uint32_t c,
-                                        double x) {
-  uint32_t h = HashId(c) & mask;
-  while (true) {
-    const uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (k == c) {
-      atomicAdd(&vals[h], x);
-      return;
-    }
-    if (k == kNoneId) {
-      const uint32_t old = atomicCAS(&keys[h], kNoneId, c);
-      if (old == kNoneId || old == c) {
-        atomicAdd(&vals[h], x);
-        return;
-      }
-    }
-    h = (h + 1) & mask;
-  }
-}
-
 template <typename KeyPtr>
 __device__ __forceinline__ int HashFindOrInsert(KeyPtr keys, uint32_t mask, uint32_t c) {
   uint32_t h = HashId(c) & mask;
@@ -360,88 +499,268 @@ __device__ __forceinline__ Cand BlockBest(Cand x, double* sd, uint32_t* sc) {
   return r;
 }
 
-// THREADS cooperating threads per vertex; TABLE > 0: LDS table of TABLE
-// entries, TABLE == 0: per-workgroup region of the global scratch table.
+// Bounded insert: the slot of c, or -1 when all cap slots hold other keys.
+template <typename KeyPtr>
+__device__ __forceinline__ int HashSlot(KeyPtr keys, uint32_t mask, uint32_t c) {
+  uint32_t h = HashId(c) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == c) return (int)h;
+    if (k == kNoneId) {
+      const uint32_t old = atomicCAS(&keys[h], kNoneId, c);
+      if (old == kNoneId || old == c) return (int)h;
+    }
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+template <typename KeyPtr>
+__device__ __forceinline__ int HashFind(KeyPtr keys, uint32_t mask, uint32_t c) {
+  uint32_t h = HashId(c) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const uint32_t k = keys[h];
+    if (k == c) return (int)h;
+    if (k == kNoneId) return -1;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// The value EfficientBestMove adds for an edge of weight wf: (double)w -
+// offset on the low-degree path, the float w -= offset on the hub path
+// (PCCI.cc:466, 529).
+__device__ __forceinline__ double EdgeValue(float wf, bool hub, double offset) {
+  if (hub) {
+    float wh = wf;
+    wh -= offset;
+    return (double)wh;
+  }
+  double x = (double)wf;
+  x -= offset;
+  return x;
+}
+
+// Block size of the pbbs reduce of curr_together_seq (hub path, PCCI.cc:545):
+// max(1024, 4 * ceil(sqrt(deg))).
+__device__ __forceinline__ int64_t PbbsBlock(int64_t deg) {
+  const int64_t bs = (int64_t)ceil(sqrt((double)deg)) * 4;
+  return bs < 1024 ? 1024 : bs;
+}
+
+// LDS staging of the order repair: one chunk of THREADS edges.
+template <int THREADS>
+struct RepairStage {
+  uint32_t slot[THREADS];
+  uint32_t pos[THREADS];
+  double x[THREADS];
+  uint32_t wsum[THREADS / kWave];
+  double T;
+};
+
+// Recomputes, in adjacency order, the sums of the slots whose summation order
+// matters (SlotInexact), after Accumulate filled the table.  Edges of those
+// slots are compacted chunk by chunk in adjacency order and folded by one
+// thread, so each such slot ends up with exactly the sequential sum.  own is
+// the slot of c(v) (or -1); the return value is the own-cluster total: on the
+// hub path the pbbs block order of curr_together_seq (block sums folded left,
+// PCCI.cc:516-545), otherwise the slot's sum.
+template <int THREADS>
+__device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const StateView& st,
+                                                  const MoveParams& mp, uint32_t v, int64_t start,
+                                                  int64_t deg, bool hub, const uint32_t* keys,
+                                                  double* vals, const uint32_t* cnt, uint32_t cap,
+                                                  uint32_t parts, uint32_t part, int own,
+                                                  RepairStage<THREADS>* stg) {
+  const uint32_t mask = cap - 1;
+  for (uint32_t s = threadIdx.x; s < cap; s += THREADS)
+    if (SlotInexact(cnt[s])) vals[s] = 0.0;
+  __syncthreads();
+  const bool own_ordered = hub && own >= 0 && SlotInexact(cnt[own]);
+  const int64_t bs = PbbsBlock(deg);
+  const int lane = LaneId(), w = threadIdx.x / kWave;
+  // Fold state of thread 0.
+  double t_blk = 0.0, t_tot = 0.0;
+  int64_t cur_blk = -1;
+  bool have_tot = false;
+  for (int64_t q0 = 0; q0 < deg; q0 += THREADS) {
+    const int64_t q = q0 + threadIdx.x;
+    int slot = -1;
+    double x = 0.0;
+    if (q < deg) {
+      const uint32_t u = g.nbr[start + q];
+      if (u != v) {
+        const uint32_t c = st.ids[u];
+        if (parts == 1 || PartOf(c, parts) == part) {
+          const int s = HashFind(keys, mask, c);
+          if (s >= 0 && SlotInexact(cnt[s])) {
+            slot = s;
+            x = EdgeValue(g.w[start + q], hub, mp.offset);
+          }
+        }
+      }
+    }
+    if (!__syncthreads_or(slot >= 0)) continue;
+    const uint64_t m = __ballot(slot >= 0);
+    if (lane == 0) stg->wsum[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int i = 0; i < THREADS / kWave; ++i) {
+      const uint32_t t = stg->wsum[i];
+      if (i < w) before += t;
+      total += t;
+    }
+    if (slot >= 0) {
+      const uint32_t k = before + (uint32_t)__popcll(m & LaneMaskLt());
+      stg->slot[k] = (uint32_t)slot;
+      stg->pos[k] = (uint32_t)q;
+      stg->x[k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t k = 0; k < total; ++k) {
+        const uint32_t sl = stg->slot[k];
+        const double xx = stg->x[k];
+        vals[sl] += xx;
+        if (own_ordered && (int)sl == own) {
+          const int64_t blk = (int64_t)stg->pos[k] / bs;
+          if (blk != cur_blk) {
+            if (cur_blk >= 0) {
+              t_tot = have_tot ? t_tot + t_blk : t_blk;
+              have_tot = true;
+            }
+            cur_blk = blk;
+            t_blk = xx;
+          } else {
+            t_blk += xx;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double T = 0.0;
+    if (own_ordered) {
+      T = t_tot;
+      if (cur_blk >= 0) T = have_tot ? t_tot + t_blk : t_blk;
+    } else if (own >= 0) {
+      T = vals[own];
+    }
+    stg->T = T;
+  }
+  __syncthreads();
+  return stg->T;
+}
+
+// Adds the neighbours of v whose cluster falls in partition `part` of
+// `parts` into the LDS table (keys / vals / cnt, cap = mask + 1 slots).  Each
+// thread keeps U edges in flight.  Returns true if an insert found the table
+// full.  The value added is EfficientBestMove's: (double)w - offset on the
+// low-degree path, the float w -= offset on the hub path (PCCI.cc:466, 529).
+template <int THREADS, bool kAsync>
+__device__ __forceinline__ bool Accumulate(const CsrView& g, const StateView& st,
+                                           const MoveParams& mp, uint32_t v, int64_t start,
+                                           int64_t deg, bool hub, int thr, uint32_t* keys,
+                                           double* vals, uint32_t* cnt, uint32_t mask,
+                                           uint32_t parts, uint32_t part) {
+  constexpr int U = 4;
+  bool overflow = false;
+  for (int64_t q0 = threadIdx.x; q0 < deg; q0 += (int64_t)THREADS * U) {
+    uint32_t c[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * THREADS;
+      c[k] = q < deg ? g.nbr[start + q] : v;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) c[k] = c[k] != v ? st.ids[c[k]] : kNoneId;
+    float wf[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * THREADS;
+      if (c[k] != kNoneId && parts > 1 && PartOf(c[k], parts) != part) c[k] = kNoneId;
+      wf[k] = c[k] != kNoneId ? g.w[start + q] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (c[k] == kNoneId) continue;
+      const double x = EdgeValue(wf[k], hub, mp.offset);
+      const int slot = HashSlot(keys, mask, c[k]);
+      if (slot < 0) {
+        overflow = true;
+        continue;
+      }
+      atomicAdd(&vals[slot], x);
+      if (!kAsync) {
+        atomicAdd(&cnt[slot], 1u);
+        if (GranExp(x) < thr) atomicOr(&cnt[slot], kFineBit);
+      }
+    }
+  }
+  return overflow;
+}
+
+// Bins 5 and 6: THREADS cooperating threads and a TABLE-entry LDS table per
+// vertex.
 template <int THREADS, int TABLE, bool kAsync>
 __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
                                                      const uint32_t* __restrict__ list,
                                                      int64_t count, MoveOut out,
                                                      uint32_t* __restrict__ fb_list,
                                                      RoundCounters* __restrict__ rc,
-                                                     HashScratch hs, uint64_t salt) {
+                                                     uint64_t salt) {
   constexpr int NW = THREADS / kWave;
-  __shared__ uint32_t skeys[TABLE > 0 ? TABLE : 1];
-  __shared__ double svals[TABLE > 0 ? TABLE : 1];
+  __shared__ uint32_t keys[TABLE];
+  __shared__ double vals[TABLE];
+  __shared__ uint32_t cnt[TABLE];
   __shared__ double sd[NW];
   __shared__ uint32_t sc[NW];
-  __shared__ int si[NW];
-  __shared__ double sdd[NW];
-  uint32_t* keys;
-  double* vals;
-  if (TABLE > 0) {
-    keys = skeys;
-    vals = svals;
-  } else {
-    keys = hs.keys + (size_t)blockIdx.x * hs.cap;
-    vals = hs.vals + (size_t)blockIdx.x * hs.cap;
-  }
+  __shared__ RepairStage<THREADS> stg;
   for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t v = list[PermuteIndex(i, count, salt)];
     if (kAsync && !InSubRound(mp, v)) continue;  // uniform across the workgroup
+    if (!kAsync && mp.force_ordered) {
+      if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
+      continue;
+    }
     const int64_t start = g.off[v];
     const int64_t deg = g.off[v + 1] - start;
     // Table sized per vertex (load factor <= 1/2) so that clearing and
     // scanning it costs O(deg), not O(TABLE).
-    const uint32_t cap = min(TABLE > 0 ? (uint32_t)TABLE : hs.cap,
-                             max(64u, NextPow2((uint64_t)(2 * deg + 2))));
+    const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * deg + 2))));
     const uint32_t mask = cap - 1;
     for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
       keys[s] = kNoneId;
       vals[s] = 0.0;
+      cnt[s] = 0;
     }
     __syncthreads();
     const uint32_t cv = st.ids[v];
     const bool hub = deg > kHubDegree;
-    int gmin = 1 << 20;
-    double sabs = 0.0;
-    for (int64_t p = threadIdx.x; p < deg; p += THREADS) {
-      const uint32_t u = g.nbr[start + p];
-      if (u == v) continue;
-      const float wf = g.w[start + p];
-      double x;
-      if (hub) {
-        float wh = wf;
-        wh -= mp.offset;  // float arithmetic on the hub path (PCCI.cc:529)
-        x = (double)wh;
-      } else {
-        x = (double)wf;
-        x -= mp.offset;
+    const int thr = kAsync ? 0 : (int)mp.gthr[v];
+    const bool overflow = Accumulate<THREADS, kAsync>(g, st, mp, v, start, deg, hub, thr, keys,
+                                                      vals, cnt, mask, 1, 0);
+    if (__syncthreads_or(overflow)) {  // cannot happen at load <= 1/2; kept as a guard
+      if (threadIdx.x == 0) {
+        if (kAsync)
+          out.moved[v] = 0;
+        else
+          fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       }
-      const uint32_t c = st.ids[u];
-      if (!kAsync) {
-        gmin = min(gmin, GranExp(x));
-        sabs += fabs(x);
-      }
-      if (TABLE > 0)
-        HashAdd(skeys, svals, mask, c, x);
-      else
-        HashAdd(keys, vals, mask, c, x);
+      continue;
     }
-    __syncthreads();
+    const int own = HashFind(keys, mask, cv);
+    double T = own >= 0 ? vals[own] : 0.0;
     if (!kAsync) {
-      const int gm = BlockReduce(gmin, si, [](int a, int b) { return min(a, b); });
-      const double sa = BlockReduce(sabs, sdd, [](double a, double b) { return a + b; });
-      const bool exact = !mp.force_ordered && SumIsOrderFree(gm, sa);
-      if (!exact) {
-        if (threadIdx.x == 0) {
-          const unsigned long long pos = atomicAdd(&rc->fb_count, 1ull);
-          fb_list[pos] = v;
-        }
-        __syncthreads();
-        continue;
+      bool inexact = false;
+      for (uint32_t s = threadIdx.x; s < cap; s += THREADS) inexact = inexact || SlotInexact(cnt[s]);
+      if (__syncthreads_or(inexact)) {
+        T = RepairInexactSlots<THREADS>(g, st, mp, v, start, deg, hub, keys, vals, cnt, cap, 1, 0,
+                                        own, &stg);
+        if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
       }
     }
-    const double T = HashGet(keys, vals, mask, cv);
     const double k = st.nw[v];
     const double res = ResOf(mp, v);
     double change = 0;
@@ -467,6 +786,144 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
         SyncEpilogue(v, t, d, mp.new_id, st, out);
     }
     __syncthreads();
+  }
+}
+
+// Bin 7, partition pass: work item t (partition p of list item i) builds the
+// LDS table of v's neighbour clusters in partition p and appends its entries
+// to v's entry region; the partition holding c(v) records S_{c(v)}.
+constexpr int kHubThreads = 512;
+constexpr int kHubTable = 8192;
+
+template <bool kAsync>
+__global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateView st,
+                                                             MoveParams mp,
+                                                             const uint32_t* __restrict__ list,
+                                                             int64_t count, HubScratch hb,
+                                                             RoundCounters* __restrict__ rc) {
+  __shared__ uint32_t keys[kHubTable];
+  __shared__ double vals[kHubTable];
+  __shared__ uint32_t cnt[kHubTable];
+  __shared__ uint32_t sw[kHubThreads / kWave];
+  __shared__ uint32_t s_base;
+  __shared__ RepairStage<kHubThreads> stg;
+  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  if (t >= hb.part_off[count]) return;
+  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const uint32_t v = list[i];
+  if (kAsync && !InSubRound(mp, v)) return;
+  if (!kAsync && mp.force_ordered) return;
+  const uint32_t parts = (uint32_t)(hb.part_off[i + 1] - hb.part_off[i]);
+  const uint32_t part = (uint32_t)(t - hb.part_off[i]);
+  const int64_t start = g.off[v];
+  const int64_t deg = g.off[v + 1] - start;
+  const int64_t per = (deg + parts - 1) / parts;
+  const uint32_t cap = min((uint32_t)kHubTable, max(64u, NextPow2((uint64_t)(2 * per + 2))));
+  const uint32_t mask = cap - 1;
+  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
+    keys[s] = kNoneId;
+    vals[s] = 0.0;
+    cnt[s] = 0;
+  }
+  __syncthreads();
+  const bool hub = deg > kHubDegree;
+  const int thr = kAsync ? 0 : (int)mp.gthr[v];
+  const bool overflow = Accumulate<kHubThreads, kAsync>(g, st, mp, v, start, deg, hub, thr, keys,
+                                                        vals, cnt, mask, parts, part);
+  if (__syncthreads_or(overflow)) {
+    if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
+    return;
+  }
+  const uint32_t cv = st.ids[v];
+  const int own = PartOf(cv, parts) == part ? HashFind(keys, mask, cv) : -1;
+  double T = own >= 0 ? vals[own] : 0.0;
+  if (!kAsync) {
+    bool inexact = false;
+    for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads)
+      inexact = inexact || SlotInexact(cnt[s]);
+    if (__syncthreads_or(inexact)) {
+      T = RepairInexactSlots<kHubThreads>(g, st, mp, v, start, deg, hub, keys, vals, cnt, cap,
+                                          parts, part, own, &stg);
+      if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
+    }
+  }
+  if (threadIdx.x == 0 && own >= 0) {
+    hb.tval[i] = T;
+    atomicOr(&hb.hflag[i], 1u);
+  }
+  // Compact the occupied slots into the entry region.
+  uint32_t mine = 0;
+  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) mine += keys[s] != kNoneId;
+  uint32_t total = 0;
+  uint32_t at = BlockExclusiveScan(mine, sw, &total);
+  if (total == 0) return;
+  if (threadIdx.x == 0) s_base = atomicAdd(&hb.ecur[i], total);
+  __syncthreads();
+  const int64_t e0 = hb.ent_off[i] + s_base;
+  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
+    const uint32_t c = keys[s];
+    if (c == kNoneId) continue;
+    hb.ekeys[e0 + at] = c;
+    hb.evals[e0 + at] = vals[s];
+    ++at;
+  }
+}
+
+// Bin 7, selection pass: one workgroup per list item reduces its entries to
+// EfficientBestMove's decision and resets the item's counters.
+constexpr int kSelThreads = 256;
+
+template <bool kAsync>
+__global__ __launch_bounds__(kSelThreads) void k_bm_hub_select(
+    CsrView g, StateView st, MoveParams mp, const uint32_t* __restrict__ list, int64_t count,
+    HubScratch hb, MoveOut out, uint32_t* __restrict__ fb_list, RoundCounters* __restrict__ rc) {
+  __shared__ double sd[kSelThreads / kWave];
+  __shared__ uint32_t sc[kSelThreads / kWave];
+  for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t v = list[i];
+    const uint32_t flag = hb.hflag[i];
+    const uint32_t ne = hb.ecur[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      hb.hflag[i] = 0;
+      hb.ecur[i] = 0;
+    }
+    if (kAsync && !InSubRound(mp, v)) continue;
+    if (!kAsync && (mp.force_ordered || (flag & 2u))) {
+      if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
+      continue;
+    }
+    if (kAsync && (flag & 2u)) {  // table overflow: no move this time
+      if (threadIdx.x == 0) out.moved[v] = 0;
+      continue;
+    }
+    const int64_t deg = g.off[v + 1] - g.off[v];
+    const bool hub = deg > kHubDegree;
+    const uint32_t cv = st.ids[v];
+    const double T = (flag & 1u) ? hb.tval[i] : 0.0;
+    const double k = st.nw[v];
+    const double res = ResOf(mp, v);
+    double change = 0;
+    change -= NetWeight(T, k * (st.K[cv] - k), res);
+    double mw = 0;
+    mw += k;
+    const int64_t e0 = hb.ent_off[i];
+    Cand best{0.0, kNoneId};
+    for (uint32_t j = threadIdx.x; j < ne; j += kSelThreads) {
+      const uint32_t c = hb.ekeys[e0 + j];
+      const Cand x{change + NetWeight(hb.evals[e0 + j], mw * st.K[c], res), c};
+      best = PickBetter(best, x);
+    }
+    best = BlockBest<kSelThreads>(best, sd, sc);
+    if (threadIdx.x == 0) {
+      uint32_t t;
+      double d;
+      Decide(best, change, hub, mp.new_id, &t, &d);
+      if (kAsync)
+        AsyncEpilogue(v, t, d, mp, st, out);
+      else
+        SyncEpilogue(v, t, d, mp.new_id, st, out);
+    }
   }
 }
 
@@ -596,17 +1053,40 @@ int GridFor(int64_t items, int per_block, int max_blocks) {
 void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThresholds& th,
                        const BinLayout& layout, uint32_t* list, RoundCounters* rc, hipStream_t s) {
   if (g.n == 0) return;
-  const int grid = GridFor(g.n, 256, 8192);
-  hipLaunchKernelGGL(k_bin_frontier, dim3(grid), dim3(256), 0, s, g.n, g.off, frontier, th, layout,
-                     list, rc);
+  const int grid = GridFor(g.n, (int)kBinChunk, 2048);
+  hipLaunchKernelGGL(k_bin_frontier, dim3(grid), dim3(kBinThreads), 0, s, g.n, g.off, frontier, th,
+                     layout, list, rc);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
                        hipStream_t s) {
   if (g.n == 0) return;
-  const int grid = GridFor(g.n, 256, 4096);
+  const int grid = GridFor(g.n, 256 * 16, 1024);
   hipLaunchKernelGGL(k_bin_capacity, dim3(grid), dim3(256), 0, s, g.n, g.off, th, caps);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s) {
+  if (g.n == 0) return;
+  const int grid = GridFor(g.n, 256, 8192);
+  hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset, gthr);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
+                   int64_t* part_off, int64_t* ent_off, hipStream_t s) {
+  hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
+                     part_edges, part_off, ent_off);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchCollectHubs(const CsrView& g, const BinThresholds& th, uint32_t* list,
+                       unsigned long long* count, hipStream_t s) {
+  if (g.n == 0) return;
+  const int grid = GridFor(g.n, 256, 8192);
+  hipLaunchKernelGGL(k_collect_hubs, dim3(grid), dim3(256), 0, s, g.n, g.off, th.block_max, list,
+                     count);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -619,19 +1099,17 @@ static void LaunchSmall(int64_t count, const CsrView& g, const StateView& st, co
 }
 
 template <int THREADS, int TABLE, bool A>
-static void LaunchHash(int64_t count, int max_grid, const CsrView& g, const StateView& st,
-                       const MoveParams& mp, const uint32_t* list, const MoveOut& out,
-                       uint32_t* fb_list, RoundCounters* rc, const HashScratch& hs, uint64_t salt,
-                       hipStream_t s) {
-  const int grid = GridFor(count, 1, max_grid);
+static void LaunchHash(int64_t count, const CsrView& g, const StateView& st, const MoveParams& mp,
+                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
+  const int grid = GridFor(count, 1, 1 << 20);
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
-                     list, count, out, fb_list, rc, hs, salt);
+                     list, count, out, fb_list, rc, salt);
 }
 
 void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, const StateView& st,
                        const MoveParams& mp, const uint32_t* list, const MoveOut& out,
-                       uint32_t* fb_list, RoundCounters* rc, const HashScratch& hs,
-                       uint64_t salt, hipStream_t s) {
+                       uint32_t* fb_list, RoundCounters* rc, uint64_t salt, hipStream_t s) {
   if (count <= 0) return;
   switch (bin) {
     case 0:
@@ -655,17 +1133,41 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, con
             : LaunchSmall<64, false>(count, g, st, mp, list, out, salt, s);
       break;
     case 5:
-      async ? LaunchHash<64, 1024, true>(count, 1 << 20, g, st, mp, list, out, fb_list, rc, hs, salt, s)
-            : LaunchHash<64, 1024, false>(count, 1 << 20, g, st, mp, list, out, fb_list, rc, hs, salt, s);
+      async ? LaunchHash<64, 1024, true>(count, g, st, mp, list, out, fb_list, rc, salt, s)
+            : LaunchHash<64, 1024, false>(count, g, st, mp, list, out, fb_list, rc, salt, s);
       break;
     case 6:
-      async ? LaunchHash<512, 8192, true>(count, 1 << 20, g, st, mp, list, out, fb_list, rc, hs, salt, s)
-            : LaunchHash<512, 8192, false>(count, 1 << 20, g, st, mp, list, out, fb_list, rc, hs, salt, s);
+      async ? LaunchHash<512, 8192, true>(count, g, st, mp, list, out, fb_list, rc, salt, s)
+            : LaunchHash<512, 8192, false>(count, g, st, mp, list, out, fb_list, rc, salt, s);
       break;
     default:
-      async ? LaunchHash<512, 0, true>(count, hs.regions, g, st, mp, list, out, fb_list, rc, hs, salt, s)
-            : LaunchHash<512, 0, false>(count, hs.regions, g, st, mp, list, out, fb_list, rc, hs, salt, s);
-      break;
+      throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: bin 7 goes through LaunchBestMoveHub",
+                     __FILE__, __LINE__};
+  }
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
+                       const StateView& st, const MoveParams& mp, const uint32_t* list,
+                       const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, hipStream_t s) {
+  if (count <= 0) return;
+  // sum_i ceil(deg_i / E) < te / E + count bounds the partition items.
+  const int64_t items = count + te / hb.part_edges + 1;
+  if (items > (int64_t)INT32_MAX)
+    throw HipError{hipErrorInvalidValue, "LaunchBestMoveHub: too many partitions", __FILE__,
+                   __LINE__};
+  const int sel = GridFor(count, 1, 1 << 20);
+  if (async) {
+    hipLaunchKernelGGL(k_bm_hub_part<true>, dim3((unsigned)items), dim3(kHubThreads), 0, s, g, st,
+                       mp, list, count, hb, rc);
+    hipLaunchKernelGGL(k_bm_hub_select<true>, dim3(sel), dim3(kSelThreads), 0, s, g, st, mp, list,
+                       count, hb, out, fb_list, rc);
+  } else {
+    hipLaunchKernelGGL(k_bm_hub_part<false>, dim3((unsigned)items), dim3(kHubThreads), 0, s, g, st,
+                       mp, list, count, hb, rc);
+    hipLaunchKernelGGL(k_bm_hub_select<false>, dim3(sel), dim3(kSelThreads), 0, s, g, st, mp, list,
+                       count, hb, out, fb_list, rc);
   }
   PCC_HIP_CHECK(hipGetLastError());
 }

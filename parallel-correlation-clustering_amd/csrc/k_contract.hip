@@ -48,64 +48,132 @@ struct GraphMap {
   }
 };
 
-// 16 lanes per vertex count the kept edges (PGU.cc:93-103).
-__global__ __launch_bounds__(256) void k_count_kept(CsrView g, const uint32_t* __restrict__ ids,
-                                                    GraphMap gm, uint32_t* __restrict__ kept) {
-  const int sub = threadIdx.x & 15;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
-  for (int64_t u = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; u < g.n; u += stride) {
-    const uint32_t cu = ids[u];
+// Kept edges (PGU.cc:93-103) are found chunk by chunk: a workgroup stages
+// kEdgeChunk consecutive CSR edges in LDS and each thread walks kEdgeItems of
+// them in order, tracking the source vertex, so hub vertices are spread over
+// many workgroups and the kept edges come out in CSR order.
+constexpr int kEdgeItems = 16;
+constexpr int kEdgeThreads = 256;
+constexpr int64_t kEdgeChunk = (int64_t)kEdgeItems * kEdgeThreads;
+
+// Source vertex of edge p (p < m): the largest u with off[u] <= p, searched in
+// [lo, n).
+__device__ __forceinline__ int64_t VertexOfEdge(const int64_t* __restrict__ off, int64_t lo,
+                                                int64_t n, int64_t p) {
+  // The next few vertices first (the common case), then bisection.
+  for (int k = 0; k < 4 && lo + 1 < n; ++k) {
+    if (off[lo + 1] > p) return lo;
+    ++lo;
+  }
+  int64_t hi = n;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo < n ? lo : n - 1;  // off[n] == m > p keeps lo < n; guard anyway
+}
+
+struct EdgeWalk {
+  int64_t u = -1, uend = 0;
+  uint32_t cu = 0;
+  int64_t shift = 0;
+  // Advances to the source of edge p.
+  __device__ __forceinline__ void To(const CsrView& g, const uint32_t* __restrict__ ids,
+                                     const GraphMap& gm, int64_t p) {
+    if (p < uend) return;
+    u = u < 0 ? VertexOfEdge(g.off, 0, g.n, p) : VertexOfEdge(g.off, u + 1, g.n, p);
+    uend = g.off[u + 1];
+    cu = ids[u];
+    shift = gm.Shift(u);
+  }
+};
+
+__global__ __launch_bounds__(kEdgeThreads) void k_count_kept(CsrView g,
+                                                             const uint32_t* __restrict__ ids,
+                                                             GraphMap gm, int64_t nchunks,
+                                                             uint32_t* __restrict__ chunk_cnt) {
+  __shared__ uint32_t snbr[kEdgeChunk];
+  __shared__ uint32_t sw[kEdgeThreads / kWave];
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t c0 = ch * kEdgeChunk;
+    const int64_t len = min(kEdgeChunk, g.m - c0);
+    __syncthreads();
+    for (int64_t k = threadIdx.x; k < len; k += kEdgeThreads) snbr[k] = g.nbr[c0 + k];
+    __syncthreads();
     uint32_t c = 0;
-    const int64_t lim = gm.Shift(u) == INT64_MIN ? g.off[u] : g.off[u + 1];
-    for (int64_t p = g.off[u] + sub; p < lim; p += 16) {
-      const uint32_t v = g.nbr[p];
-      if ((int64_t)v <= u || ids[v] != cu) ++c;
+    const int64_t q0 = (int64_t)threadIdx.x * kEdgeItems;
+    if (q0 < len) {
+      EdgeWalk wk;
+      const int64_t q1 = min(q0 + kEdgeItems, len);
+      for (int64_t q = q0; q < q1; ++q) {
+        wk.To(g, ids, gm, c0 + q);
+        if (wk.shift == INT64_MIN) continue;
+        const uint32_t v = snbr[q];
+        if ((int64_t)v <= wk.u || ids[v] != wk.cu) ++c;
+      }
     }
-    for (int s = 8; s > 0; s >>= 1) c += __shfl_xor(c, s, 16);
-    if (sub == 0) kept[u] = c;
+    uint32_t total = 0;
+    (void)BlockExclusiveScan(c, sw, &total);
+    if (threadIdx.x == 0) chunk_cnt[ch] = total;
   }
 }
 
-// Writes the kept edges of each vertex, in CSR order, as (key, weight).
-__global__ __launch_bounds__(256) void k_emit_kept(CsrView g, const uint32_t* __restrict__ ids,
-                                                   GraphMap gm,
-                                                   const int64_t* __restrict__ kept_off, int vbits,
-                                                   uint64_t* __restrict__ keys,
-                                                   float* __restrict__ vals) {
-  const int sub = threadIdx.x & 15;
-  const int wl = LaneId();
-  const uint64_t gmask = 0xFFFFull << (wl & ~15);
-  const uint64_t lt = wl == 0 ? 0ull : (~0ull >> (64 - wl));
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
-  for (int64_t ub = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) / 16; ub < g.n;
-       ub += stride) {
-    const int64_t u = ub + (wl >> 4);
-    const int64_t shift = u < g.n ? gm.Shift(u) : INT64_MIN;
-    const bool active = u < g.n && shift != INT64_MIN;
-    const int64_t b = active ? g.off[u] : 0, e = active ? g.off[u + 1] : 0;
-    const uint32_t cu = active ? ids[u] : 0u;
-    int64_t pos = active ? kept_off[u] : 0;
-    const int64_t len = e - b;
-    int64_t maxlen = len;
-    for (int s = 32; s > 0; s >>= 1) maxlen = max(maxlen, (int64_t)__shfl_xor(maxlen, s));
-    for (int64_t c0 = 0; c0 < maxlen; c0 += 16) {
-      const int64_t p = b + c0 + sub;
-      bool keep = false;
-      uint32_t v = 0, cv = 0;
-      float w = 0.0f;
-      if (active && p < e) {
-        v = g.nbr[p];
-        cv = ids[v];
-        w = g.w[p];
-        keep = (int64_t)v <= u || cv != cu;
+// Writes the kept edges as (key, weight) in CSR order; chunk ch starts at
+// chunk_off[ch].
+__global__ __launch_bounds__(kEdgeThreads) void k_emit_kept(CsrView g,
+                                                            const uint32_t* __restrict__ ids,
+                                                            GraphMap gm, int64_t nchunks,
+                                                            const int64_t* __restrict__ chunk_off,
+                                                            int vbits, uint64_t* __restrict__ keys,
+                                                            float* __restrict__ vals) {
+  __shared__ uint32_t snbr[kEdgeChunk];
+  __shared__ float sw8[kEdgeChunk];
+  __shared__ uint32_t sw[kEdgeThreads / kWave];
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t c0 = ch * kEdgeChunk;
+    const int64_t len = min(kEdgeChunk, g.m - c0);
+    __syncthreads();
+    for (int64_t k = threadIdx.x; k < len; k += kEdgeThreads) {
+      snbr[k] = g.nbr[c0 + k];
+      sw8[k] = g.w[c0 + k];
+    }
+    __syncthreads();
+    uint64_t key[kEdgeItems];
+    uint32_t keep = 0;  // bit j: edge q0 + j kept
+    uint32_t c = 0;
+    const int64_t q0 = (int64_t)threadIdx.x * kEdgeItems;
+    if (q0 < len) {
+      EdgeWalk wk;
+#pragma unroll
+      for (int j = 0; j < kEdgeItems; ++j) {
+        const int64_t q = q0 + j;
+        key[j] = 0;
+        if (q >= len) continue;
+        wk.To(g, ids, gm, c0 + q);
+        if (wk.shift == INT64_MIN) continue;
+        const uint32_t v = snbr[q];
+        const uint32_t cv = ids[v];
+        if ((int64_t)v <= wk.u || cv != wk.cu) {
+          key[j] = ((uint64_t)(wk.cu + wk.shift) << vbits) | (uint64_t)(cv + wk.shift);
+          keep |= 1u << j;
+          ++c;
+        }
       }
-      const uint64_t bal = __ballot(keep) & gmask;
-      if (keep) {
-        const int64_t q = pos + __popcll(bal & lt);
-        keys[q] = ((uint64_t)(cu + shift) << vbits) | (uint64_t)(cv + shift);
-        vals[q] = w;
+    }
+    uint32_t total = 0;
+    uint32_t at = BlockExclusiveScan(c, sw, &total);
+    const int64_t base = chunk_off[ch] + at;
+    int64_t k = 0;
+#pragma unroll
+    for (int j = 0; j < kEdgeItems; ++j) {
+      if (keep & (1u << j)) {
+        keys[base + k] = key[j];
+        vals[base + k] = sw8[q0 + j];
+        ++k;
       }
-      pos += __popcll(bal);
     }
   }
 }
@@ -198,12 +266,29 @@ __global__ __launch_bounds__(256) void k_emit_compressed(const uint64_t* __restr
                                                          int64_t nseg, int vbits,
                                                          uint32_t* __restrict__ nbr,
                                                          uint32_t* __restrict__ cnt) {
+  // Segments are sorted by source, so a wave's sources form runs; the first
+  // lane of each run adds the run length.
   const uint64_t vmask = (1ull << vbits) - 1ull;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = keys[start[s]];
-    nbr[s] = (uint32_t)(k & vmask);
-    atomicAdd(&cnt[k >> vbits], 1u);
+  const int lane = LaneId();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nseg;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = base + threadIdx.x;
+    const bool in = s < nseg;
+    uint64_t src = ~0ull;
+    if (in) {
+      const uint64_t k = keys[start[s]];
+      nbr[s] = (uint32_t)(k & vmask);
+      src = k >> vbits;
+    }
+    const uint64_t prev = __shfl_up(src, 1);
+    const bool head = in && (lane == 0 || prev != src);
+    const uint64_t heads = __ballot(head);
+    const uint64_t valid = __ballot(in);  // a prefix of the wave's lanes
+    if (head) {
+      const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+      const int end = later ? __ffsll((unsigned long long)later) - 1 : 64 - __clzll(valid);
+      atomicAdd(&cnt[src], (uint32_t)(end - lane));
+    }
   }
 }
 
@@ -236,9 +321,37 @@ __global__ __launch_bounds__(256) void k_scatter_node_weights(const uint64_t* __
 __global__ __launch_bounds__(256) void k_graph_max(int64_t n, const uint32_t* __restrict__ ids,
                                                    const uint32_t* __restrict__ vgraph,
                                                    uint32_t* __restrict__ gmax) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x)
-    atomicMax(&gmax[vgraph ? vgraph[v] : 0], ids[v]);
+  if (vgraph == nullptr) {  // one graph: a block maximum, one atomic per block
+    __shared__ uint32_t smax;
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    uint32_t mx = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+      mx = max(mx, ids[v]);
+    for (int s = 32; s > 0; s >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, s));
+    if (LaneId() == 0) atomicMax(&smax, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&gmax[0], smax);
+    return;
+  }
+  // Batches: one atomic per wave when the wave's vertices share a graph,
+  // otherwise one per vertex.
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    const bool in = v < n;
+    const uint32_t x = in ? ids[v] : 0u;
+    const uint32_t gi = (in && vgraph) ? vgraph[v] : 0u;
+    const uint32_t g0 = __shfl(gi, 0);
+    if (__all(!in || gi == g0)) {
+      uint32_t mx = x;
+      for (int s = 32; s > 0; s >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, s));
+      if (LaneId() == 0) atomicMax(&gmax[g0], mx);
+    } else if (in) {
+      atomicMax(&gmax[gi], x);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_flatten_shift(int64_t n, uint32_t* __restrict__ ids,
@@ -374,34 +487,72 @@ __global__ __launch_bounds__(256) void k_cluster_weights(int64_t n, const uint32
 
 // Per-vertex terms of ComputeObjective (PCCI.cc:124-145): the intra-cluster
 // half weights folded in pbbs order, and k_v (K_c(v) - k_v).
+// Per-vertex intra-cluster edge weight (halved) in pbbslib's reduce order:
+// a left fold for degrees up to one reduce block, blocked otherwise
+// (PCCI.cc:128-138).  Degrees above 1024 are handled by the whole wave, one
+// lane per reduce block.
+__device__ __forceinline__ double ObjTerm(const CsrView& g, const uint32_t* __restrict__ ids,
+                                          uint32_t cv, double offset, int64_t p) {
+  const float w = g.w[p];
+  return ids[g.nbr[p]] == cv ? (w - offset) / 2 : 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32_t* __restrict__ ids,
                                                          const double* __restrict__ nw,
                                                          const double* __restrict__ K,
                                                          double offset, double* __restrict__ e,
                                                          double* __restrict__ r) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t cv = ids[v];
-    const int64_t b = g.off[v], deg = g.off[v + 1] - b;
-    double tot = 0.0;
-    if (deg > 0) {
-      int64_t bs = (int64_t)ceil(sqrt((double)deg)) * 4;
-      if (bs < 1024) bs = 1024;
-      bool have = false;
-      for (int64_t s = 0; s < deg; s += bs) {
-        const int64_t en = (s + bs < deg) ? s + bs : deg;
-        double blk = 0.0;
-        for (int64_t p = s; p < en; ++p) {
-          const float w = g.w[b + p];
-          const double x = ids[g.nbr[b + p]] == cv ? (w - offset) / 2 : 0.0;
-          blk = (p == s) ? x : blk + x;
+  const int lane = LaneId();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    const bool in = v < g.n;
+    uint32_t cv = 0;
+    int64_t b = 0, deg = 0;
+    if (in) {
+      cv = ids[v];
+      b = g.off[v];
+      deg = g.off[v + 1] - b;
+      r[v] = nw[v] * (K[cv] - nw[v]);
+      if (deg <= 1024) {
+        double tot = 0.0;
+        for (int64_t p = 0; p < deg; ++p) {
+          const double x = ObjTerm(g, ids, cv, offset, b + p);
+          tot = p == 0 ? x : tot + x;
         }
-        tot = have ? tot + blk : blk;
-        have = true;
+        e[v] = tot;
       }
     }
-    e[v] = tot;
-    r[v] = nw[v] * (K[cv] - nw[v]);
+    uint64_t big = __ballot(in && deg > 1024);
+    const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
+    while (big) {
+      const int l = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
+      const uint32_t vc = __shfl(cv, l);
+      int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
+      if (bs < 1024) bs = 1024;
+      const int64_t nblk = (vd + bs - 1) / bs;
+      double tot = 0.0;
+      bool have = false;
+      for (int64_t g0 = 0; g0 < nblk; g0 += kWave) {
+        const int64_t blk = g0 + lane;
+        double x = 0.0;
+        if (blk < nblk) {
+          const int64_t s0 = blk * bs;
+          const int64_t en = min(s0 + bs, vd);
+          x = ObjTerm(g, ids, vc, offset, vb + s0);
+          for (int64_t p = s0 + 1; p < en; ++p) x += ObjTerm(g, ids, vc, offset, vb + p);
+        }
+        const int64_t nb = min((int64_t)kWave, nblk - g0);
+        for (int j = 0; j < nb; ++j) {
+          const double y = __shfl(x, j);
+          tot = have ? tot + y : y;
+          have = true;
+        }
+      }
+      if (lane == 0) e[wave_v0 + l] = tot;
+    }
   }
 }
 
@@ -410,13 +561,38 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
 __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint32_t* __restrict__ ids,
                                                           double* __restrict__ intra,
                                                           double* __restrict__ dsum) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t cv = ids[v];
-    int64_t cnt = 0;
-    for (int64_t p = g.off[v]; p < g.off[v + 1]; ++p) cnt += ids[g.nbr[p]] == cv ? 1 : 0;
-    intra[v] = (double)cnt;
-    atomicAdd(&dsum[cv], (double)(g.off[v + 1] - g.off[v]));
+  // Integer counts: any order is exact.  Degrees above 64 are counted by the
+  // whole wave.
+  const int lane = LaneId();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    const bool in = v < g.n;
+    uint32_t cv = 0;
+    int64_t b = 0, deg = 0;
+    if (in) {
+      cv = ids[v];
+      b = g.off[v];
+      deg = g.off[v + 1] - b;
+      atomicAdd(&dsum[cv], (double)deg);
+      if (deg <= 64) {
+        int64_t cnt = 0;
+        for (int64_t p = b; p < b + deg; ++p) cnt += ids[g.nbr[p]] == cv ? 1 : 0;
+        intra[v] = (double)cnt;
+      }
+    }
+    uint64_t big = __ballot(in && deg > 64);
+    const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
+    while (big) {
+      const int l = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
+      const uint32_t vc = __shfl(cv, l);
+      unsigned long long cnt = 0;
+      for (int64_t p = lane; p < vd; p += kWave) cnt += ids[g.nbr[vb + p]] == vc ? 1ull : 0ull;
+      for (int s2 = 32; s2 > 0; s2 >>= 1) cnt += __shfl_xor(cnt, s2);
+      if (lane == 0) intra[wave_v0 + l] = (double)cnt;
+    }
   }
 }
 
@@ -535,7 +711,7 @@ int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<in
 void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, uint32_t* gmax,
                        hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_graph_max, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids, vgraph, gmax);
+  hipLaunchKernelGGL(k_graph_max, dim3(Grid(n, 256, vgraph ? 8192 : 1024)), dim3(256), 0, s, n, ids, vgraph, gmax);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -547,23 +723,30 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
   const int vbits = BitsFor((uint64_t)(n2 > 1 ? n2 - 1 : 1));
 
   // Kept edges in CSR order.
-  DevBuf<uint32_t> kept(al, n);
-  DevBuf<int64_t> kept_off(al, n + 1);
-  hipLaunchKernelGGL(k_count_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids, gm, kept.get());
-  ExclusiveScanU32toI64(al, kept.get(), kept_off.get(), n, kept_off.get() + n, s);
+  const int64_t nchunks = (g.m + kEdgeChunk - 1) / kEdgeChunk;
   int64_t mk = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&mk, kept_off.get() + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
-  DevBuf<uint64_t> keys(al, mk);
-  DevBuf<float> vals(al, mk);
-  if (mk > 0) {
-    hipLaunchKernelGGL(k_emit_kept, dim3(Grid(n, 16, 8192)), dim3(256), 0, s, g, ids, gm,
-                       kept_off.get(), vbits, keys.get(), vals.get());
+  DevBuf<uint64_t> keys;
+  DevBuf<float> vals;
+  if (nchunks > 0) {
+    DevBuf<uint32_t> ccnt(al, nchunks);
+    DevBuf<int64_t> coff(al, nchunks + 1);
+    const int grid = (int)std::min<int64_t>(nchunks, 1 << 16);
+    hipLaunchKernelGGL(k_count_kept, dim3(grid), dim3(kEdgeThreads), 0, s, g, ids, gm, nchunks,
+                       ccnt.get());
     PCC_HIP_CHECK(hipGetLastError());
-    RadixSortPairsF32(al, keys.get(), vals.get(), mk, 0, 2 * vbits, s);
+    ExclusiveScanU32toI64(al, ccnt.get(), coff.get(), nchunks, coff.get() + nchunks, s);
+    PCC_HIP_CHECK(hipMemcpyAsync(&mk, coff.get() + nchunks, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                 s));
+    PCC_HIP_CHECK(hipStreamSynchronize(s));
+    keys.Reset(al, mk);
+    vals.Reset(al, mk);
+    if (mk > 0) {
+      hipLaunchKernelGGL(k_emit_kept, dim3(grid), dim3(kEdgeThreads), 0, s, g, ids, gm, nchunks,
+                         coff.get(), vbits, keys.get(), vals.get());
+      PCC_HIP_CHECK(hipGetLastError());
+      RadixSortPairsF32(al, keys.get(), vals.get(), mk, 0, 2 * vbits, s);
+    }
   }
-  kept.Release();
-  kept_off.Release();
 
   // Runs of equal (c(u), c(v)) -> compressed edges.
   DevBuf<int64_t> start;
@@ -582,6 +765,13 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
     PCC_HIP_CHECK(hipGetLastError());
   }
   ExclusiveScanU32toI64(al, cnt.get(), out->off.get(), n2, out->off.get() + n2, s);
+  int64_t mend = -1;
+  PCC_HIP_CHECK(hipMemcpyAsync(&mend, out->off.get() + n2, sizeof(int64_t), hipMemcpyDeviceToHost,
+                               s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  if (mend != m2)
+    throw HipError{hipErrorUnknown, "CompressGraph: contracted offsets do not cover the edges",
+                   __FILE__, __LINE__};
   keys.Release();
   vals.Release();
   start.Release();

@@ -113,34 +113,94 @@ __global__ __launch_bounds__(256) void k_new_assign(uint32_t n, StateView st,
 }
 
 // 16 lanes per seed vertex mark its neighbours.
-__global__ __launch_bounds__(256) void k_expand_moved(Bins bins, const uint32_t* __restrict__ list,
+// Marks the neighbours of the selected vertices.  Each thread owns one
+// vertex; degrees up to 16 are walked by the thread itself, larger ones by
+// the whole wave in turn.
+__device__ __forceinline__ void MarkNeighbours(bool sel, int64_t st, int64_t deg,
+                                               const uint32_t* __restrict__ nbr,
+                                               uint8_t* __restrict__ next) {
+  if (sel && deg <= 16)
+    for (int64_t p = 0; p < deg; ++p) next[nbr[st + p]] = 1;
+  uint64_t big = __ballot(sel && deg > 16);
+  while (big) {
+    const int l = __ffsll((unsigned long long)big) - 1;
+    big &= big - 1;
+    const int64_t vs = __shfl(st, l);
+    const int64_t vd = __shfl(deg, l);
+    for (int64_t p = LaneId(); p < vd; p += kWave) next[nbr[vs + p]] = 1;
+  }
+}
+
+// Moved vertices of bins 0..6 (bin 7 goes through k_expand_hub).
+__global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
+                                                      const uint32_t* __restrict__ list,
                                                       const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint8_t* __restrict__ moved,
                                                       uint8_t* __restrict__ next) {
-  const int sub = threadIdx.x & 15;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
-  for (int b = 0; b < kBins; ++b) {
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; i < bins.count[b];
-         i += stride) {
-      const uint32_t v = list[bins.base[b] + i];
-      if (!moved[v]) continue;
-      for (int64_t p = off[v] + sub; p < off[v + 1]; p += 16) next[nbr[p]] = 1;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = base + threadIdx.x;
+    bool sel = false;
+    int64_t st = 0, deg = 0;
+    if (t < total) {
+      int64_t r = t;
+      int b = 0;
+      while (r >= bins.count[b]) r -= bins.count[b++];
+      const uint32_t v = list[bins.base[b] + r];
+      if (moved[v]) {
+        sel = true;
+        st = off[v];
+        deg = off[v + 1] - st;
+      }
     }
+    MarkNeighbours(sel, st, deg, nbr, next);
   }
 }
 
+// Vertices with deg <= block_max whose cluster is modified.
 __global__ __launch_bounds__(256) void k_expand_clusters(int64_t n, const int64_t* __restrict__ off,
                                                          const uint32_t* __restrict__ nbr,
                                                          const uint32_t* __restrict__ ids,
                                                          const uint8_t* __restrict__ modified,
+                                                         int64_t block_max,
                                                          uint8_t* __restrict__ next) {
-  const int sub = threadIdx.x & 15;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / 16);
-  for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 16; v < n; v += stride) {
-    if (!modified[ids[v]]) continue;
-    for (int64_t p = off[v] + sub; p < off[v + 1]; p += 16) next[nbr[p]] = 1;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    bool sel = false;
+    int64_t st = 0, deg = 0;
+    if (v < n) {
+      st = off[v];
+      deg = off[v + 1] - st;
+      sel = deg <= block_max && modified[ids[v]] != 0;
+    }
+    MarkNeighbours(sel, st, deg, nbr, next);
   }
+}
+
+// Bin-7 vertices: one workgroup per partition (part_off prefix of the list)
+// marks that partition's share of the adjacency.  The seed test is moved[v]
+// (ids null) or modified[ids[v]].
+__global__ __launch_bounds__(256) void k_expand_hub(const uint32_t* __restrict__ list, int64_t count,
+                                                    const int64_t* __restrict__ part_off,
+                                                    const int64_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ nbr,
+                                                    const uint8_t* __restrict__ moved,
+                                                    const uint32_t* __restrict__ ids,
+                                                    const uint8_t* __restrict__ modified,
+                                                    uint8_t* __restrict__ next) {
+  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  if (t >= part_off[count]) return;
+  const int64_t i = HubItemOf(part_off, count, t);
+  const uint32_t v = list[i];
+  if (ids ? modified[ids[v]] == 0 : moved[v] == 0) return;
+  const int64_t parts = part_off[i + 1] - part_off[i];
+  const int64_t part = t - part_off[i];
+  const int64_t st = off[v], deg = off[v + 1] - st;
+  const int64_t per = (deg + parts - 1) / parts;
+  const int64_t e1 = min(deg, (part + 1) * per);
+  for (int64_t p = part * per + threadIdx.x; p < e1; p += blockDim.x) next[nbr[st + p]] = 1;
 }
 
 int Grid(int64_t items, int per_block, int max_blocks) {
@@ -208,22 +268,43 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchExpandMoved(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
-                       const CsrView& g, const uint8_t* moved, uint8_t* next, hipStream_t s) {
+void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
+                       const BinLayout& layout, const uint32_t* list,
+                       const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,
+                       const uint8_t* moved, uint8_t* next, hipStream_t s) {
   int64_t total;
-  const Bins b = MakeBins(counts, layout, &total);
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_expand_moved, dim3(Grid(total, 16, 8192)), dim3(256), 0, s, b, list, g.off,
-                     g.nbr, moved, next);
-  PCC_HIP_CHECK(hipGetLastError());
+  Bins b = MakeBins(counts, layout, &total);
+  const int64_t small = total - counts[kBins - 1];
+  b.count[kBins - 1] = 0;
+  if (small > 0) {
+    hipLaunchKernelGGL(k_expand_moved, dim3(Grid(small, 256, 8192)), dim3(256), 0, s, b, small,
+                       list, g.off, g.nbr, moved, next);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t c7 = counts[kBins - 1];
+  if (c7 > 0) {
+    const int64_t items = c7 + te[kBins - 1] / part_edges + 1;
+    hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s,
+                       list + layout.base[kBins - 1], c7, hub_part_off, g.off, g.nbr, moved,
+                       (const uint32_t*)nullptr, (const uint8_t*)nullptr, next);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                          int64_t block_max, const uint32_t* hubs, int64_t hub_count,
+                          int64_t hub_te, const int64_t* hub_part_off, int64_t part_edges,
                           uint8_t* next, hipStream_t s) {
   if (g.n == 0) return;
-  hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 16, 8192)), dim3(256), 0, s, g.n, g.off,
-                     g.nbr, ids, modified, next);
+  hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
+                     g.nbr, ids, modified, block_max, next);
   PCC_HIP_CHECK(hipGetLastError());
+  if (hub_count > 0) {
+    const int64_t items = hub_count + hub_te / part_edges + 1;
+    hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s, hubs, hub_count,
+                       hub_part_off, g.off, g.nbr, (const uint8_t*)nullptr, ids, modified, next);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
 }
 
 }  // namespace pcc

@@ -113,7 +113,8 @@ constexpr int kBins = 8;
 // 0..4: deg <= 4, 8, 16, 32, 64 -> 4..64 lanes per vertex, ordered shuffles
 // 5: deg <= wave_max   -> one wave, LDS hash
 // 6: deg <= block_max  -> one workgroup, LDS hash
-// 7: larger            -> one workgroup, global-memory hash
+// 7: larger            -> several workgroups per vertex, each an LDS hash over
+//                         one hash partition of the neighbour clusters
 struct BinThresholds {
   int64_t wave_max = 512;
   int64_t block_max = 4096;
@@ -123,10 +124,10 @@ struct BinThresholds {
 struct RoundCounters {
   unsigned long long cnt[kBins];
   unsigned long long te[kBins];
-  unsigned long long fb_count;
+  unsigned long long fb_count;      // vertices listed for the ordered kernel
   unsigned long long new_count;
+  unsigned long long repair_count;  // hash tables with order-repaired slots
   unsigned long long moved_count;
-  unsigned long long pad;
 };
 
 struct BinLayout {
@@ -154,6 +155,10 @@ struct MoveParams {
   uint32_t sub_index = 0;
   // Async: apply the sync singleton rule before moving (0/1).
   int async_singleton_rule = 0;
+  // Per-vertex granularity threshold of the hashed bins (deg > 64): a weight
+  // x with GranExp(x) >= gthr[v] cannot make any partial sum of v's
+  // per-cluster totals inexact.
+  const int16_t* gthr = nullptr;
 };
 
 struct MoveOut {
@@ -164,7 +169,26 @@ struct MoveOut {
   double* dbg_delta = nullptr;
 };
 
-// Scratch for the global-memory hash bin and the ordered fallback path.
+// Default edges per partition of a bin-7 vertex.
+constexpr int64_t kHubPartEdges = 4096;
+
+// Bin-7 scratch.  part_off / ent_off: exclusive prefix sums over the bin's
+// list of the partition counts ceil(deg / kHubPartEdges) and of the degrees;
+// ekeys / evals[ent_off[i] ...]: the (cluster, sum) entries of list item i;
+// ecur / hflag (bit 0: own-cluster sum found in tval, bit 1: inexact or table
+// overflow) are zero between launches.
+struct HubScratch {
+  int64_t part_edges = kHubPartEdges;
+  int64_t* part_off = nullptr;
+  int64_t* ent_off = nullptr;
+  uint32_t* ekeys = nullptr;
+  double* evals = nullptr;
+  uint32_t* ecur = nullptr;
+  uint32_t* hflag = nullptr;
+  double* tval = nullptr;
+};
+
+// Scratch for the ordered fallback path.
 struct HashScratch {
   uint32_t* keys = nullptr;
   double* vals = nullptr;
@@ -180,15 +204,37 @@ void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThres
                        const BinLayout& layout, uint32_t* list, RoundCounters* rc,
                        hipStream_t s);
 
-// Histogram of bins over all vertices (capacity of each list region).
+// Histogram of bins over all vertices: caps[b] vertices and caps[kBins + b]
+// edges of bin b (capacity of each list region).
 void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
                        hipStream_t s);
 
-// One best-move kernel per bin.  count = number of listed vertices in the bin.
+// gthr[v] for every vertex with deg > 64 (see MoveParams::gthr).
+void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s);
+
+// Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
+// partitions of part_edges edges.
+void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
+                   int64_t* part_off, int64_t* ent_off, hipStream_t s);
+
+// Every vertex of bin 7 (deg > th.block_max) appended to list; *count zeroed
+// by the caller.
+void LaunchCollectHubs(const CsrView& g, const BinThresholds& th, uint32_t* list,
+                       unsigned long long* count, hipStream_t s);
+
+// One best-move kernel per bin 0..6.  count = number of listed vertices in
+// the bin.
 void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, const StateView& st,
                        const MoveParams& mp, const uint32_t* list, const MoveOut& out,
-                       uint32_t* fb_list, RoundCounters* rc, const HashScratch& hs,
-                       uint64_t permute_salt, hipStream_t s);
+                       uint32_t* fb_list, RoundCounters* rc, uint64_t permute_salt,
+                       hipStream_t s);
+
+// Bin 7: partition pass and selection pass over a list whose HubScratch
+// prefix sums are current; te = sum of the listed degrees.
+void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
+                       const StateView& st, const MoveParams& mp, const uint32_t* list,
+                       const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, hipStream_t s);
 
 // Ordered (sequential-order-exact) best moves for the fallback list.
 void LaunchBestMoveOrdered(const CsrView& g, const StateView& st, const MoveParams& mp,
@@ -210,11 +256,19 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
                       hipStream_t s);
 
-// Next frontier = neighbours of the seeds.
-void LaunchExpandMoved(const int64_t counts[kBins], const BinLayout& layout,
-                       const uint32_t* list, const CsrView& g, const uint8_t* moved,
-                       uint8_t* next, hipStream_t s);
+// Next frontier = neighbours of the seeds.  Bin 7 of the list is walked in
+// the partitions of hub_part_off (its HubScratch::part_off, partitions of
+// part_edges edges); te[7] = sum of its degrees.
+void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
+                       const BinLayout& layout, const uint32_t* list,
+                       const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,
+                       const uint8_t* moved, uint8_t* next, hipStream_t s);
+// Neighbours of every vertex whose cluster is modified.  Vertices with
+// deg > block_max are taken from hubs (hub_count of them, hub_te edges, with
+// partition prefix hub_part_off) instead of the vertex sweep.
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                          int64_t block_max, const uint32_t* hubs, int64_t hub_count,
+                          int64_t hub_te, const int64_t* hub_part_off, int64_t part_edges,
                           uint8_t* next, hipStream_t s);
 
 // ------------------------------------------------------------- primitives --

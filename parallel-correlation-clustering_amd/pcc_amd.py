@@ -48,6 +48,7 @@ OPT_WAVE_MAX_DEG = 3
 OPT_BLOCK_MAX_DEG = 4
 OPT_ASYNC_SUBROUNDS = 5
 OPT_ASYNC_SINGLETON_RULE = 6
+OPT_HUB_PART_EDGES = 7
 
 
 class StatusError(RuntimeError):

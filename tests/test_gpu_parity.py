@@ -172,3 +172,115 @@ def test_cluster_bit_exact(pcc, orc, case):
     assert st.objective == pytest.approx(r.objective, rel=1e-9, abs=1e-6)
     if modularity:
         assert st.modularity == pytest.approx(r.modularity, rel=1e-9, abs=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Hub partitions and the order-free test of the hashed bins.
+
+HUB_OPTS = [(64, 64, 16), (64, 128, 64), (512, 4096, 4096)]
+
+
+def _hub_dev(pcc, csr, opts):
+    wave_max, block_max, part = opts
+    dg = _dev(pcc, csr)
+    dg.set_option(pcc.OPT_WAVE_MAX_DEG, wave_max)
+    dg.set_option(pcc.OPT_BLOCK_MAX_DEG, block_max)
+    dg.set_option(pcc.OPT_HUB_PART_EDGES, part)
+    return dg
+
+
+@pytest.mark.parametrize("opts", HUB_OPTS)
+@pytest.mark.parametrize("name", ["rmat_signed", "rmat_hubs"])
+def test_best_moves_hub_partitions(pcc, orc, name, opts):
+    """Every vertex above the block bin split into many hash partitions."""
+    csr = _graphs(pcc)[name]
+    print(f"hub partitions {name} opts={opts}", flush=True)
+    rng = np.random.default_rng(17)
+    for kclusters in (csr.n, max(2, csr.n // 5), 3):
+        ids, sizes, nw, K = random_state(rng, csr.n, kclusters)
+        for res, off in [(0.05, 0.0), (0.3, 0.25)]:
+            cfg = pcc.make_config(resolution=res, edge_weight_offset=off)
+            g = _hub_dev(pcc, csr, opts).best_moves(cfg, ids, sizes, nw, K)
+            o = orc.sync_decisions(orc.G.from_csr(csr), orc.S(ids, sizes, nw, K),
+                                   orc.config_from(cfg))
+            for a, b in zip(g, o):
+                assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), \
+                    f"k={kclusters} res={res}"
+
+
+def _scaled_weights(csr, every, scale):
+    """Scales the weight of every `every`-th undirected edge (by a hash of the
+    endpoint pair, so both directions agree) by `scale`."""
+    deg = np.diff(csr.offsets)
+    src = np.repeat(np.arange(csr.n, dtype=np.int64), deg)
+    a = np.minimum(src, csr.nbrs.astype(np.int64))
+    b = np.maximum(src, csr.nbrs.astype(np.int64))
+    h = ((a * 1000003 + b) * 0x9E3779B1) & 0xFFFFFFFF
+    w = csr.weights.copy()
+    sel = (h >> 7) % every == 0
+    w[sel] = (w[sel].astype(np.float64) * scale).astype(np.float32)
+    return sel, w
+
+
+def _with_weights(pcc, csr, w):
+    return pcc.Csr(csr.n, csr.offsets, csr.nbrs, w)
+
+
+@pytest.mark.parametrize("opts", [(512, 4096, 4096), (64, 64, 16)])
+def test_fine_weights_take_ordered_path(pcc, orc, opts):
+    """Weights of 2^-40 next to weights near 1 make multi-term cluster sums
+    order-sensitive; those vertices must still match the sequential order."""
+    base = _graphs(pcc)["rmat_hubs"]
+    _, w = _scaled_weights(base, 16, 2.0 ** -40)
+    csr = _with_weights(pcc, base, w)
+    print(f"fine weights opts={opts} n={csr.n} m={csr.m}", flush=True)
+    rng = np.random.default_rng(23)
+    ids, sizes, nw, K = random_state(rng, csr.n, max(2, csr.n // 50))
+    cfg = pcc.make_config(resolution=0.05)
+    g = _hub_dev(pcc, csr, opts).best_moves(cfg, ids, sizes, nw, K)
+    o = orc.sync_decisions(orc.G.from_csr(csr), orc.S(ids, sizes, nw, K), orc.config_from(cfg))
+    for a, b in zip(g, o):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    # End to end, with the number of ordered-path vertices reported.
+    cfg = pcc.make_config(resolution=0.05, move_method=1, refine=1)
+    ids2, st = _hub_dev(pcc, csr, opts).cluster(cfg, pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), False)
+    print(f"  ordered-path vertices {st.fallback_vertices}", flush=True)
+    assert np.array_equal(ids2, r.ids)
+    assert st.fallback_vertices > 0
+
+
+def test_dyadic_weights_stay_on_hash_path(pcc, orc):
+    """Weights that are multiples of 2^-10 sum exactly in any order: no vertex
+    needs the ordered path, and the hashed result is still bit-exact."""
+    base = _graphs(pcc)["rmat_hubs"]
+    w = (np.round(base.weights.astype(np.float64) * 1024.0) / 1024.0).astype(np.float32)
+    csr = _with_weights(pcc, base, w)
+    cfg = pcc.make_config(resolution=0.05, move_method=1, refine=1)
+    dg = _dev(pcc, csr)
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), False)
+    assert np.array_equal(ids, r.ids)
+    assert st.fallback_vertices == 0
+
+
+HUB_CLUSTER_CASES = [
+    ("rmat_signed", dict(resolution=0.05, move_method=1, refine=1), False),
+    ("rmat_hubs", dict(resolution=0.05, move_method=2, refine=1), False),
+    ("rmat_hubs", dict(resolution=0.05, move_method=0, refine=0), False),
+    ("rmat_unit_hubs", dict(resolution=1.0, move_method=2, refine=1), True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(HUB_CLUSTER_CASES)))
+def test_cluster_hub_partitions(pcc, orc, case):
+    """End-to-end parity with every vertex of degree > 64 in the partitioned
+    bin (partition expansion of the frontier included)."""
+    name, kw, modularity = HUB_CLUSTER_CASES[case]
+    csr = _graphs(pcc)[name]
+    print(f"cluster hubs {name} {kw}", flush=True)
+    cfg = pcc.make_config(**kw)
+    dg = _hub_dev(pcc, csr, (64, 64, 16))
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), modularity)
+    assert np.array_equal(ids, r.ids)
