@@ -26,10 +26,23 @@ sys.path.insert(0, os.path.join(ROOT, "parallel-correlation-clustering_amd"))
 HBM_PEAK_GBS = 8000.0
 BIN_NAMES = ["deg<=4 (4 lanes)", "deg<=8 (8 lanes)", "deg<=16 (16 lanes)", "deg<=32 (32 lanes)",
              "deg<=64 (64 lanes)", "one-wave LDS hash", "one-workgroup LDS hash",
-             "global-memory hash"]
-BIN_KERNELS = ["k_bm_small<4, false>", "k_bm_small<8, false>", "k_bm_small<16, false>",
-               "k_bm_small<32, false>", "k_bm_small<64, false>", "k_bm_hash<64, 1024, false>",
-               "k_bm_hash<512, 8192, false>", "k_bm_hash<512, 0, false>"]
+             "partitioned LDS hash (hub vertices)"]
+BIN_KERNELS = ["k_bm_small<4>", "k_bm_small<8>", "k_bm_small<16>", "k_bm_small<32>",
+               "k_bm_small<64>", "k_bm_hash<64,256> + k_bm_hash<64,1024>",
+               "k_bm_hash<128,2048> + k_bm_hash<256,4096> + k_bm_hash<512,8192>",
+               "k_hub_scan + k_bm_hub_part + k_bm_hub_select"]
+
+
+def apply_tuning(g, args, pcc_amd):
+    """Degree-bin thresholds and hub partition size (defaults: 512 / 4096 / 4096)."""
+    if args.wave_max:
+        g.set_option(pcc_amd.OPT_WAVE_MAX_DEG, args.wave_max)
+    if args.block_max:
+        g.set_option(pcc_amd.OPT_BLOCK_MAX_DEG, args.block_max)
+    if args.hub_part_edges:
+        g.set_option(pcc_amd.OPT_HUB_PART_EDGES, args.hub_part_edges)
+    if args.async_subrounds is not None:
+        g.set_option(pcc_amd.OPT_ASYNC_SUBROUNDS, args.async_subrounds)
 
 
 def main() -> int:
@@ -37,8 +50,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--edge-factor", type=float, default=16.0)
+    ap.add_argument("--scale", type=int, default=None,
+                    help="RMAT scale (default 22; modularity-async: 24)")
+    ap.add_argument("--edge-factor", type=float, default=None,
+                    help="RMAT edge factor (default 16; modularity-async: 32)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads for the CPU baseline (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -50,7 +65,16 @@ def main() -> int:
                     help="batched: per-graph runs timed for the comparison (0: all)")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests in the stream")
     ap.add_argument("--qps", type=float, default=8.0, help="serving: Poisson arrival rate")
+    ap.add_argument("--wave-max", type=int, default=0, help="upper degree of the one-wave bin")
+    ap.add_argument("--block-max", type=int, default=0, help="upper degree of the workgroup bin")
+    ap.add_argument("--hub-part-edges", type=int, default=0, help="edges per hub partition")
+    ap.add_argument("--async-subrounds", type=int, default=None, help="async sub-rounds (0: auto)")
     args = ap.parse_args()
+    big = args.workload == "modularity-async"
+    if args.scale is None:
+        args.scale = 24 if big else 22
+    if args.edge_factor is None:
+        args.edge_factor = 32.0 if big else 16.0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the clustering path is single-GPU in this build)", file=sys.stderr)
@@ -73,6 +97,7 @@ def main() -> int:
     g = pcc_amd.DeviceGraph.from_csr(csr)
     upload_s = time.time() - t0
     g.set_option(pcc_amd.OPT_TIMING, 1)
+    apply_tuning(g, args, pcc_amd)
     print(f"# graph n={csr.n} m={csr.m} gen={gen_s:.1f}s upload={upload_s:.2f}s", file=sys.stderr,
           flush=True)
 
@@ -244,19 +269,28 @@ def bench_batched(args):
 def bench_modularity_async(args):
     import pcc_amd
 
+    t0 = time.time()
     csr = pcc_amd.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, 1.0, 1.0, seed=3)
+    print(f"# generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     cfg = pcc_amd.make_config(resolution=1.0, move_method=pcc_amd.NBHR_CLUSTER_MOVE, refine=1,
                               permute=1)
     cfg.async_ = 1
+    t0 = time.time()
     g = pcc_amd.DeviceGraph.from_csr(csr)
+    print(f"# graph n={csr.n} m={csr.m} upload={time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     g.set_option(pcc_amd.OPT_TIMING, 1)
-    for _ in range(args.warmup):
-        g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
+    apply_tuning(g, args, pcc_amd)
+    for i in range(args.warmup):
+        _, st = g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
+        print(f"# warmup {i}: {st.cluster_seconds*1e3:.1f} ms modularity={st.modularity:.5f}",
+              file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     stats = []
-    for _ in range(args.steps):
+    for i in range(args.steps):
         ids, st = g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
         stats.append(st)
+        print(f"# step {i}: {st.cluster_seconds*1e3:.1f} ms rounds={st.rounds} "
+              f"modularity={st.modularity:.5f} clusters={st.num_clusters}", file=sys.stderr, flush=True)
     wall = (time.perf_counter() - t0) / max(args.steps, 1)
     te = sum(s.edges_scanned for s in stats)
     bm = sum(s.bestmove_seconds for s in stats)

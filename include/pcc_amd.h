@@ -285,8 +285,8 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_FORCE_ORDERED 2  /* 1: route every hashed vertex to the ordered path */
 #define PCC_OPT_WAVE_MAX_DEG 3   /* upper degree of the one-wave LDS-hash bin */
 #define PCC_OPT_BLOCK_MAX_DEG 4  /* upper degree of the one-workgroup LDS-hash bin */
-#define PCC_OPT_ASYNC_SUBROUNDS 5 /* async: launches per round, each on 1/k of the frontier */
-#define PCC_OPT_ASYNC_SINGLETON_RULE 6 /* async: 1 applies the sync singleton rule */
+#define PCC_OPT_ASYNC_SUBROUNDS 5 /* async: launches per round, each on 1/k of the frontier (0: automatic) */
+#define PCC_OPT_ASYNC_SINGLETON_RULE 6 /* async: 1 applies the sync singleton rule (default 0) */
 #define PCC_OPT_HUB_PART_EDGES 7 /* edges per hash partition of the largest-degree bin */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 

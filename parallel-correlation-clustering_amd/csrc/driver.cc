@@ -307,8 +307,8 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
       g->force_ordered = value != 0;
       return PCC_OK;
     case PCC_OPT_ASYNC_SUBROUNDS:
-      if (value < 1 || value > 1024)
-        return SetError(PCC_INVALID_ARGUMENT, "async sub-rounds must lie in [1, 1024]");
+      if (value < 0 || value > 1024)
+        return SetError(PCC_INVALID_ARGUMENT, "async sub-rounds must lie in [0, 1024]");
       g->async_subrounds = (int)value;
       return PCC_OK;
     case PCC_OPT_ASYNC_SINGLETON_RULE:

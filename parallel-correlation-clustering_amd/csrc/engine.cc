@@ -423,8 +423,8 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
   for (int b = 0; b + 1 < kBins; ++b) {
     if (counts[b] == 0) continue;
     Timer tb = StartTimer();
-    LaunchBestMoveBin(b, counts[b], async, c.g, c.st, mp, ws.list.get() + c.layout.base[b], out,
-                      ws.fb_list.get(), ws.rc.get(), salt, s_);
+    LaunchBestMoveBin(b, counts[b], async, h_->th, c.g, c.st, mp, ws.list.get() + c.layout.base[b],
+                      out, ws.fb_list.get(), ws.rc.get(), salt, s_);
     StopTimer(tb, b);
   }
   const int64_t c7 = counts[kBins - 1];
@@ -467,10 +467,15 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
   Timer all = StartTimer();
   // Async rounds run as sub-rounds over disjoint hashed subsets of the
-  // frontier, each launch seeing the moves of the previous ones, which keeps
-  // the number of simultaneously deciding vertices closer to a multicore
-  // run's.
-  const int subs = async ? std::max(1, h_->async_subrounds) : 1;
+  // frontier, each launch seeing the moves of the previous ones.  The
+  // quality of an async round falls as the fraction of the frontier deciding
+  // at once grows; about 1/64 of the frontier per sub-round (and at least 64
+  // vertices) keeps the objective at the level of a sequential run.
+  int subs = 1;
+  if (async) {
+    subs = h_->async_subrounds > 0 ? h_->async_subrounds
+                                   : (int)std::min<int64_t>(64, std::max<int64_t>(1, (F + 63) / 64));
+  }
   for (int sub = 0; sub < subs; ++sub) {
     MoveParams mp = c.mp;
     mp.sub_count = (uint32_t)subs;

@@ -38,8 +38,8 @@ struct pcc_graph {
   // Options.
   bool timing = false;
   bool force_ordered = false;
-  int async_subrounds = 8;
-  int async_singleton_rule = 1;
+  int async_subrounds = 0;  // 0: automatic (see Engine::RoundStep)
+  int async_singleton_rule = 0;
   int64_t hub_part_edges = pcc::kHubPartEdges;
   pcc::BinThresholds th;
   int num_graphs() const { return gbase.empty() ? 1 : (int)gbase.size() - 1; }

@@ -34,6 +34,8 @@
 //   serves the forced-ordered test mode and the (load-factor-excluded) case of
 //   a full partition table.
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -706,7 +708,8 @@ __device__ __forceinline__ bool Accumulate(const CsrView& g, const StateView& st
 template <int THREADS, int TABLE, bool kAsync>
 __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
                                                      const uint32_t* __restrict__ list,
-                                                     int64_t count, MoveOut out,
+                                                     int64_t count, int64_t dlo, int64_t dhi,
+                                                     MoveOut out,
                                                      uint32_t* __restrict__ fb_list,
                                                      RoundCounters* __restrict__ rc,
                                                      uint64_t salt) {
@@ -720,12 +723,13 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
   for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t v = list[PermuteIndex(i, count, salt)];
     if (kAsync && !InSubRound(mp, v)) continue;  // uniform across the workgroup
+    const int64_t start = g.off[v];
+    const int64_t deg = g.off[v + 1] - start;
+    if (deg <= dlo || deg > dhi) continue;  // another launch's degree range
     if (!kAsync && mp.force_ordered) {
       if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       continue;
     }
-    const int64_t start = g.off[v];
-    const int64_t deg = g.off[v + 1] - start;
     // Table sized per vertex (load factor <= 1/2) so that clearing and
     // scanning it costs O(deg), not O(TABLE).
     const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * deg + 2))));
@@ -1098,18 +1102,45 @@ static void LaunchSmall(int64_t count, const CsrView& g, const StateView& st, co
                      salt);
 }
 
+// One launch per degree range (dlo, dhi] of a hashed bin, each with the
+// table size that range needs at load factor <= 1/2: smaller tables leave
+// LDS for more workgroups per CU.  The workgroups stride over the bin's list
+// and skip other ranges' vertices.
 template <int THREADS, int TABLE, bool A>
-static void LaunchHash(int64_t count, const CsrView& g, const StateView& st, const MoveParams& mp,
-                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
-  const int grid = GridFor(count, 1, 1 << 20);
+static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, const CsrView& g,
+                       const StateView& st, const MoveParams& mp, const uint32_t* list,
+                       const MoveOut& out, uint32_t* fb_list, RoundCounters* rc, uint64_t salt,
+                       bool whole_bin, hipStream_t s) {
+  if (dhi <= dlo) return;
+  const int grid = GridFor(count, 1, whole_bin ? (1 << 20) : 8192);
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
-                     list, count, out, fb_list, rc, salt);
+                     list, count, dlo, dhi, out, fb_list, rc, salt);
 }
 
-void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, const StateView& st,
-                       const MoveParams& mp, const uint32_t* list, const MoveOut& out,
-                       uint32_t* fb_list, RoundCounters* rc, uint64_t salt, hipStream_t s) {
+template <bool A>
+static void LaunchHashBins(int bin, int64_t count, const BinThresholds& th, const CsrView& g,
+                           const StateView& st, const MoveParams& mp, const uint32_t* list,
+                           const MoveOut& out, uint32_t* fb_list, RoundCounters* rc,
+                           uint64_t salt, hipStream_t s) {
+  if (bin == 5) {
+    const int64_t lo = 64, hi = th.wave_max;
+    const int64_t cut = std::min<int64_t>(127, hi);
+    LaunchHash<64, 256, A>(count, lo, cut, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<64, 1024, A>(count, cut, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
+  } else {
+    const int64_t lo = th.wave_max, hi = th.block_max;
+    const int64_t c1 = std::max(lo, std::min<int64_t>(1023, hi));
+    const int64_t c2 = std::max(c1, std::min<int64_t>(2047, hi));
+    LaunchHash<128, 2048, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<256, 4096, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<512, 8192, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
+  }
+}
+
+void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& th,
+                       const CsrView& g, const StateView& st, const MoveParams& mp,
+                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
   if (count <= 0) return;
   switch (bin) {
     case 0:
@@ -1133,12 +1164,9 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, con
             : LaunchSmall<64, false>(count, g, st, mp, list, out, salt, s);
       break;
     case 5:
-      async ? LaunchHash<64, 1024, true>(count, g, st, mp, list, out, fb_list, rc, salt, s)
-            : LaunchHash<64, 1024, false>(count, g, st, mp, list, out, fb_list, rc, salt, s);
-      break;
     case 6:
-      async ? LaunchHash<512, 8192, true>(count, g, st, mp, list, out, fb_list, rc, salt, s)
-            : LaunchHash<512, 8192, false>(count, g, st, mp, list, out, fb_list, rc, salt, s);
+      async ? LaunchHashBins<true>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s)
+            : LaunchHashBins<false>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s);
       break;
     default:
       throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: bin 7 goes through LaunchBestMoveHub",

@@ -131,7 +131,25 @@ __device__ __forceinline__ void MarkNeighbours(bool sel, int64_t st, int64_t deg
   }
 }
 
-// Moved vertices of bins 0..6 (bin 7 goes through k_expand_hub).
+// Moved vertices of bins 5 and 6: one workgroup per vertex.
+__global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
+                                                      const uint32_t* __restrict__ list,
+                                                      const int64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint8_t* __restrict__ moved,
+                                                      uint8_t* __restrict__ next) {
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    int64_t r = t;
+    int b = 0;
+    while (r >= bins.count[b]) r -= bins.count[b++];
+    const uint32_t v = list[bins.base[b] + r];
+    if (!moved[v]) continue;
+    const int64_t st = off[v], en = off[v + 1];
+    for (int64_t p = st + threadIdx.x; p < en; p += blockDim.x) next[nbr[p]] = 1;
+  }
+}
+
+// Moved vertices of bins 0..4 (bins 5-6: k_expand_block, bin 7: k_expand_hub).
 __global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
                                                       const uint32_t* __restrict__ list,
                                                       const int64_t* __restrict__ off,
@@ -273,12 +291,24 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,
                        const uint8_t* moved, uint8_t* next, hipStream_t s) {
   int64_t total;
-  Bins b = MakeBins(counts, layout, &total);
-  const int64_t small = total - counts[kBins - 1];
-  b.count[kBins - 1] = 0;
-  if (small > 0) {
-    hipLaunchKernelGGL(k_expand_moved, dim3(Grid(small, 256, 8192)), dim3(256), 0, s, b, small,
-                       list, g.off, g.nbr, moved, next);
+  const Bins all = MakeBins(counts, layout, &total);
+  Bins lo = all, mid = all;
+  int64_t nlo = 0, nmid = 0;
+  for (int b = 0; b < kBins; ++b) {
+    const bool is_lo = b <= 4, is_mid = b == 5 || b == 6;
+    lo.count[b] = is_lo ? all.count[b] : 0;
+    mid.count[b] = is_mid ? all.count[b] : 0;
+    nlo += lo.count[b];
+    nmid += mid.count[b];
+  }
+  if (nlo > 0) {
+    hipLaunchKernelGGL(k_expand_moved, dim3(Grid(nlo, 256, 8192)), dim3(256), 0, s, lo, nlo, list,
+                       g.off, g.nbr, moved, next);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  if (nmid > 0) {
+    hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 16384)), dim3(256), 0, s, mid, nmid, list,
+                       g.off, g.nbr, moved, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
   const int64_t c7 = counts[kBins - 1];

@@ -224,10 +224,10 @@ void LaunchCollectHubs(const CsrView& g, const BinThresholds& th, uint32_t* list
 
 // One best-move kernel per bin 0..6.  count = number of listed vertices in
 // the bin.
-void LaunchBestMoveBin(int bin, int64_t count, bool async, const CsrView& g, const StateView& st,
-                       const MoveParams& mp, const uint32_t* list, const MoveOut& out,
-                       uint32_t* fb_list, RoundCounters* rc, uint64_t permute_salt,
-                       hipStream_t s);
+void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& th,
+                       const CsrView& g, const StateView& st, const MoveParams& mp,
+                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, uint64_t permute_salt, hipStream_t s);
 
 // Bin 7: partition pass and selection pass over a list whose HubScratch
 // prefix sums are current; te = sum of the listed degrees.
