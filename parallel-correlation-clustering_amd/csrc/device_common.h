@@ -63,6 +63,15 @@ __device__ __forceinline__ uint32_t HashId(uint32_t c) {
   return h ^ (h >> 15);
 }
 
+// Async sub-round of vertex v among S (a fixed hash of the id).
+__device__ __forceinline__ uint32_t SubRoundOf(uint32_t v, uint32_t S) {
+  uint32_t h = v * 0x9E3779B1u;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  return h % S;
+}
+
 // Partition of cluster c among `parts` partitions (a hash independent of
 // HashId's low bits, range-reduced by multiply-shift).
 __device__ __forceinline__ uint32_t PartOf(uint32_t c, uint32_t parts) {

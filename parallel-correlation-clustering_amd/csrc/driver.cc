@@ -449,7 +449,9 @@ int pcc_best_moves(pcc_graph* g, const pcc_config* cfg, const uint32_t* cluster_
     out.moved_clusters = ws.flags.get();
     out.dbg_target = tgt.get();
     out.dbg_delta = dlt.get();
-    e.LaunchMoves(c, ws, c.mp, false, counts, te, out, 0, true);
+    const uint32_t* lists[kBins];
+    for (int b = 0; b < kBins; ++b) lists[b] = ws.list.get() + c.layout.base[b];
+    e.LaunchMoves(c, ws, c.mp, false, lists, counts, te, out, 0, true);
     LaunchBestMoveOrdered(c.g, c.st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s);
     PCC_HIP_CHECK(hipMemcpyAsync(out_moves, ws.moves.get(), 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     PCC_HIP_CHECK(hipMemcpyAsync(out_moved, ws.moved.get(), (size_t)n, hipMemcpyDeviceToHost, s));

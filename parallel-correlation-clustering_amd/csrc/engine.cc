@@ -248,6 +248,9 @@ Engine::~Engine() {
 Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) {
   const int64_t nn = n > 0 ? n : 1;
   list.Reset(al, nn);
+  list2.Reset(al, nn);
+  sub_cnt.Reset(al, kBins * 1024);
+  sub_cur.Reset(al, kBins * 1024);
   fb_list.Reset(al, nn);
   moves.Reset(al, nn);
   moved.Reset(al, nn);
@@ -364,21 +367,33 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   if (c7 > 0) {
     c->hb_part.Reset(al_, c7 + 1);
     c->hb_ent.Reset(al_, c7 + 1);
+    c->hb_cbuf.Reset(al_, t7);
     c->hb_ekeys.Reset(al_, t7);
     c->hb_evals.Reset(al_, t7);
     c->hb_ecur.Reset(al_, c7);
     c->hb_flag.Reset(al_, c7);
     c->hb_tval.Reset(al_, c7);
+    // Partition items of any sub-list: sum ceil(deg / E) < t7 / E + c7.
+    const int64_t items = c7 + t7 / h_->hub_part_edges + 1;
+    c->hb_pe_base.Reset(al_, items);
+    c->hb_pe_cnt.Reset(al_, items);
+    c->hb_pbest_c.Reset(al_, items);
+    c->hb_pbest_d.Reset(al_, items);
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_ecur.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_flag.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     c->hb.part_edges = h_->hub_part_edges;
     c->hb.part_off = c->hb_part.get();
     c->hb.ent_off = c->hb_ent.get();
+    c->hb.cbuf = c->hb_cbuf.get();
     c->hb.ekeys = c->hb_ekeys.get();
     c->hb.evals = c->hb_evals.get();
     c->hb.ecur = c->hb_ecur.get();
     c->hb.hflag = c->hb_flag.get();
     c->hb.tval = c->hb_tval.get();
+    c->hb.pe_base = c->hb_pe_base.get();
+    c->hb.pe_cnt = c->hb_pe_cnt.get();
+    c->hb.pbest_c = c->hb_pbest_c.get();
+    c->hb.pbest_d = c->hb_pbest_d.get();
     if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
       c->hubs.Reset(al_, c7);
       c->hubs_part.Reset(al_, c7 + 1);
@@ -418,18 +433,19 @@ void Engine::BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, in
 }
 
 void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
-                         const int64_t counts[kBins], const int64_t te[kBins], const MoveOut& out,
-                         uint64_t salt, bool scan_hubs) {
+                         const uint32_t* const lists[kBins], const int64_t counts[kBins],
+                         const int64_t te[kBins], const MoveOut& out, uint64_t salt,
+                         bool scan_hubs) {
   for (int b = 0; b + 1 < kBins; ++b) {
     if (counts[b] == 0) continue;
     Timer tb = StartTimer();
-    LaunchBestMoveBin(b, counts[b], async, h_->th, c.g, c.st, mp, ws.list.get() + c.layout.base[b],
-                      out, ws.fb_list.get(), ws.rc.get(), salt, s_);
+    LaunchBestMoveBin(b, counts[b], async, h_->th, c.g, c.st, mp, lists[b], out, ws.fb_list.get(),
+                      ws.rc.get(), salt, s_);
     StopTimer(tb, b);
   }
   const int64_t c7 = counts[kBins - 1];
   if (c7 > 0) {
-    const uint32_t* list7 = ws.list.get() + c.layout.base[kBins - 1];
+    const uint32_t* list7 = lists[kBins - 1];
     Timer tb = StartTimer();
     if (scan_hubs) LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, s_);
     LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
@@ -440,7 +456,6 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
 
 int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, int G,
                           uint8_t* gmoved, int64_t counts[kBins], int64_t te[kBins]) {
-  (void)G;
   const CsrView& g = c.g;
   const StateView& st = c.st;
   const int64_t n = g.n;
@@ -476,12 +491,61 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     subs = h_->async_subrounds > 0 ? h_->async_subrounds
                                    : (int)std::min<int64_t>(64, std::max<int64_t>(1, (F + 63) / 64));
   }
+  const uint32_t* lists[kBins];
+  int64_t scount[kBins];
+  for (int b = 0; b < kBins; ++b) {
+    lists[b] = ws.list.get() + c.layout.base[b];
+    scount[b] = counts[b];
+  }
+  std::vector<uint32_t> hcnt, hoff;
+  if (subs > 1) {
+    // Group every bin's list by sub-round so that each launch sees only its
+    // own vertices.
+    const size_t K = (size_t)kBins * (size_t)subs;
+    PCC_HIP_CHECK(hipMemsetAsync(ws.sub_cnt.get(), 0, sizeof(uint32_t) * K, s_));
+    LaunchSubCount(ws.list.get(), c.layout, counts, (uint32_t)subs, ws.sub_cnt.get(), s_);
+    hcnt.resize(K);
+    hoff.resize(K);
+    PCC_HIP_CHECK(hipMemcpyAsync(hcnt.data(), ws.sub_cnt.get(), sizeof(uint32_t) * K,
+                                 hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    for (int b = 0; b < kBins; ++b) {
+      uint32_t run = 0;
+      for (int j = 0; j < subs; ++j) {
+        hoff[(size_t)b * subs + j] = run;
+        run += hcnt[(size_t)b * subs + j];
+      }
+    }
+    PCC_HIP_CHECK(hipMemcpyAsync(ws.sub_cur.get(), hoff.data(), sizeof(uint32_t) * K,
+                                 hipMemcpyHostToDevice, s_));
+    LaunchSubScatter(ws.list.get(), c.layout, counts, (uint32_t)subs, ws.sub_cur.get(),
+                     ws.list2.get(), s_);
+  }
+  FreeIds fids;
+  if (async) PrepareFreeIds(al_, st, (uint32_t)n, G, c.mp.gbase, &fids, s_);
   for (int sub = 0; sub < subs; ++sub) {
     MoveParams mp = c.mp;
-    mp.sub_count = (uint32_t)subs;
-    mp.sub_index = (uint32_t)sub;
     mp.async_singleton_rule = async ? h_->async_singleton_rule : 0;
-    LaunchMoves(c, ws, mp, async, counts, te, out, salt, sub == 0);
+    if (async) {
+      mp.free_ids = fids.ids.get();
+      mp.free_cur = fids.cur.get();
+      mp.free_end = fids.end.get();
+    }
+    if (subs > 1) {
+      for (int b = 0; b < kBins; ++b) {
+        const size_t k = (size_t)b * subs + sub;
+        lists[b] = ws.list2.get() + c.layout.base[b] + hoff[k];
+        scount[b] = hcnt[k];
+      }
+    }
+    LaunchMoves(c, ws, mp, async, lists, scount, te, out, salt, subs > 1 || sub == 0);
+  }
+  if (subs > 1) {
+    // The frontier expansion walks bin 7 of the whole list.
+    PCC_HIP_CHECK(hipStreamSynchronize(s_));  // hoff / hcnt stay valid until here
+    if (counts[kBins - 1] > 0)
+      LaunchHubScan(g, ws.list.get() + c.layout.base[kBins - 1], counts[kBins - 1],
+                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, s_);
   }
   if (!async && counts[5] + counts[6] + counts[7] > 0) {
     Timer tf = StartTimer();

@@ -105,7 +105,8 @@ class Engine {
           std::vector<int64_t>* out_iter);
 
   struct Workspace {
-    DevBuf<uint32_t> list, fb_list, moves;
+    DevBuf<uint32_t> list, list2, fb_list, moves;
+    DevBuf<uint32_t> sub_cnt, sub_cur;  // async sub-round segments, kBins * 1024
     DevBuf<uint8_t> moved, flags, next, new_flag;
     DevBuf<RoundCounters> rc;
     Workspace(DeviceAllocator* al, int64_t n, hipStream_t s);
@@ -127,8 +128,8 @@ class Engine {
     // Bin-7 scratch, sized for every bin-7 vertex of the level.
     HubScratch hb;
     DevBuf<int64_t> hb_part, hb_ent;
-    DevBuf<uint32_t> hb_ekeys, hb_ecur, hb_flag;
-    DevBuf<double> hb_evals, hb_tval;
+    DevBuf<uint32_t> hb_cbuf, hb_ekeys, hb_ecur, hb_flag, hb_pe_base, hb_pe_cnt, hb_pbest_c;
+    DevBuf<double> hb_evals, hb_tval, hb_pbest_d;
     // Every bin-7 vertex with its partition prefix (NBHR_CLUSTER_MOVE
     // expansion).
     DevBuf<uint32_t> hubs;
@@ -136,11 +137,12 @@ class Engine {
     int64_t hub_count = 0, hub_te = 0;
   };
   void PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay, LevelCtx* ctx);
-  // Best-move kernels of every bin for the binned list (counts / te per bin);
-  // scan_hubs: refresh the bin-7 prefix sums first.
+  // Best-move kernels of every bin; lists[b] holds counts[b] vertices of bin
+  // b, te[7] bounds the degree sum of bin 7's list.  scan_hubs: refresh the
+  // bin-7 prefix sums first.
   void LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
-                   const int64_t counts[kBins], const int64_t te[kBins], const MoveOut& out,
-                   uint64_t salt, bool scan_hubs);
+                   const uint32_t* const lists[kBins], const int64_t counts[kBins],
+                   const int64_t te[kBins], const MoveOut& out, uint64_t salt, bool scan_hubs);
   // IterateBestMoves (parallel-correlation-clusterer.cc:340-365) over the
   // graphs selected by sel (present-graph flags; null = all).  Returns the
   // per-present-graph "moved" flags.

@@ -188,6 +188,78 @@ __global__ __launch_bounds__(256) void k_collect_hubs(int64_t n, const int64_t* 
   }
 }
 
+// Splitting the binned list by async sub-round: list item t of the bins'
+// concatenation lies in bin b; its (b, SubRoundOf(v, S)) segment is counted
+// (k_sub_count) and, after the host turned counts into segment starts in
+// cur, filled (k_sub_scatter).  Each workgroup handles a slab of items and
+// reserves its share of every segment with one atomic.
+constexpr int kSubThreads = 256;
+constexpr int64_t kSubSlab = 16 * kSubThreads;
+constexpr int kMaxSub = 1024;
+
+struct BinCounts {
+  int64_t base[kBins];
+  int64_t count[kBins];
+};
+
+__device__ __forceinline__ int BinOfItem(const BinCounts& bc, int64_t t, int64_t* r) {
+  int b = 0;
+  while (t >= bc.count[b]) t -= bc.count[b++];
+  *r = t;
+  return b;
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_sub_count(const uint32_t* __restrict__ list,
+                                                           BinCounts bc, int64_t total, uint32_t S,
+                                                           uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[kBins * kMaxSub];
+  for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads) h[k] = 0;
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * kSubThreads + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kSubThreads) {
+    int64_t r;
+    const int b = BinOfItem(bc, t, &r);
+    atomicAdd(&h[b * S + SubRoundOf(list[bc.base[b] + r], S)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads)
+    if (h[k]) atomicAdd(&cnt[k], h[k]);
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_sub_scatter(const uint32_t* __restrict__ list,
+                                                             BinCounts bc, int64_t total,
+                                                             uint32_t S,
+                                                             uint32_t* __restrict__ cur,
+                                                             uint32_t* __restrict__ out) {
+  __shared__ uint32_t h[kBins * kMaxSub];
+  __shared__ uint32_t hb[kBins * kMaxSub];
+  for (int64_t s0 = (int64_t)blockIdx.x * kSubSlab; s0 < total;
+       s0 += (int64_t)gridDim.x * kSubSlab) {
+    const int64_t s1 = min(total, s0 + kSubSlab);
+    for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads) h[k] = 0;
+    __syncthreads();
+    for (int64_t t = s0 + threadIdx.x; t < s1; t += kSubThreads) {
+      int64_t r;
+      const int b = BinOfItem(bc, t, &r);
+      atomicAdd(&h[b * S + SubRoundOf(list[bc.base[b] + r], S)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads) {
+      hb[k] = h[k] ? atomicAdd(&cur[k], h[k]) : 0u;
+      h[k] = 0;
+    }
+    __syncthreads();
+    for (int64_t t = s0 + threadIdx.x; t < s1; t += kSubThreads) {
+      int64_t r;
+      const int b = BinOfItem(bc, t, &r);
+      const uint32_t v = list[bc.base[b] + r];
+      const uint32_t k = b * S + SubRoundOf(v, S);
+      out[bc.base[b] + hb[k] + atomicAdd(&h[k], 1u)] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // gthr[v] = ThresholdFor(sum over v's edges of |w| + |offset|) for deg > 64;
 // one wave per such vertex.
 __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t* __restrict__ off,
@@ -332,26 +404,39 @@ __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d,
     out.moved[v] = 1;
     return;
   }
-  // Empty ids are searched within the vertex's own graph.
-  uint32_t lo = 0, hi = mp.n;
+  // Empty ids are searched within the vertex's own graph: v itself first, as
+  // the reference's probe does, then the round's list of empty ids, then the
+  // probe v + 1, v + 2, ... (PCCI.cc:387-399).
+  uint32_t lo = 0, hi = mp.n, gi = 0;
   if (mp.vgraph) {
-    const uint32_t g = mp.vgraph[v];
-    lo = (uint32_t)mp.gbase[g];
-    hi = (uint32_t)mp.gbase[g + 1];
+    gi = mp.vgraph[v];
+    lo = (uint32_t)mp.gbase[gi];
+    hi = (uint32_t)mp.gbase[gi + 1];
+  }
+  auto claim = [&](uint32_t i) {
+    if (__hip_atomic_load(&st.sizes[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+        atomicCAS(&st.sizes[i], 0u, 1u) != 0u)
+      return false;
+    atomicSub(&st.sizes[cur], 1u);
+    atomicAdd(&st.K[cur], -1 * k);
+    atomicAdd(&st.K[i], k);
+    st.ids[v] = i;
+    out.moved_clusters[i] = 1;
+    out.moved[v] = 1;
+    return true;
+  };
+  if (claim(v)) return;
+  if (mp.free_ids) {
+    while (true) {
+      const unsigned long long x = atomicAdd(&mp.free_cur[gi], 1ull);
+      if (x >= mp.free_end[gi]) break;
+      if (claim(mp.free_ids[x])) return;
+    }
   }
   uint32_t i = v;
   for (uint32_t probe = lo; probe < hi; ++probe) {
-    if (__hip_atomic_load(&st.sizes[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-        atomicCAS(&st.sizes[i], 0u, 1u) == 0u) {
-      atomicSub(&st.sizes[cur], 1u);
-      atomicAdd(&st.K[cur], -1 * k);
-      atomicAdd(&st.K[i], k);
-      st.ids[v] = i;
-      out.moved_clusters[i] = 1;
-      out.moved[v] = 1;
-      return;
-    }
     i = (i + 1 == hi) ? lo : i + 1;
+    if (claim(i)) return;
   }
   out.moved[v] = 0;
 }
@@ -359,11 +444,7 @@ __device__ __forceinline__ void AsyncEpilogue(uint32_t v, uint32_t t, double d,
 // Sub-round of a vertex in async mode (a fixed hash of the id).
 __device__ __forceinline__ bool InSubRound(const MoveParams& mp, uint32_t v) {
   if (mp.sub_count <= 1) return true;
-  uint32_t h = v * 0x9E3779B1u;
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  return h % mp.sub_count == mp.sub_index;
+  return SubRoundOf(v, mp.sub_count) == mp.sub_index;
 }
 
 // Item -> list position; with a salt, a bijection of [0, count) that changes
@@ -567,11 +648,11 @@ struct RepairStage {
 // the slot of c(v) (or -1); the return value is the own-cluster total: on the
 // hub path the pbbs block order of curr_together_seq (block sums folded left,
 // PCCI.cc:516-545), otherwise the slot's sum.
-template <int THREADS>
-__device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const StateView& st,
-                                                  const MoveParams& mp, uint32_t v, int64_t start,
-                                                  int64_t deg, bool hub, const uint32_t* keys,
-                                                  double* vals, const uint32_t* cnt, uint32_t cap,
+template <int THREADS, class Src>
+__device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const MoveParams& mp,
+                                                  Src src, int64_t start, int64_t deg, bool hub,
+                                                  const uint32_t* keys, double* vals,
+                                                  const uint32_t* cnt, uint32_t cap,
                                                   uint32_t parts, uint32_t part, int own,
                                                   RepairStage<THREADS>* stg) {
   const uint32_t mask = cap - 1;
@@ -590,9 +671,8 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Sta
     int slot = -1;
     double x = 0.0;
     if (q < deg) {
-      const uint32_t u = g.nbr[start + q];
-      if (u != v) {
-        const uint32_t c = st.ids[u];
+      const uint32_t c = src.Cluster(src.Neighbour(q));
+      if (c != kNoneId) {
         if (parts == 1 || PartOf(c, parts) == part) {
           const int s = HashFind(keys, mask, c);
           if (s >= 0 && SlotInexact(cnt[s])) {
@@ -655,28 +735,47 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Sta
   return stg->T;
 }
 
+// Where the hashed kernels take the cluster of v's q-th neighbour from:
+// the adjacency and the cluster ids (one gather per edge), or a buffer the
+// hub gather pass filled with those clusters (kNoneId for self-loops).
+struct AdjacencyClusters {
+  const uint32_t* nbr;  // v's adjacency
+  const uint32_t* ids;
+  uint32_t v;
+  __device__ __forceinline__ uint32_t Neighbour(int64_t q) const { return nbr[q]; }
+  __device__ __forceinline__ uint32_t Cluster(uint32_t u) const {
+    return u != v ? ids[u] : kNoneId;
+  }
+};
+struct BufferedClusters {
+  const uint32_t* c;  // v's gathered clusters
+  __device__ __forceinline__ uint32_t Neighbour(int64_t q) const { return c[q]; }
+  __device__ __forceinline__ uint32_t Cluster(uint32_t x) const { return x; }
+};
+
 // Adds the neighbours of v whose cluster falls in partition `part` of
 // `parts` into the LDS table (keys / vals / cnt, cap = mask + 1 slots).  Each
 // thread keeps U edges in flight.  Returns true if an insert found the table
 // full.  The value added is EfficientBestMove's: (double)w - offset on the
 // low-degree path, the float w -= offset on the hub path (PCCI.cc:466, 529).
-template <int THREADS, bool kAsync>
-__device__ __forceinline__ bool Accumulate(const CsrView& g, const StateView& st,
-                                           const MoveParams& mp, uint32_t v, int64_t start,
-                                           int64_t deg, bool hub, int thr, uint32_t* keys,
-                                           double* vals, uint32_t* cnt, uint32_t mask,
-                                           uint32_t parts, uint32_t part) {
+template <int THREADS, bool kAsync, class Src>
+__device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& mp, Src src,
+                                           int64_t start, int64_t deg, bool hub, int thr,
+                                           uint32_t* keys, double* vals, uint32_t* cnt,
+                                           uint32_t mask, uint32_t parts, uint32_t part) {
   constexpr int U = 4;
   bool overflow = false;
   for (int64_t q0 = threadIdx.x; q0 < deg; q0 += (int64_t)THREADS * U) {
     uint32_t c[U];
+    bool in[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t q = q0 + (int64_t)k * THREADS;
-      c[k] = q < deg ? g.nbr[start + q] : v;
+      in[k] = q < deg;
+      c[k] = in[k] ? src.Neighbour(q) : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < U; ++k) c[k] = c[k] != v ? st.ids[c[k]] : kNoneId;
+    for (int k = 0; k < U; ++k) c[k] = in[k] ? src.Cluster(c[k]) : kNoneId;
     float wf[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -699,6 +798,79 @@ __device__ __forceinline__ bool Accumulate(const CsrView& g, const StateView& st
         if (GranExp(x) < thr) atomicOr(&cnt[slot], kFineBit);
       }
     }
+  }
+  return overflow;
+}
+
+// Accumulate over gathered clusters cb[0, deg) (partition `part` of
+// `parts`): the buffer is streamed with 16-byte loads, U per thread in
+// flight; only the edges of the partition load their weight.
+template <int THREADS, bool kAsync>
+__device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveParams& mp,
+                                                   const uint32_t* __restrict__ cb, int64_t start,
+                                                   int64_t deg, bool hub, int thr, uint32_t* keys,
+                                                   double* vals, uint32_t* cnt, uint32_t mask,
+                                                   uint32_t parts, uint32_t part) {
+  bool overflow = false;
+  auto insert = [&](uint32_t c, float wf) {
+    const double x = EdgeValue(wf, hub, mp.offset);
+    const int slot = HashSlot(keys, mask, c);
+    if (slot < 0) {
+      overflow = true;
+      return;
+    }
+    atomicAdd(&vals[slot], x);
+    if (!kAsync) {
+      atomicAdd(&cnt[slot], 1u);
+      if (GranExp(x) < thr) atomicOr(&cnt[slot], kFineBit);
+    }
+  };
+  auto mine = [&](uint32_t c) {
+    return c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
+  };
+  const int64_t lead = (int64_t)((4 - (((uintptr_t)cb >> 2) & 3)) & 3);
+  const int64_t head = lead < deg ? lead : deg;
+  if ((int64_t)threadIdx.x < head) {
+    const uint32_t c = cb[threadIdx.x];
+    if (mine(c)) insert(c, g.w[start + threadIdx.x]);
+  }
+  const int64_t nvec = (deg - head) >> 2;
+  const uint4* vp = reinterpret_cast<const uint4*>(cb + head);
+  constexpr int U = 4;
+  for (int64_t j0 = threadIdx.x; j0 < nvec; j0 += (int64_t)THREADS * U) {
+    uint4 x4[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t j = j0 + (int64_t)k * THREADS;
+      x4[k] = j < nvec ? vp[j] : make_uint4(kNoneId, kNoneId, kNoneId, kNoneId);
+    }
+    // Partition filter first, then every weight load of the batch, then the
+    // inserts, so the loads of one thread are in flight together.
+    uint32_t c[4 * U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      c[4 * k] = x4[k].x;
+      c[4 * k + 1] = x4[k].y;
+      c[4 * k + 2] = x4[k].z;
+      c[4 * k + 3] = x4[k].w;
+    }
+#pragma unroll
+    for (int e = 0; e < 4 * U; ++e)
+      if (!mine(c[e])) c[e] = kNoneId;
+    float wf[4 * U];
+#pragma unroll
+    for (int e = 0; e < 4 * U; ++e) {
+      const int64_t q = head + 4 * (j0 + (int64_t)(e >> 2) * THREADS) + (e & 3);
+      wf[e] = c[e] != kNoneId ? g.w[start + q] : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4 * U; ++e)
+      if (c[e] != kNoneId) insert(c[e], wf[e]);
+  }
+  const int64_t t0 = head + 4 * nvec;
+  if (t0 + (int64_t)threadIdx.x < deg) {
+    const uint32_t c = cb[t0 + threadIdx.x];
+    if (mine(c)) insert(c, g.w[start + t0 + threadIdx.x]);
   }
   return overflow;
 }
@@ -743,8 +915,9 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     const uint32_t cv = st.ids[v];
     const bool hub = deg > kHubDegree;
     const int thr = kAsync ? 0 : (int)mp.gthr[v];
-    const bool overflow = Accumulate<THREADS, kAsync>(g, st, mp, v, start, deg, hub, thr, keys,
-                                                      vals, cnt, mask, 1, 0);
+    const AdjacencyClusters src{g.nbr + start, st.ids, v};
+    const bool overflow = Accumulate<THREADS, kAsync>(g, mp, src, start, deg, hub, thr, keys, vals,
+                                                      cnt, mask, 1, 0);
     if (__syncthreads_or(overflow)) {  // cannot happen at load <= 1/2; kept as a guard
       if (threadIdx.x == 0) {
         if (kAsync)
@@ -760,7 +933,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
       bool inexact = false;
       for (uint32_t s = threadIdx.x; s < cap; s += THREADS) inexact = inexact || SlotInexact(cnt[s]);
       if (__syncthreads_or(inexact)) {
-        T = RepairInexactSlots<THREADS>(g, st, mp, v, start, deg, hub, keys, vals, cnt, cap, 1, 0,
+        T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
                                         own, &stg);
         if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
       }
@@ -793,21 +966,57 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
   }
 }
 
+// Bin 7, gather pass: the cluster of every neighbour of the listed vertices,
+// cbuf[ent_off[i] + q] (kNoneId for self-loops), one workgroup per partition
+// of edges, so the partition pass reads clusters without gathers.
+template <bool kAsync>
+__global__ __launch_bounds__(256) void k_hub_gather(CsrView g, StateView st, MoveParams mp,
+                                                    const uint32_t* __restrict__ list,
+                                                    int64_t count, HubScratch hb) {
+  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  if (t >= hb.part_off[count]) return;
+  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const uint32_t v = list[i];
+  if (kAsync && !InSubRound(mp, v)) return;
+  if (!kAsync && mp.force_ordered) return;
+  const int64_t parts = hb.part_off[i + 1] - hb.part_off[i];
+  const int64_t part = t - hb.part_off[i];
+  const int64_t start = g.off[v], deg = g.off[v + 1] - start;
+  const int64_t per = (deg + parts - 1) / parts;
+  const int64_t e0 = part * per, e1 = min(deg, (part + 1) * per);
+  uint32_t* out = hb.cbuf + hb.ent_off[i];
+  constexpr int U = 8;
+  for (int64_t q0 = e0 + threadIdx.x; q0 < e1; q0 += (int64_t)blockDim.x * U) {
+    uint32_t u[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * blockDim.x;
+      u[k] = q < e1 ? g.nbr[start + q] : v;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) u[k] = u[k] != v ? st.ids[u[k]] : kNoneId;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * blockDim.x;
+      if (q < e1) out[q] = u[k];
+    }
+  }
+}
+
 // Bin 7, partition pass: work item t (partition p of list item i) builds the
 // LDS table of v's neighbour clusters in partition p and appends its entries
 // to v's entry region; the partition holding c(v) records S_{c(v)}.
 constexpr int kHubThreads = 512;
-constexpr int kHubTable = 8192;
 
-template <bool kAsync>
+template <int TABLE, bool kAsync>
 __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateView st,
                                                              MoveParams mp,
                                                              const uint32_t* __restrict__ list,
                                                              int64_t count, HubScratch hb,
                                                              RoundCounters* __restrict__ rc) {
-  __shared__ uint32_t keys[kHubTable];
-  __shared__ double vals[kHubTable];
-  __shared__ uint32_t cnt[kHubTable];
+  __shared__ uint32_t keys[TABLE];
+  __shared__ double vals[TABLE];
+  __shared__ uint32_t cnt[TABLE];
   __shared__ uint32_t sw[kHubThreads / kWave];
   __shared__ uint32_t s_base;
   __shared__ RepairStage<kHubThreads> stg;
@@ -822,7 +1031,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const int64_t start = g.off[v];
   const int64_t deg = g.off[v + 1] - start;
   const int64_t per = (deg + parts - 1) / parts;
-  const uint32_t cap = min((uint32_t)kHubTable, max(64u, NextPow2((uint64_t)(2 * per + 2))));
+  const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * per + 2))));
   const uint32_t mask = cap - 1;
   for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
     keys[s] = kNoneId;
@@ -832,10 +1041,14 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   __syncthreads();
   const bool hub = deg > kHubDegree;
   const int thr = kAsync ? 0 : (int)mp.gthr[v];
-  const bool overflow = Accumulate<kHubThreads, kAsync>(g, st, mp, v, start, deg, hub, thr, keys,
-                                                        vals, cnt, mask, parts, part);
+  const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
+  const bool overflow = AccumulateBuffered<kHubThreads, kAsync>(
+      g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part);
   if (__syncthreads_or(overflow)) {
-    if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
+    if (threadIdx.x == 0) {
+      atomicOr(&hb.hflag[i], 2u);
+      hb.pe_cnt[t] = 0;
+    }
     return;
   }
   const uint32_t cv = st.ids[v];
@@ -846,7 +1059,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads)
       inexact = inexact || SlotInexact(cnt[s]);
     if (__syncthreads_or(inexact)) {
-      T = RepairInexactSlots<kHubThreads>(g, st, mp, v, start, deg, hub, keys, vals, cnt, cap,
+      T = RepairInexactSlots<kHubThreads>(g, mp, src, start, deg, hub, keys, vals, cnt, cap,
                                           parts, part, own, &stg);
       if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
     }
@@ -860,8 +1073,15 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) mine += keys[s] != kNoneId;
   uint32_t total = 0;
   uint32_t at = BlockExclusiveScan(mine, sw, &total);
-  if (total == 0) return;
-  if (threadIdx.x == 0) s_base = atomicAdd(&hb.ecur[i], total);
+  if (total == 0) {
+    if (threadIdx.x == 0) hb.pe_cnt[t] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    s_base = atomicAdd(&hb.ecur[i], total);
+    hb.pe_base[t] = s_base;
+    hb.pe_cnt[t] = total;
+  }
   __syncthreads();
   const int64_t e0 = hb.ent_off[i] + s_base;
   for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
@@ -873,60 +1093,98 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   }
 }
 
-// Bin 7, selection pass: one workgroup per list item reduces its entries to
-// EfficientBestMove's decision and resets the item's counters.
+// Own-cluster term of EfficientBestMove for a bin-7 list item.
+__device__ __forceinline__ double HubChange(const StateView& st, const MoveParams& mp,
+                                            const HubScratch& hb, int64_t i, uint32_t v,
+                                            uint32_t flag) {
+  const uint32_t cv = st.ids[v];
+  const double T = (flag & 1u) ? hb.tval[i] : 0.0;
+  const double k = st.nw[v];
+  double change = 0;
+  change -= NetWeight(T, k * (st.K[cv] - k), ResOf(mp, v));
+  return change;
+}
+
+// Bin 7, candidate pass: one workgroup per partition item reduces that
+// partition's entries to its best candidate (pbest_*[t]).
 constexpr int kSelThreads = 256;
 
 template <bool kAsync>
-__global__ __launch_bounds__(kSelThreads) void k_bm_hub_select(
-    CsrView g, StateView st, MoveParams mp, const uint32_t* __restrict__ list, int64_t count,
-    HubScratch hb, MoveOut out, uint32_t* __restrict__ fb_list, RoundCounters* __restrict__ rc) {
+__global__ __launch_bounds__(kSelThreads) void k_bm_hub_cand(CsrView g, StateView st,
+                                                             MoveParams mp,
+                                                             const uint32_t* __restrict__ list,
+                                                             int64_t count, HubScratch hb) {
   __shared__ double sd[kSelThreads / kWave];
   __shared__ uint32_t sc[kSelThreads / kWave];
-  for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  if (t >= hb.part_off[count]) return;
+  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const uint32_t v = list[i];
+  if (kAsync && !InSubRound(mp, v)) return;
+  if (!kAsync && mp.force_ordered) return;
+  const uint32_t flag = hb.hflag[i];
+  if (flag & 2u) return;
+  const double change = HubChange(st, mp, hb, i, v, flag);
+  const double res = ResOf(mp, v);
+  double mw = 0;
+  mw += st.nw[v];
+  const int64_t e0 = hb.ent_off[i] + hb.pe_base[t];
+  const uint32_t ne = hb.pe_cnt[t];
+  Cand best{0.0, kNoneId};
+  for (uint32_t j = threadIdx.x; j < ne; j += kSelThreads) {
+    const uint32_t c = hb.ekeys[e0 + j];
+    const Cand x{change + NetWeight(hb.evals[e0 + j], mw * st.K[c], res), c};
+    best = PickBetter(best, x);
+  }
+  best = BlockBest<kSelThreads>(best, sd, sc);
+  if (threadIdx.x == 0) {
+    hb.pbest_d[t] = best.d;
+    hb.pbest_c[t] = best.c;
+  }
+}
+
+// Bin 7, decision pass: one wave per list item reduces the partition bests to
+// EfficientBestMove's decision, applies the sync or async epilogue (or lists
+// the vertex for the ordered kernel) and resets the item's counters.
+template <bool kAsync>
+__global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, MoveParams mp,
+                                                      const uint32_t* __restrict__ list,
+                                                      int64_t count, HubScratch hb, MoveOut out,
+                                                      uint32_t* __restrict__ fb_list,
+                                                      RoundCounters* __restrict__ rc) {
+  const int lane = LaneId();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = wave; i < count; i += nwaves) {
     const uint32_t v = list[i];
     const uint32_t flag = hb.hflag[i];
-    const uint32_t ne = hb.ecur[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       hb.hflag[i] = 0;
       hb.ecur[i] = 0;
     }
     if (kAsync && !InSubRound(mp, v)) continue;
     if (!kAsync && (mp.force_ordered || (flag & 2u))) {
-      if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
+      if (lane == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       continue;
     }
     if (kAsync && (flag & 2u)) {  // table overflow: no move this time
-      if (threadIdx.x == 0) out.moved[v] = 0;
+      if (lane == 0) out.moved[v] = 0;
       continue;
     }
-    const int64_t deg = g.off[v + 1] - g.off[v];
-    const bool hub = deg > kHubDegree;
-    const uint32_t cv = st.ids[v];
-    const double T = (flag & 1u) ? hb.tval[i] : 0.0;
-    const double k = st.nw[v];
-    const double res = ResOf(mp, v);
-    double change = 0;
-    change -= NetWeight(T, k * (st.K[cv] - k), res);
-    double mw = 0;
-    mw += k;
-    const int64_t e0 = hb.ent_off[i];
     Cand best{0.0, kNoneId};
-    for (uint32_t j = threadIdx.x; j < ne; j += kSelThreads) {
-      const uint32_t c = hb.ekeys[e0 + j];
-      const Cand x{change + NetWeight(hb.evals[e0 + j], mw * st.K[c], res), c};
-      best = PickBetter(best, x);
-    }
-    best = BlockBest<kSelThreads>(best, sd, sc);
-    if (threadIdx.x == 0) {
-      uint32_t t;
+    for (int64_t t = hb.part_off[i] + lane; t < hb.part_off[i + 1]; t += kWave)
+      best = PickBetter(best, Cand{hb.pbest_d[t], hb.pbest_c[t]});
+    best = GroupBest(best, kWave);
+    if (lane == 0) {
+      const int64_t deg = g.off[v + 1] - g.off[v];
+      const double change = HubChange(st, mp, hb, i, v, flag);
+      uint32_t tt;
       double d;
-      Decide(best, change, hub, mp.new_id, &t, &d);
+      Decide(best, change, deg > kHubDegree, mp.new_id, &tt, &d);
       if (kAsync)
-        AsyncEpilogue(v, t, d, mp, st, out);
+        AsyncEpilogue(v, tt, d, mp, st, out);
       else
-        SyncEpilogue(v, t, d, mp.new_id, st, out);
+        SyncEpilogue(v, tt, d, mp.new_id, st, out);
     }
   }
 }
@@ -1071,6 +1329,36 @@ void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long 
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
+                    uint32_t S, uint32_t* cnt, hipStream_t s) {
+  BinCounts bc;
+  int64_t total = 0;
+  for (int b = 0; b < kBins; ++b) {
+    bc.base[b] = layout.base[b];
+    bc.count[b] = counts[b];
+    total += counts[b];
+  }
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_sub_count, dim3(GridFor(total, (int)kSubSlab, 1024)), dim3(kSubThreads), 0, s,
+                     list, bc, total, S, cnt);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
+                      uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s) {
+  BinCounts bc;
+  int64_t total = 0;
+  for (int b = 0; b < kBins; ++b) {
+    bc.base[b] = layout.base[b];
+    bc.count[b] = counts[b];
+    total += counts[b];
+  }
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_sub_scatter, dim3(GridFor(total, (int)kSubSlab, 1024)), dim3(kSubThreads), 0,
+                     s, list, bc, total, S, cur, out);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s) {
   if (g.n == 0) return;
   const int grid = GridFor(g.n, 256, 8192);
@@ -1175,6 +1463,21 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+// The partition table holds 2 * part_edges slots (load factor <= 1/2), from
+// 2048 to 8192: smaller partitions leave LDS for more workgroups per CU.
+template <bool A>
+static void LaunchHubPart(int64_t items, const CsrView& g, const StateView& st,
+                          const MoveParams& mp, const uint32_t* list, int64_t count,
+                          const HubScratch& hb, RoundCounters* rc, hipStream_t s) {
+  const dim3 grid((unsigned)items), block(kHubThreads);
+  if (hb.part_edges <= 1024)
+    hipLaunchKernelGGL((k_bm_hub_part<2048, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+  else if (hb.part_edges <= 2048)
+    hipLaunchKernelGGL((k_bm_hub_part<4096, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+  else
+    hipLaunchKernelGGL((k_bm_hub_part<8192, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+}
+
 void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
                        const StateView& st, const MoveParams& mp, const uint32_t* list,
                        const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
@@ -1185,17 +1488,23 @@ void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
   if (items > (int64_t)INT32_MAX)
     throw HipError{hipErrorInvalidValue, "LaunchBestMoveHub: too many partitions", __FILE__,
                    __LINE__};
-  const int sel = GridFor(count, 1, 1 << 20);
+  const int fin = GridFor(count, 256 / kWave, 1 << 16);
   if (async) {
-    hipLaunchKernelGGL(k_bm_hub_part<true>, dim3((unsigned)items), dim3(kHubThreads), 0, s, g, st,
-                       mp, list, count, hb, rc);
-    hipLaunchKernelGGL(k_bm_hub_select<true>, dim3(sel), dim3(kSelThreads), 0, s, g, st, mp, list,
-                       count, hb, out, fb_list, rc);
+    hipLaunchKernelGGL(k_hub_gather<true>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
+                       count, hb);
+    LaunchHubPart<true>(items, g, st, mp, list, count, hb, rc, s);
+    hipLaunchKernelGGL(k_bm_hub_cand<true>, dim3((unsigned)items), dim3(kSelThreads), 0, s, g, st,
+                       mp, list, count, hb);
+    hipLaunchKernelGGL(k_bm_hub_final<true>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count, hb,
+                       out, fb_list, rc);
   } else {
-    hipLaunchKernelGGL(k_bm_hub_part<false>, dim3((unsigned)items), dim3(kHubThreads), 0, s, g, st,
-                       mp, list, count, hb, rc);
-    hipLaunchKernelGGL(k_bm_hub_select<false>, dim3(sel), dim3(kSelThreads), 0, s, g, st, mp, list,
-                       count, hb, out, fb_list, rc);
+    hipLaunchKernelGGL(k_hub_gather<false>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
+                       count, hb);
+    LaunchHubPart<false>(items, g, st, mp, list, count, hb, rc, s);
+    hipLaunchKernelGGL(k_bm_hub_cand<false>, dim3((unsigned)items), dim3(kSelThreads), 0, s, g, st,
+                       mp, list, count, hb);
+    hipLaunchKernelGGL(k_bm_hub_final<false>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count,
+                       hb, out, fb_list, rc);
   }
   PCC_HIP_CHECK(hipGetLastError());
 }

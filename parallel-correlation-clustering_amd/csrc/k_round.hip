@@ -286,6 +286,57 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+// Ranges of each graph's ids in the ascending empty-id list: free_cur[g] /
+// free_end[g] = first index with id >= gbase[g] / gbase[g + 1].
+__global__ void k_free_ranges(int G, const int64_t* __restrict__ gbase, uint32_t n,
+                              const uint8_t* __restrict__ z, const uint32_t* __restrict__ zrank,
+                              const uint32_t* __restrict__ zero_list,
+                              unsigned long long* __restrict__ free_cur,
+                              unsigned long long* __restrict__ free_end) {
+  const uint32_t total = n ? zrank[n - 1] + z[n - 1] : 0;
+  for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi <= G; gi += gridDim.x * blockDim.x) {
+    const int64_t key = gi == G ? (int64_t)n : (gbase ? gbase[gi] : (gi == 0 ? 0 : (int64_t)n));
+    uint32_t lo = 0, hi = total;  // first index with zero_list[idx] >= key
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((int64_t)zero_list[mid] < key)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (gi < G) free_cur[gi] = lo;
+    if (gi > 0) free_end[gi - 1] = lo;
+  }
+}
+
+void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
+                    const int64_t* gbase, FreeIds* f, hipStream_t s) {
+  f->ids.Reset(al, n > 0 ? n : 1);
+  f->cur.Reset(al, G);
+  f->end.Reset(al, G);
+  if (n == 0) {
+    PCC_HIP_CHECK(hipMemsetAsync(f->cur.get(), 0, sizeof(unsigned long long) * (size_t)G, s));
+    PCC_HIP_CHECK(hipMemsetAsync(f->end.get(), 0, sizeof(unsigned long long) * (size_t)G, s));
+    return;
+  }
+  DevBuf<uint8_t> z(al, n);
+  DevBuf<uint32_t> zrank(al, n);
+  DevBuf<unsigned long long> one(al, 1);
+  const unsigned long long kOne = 1;
+  PCC_HIP_CHECK(hipMemcpyAsync(one.get(), &kOne, sizeof(kOne), hipMemcpyHostToDevice, s));
+  const int grid = Grid(n, 256, 4096);
+  hipLaunchKernelGGL(k_zero_flags, dim3(grid), dim3(256), 0, s, n, st.sizes, z.get(), one.get());
+  ExclusiveScanU8toU32(al, z.get(), zrank.get(), n, nullptr, s);
+  hipLaunchKernelGGL(k_zero_compact, dim3(grid), dim3(256), 0, s, n, z.get(), zrank.get(),
+                     f->ids.get(), one.get());
+  hipLaunchKernelGGL(k_free_ranges, dim3((G + 256) / 256), dim3(256), 0, s, G, gbase, n, z.get(),
+                     zrank.get(), f->ids.get(), f->cur.get(), f->end.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  // z, zrank and one go back to the caching allocator; the stream orders
+  // their reuse after these kernels.
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
                        const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,

@@ -155,6 +155,11 @@ struct MoveParams {
   uint32_t sub_index = 0;
   // Async: apply the sync singleton rule before moving (0/1).
   int async_singleton_rule = 0;
+  // Async moves to a new cluster: ascending empty ids of the round and, per
+  // graph, the next / end index into them (null: probe only).
+  const uint32_t* free_ids = nullptr;
+  unsigned long long* free_cur = nullptr;
+  const unsigned long long* free_end = nullptr;
   // Per-vertex granularity threshold of the hashed bins (deg > 64): a weight
   // x with GranExp(x) >= gthr[v] cannot make any partial sum of v's
   // per-cluster totals inexact.
@@ -170,7 +175,7 @@ struct MoveOut {
 };
 
 // Default edges per partition of a bin-7 vertex.
-constexpr int64_t kHubPartEdges = 4096;
+constexpr int64_t kHubPartEdges = 2048;
 
 // Bin-7 scratch.  part_off / ent_off: exclusive prefix sums over the bin's
 // list of the partition counts ceil(deg / kHubPartEdges) and of the degrees;
@@ -181,11 +186,18 @@ struct HubScratch {
   int64_t part_edges = kHubPartEdges;
   int64_t* part_off = nullptr;
   int64_t* ent_off = nullptr;
+  uint32_t* cbuf = nullptr;  // neighbour clusters, at ent_off like the entries
   uint32_t* ekeys = nullptr;
   double* evals = nullptr;
   uint32_t* ecur = nullptr;
   uint32_t* hflag = nullptr;
   double* tval = nullptr;
+  // Per partition item t: its entries ent_off[i] + pe_base[t] (pe_cnt[t]
+  // of them) and its best candidate.
+  uint32_t* pe_base = nullptr;
+  uint32_t* pe_cnt = nullptr;
+  double* pbest_d = nullptr;
+  uint32_t* pbest_c = nullptr;
 };
 
 // Scratch for the ordered fallback path.
@@ -208,6 +220,15 @@ void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThres
 // edges of bin b (capacity of each list region).
 void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
                        hipStream_t s);
+
+// Async sub-rounds: cnt[b * S + s] (zeroed by the caller) counts the listed
+// vertices of bin b in sub-round s; with cur[b * S + s] set to the segment
+// starts within each bin region, LaunchSubScatter copies the list into out
+// grouped by (bin, sub-round).  S <= 1024.
+void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
+                    uint32_t S, uint32_t* cnt, hipStream_t s);
+void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
+                      uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s);
 
 // gthr[v] for every vertex with deg > 64 (see MoveParams::gthr).
 void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s);
@@ -251,6 +272,14 @@ void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uin
 void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
                          uint8_t* modified, RoundCounters* rc, hipStream_t s);
+
+// Empty cluster ids at the start of an async round (MoveParams::free_*).
+struct FreeIds {
+  DevBuf<uint32_t> ids;
+  DevBuf<unsigned long long> cur, end;
+};
+void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
+                    const int64_t* gbase, FreeIds* f, hipStream_t s);
 
 // flags[graph of v] = 1 for every v with next[v] != 0.
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
