@@ -30,7 +30,7 @@ BIN_NAMES = ["deg<=4 (4 lanes)", "deg<=8 (8 lanes)", "deg<=16 (16 lanes)", "deg<
 BIN_KERNELS = ["k_bm_small<4>", "k_bm_small<8>", "k_bm_small<16>", "k_bm_small<32>",
                "k_bm_small<64>", "k_bm_hash<64,256> + k_bm_hash<64,1024>",
                "k_bm_hash<128,2048> + k_bm_hash<256,4096> + k_bm_hash<512,8192>",
-               "k_hub_scan + k_bm_hub_part + k_bm_hub_select"]
+               "k_hub_scan + k_hub_gather + k_bm_hub_part + k_bm_hub_cand + k_bm_hub_final"]
 
 
 def apply_tuning(g, args, pcc_amd):
@@ -58,7 +58,7 @@ def main() -> int:
                     help="host threads for the CPU baseline (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="rmat22",
-                    choices=["rmat22", "batched", "modularity-async", "serving"],
+                    choices=["rmat22", "batched", "modularity-async", "serving", "copy"],
                     help="rmat22: the headline line; the others are secondary measurements")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of SBM graphs")
     ap.add_argument("--batch-edges", type=int, default=1 << 30,
@@ -87,6 +87,8 @@ def main() -> int:
         return bench_modularity_async(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "copy":
+        return bench_copy(args)
 
     import numpy as np
     import pcc_amd
@@ -181,12 +183,26 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args, dom),
+            "algorithmic_bytes_per_launch": round(dom_bytes / max(bin_l[dom], 1)),
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def measured_traffic(args, dom):
+    """HBM bytes per launch of the dominant kernel group, from the calibrated
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/ for the default
+    workload and tuning (null otherwise)."""
+    default = (args.scale == 22 and args.edge_factor == 16.0 and not args.wave_max
+               and not args.block_max and not args.hub_part_edges)
+    path = os.path.join(ROOT, "profiles", "traffic_rmat22.json")
+    if not default or dom != 7 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_round"]
 
 
 def c4_graphs(pcc_amd, count, seed=4):
@@ -383,6 +399,30 @@ def bench_serving(args):
         "batches": st.batches, "mean_batch_graphs": round(st.mean_batch_graphs, 2),
         "max_graphs_in_flight": st.max_graphs_in_flight, "failed": st.failed,
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_copy(args):
+    """Plain device-to-device copy of a known size: the calibration of the
+    FETCH_SIZE / WRITE_SIZE counters used for roofline.traffic."""
+    import torch
+
+    n = 1 << 28  # 1 GiB of float32
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    line = {"metric": "device copy bandwidth (counter calibration)", "value": round(8 * n / dt / 1e9, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"copy of {4 * n} bytes (read {4 * n} + write {4 * n} per step)"},
+            "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n}
     print(json.dumps(line), flush=True)
     return 0
 
