@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
   const int wlane = LaneId();
   const int gbase_lane = wlane & ~(G - 1);
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase_lane);
-  for (int64_t gb = (int64_t)blockIdx.x * kGroups; gb < count;
+  for (int64_t gb = XcdItem(blockIdx.x, gridDim.x) * kGroups; gb < count;
        gb += (int64_t)gridDim.x * kGroups) {
     const int64_t gi = gb + threadIdx.x / G;
     const uint32_t v = gi < count ? list[PermuteIndex(gi, count, salt)] : 0u;
@@ -892,7 +892,9 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
   __shared__ double sd[NW];
   __shared__ uint32_t sc[NW];
   __shared__ RepairStage<THREADS> stg;
-  for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+  // Consecutive list items (nearby vertex ids, which share neighbourhoods in
+  // RMAT-like graphs) go to workgroups of one XCD.
+  for (int64_t i = XcdItem(blockIdx.x, gridDim.x); i < count; i += gridDim.x) {
     const uint32_t v = list[PermuteIndex(i, count, salt)];
     if (kAsync && !InSubRound(mp, v)) continue;  // uniform across the workgroup
     const int64_t start = g.off[v];
@@ -1412,16 +1414,18 @@ static void LaunchHashBins(int bin, int64_t count, const BinThresholds& th, cons
                            uint64_t salt, hipStream_t s) {
   if (bin == 5) {
     const int64_t lo = 64, hi = th.wave_max;
-    const int64_t cut = std::min<int64_t>(127, hi);
-    LaunchHash<64, 256, A>(count, lo, cut, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<64, 1024, A>(count, cut, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    const int64_t c1 = std::min<int64_t>(127, hi);
+    const int64_t c2 = std::max(c1, std::min<int64_t>(255, hi));
+    LaunchHash<64, 256, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<64, 512, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<128, 1024, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
   } else {
     const int64_t lo = th.wave_max, hi = th.block_max;
     const int64_t c1 = std::max(lo, std::min<int64_t>(1023, hi));
     const int64_t c2 = std::max(c1, std::min<int64_t>(2047, hi));
-    LaunchHash<128, 2048, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<256, 4096, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<512, 8192, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<256, 2048, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<512, 4096, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
+    LaunchHash<1024, 8192, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
   }
 }
 

@@ -8,10 +8,12 @@ RMAT graphs.  The contract is therefore statistical, against the oracle's
 async runs over eight seeded processing orders (permute seeds 1..8):
 
 * on graphs whose order-to-order spread is small (std <= 0.25%), the GPU
-  objective lies within 0.5% of the oracle's mean;
+  objective is at most 0.5% below the oracle's mean;
 * on the noisier small graphs, it is no lower than the oracle's worst order
-  minus 0.5% (a result better than every sequential order is accepted: it is
-  not a loss of quality).
+  minus 0.5%.
+A result above the sequential runs is accepted (it is not a loss of quality);
+the objective the GPU reports is checked separately against the oracle's own
+evaluation of the GPU's cluster ids, so an inflated value cannot pass.
 """
 
 import numpy as np
@@ -72,6 +74,12 @@ def _run(pcc, orc, spec, kw, modularity):
         ref.append(_value(orc.cluster(g, orc.config_from(ocfg), modularity), modularity))
     ref = np.array(ref)
     gpu = _value(st, modularity)
+    if modularity:
+        again = orc.modularity(g, ids, kw["resolution"])
+    else:
+        again = orc.objective(g, ids, np.ones(csr.n), kw["resolution"],
+                              kw.get("edge_weight_offset", 0.0))
+    assert gpu == pytest.approx(again, rel=1e-9, abs=1e-9)
     print(f"  gpu {gpu:.6f} clusters {st.num_clusters}; oracle mean {ref.mean():.6f} "
           f"std {ref.std() / abs(ref.mean()) * 100:.3f}% range [{ref.min():.6f}, {ref.max():.6f}]",
           flush=True)
@@ -83,7 +91,7 @@ def test_async_within_half_percent_of_oracle_mean(pcc, orc, case):
     spec, kw, modularity = STRICT_CASES[case]
     gpu, ref = _run(pcc, orc, spec, kw, modularity)
     mean = ref.mean()
-    assert abs(gpu - mean) <= 0.005 * abs(mean)
+    assert gpu >= mean - 0.005 * abs(mean)
 
 
 @pytest.mark.parametrize("case", range(len(RANGE_CASES)))
