@@ -36,13 +36,13 @@ __device__ __forceinline__ int GranExp(double x) {
   return e - 1075 + __builtin_ctzll(mant | (1ull << 52));
 }
 
-// Order-free summation test of the hashed bins.  Let B bound the sum of the
-// magnitudes of every value a vertex adds into its per-cluster totals.  A
-// value x with GranExp(x) >= ThresholdFor(B) is a multiple of 2^thr, and every
-// partial sum of such values is a multiple of 2^thr of magnitude below
-// 2^(thr + 51), hence exactly representable: such totals come out the same in
-// every summation order.  A total with a finer value is order-free only if it
-// has at most two terms (floating-point addition commutes).
+// Order-free summation test of the hashed bins.  Per vertex: B bounds the
+// sum and M the largest of the magnitudes of the values the vertex adds into
+// its per-cluster totals; thr = ThresholdFor(B) and every |value| < 2^mexp.
+// A value x is "fine" if GranExp(x) < thr.  A per-cluster total whose values
+// are all multiples of 2^g, and whose partial sums stay below 2^(g + 50) in
+// magnitude, is exact in every summation order; so is any total of at most two
+// values (floating-point addition commutes).
 __device__ __forceinline__ int ThresholdFor(double b) {
   if (!(b < INFINITY)) return 32767;  // NaN / inf: no value counts as coarse
   if (b == 0.0) return -32768;
@@ -50,12 +50,42 @@ __device__ __forceinline__ int ThresholdFor(double b) {
   (void)frexp(b * (1.0 + 0x1p-20), &e);  // b' = f * 2^e, f in [0.5, 1)
   return (e - 1) - 50;                    // floor(log2 b') - 50
 }
+__device__ __forceinline__ int MagExpFor(double m) {
+  if (!(m < INFINITY)) return 32767;
+  if (m == 0.0) return -32768;
+  int e = 0;
+  (void)frexp(m * (1.0 + 0x1p-20), &e);  // m' < 2^e
+  return e;
+}
+__device__ __forceinline__ int32_t PackScale(int thr, int mexp) {
+  return (int32_t)(((uint32_t)(thr & 0xFFFF) << 16) | (uint32_t)(mexp & 0xFFFF));
+}
+__device__ __forceinline__ int ScaleThr(int32_t s) { return (int16_t)((uint32_t)s >> 16); }
+__device__ __forceinline__ int ScaleMexp(int32_t s) { return (int16_t)((uint32_t)s & 0xFFFF); }
 
-// Per-slot term counter of the hashed bins: bits 0..30 count the terms,
-// bit 31 marks a term finer than the vertex's threshold.
-constexpr uint32_t kFineBit = 0x80000000u;
-__device__ __forceinline__ bool SlotInexact(uint32_t cnt) {
-  return (cnt & kFineBit) != 0 && (cnt & ~kFineBit) >= 3;
+// Per-slot term record of the hashed bins: bits 0..23 count the terms
+// (capped well below 2^24), bit 24 + i is set when a term has
+// GranExp < thr - 8 i (i < 8).
+constexpr uint32_t kCountMask = 0xFFFFFFu;
+constexpr uint32_t kCountCap = 0xFFF000u;
+__device__ __forceinline__ uint32_t FineBits(int ge, int thr) {
+  if (ge >= thr) return 0u;
+  int j = (thr - 1 - ge) >> 3;
+  if (j > 7) j = 7;
+  return ((2u << j) - 1u) << 24;
+}
+// True when the slot's sum may depend on the summation order: at least three
+// terms, a fine one among them, and partial sums (below count * 2^mexp) that
+// may exceed 2^(g + 50) for the granularity 2^g the fine bits guarantee.
+__device__ __forceinline__ bool SlotInexact(uint32_t cnt, int thr, int mexp) {
+  const uint32_t bits = cnt >> 24, count = cnt & kCountMask;
+  if (bits == 0 || count <= 2) return false;
+  const int j = 31 - __clz((int)bits);
+  if (j >= 7) return true;
+  const int e = thr - 8 * (j + 1) + 50 - mexp;
+  if (e >= 31) return false;
+  if (e <= 0) return true;
+  return count >= (1u << e);
 }
 
 __device__ __forceinline__ uint32_t HashId(uint32_t c) {
@@ -85,12 +115,24 @@ __device__ __forceinline__ uint32_t PartOf(uint32_t c, uint32_t parts) {
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip.  This
 // bijection of [0, grid) gives each XCD a contiguous range of work items, so
-// consecutive items (the partitions of one vertex) share an L2.
+// consecutive items share an L2 (for items of similar cost).
 __device__ __forceinline__ int64_t XcdItem(int64_t block, int64_t grid) {
   constexpr int64_t kXcds = 8;
   const int64_t x = block % kXcds, j = block / kXcds;
   const int64_t q = grid / kXcds, r = grid % kXcds;
   return x * q + (x < r ? x : r) + j;
+}
+
+// Items of very different cost: chunks of kXcdChunk consecutive items go to
+// the XCDs in turn, so a run of expensive items (the partitions of one large
+// vertex) is spread over every XCD while each chunk still shares an L2.  A
+// bijection of [0, grid) when grid is a multiple of kXcdGrid.
+constexpr int64_t kXcdChunk = 16;
+constexpr int64_t kXcdGrid = 8 * kXcdChunk;
+__device__ __forceinline__ int64_t XcdChunkItem(int64_t block) {
+  constexpr int64_t kXcds = 8;
+  const int64_t x = block % kXcds, j = block / kXcds;
+  return (x + kXcds * (j / kXcdChunk)) * kXcdChunk + (j % kXcdChunk);
 }
 
 // Selection order of EfficientBestMove among candidate clusters
@@ -152,21 +194,6 @@ __device__ __forceinline__ T BlockReduce(T x, T* smem, Op op) {
   for (int i = 1; i < nw; ++i) r = op(r, smem[i]);
   __syncthreads();
   return r;
-}
-
-// List item of a bin-7 partition work item t: i with part_off[i] <= t <
-// part_off[i + 1].
-__device__ __forceinline__ int64_t HubItemOf(const int64_t* __restrict__ part_off, int64_t count,
-                                             int64_t t) {
-  int64_t lo = 0, hi = count;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (part_off[mid] <= t)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
 }
 
 // Exclusive block scan; `smem` needs blockDim.x / 64 slots.  *total receives

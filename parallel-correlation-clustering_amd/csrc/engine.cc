@@ -359,8 +359,8 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   c->hs.keys = c->hkeys.get();
   c->hs.vals = c->hvals.get();
   if (hashed) {
-    c->gthr.Reset(al_, g.n);
-    LaunchGranThreshold(g, cfg_.edge_weight_offset, c->gthr.get(), s_);
+    c->gscale.Reset(al_, g.n);
+    LaunchGranThreshold(g, cfg_.edge_weight_offset, c->gscale.get(), s_);
   }
   c->hb = HubScratch();
   const int64_t c7 = c->caps[kBins - 1], t7 = c->caps_te[kBins - 1];
@@ -379,6 +379,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb_pe_cnt.Reset(al_, items);
     c->hb_pbest_c.Reset(al_, items);
     c->hb_pbest_d.Reset(al_, items);
+    c->hb_item_of.Reset(al_, items);
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_ecur.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_flag.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     c->hb.part_edges = h_->hub_part_edges;
@@ -394,15 +395,17 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.pe_cnt = c->hb_pe_cnt.get();
     c->hb.pbest_c = c->hb_pbest_c.get();
     c->hb.pbest_d = c->hb_pbest_d.get();
+    c->hb.item_of = c->hb_item_of.get();
     if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
       c->hubs.Reset(al_, c7);
       c->hubs_part.Reset(al_, c7 + 1);
       c->hubs_ent.Reset(al_, c7 + 1);
+      c->hubs_item_of.Reset(al_, items);
       DevBuf<unsigned long long> cnt(al_, 1);
       PCC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s_));
       LaunchCollectHubs(g, h_->th, c->hubs.get(), cnt.get(), s_);
       LaunchHubScan(g, c->hubs.get(), c7, h_->hub_part_edges, c->hubs_part.get(),
-                    c->hubs_ent.get(), s_);
+                    c->hubs_ent.get(), c->hubs_item_of.get(), s_);
       c->hub_count = c7;
       c->hub_te = t7;
     }
@@ -413,7 +416,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   mp.offset = cfg_.edge_weight_offset;
   mp.n = (uint32_t)g.n;
   mp.force_ordered = h_->force_ordered ? 1 : 0;
-  mp.gthr = c->gthr.get();
+  mp.gscale = c->gscale.get();
   if (lay.batch) {
     // Batches use a sentinel outside every graph's id range.
     mp.new_id = 0xFFFFFFFEu;
@@ -447,7 +450,8 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
   if (c7 > 0) {
     const uint32_t* list7 = lists[kBins - 1];
     Timer tb = StartTimer();
-    if (scan_hubs) LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, s_);
+    if (scan_hubs)
+      LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
     LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_);
     StopTimer(tb, kBins - 1);
@@ -545,7 +549,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     PCC_HIP_CHECK(hipStreamSynchronize(s_));  // hoff / hcnt stay valid until here
     if (counts[kBins - 1] > 0)
       LaunchHubScan(g, ws.list.get() + c.layout.base[kBins - 1], counts[kBins - 1],
-                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, s_);
+                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
   }
   if (!async && counts[5] + counts[6] + counts[7] > 0) {
     Timer tf = StartTimer();
@@ -575,10 +579,11 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   PCC_HIP_CHECK(hipMemsetAsync(ws.next.get(), 0, (size_t)n, s_));
   if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
     LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.hubs.get(), c.hub_count,
-                         c.hub_te, c.hubs_part.get(), h_->hub_part_edges, ws.next.get(), s_);
+                         c.hub_te, c.hubs_part.get(), c.hubs_item_of.get(), h_->hub_part_edges,
+                         ws.next.get(), s_);
   } else {
-    LaunchExpandMoved(counts, te, c.layout, ws.list.get(), c.hb.part_off, c.hb.part_edges, g,
-                      ws.moved.get(), ws.next.get(), s_);
+    LaunchExpandMoved(counts, te, c.layout, ws.list.get(), c.hb.part_off, c.hb.item_of,
+                      c.hb.part_edges, g, ws.moved.get(), ws.next.get(), s_);
   }
   LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
   ResetCounters(ws.rc.get(), false);

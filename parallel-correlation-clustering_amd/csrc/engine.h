@@ -120,7 +120,7 @@ class Engine {
     BinLayout layout;
     int64_t caps[kBins];     // vertices of each bin on this level
     int64_t caps_te[kBins];  // edges of each bin on this level
-    DevBuf<int16_t> gthr;    // MoveParams::gthr
+    DevBuf<int32_t> gscale;  // MoveParams::gscale
     // Ordered fallback scratch.
     HashScratch hs;
     DevBuf<uint32_t> hkeys;
@@ -128,11 +128,12 @@ class Engine {
     // Bin-7 scratch, sized for every bin-7 vertex of the level.
     HubScratch hb;
     DevBuf<int64_t> hb_part, hb_ent;
-    DevBuf<uint32_t> hb_cbuf, hb_ekeys, hb_ecur, hb_flag, hb_pe_base, hb_pe_cnt, hb_pbest_c;
+    DevBuf<uint32_t> hb_cbuf, hb_ekeys, hb_ecur, hb_flag, hb_pe_base, hb_pe_cnt, hb_pbest_c,
+        hb_item_of;
     DevBuf<double> hb_evals, hb_tval, hb_pbest_d;
     // Every bin-7 vertex with its partition prefix (NBHR_CLUSTER_MOVE
     // expansion).
-    DevBuf<uint32_t> hubs;
+    DevBuf<uint32_t> hubs, hubs_item_of;
     DevBuf<int64_t> hubs_part, hubs_ent;
     int64_t hub_count = 0, hub_te = 0;
   };

@@ -260,11 +260,12 @@ __global__ __launch_bounds__(kSubThreads) void k_sub_scatter(const uint32_t* __r
   }
 }
 
-// gthr[v] = ThresholdFor(sum over v's edges of |w| + |offset|) for deg > 64;
-// one wave per such vertex.
+// gscale[v] = PackScale(ThresholdFor(B), MagExpFor(M)) for deg > 64, with B
+// and M the sum and the maximum over v's edges of |w| + |offset|; one wave
+// per such vertex.
 __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t* __restrict__ off,
                                                         const float* __restrict__ w, double offset,
-                                                        int16_t* __restrict__ gthr) {
+                                                        int32_t* __restrict__ gscale) {
   const double ao = fabs(offset);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -281,17 +282,27 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
       big &= big - 1;
       const int64_t vs = __shfl(st, l);
       const int64_t vd = __shfl(deg, l);
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, mx = 0;
       int64_t p = LaneId();
       for (; p + 3 * kWave < vd; p += 4 * kWave) {
-        s0 += fabs((double)w[vs + p]) + ao;
-        s1 += fabs((double)w[vs + p + kWave]) + ao;
-        s2 += fabs((double)w[vs + p + 2 * kWave]) + ao;
-        s3 += fabs((double)w[vs + p + 3 * kWave]) + ao;
+        const double a0 = fabs((double)w[vs + p]) + ao;
+        const double a1 = fabs((double)w[vs + p + kWave]) + ao;
+        const double a2 = fabs((double)w[vs + p + 2 * kWave]) + ao;
+        const double a3 = fabs((double)w[vs + p + 3 * kWave]) + ao;
+        s0 += a0;
+        s1 += a1;
+        s2 += a2;
+        s3 += a3;
+        mx = fmax(mx, fmax(fmax(a0, a1), fmax(a2, a3)));
       }
-      for (; p < vd; p += kWave) s0 += fabs((double)w[vs + p]) + ao;
+      for (; p < vd; p += kWave) {
+        const double a = fabs((double)w[vs + p]) + ao;
+        s0 += a;
+        mx = fmax(mx, a);
+      }
       const double s = WaveSum((s0 + s1) + (s2 + s3));
-      if (LaneId() == 0) gthr[wave_v0 + l] = (int16_t)ThresholdFor(s);
+      for (int sh = 32; sh > 0; sh >>= 1) mx = fmax(mx, __shfl_xor(mx, sh));
+      if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(ThresholdFor(s), MagExpFor(mx));
     }
   }
 }
@@ -302,7 +313,8 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            const uint32_t* __restrict__ list,
                                                            int64_t count, int64_t part_edges,
                                                            int64_t* __restrict__ part_off,
-                                                           int64_t* __restrict__ ent_off) {
+                                                           int64_t* __restrict__ ent_off,
+                                                           uint32_t* __restrict__ item_of) {
   __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
   int64_t carry_p = 0, carry_e = 0;
   for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
@@ -319,6 +331,7 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
     if (i < count) {
       part_off[i] = carry_p + xp;
       ent_off[i] = carry_e + xe;
+      for (int64_t q = 0; q < parts; ++q) item_of[carry_p + xp + q] = (uint32_t)i;
     }
     carry_p += tp;
     carry_e += tdeg;
@@ -653,13 +666,13 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
                                                   Src src, int64_t start, int64_t deg, bool hub,
                                                   const uint32_t* keys, double* vals,
                                                   const uint32_t* cnt, uint32_t cap,
-                                                  uint32_t parts, uint32_t part, int own,
-                                                  RepairStage<THREADS>* stg) {
+                                                  uint32_t parts, uint32_t part, int own, int thr,
+                                                  int mexp, RepairStage<THREADS>* stg) {
   const uint32_t mask = cap - 1;
   for (uint32_t s = threadIdx.x; s < cap; s += THREADS)
-    if (SlotInexact(cnt[s])) vals[s] = 0.0;
+    if (SlotInexact(cnt[s], thr, mexp)) vals[s] = 0.0;
   __syncthreads();
-  const bool own_ordered = hub && own >= 0 && SlotInexact(cnt[own]);
+  const bool own_ordered = hub && own >= 0 && SlotInexact(cnt[own], thr, mexp);
   const int64_t bs = PbbsBlock(deg);
   const int lane = LaneId(), w = threadIdx.x / kWave;
   // Fold state of thread 0.
@@ -675,7 +688,7 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
       if (c != kNoneId) {
         if (parts == 1 || PartOf(c, parts) == part) {
           const int s = HashFind(keys, mask, c);
-          if (s >= 0 && SlotInexact(cnt[s])) {
+          if (s >= 0 && SlotInexact(cnt[s], thr, mexp)) {
             slot = s;
             x = EdgeValue(g.w[start + q], hub, mp.offset);
           }
@@ -733,6 +746,13 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
   }
   __syncthreads();
   return stg->T;
+}
+
+// Counts a term of a slot (capped) and records how fine it is.
+__device__ __forceinline__ void RecordTerm(uint32_t* cnt, int slot, double x, int thr) {
+  if ((cnt[slot] & kCountMask) < kCountCap) atomicAdd(&cnt[slot], 1u);
+  const uint32_t fb = FineBits(GranExp(x), thr);
+  if (fb) atomicOr(&cnt[slot], fb);
 }
 
 // Where the hashed kernels take the cluster of v's q-th neighbour from:
@@ -793,10 +813,7 @@ __device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& m
         continue;
       }
       atomicAdd(&vals[slot], x);
-      if (!kAsync) {
-        atomicAdd(&cnt[slot], 1u);
-        if (GranExp(x) < thr) atomicOr(&cnt[slot], kFineBit);
-      }
+      if (!kAsync) RecordTerm(cnt, slot, x, thr);
     }
   }
   return overflow;
@@ -820,10 +837,7 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
       return;
     }
     atomicAdd(&vals[slot], x);
-    if (!kAsync) {
-      atomicAdd(&cnt[slot], 1u);
-      if (GranExp(x) < thr) atomicOr(&cnt[slot], kFineBit);
-    }
+    if (!kAsync) RecordTerm(cnt, slot, x, thr);
   };
   auto mine = [&](uint32_t c) {
     return c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
@@ -916,7 +930,8 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     __syncthreads();
     const uint32_t cv = st.ids[v];
     const bool hub = deg > kHubDegree;
-    const int thr = kAsync ? 0 : (int)mp.gthr[v];
+    const int32_t scale = kAsync ? 0 : mp.gscale[v];
+    const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
     const AdjacencyClusters src{g.nbr + start, st.ids, v};
     const bool overflow = Accumulate<THREADS, kAsync>(g, mp, src, start, deg, hub, thr, keys, vals,
                                                       cnt, mask, 1, 0);
@@ -933,10 +948,11 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     double T = own >= 0 ? vals[own] : 0.0;
     if (!kAsync) {
       bool inexact = false;
-      for (uint32_t s = threadIdx.x; s < cap; s += THREADS) inexact = inexact || SlotInexact(cnt[s]);
+      for (uint32_t s = threadIdx.x; s < cap; s += THREADS)
+        inexact = inexact || SlotInexact(cnt[s], thr, mexp);
       if (__syncthreads_or(inexact)) {
         T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
-                                        own, &stg);
+                                        own, thr, mexp, &stg);
         if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
       }
     }
@@ -975,9 +991,9 @@ template <bool kAsync>
 __global__ __launch_bounds__(256) void k_hub_gather(CsrView g, StateView st, MoveParams mp,
                                                     const uint32_t* __restrict__ list,
                                                     int64_t count, HubScratch hb) {
-  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const int64_t i = hb.item_of[t];
   const uint32_t v = list[i];
   if (kAsync && !InSubRound(mp, v)) return;
   if (!kAsync && mp.force_ordered) return;
@@ -1022,9 +1038,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   __shared__ uint32_t sw[kHubThreads / kWave];
   __shared__ uint32_t s_base;
   __shared__ RepairStage<kHubThreads> stg;
-  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const int64_t i = hb.item_of[t];
   const uint32_t v = list[i];
   if (kAsync && !InSubRound(mp, v)) return;
   if (!kAsync && mp.force_ordered) return;
@@ -1042,7 +1058,8 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   }
   __syncthreads();
   const bool hub = deg > kHubDegree;
-  const int thr = kAsync ? 0 : (int)mp.gthr[v];
+  const int32_t scale = kAsync ? 0 : mp.gscale[v];
+    const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
   const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
   const bool overflow = AccumulateBuffered<kHubThreads, kAsync>(
       g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part);
@@ -1059,10 +1076,10 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   if (!kAsync) {
     bool inexact = false;
     for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads)
-      inexact = inexact || SlotInexact(cnt[s]);
+      inexact = inexact || SlotInexact(cnt[s], thr, mexp);
     if (__syncthreads_or(inexact)) {
       T = RepairInexactSlots<kHubThreads>(g, mp, src, start, deg, hub, keys, vals, cnt, cap,
-                                          parts, part, own, &stg);
+                                          parts, part, own, thr, mexp, &stg);
       if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
     }
   }
@@ -1118,9 +1135,9 @@ __global__ __launch_bounds__(kSelThreads) void k_bm_hub_cand(CsrView g, StateVie
                                                              int64_t count, HubScratch hb) {
   __shared__ double sd[kSelThreads / kWave];
   __shared__ uint32_t sc[kSelThreads / kWave];
-  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const int64_t i = HubItemOf(hb.part_off, count, t);
+  const int64_t i = hb.item_of[t];
   const uint32_t v = list[i];
   if (kAsync && !InSubRound(mp, v)) return;
   if (!kAsync && mp.force_ordered) return;
@@ -1361,17 +1378,18 @@ void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s) {
+void LaunchGranThreshold(const CsrView& g, double offset, int32_t* gscale, hipStream_t s) {
   if (g.n == 0) return;
   const int grid = GridFor(g.n, 256, 8192);
-  hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset, gthr);
+  hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset,
+                     gscale);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
-                   int64_t* part_off, int64_t* ent_off, hipStream_t s) {
+                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s) {
   hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
-                     part_edges, part_off, ent_off);
+                     part_edges, part_off, ent_off, item_of);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -1487,8 +1505,10 @@ void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
                        const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
                        RoundCounters* rc, hipStream_t s) {
   if (count <= 0) return;
-  // sum_i ceil(deg_i / E) < te / E + count bounds the partition items.
-  const int64_t items = count + te / hb.part_edges + 1;
+  // sum_i ceil(deg_i / E) < te / E + count bounds the partition items; the
+  // grid is a whole number of XCD chunks (XcdChunkItem).
+  int64_t items = count + te / hb.part_edges + 1;
+  items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
   if (items > (int64_t)INT32_MAX)
     throw HipError{hipErrorInvalidValue, "LaunchBestMoveHub: too many partitions", __FILE__,
                    __LINE__};

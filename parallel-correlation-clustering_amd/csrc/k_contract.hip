@@ -487,14 +487,44 @@ __global__ __launch_bounds__(256) void k_cluster_weights(int64_t n, const uint32
 
 // Per-vertex terms of ComputeObjective (PCCI.cc:124-145): the intra-cluster
 // half weights folded in pbbs order, and k_v (K_c(v) - k_v).
-// Per-vertex intra-cluster edge weight (halved) in pbbslib's reduce order:
-// a left fold for degrees up to one reduce block, blocked otherwise
-// (PCCI.cc:128-138).  Degrees above 1024 are handled by the whole wave, one
-// lane per reduce block.
+// Per-vertex intra-cluster edge weight (halved) in pbbslib's reduce order
+// (PCCI.cc:128-138): a left fold for degrees up to one reduce block of
+// max(1024, 4 ceil(sqrt(deg))) terms, block sums folded left otherwise.
+// Degrees up to 16 are folded by the vertex's own thread; larger ones by the
+// whole wave, which loads kObjChunk consecutive terms at once and folds them in
+// order through shuffles.
 __device__ __forceinline__ double ObjTerm(const CsrView& g, const uint32_t* __restrict__ ids,
                                           uint32_t cv, double offset, int64_t p) {
   const float w = g.w[p];
   return ids[g.nbr[p]] == cv ? (w - offset) / 2 : 0.0;
+}
+
+constexpr int kObjChunk = 4 * kWave;
+
+// Ordered left fold of terms [s0, en) of vertex (vb, vc) onto acc (have: acc
+// holds a value); wave-uniform.
+__device__ __forceinline__ double ObjFold(const CsrView& g, const uint32_t* __restrict__ ids,
+                                          uint32_t vc, double offset, int64_t vb, int64_t s0,
+                                          int64_t en, double acc, bool have) {
+  const int lane = LaneId();
+  for (int64_t c0 = s0; c0 < en; c0 += kObjChunk) {
+    double x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t p = c0 + k * kWave + lane;
+      x[k] = p < en ? ObjTerm(g, ids, vc, offset, vb + p) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t nk = min((int64_t)kWave, en - c0 - k * kWave);
+      for (int j = 0; j < nk; ++j) {
+        const double y = __shfl(x[k], j);
+        acc = have ? acc + y : y;
+        have = true;
+      }
+    }
+  }
+  return acc;
 }
 
 __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32_t* __restrict__ ids,
@@ -514,7 +544,7 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
       b = g.off[v];
       deg = g.off[v + 1] - b;
       r[v] = nw[v] * (K[cv] - nw[v]);
-      if (deg <= 1024) {
+      if (deg <= 16) {
         double tot = 0.0;
         for (int64_t p = 0; p < deg; ++p) {
           const double x = ObjTerm(g, ids, cv, offset, b + p);
@@ -523,31 +553,46 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
         e[v] = tot;
       }
     }
-    uint64_t big = __ballot(in && deg > 1024);
+    uint64_t big = __ballot(in && deg > 16);
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
       const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
       const uint32_t vc = __shfl(cv, l);
-      int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
-      if (bs < 1024) bs = 1024;
-      const int64_t nblk = (vd + bs - 1) / bs;
-      double tot = 0.0;
-      bool have = false;
-      for (int64_t g0 = 0; g0 < nblk; g0 += kWave) {
-        const int64_t blk = g0 + lane;
-        double x = 0.0;
-        if (blk < nblk) {
-          const int64_t s0 = blk * bs;
-          const int64_t en = min(s0 + bs, vd);
-          x = ObjTerm(g, ids, vc, offset, vb + s0);
-          for (int64_t p = s0 + 1; p < en; ++p) x += ObjTerm(g, ids, vc, offset, vb + p);
+      // First an any-order sum with the order-free test: at most two nonzero
+      // terms, or all terms multiples of 2^gmin with sum of magnitudes below
+      // 2^(gmin + 52), make every partial sum exact, so the fold order of the
+      // reference cannot change the result.
+      double sum = 0.0, sabs = 0.0;
+      int gmin = 1 << 20, nz = 0;
+      for (int64_t c0 = 0; c0 < vd; c0 += kObjChunk) {
+        double x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t p = c0 + k * kWave + lane;
+          x[k] = p < vd ? ObjTerm(g, ids, vc, offset, vb + p) : 0.0;
         }
-        const int64_t nb = min((int64_t)kWave, nblk - g0);
-        for (int j = 0; j < nb; ++j) {
-          const double y = __shfl(x, j);
-          tot = have ? tot + y : y;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          sum += x[k];
+          sabs += fabs(x[k]);
+          gmin = min(gmin, GranExp(x[k]));
+          nz += x[k] != 0.0;
+        }
+      }
+      sum = WaveSum(sum);
+      sabs = WaveSum(sabs);
+      gmin = WaveMinInt(gmin);
+      for (int sh = 32; sh > 0; sh >>= 1) nz += __shfl_xor(nz, sh);
+      double tot = sum;
+      if (nz > 2 && !(gmin >= (1 << 19) || sabs < ldexp(1.0, gmin + 52))) {
+        int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
+        if (bs < 1024) bs = 1024;
+        bool have = false;
+        for (int64_t s0 = 0; s0 < vd; s0 += bs) {
+          const double blk = ObjFold(g, ids, vc, offset, vb, s0, min(s0 + bs, vd), 0.0, false);
+          tot = have ? tot + blk : blk;
           have = true;
         }
       }

@@ -202,15 +202,16 @@ __global__ __launch_bounds__(256) void k_expand_clusters(int64_t n, const int64_
 // (ids null) or modified[ids[v]].
 __global__ __launch_bounds__(256) void k_expand_hub(const uint32_t* __restrict__ list, int64_t count,
                                                     const int64_t* __restrict__ part_off,
+                                                    const uint32_t* __restrict__ item_of,
                                                     const int64_t* __restrict__ off,
                                                     const uint32_t* __restrict__ nbr,
                                                     const uint8_t* __restrict__ moved,
                                                     const uint32_t* __restrict__ ids,
                                                     const uint8_t* __restrict__ modified,
                                                     uint8_t* __restrict__ next) {
-  const int64_t t = XcdItem(blockIdx.x, gridDim.x);
+  const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= part_off[count]) return;
-  const int64_t i = HubItemOf(part_off, count, t);
+  const int64_t i = item_of[t];
   const uint32_t v = list[i];
   if (ids ? modified[ids[v]] == 0 : moved[v] == 0) return;
   const int64_t parts = part_off[i + 1] - part_off[i];
@@ -339,8 +340,9 @@ void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
 
 void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
-                       const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,
-                       const uint8_t* moved, uint8_t* next, hipStream_t s) {
+                       const int64_t* hub_part_off, const uint32_t* hub_item_of,
+                       int64_t part_edges, const CsrView& g, const uint8_t* moved, uint8_t* next,
+                       hipStream_t s) {
   int64_t total;
   const Bins all = MakeBins(counts, layout, &total);
   Bins lo = all, mid = all;
@@ -364,9 +366,11 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
   }
   const int64_t c7 = counts[kBins - 1];
   if (c7 > 0) {
-    const int64_t items = c7 + te[kBins - 1] / part_edges + 1;
+    int64_t items = c7 + te[kBins - 1] / part_edges + 1;
+    items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
     hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s,
-                       list + layout.base[kBins - 1], c7, hub_part_off, g.off, g.nbr, moved,
+                       list + layout.base[kBins - 1], c7, hub_part_off, hub_item_of, g.off, g.nbr,
+                       moved,
                        (const uint32_t*)nullptr, (const uint8_t*)nullptr, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
@@ -374,16 +378,19 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
 
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
                           int64_t block_max, const uint32_t* hubs, int64_t hub_count,
-                          int64_t hub_te, const int64_t* hub_part_off, int64_t part_edges,
-                          uint8_t* next, hipStream_t s) {
+                          int64_t hub_te, const int64_t* hub_part_off,
+                          const uint32_t* hub_item_of, int64_t part_edges, uint8_t* next,
+                          hipStream_t s) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
                      g.nbr, ids, modified, block_max, next);
   PCC_HIP_CHECK(hipGetLastError());
   if (hub_count > 0) {
-    const int64_t items = hub_count + hub_te / part_edges + 1;
+    int64_t items = hub_count + hub_te / part_edges + 1;
+    items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
     hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s, hubs, hub_count,
-                       hub_part_off, g.off, g.nbr, (const uint8_t*)nullptr, ids, modified, next);
+                       hub_part_off, hub_item_of, g.off, g.nbr, (const uint8_t*)nullptr, ids,
+                       modified, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
 }

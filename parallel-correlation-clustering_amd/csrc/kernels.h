@@ -160,10 +160,9 @@ struct MoveParams {
   const uint32_t* free_ids = nullptr;
   unsigned long long* free_cur = nullptr;
   const unsigned long long* free_end = nullptr;
-  // Per-vertex granularity threshold of the hashed bins (deg > 64): a weight
-  // x with GranExp(x) >= gthr[v] cannot make any partial sum of v's
-  // per-cluster totals inexact.
-  const int16_t* gthr = nullptr;
+  // Per-vertex scale of the order-free test of the hashed bins (deg > 64):
+  // PackScale(threshold, magnitude exponent), see SlotInexact.
+  const int32_t* gscale = nullptr;
 };
 
 struct MoveOut {
@@ -185,6 +184,7 @@ constexpr int64_t kHubPartEdges = 2048;
 struct HubScratch {
   int64_t part_edges = kHubPartEdges;
   int64_t* part_off = nullptr;
+  uint32_t* item_of = nullptr;  // list item of each partition
   int64_t* ent_off = nullptr;
   uint32_t* cbuf = nullptr;  // neighbour clusters, at ent_off like the entries
   uint32_t* ekeys = nullptr;
@@ -230,13 +230,14 @@ void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t
 void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
                       uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s);
 
-// gthr[v] for every vertex with deg > 64 (see MoveParams::gthr).
-void LaunchGranThreshold(const CsrView& g, double offset, int16_t* gthr, hipStream_t s);
+// gscale[v] for every vertex with deg > 64 (see MoveParams::gscale).
+void LaunchGranThreshold(const CsrView& g, double offset, int32_t* gscale, hipStream_t s);
 
 // Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
-// partitions of part_edges edges.
+// partitions of part_edges edges, and the list item of every partition
+// (item_of[part_off[i] + p] = i).
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
-                   int64_t* part_off, int64_t* ent_off, hipStream_t s);
+                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s);
 
 // Every vertex of bin 7 (deg > th.block_max) appended to list; *count zeroed
 // by the caller.
@@ -290,15 +291,17 @@ void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, ui
 // part_edges edges); te[7] = sum of its degrees.
 void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
-                       const int64_t* hub_part_off, int64_t part_edges, const CsrView& g,
-                       const uint8_t* moved, uint8_t* next, hipStream_t s);
+                       const int64_t* hub_part_off, const uint32_t* hub_item_of,
+                       int64_t part_edges, const CsrView& g, const uint8_t* moved,
+                       uint8_t* next, hipStream_t s);
 // Neighbours of every vertex whose cluster is modified.  Vertices with
 // deg > block_max are taken from hubs (hub_count of them, hub_te edges, with
 // partition prefix hub_part_off) instead of the vertex sweep.
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
                           int64_t block_max, const uint32_t* hubs, int64_t hub_count,
-                          int64_t hub_te, const int64_t* hub_part_off, int64_t part_edges,
-                          uint8_t* next, hipStream_t s);
+                          int64_t hub_te, const int64_t* hub_part_off,
+                          const uint32_t* hub_item_of, int64_t part_edges, uint8_t* next,
+                          hipStream_t s);
 
 // ------------------------------------------------------------- primitives --
 
