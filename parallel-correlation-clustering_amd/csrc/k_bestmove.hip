@@ -596,20 +596,40 @@ __device__ __forceinline__ Cand BlockBest(Cand x, double* sd, uint32_t* sc) {
 }
 
 // Bounded insert: the slot of c, or -1 when all cap slots hold other keys.
+// *fresh tells whether this call stored the key.
 template <typename KeyPtr>
-__device__ __forceinline__ int HashSlot(KeyPtr keys, uint32_t mask, uint32_t c) {
+__device__ __forceinline__ int HashSlot(KeyPtr keys, uint32_t mask, uint32_t c, bool* fresh) {
   uint32_t h = HashId(c) & mask;
+  *fresh = false;
   for (uint32_t probe = 0; probe <= mask; ++probe) {
     const uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == c) return (int)h;
     if (k == kNoneId) {
       const uint32_t old = atomicCAS(&keys[h], kNoneId, c);
-      if (old == kNoneId || old == c) return (int)h;
+      if (old == kNoneId) {
+        *fresh = true;
+        return (int)h;
+      }
+      if (old == c) return (int)h;
     }
     h = (h + 1) & mask;
   }
   return -1;
 }
+
+// Occupied slots of an LDS table in insertion order, occ[0 .. *n), at most
+// cap_list of them; a table whose list fills up reports an overflow.
+struct SlotList {
+  uint16_t* occ;
+  uint32_t* n;
+  uint32_t cap_list;
+  __device__ __forceinline__ bool Add(int slot) const {
+    const uint32_t k = atomicAdd(n, 1u);
+    if (k >= cap_list) return false;
+    occ[k] = (uint16_t)slot;
+    return true;
+  }
+};
 
 template <typename KeyPtr>
 __device__ __forceinline__ int HashFind(KeyPtr keys, uint32_t mask, uint32_t c) {
@@ -653,6 +673,9 @@ struct RepairStage {
   uint32_t wsum[THREADS / kWave];
   double T;
 };
+// The partition kernel reuses the first 4 * THREADS words as its compaction
+// queue (AccumulateBuffered).
+static_assert(sizeof(RepairStage<512>) >= 4 * 512 * sizeof(uint32_t), "queue fits");
 
 // Recomputes, in adjacency order, the sums of the slots whose summation order
 // matters (SlotInexact), after Accumulate filled the table.  Edges of those
@@ -667,10 +690,13 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
                                                   const uint32_t* keys, double* vals,
                                                   const uint32_t* cnt, uint32_t cap,
                                                   uint32_t parts, uint32_t part, int own, int thr,
-                                                  int mexp, RepairStage<THREADS>* stg) {
+                                                  int mexp, const uint16_t* occ, uint32_t nocc,
+                                                  RepairStage<THREADS>* stg) {
   const uint32_t mask = cap - 1;
-  for (uint32_t s = threadIdx.x; s < cap; s += THREADS)
+  for (uint32_t k = threadIdx.x; k < nocc; k += THREADS) {
+    const uint32_t s = occ[k];
     if (SlotInexact(cnt[s], thr, mexp)) vals[s] = 0.0;
+  }
   __syncthreads();
   const bool own_ordered = hub && own >= 0 && SlotInexact(cnt[own], thr, mexp);
   const int64_t bs = PbbsBlock(deg);
@@ -782,7 +808,8 @@ template <int THREADS, bool kAsync, class Src>
 __device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& mp, Src src,
                                            int64_t start, int64_t deg, bool hub, int thr,
                                            uint32_t* keys, double* vals, uint32_t* cnt,
-                                           uint32_t mask, uint32_t parts, uint32_t part) {
+                                           uint32_t mask, uint32_t parts, uint32_t part,
+                                           const SlotList& sl) {
   constexpr int U = 4;
   bool overflow = false;
   for (int64_t q0 = threadIdx.x; q0 < deg; q0 += (int64_t)THREADS * U) {
@@ -807,8 +834,9 @@ __device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& m
     for (int k = 0; k < U; ++k) {
       if (c[k] == kNoneId) continue;
       const double x = EdgeValue(wf[k], hub, mp.offset);
-      const int slot = HashSlot(keys, mask, c[k]);
-      if (slot < 0) {
+      bool fresh;
+      const int slot = HashSlot(keys, mask, c[k], &fresh);
+      if (slot < 0 || (fresh && !sl.Add(slot))) {
         overflow = true;
         continue;
       }
@@ -820,19 +848,25 @@ __device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& m
 }
 
 // Accumulate over gathered clusters cb[0, deg) (partition `part` of
-// `parts`): the buffer is streamed with 16-byte loads, U per thread in
-// flight; only the edges of the partition load their weight.
+// `parts`).  Only about 1/parts of the entries belong to the partition, so
+// each wave first compacts the positions of its matching entries into a
+// per-wave LDS queue (wq, 4 * kWave entries per wave) and then inserts them
+// with all lanes busy, instead of running the insert code masked for every
+// entry.  The buffer is streamed with 16-byte loads, two per thread in
+// flight.
 template <int THREADS, bool kAsync>
 __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveParams& mp,
                                                    const uint32_t* __restrict__ cb, int64_t start,
                                                    int64_t deg, bool hub, int thr, uint32_t* keys,
                                                    double* vals, uint32_t* cnt, uint32_t mask,
-                                                   uint32_t parts, uint32_t part) {
+                                                   uint32_t parts, uint32_t part, uint32_t* wq,
+                                                   const SlotList& sl) {
   bool overflow = false;
   auto insert = [&](uint32_t c, float wf) {
     const double x = EdgeValue(wf, hub, mp.offset);
-    const int slot = HashSlot(keys, mask, c);
-    if (slot < 0) {
+    bool fresh;
+    const int slot = HashSlot(keys, mask, c, &fresh);
+    if (slot < 0 || (fresh && !sl.Add(slot))) {
       overflow = true;
       return;
     }
@@ -842,49 +876,57 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
   auto mine = [&](uint32_t c) {
     return c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
   };
+  const int lane = LaneId();
+  uint32_t* q = wq + (threadIdx.x / kWave) * (4 * kWave);
+  const float* wv = g.w + start;
+  // Compacts the matches among entries q0 .. q0 + 3 (m: bit e = entry q0 + e)
+  // of every lane, then inserts them.
+  auto drain = [&](uint32_t m, uint32_t q0) {
+    const uint32_t n = (uint32_t)__popc(m);
+    uint32_t inc = n;
+    for (int sh = 1; sh < kWave; sh <<= 1) {
+      const uint32_t y = __shfl_up(inc, sh);
+      if (lane >= sh) inc += y;
+    }
+    const uint32_t tot = __shfl(inc, kWave - 1);
+    if (tot == 0) return;
+    uint32_t at = inc - n;
+    for (uint32_t e = 0; e < 4; ++e)
+      if (m & (1u << e)) q[at++] = q0 + e;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < tot; k += kWave) {
+      const uint32_t qq = q[k];
+      insert(cb[qq], wv[qq]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
   const int64_t lead = (int64_t)((4 - (((uintptr_t)cb >> 2) & 3)) & 3);
   const int64_t head = lead < deg ? lead : deg;
-  if ((int64_t)threadIdx.x < head) {
-    const uint32_t c = cb[threadIdx.x];
-    if (mine(c)) insert(c, g.w[start + threadIdx.x]);
-  }
   const int64_t nvec = (deg - head) >> 2;
   const uint4* vp = reinterpret_cast<const uint4*>(cb + head);
-  constexpr int U = 4;
-  for (int64_t j0 = threadIdx.x; j0 < nvec; j0 += (int64_t)THREADS * U) {
-    uint4 x4[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t j = j0 + (int64_t)k * THREADS;
-      x4[k] = j < nvec ? vp[j] : make_uint4(kNoneId, kNoneId, kNoneId, kNoneId);
+  const uint4 none = make_uint4(kNoneId, kNoneId, kNoneId, kNoneId);
+  // Head and tail entries (fewer than 4 each) go through the first drain.
+  {
+    uint32_t m = 0, q0 = 0;
+    if ((int64_t)threadIdx.x < head) {
+      q0 = threadIdx.x;
+      m = mine(cb[q0]) ? 1u : 0u;
+    } else if ((int64_t)threadIdx.x < head + (deg - head - 4 * nvec)) {
+      q0 = (uint32_t)(head + 4 * nvec + (threadIdx.x - head));
+      m = mine(cb[q0]) ? 1u : 0u;
     }
-    // Partition filter first, then every weight load of the batch, then the
-    // inserts, so the loads of one thread are in flight together.
-    uint32_t c[4 * U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      c[4 * k] = x4[k].x;
-      c[4 * k + 1] = x4[k].y;
-      c[4 * k + 2] = x4[k].z;
-      c[4 * k + 3] = x4[k].w;
-    }
-#pragma unroll
-    for (int e = 0; e < 4 * U; ++e)
-      if (!mine(c[e])) c[e] = kNoneId;
-    float wf[4 * U];
-#pragma unroll
-    for (int e = 0; e < 4 * U; ++e) {
-      const int64_t q = head + 4 * (j0 + (int64_t)(e >> 2) * THREADS) + (e & 3);
-      wf[e] = c[e] != kNoneId ? g.w[start + q] : 0.0f;
-    }
-#pragma unroll
-    for (int e = 0; e < 4 * U; ++e)
-      if (c[e] != kNoneId) insert(c[e], wf[e]);
+    drain(m, q0);
   }
-  const int64_t t0 = head + 4 * nvec;
-  if (t0 + (int64_t)threadIdx.x < deg) {
-    const uint32_t c = cb[t0 + threadIdx.x];
-    if (mine(c)) insert(c, g.w[start + t0 + threadIdx.x]);
+  for (int64_t b0 = 0; b0 < nvec; b0 += 2 * THREADS) {
+    const int64_t j0 = b0 + threadIdx.x, j1 = j0 + THREADS;
+    const uint4 x0 = j0 < nvec ? vp[j0] : none;
+    const uint4 x1 = j1 < nvec ? vp[j1] : none;
+    uint32_t m0 = (mine(x0.x) ? 1u : 0u) | (mine(x0.y) ? 2u : 0u) | (mine(x0.z) ? 4u : 0u) |
+                  (mine(x0.w) ? 8u : 0u);
+    uint32_t m1 = (mine(x1.x) ? 1u : 0u) | (mine(x1.y) ? 2u : 0u) | (mine(x1.z) ? 4u : 0u) |
+                  (mine(x1.w) ? 8u : 0u);
+    drain(m0, (uint32_t)(head + 4 * j0));
+    drain(m1, (uint32_t)(head + 4 * j1));
   }
   return overflow;
 }
@@ -900,12 +942,25 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
                                                      RoundCounters* __restrict__ rc,
                                                      uint64_t salt) {
   constexpr int NW = THREADS / kWave;
+  constexpr uint32_t kList = TABLE * 3 / 4;
   __shared__ uint32_t keys[TABLE];
   __shared__ double vals[TABLE];
   __shared__ uint32_t cnt[TABLE];
+  __shared__ uint16_t occ[kList];
+  __shared__ uint32_t nocc;
   __shared__ double sd[NW];
   __shared__ uint32_t sc[NW];
   __shared__ RepairStage<THREADS> stg;
+  // The table is cleared once; after each vertex only its occupied slots are
+  // reset, so a vertex costs O(its distinct clusters), not O(TABLE).
+  for (uint32_t s = threadIdx.x; s < TABLE; s += THREADS) {
+    keys[s] = kNoneId;
+    vals[s] = 0.0;
+    cnt[s] = 0;
+  }
+  if (threadIdx.x == 0) nocc = 0;
+  __syncthreads();
+  const SlotList sl{occ, &nocc, kList};
   // Consecutive list items (nearby vertex ids, which share neighbourhoods in
   // RMAT-like graphs) go to workgroups of one XCD.
   for (int64_t i = XcdItem(blockIdx.x, gridDim.x); i < count; i += gridDim.x) {
@@ -918,68 +973,80 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
       if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       continue;
     }
-    // Table sized per vertex (load factor <= 1/2) so that clearing and
-    // scanning it costs O(deg), not O(TABLE).
+    // Probe range sized per vertex (load factor <= 1/2).
     const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * deg + 2))));
     const uint32_t mask = cap - 1;
-    for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
-      keys[s] = kNoneId;
-      vals[s] = 0.0;
-      cnt[s] = 0;
-    }
-    __syncthreads();
     const uint32_t cv = st.ids[v];
     const bool hub = deg > kHubDegree;
     const int32_t scale = kAsync ? 0 : mp.gscale[v];
     const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
     const AdjacencyClusters src{g.nbr + start, st.ids, v};
     const bool overflow = Accumulate<THREADS, kAsync>(g, mp, src, start, deg, hub, thr, keys, vals,
-                                                      cnt, mask, 1, 0);
-    if (__syncthreads_or(overflow)) {  // cannot happen at load <= 1/2; kept as a guard
+                                                      cnt, mask, 1, 0, sl);
+    const bool ovf = __syncthreads_or(overflow);
+    const uint32_t n_occ = min(nocc, kList);
+    if (ovf) {  // cannot happen at load <= 1/2; kept as a guard
       if (threadIdx.x == 0) {
         if (kAsync)
           out.moved[v] = 0;
         else
           fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       }
-      continue;
-    }
-    const int own = HashFind(keys, mask, cv);
-    double T = own >= 0 ? vals[own] : 0.0;
-    if (!kAsync) {
-      bool inexact = false;
-      for (uint32_t s = threadIdx.x; s < cap; s += THREADS)
-        inexact = inexact || SlotInexact(cnt[s], thr, mexp);
-      if (__syncthreads_or(inexact)) {
-        T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
-                                        own, thr, mexp, &stg);
-        if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
+    } else {
+      const int own = HashFind(keys, mask, cv);
+      double T = own >= 0 ? vals[own] : 0.0;
+      if (!kAsync) {
+        bool inexact = false;
+        for (uint32_t k = threadIdx.x; k < n_occ; k += THREADS)
+          inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
+        if (__syncthreads_or(inexact)) {
+          T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
+                                          own, thr, mexp, occ, n_occ, &stg);
+          if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
+        }
       }
-    }
-    const double k = st.nw[v];
-    const double res = ResOf(mp, v);
-    double change = 0;
-    change -= NetWeight(T, k * (st.K[cv] - k), res);
-    double mw = 0;
-    mw += k;
-    Cand best{0.0, kNoneId};
-    for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
-      const uint32_t c = keys[s];
-      if (c != kNoneId) {
+      const double k = st.nw[v];
+      const double res = ResOf(mp, v);
+      double change = 0;
+      change -= NetWeight(T, k * (st.K[cv] - k), res);
+      double mw = 0;
+      mw += k;
+      Cand best{0.0, kNoneId};
+      for (uint32_t q = threadIdx.x; q < n_occ; q += THREADS) {
+        const uint32_t s = occ[q];
+        const uint32_t c = keys[s];
         const Cand x{change + NetWeight(vals[s], mw * st.K[c], res), c};
         best = PickBetter(best, x);
       }
+      best = BlockBest<THREADS>(best, sd, sc);
+      if (threadIdx.x == 0) {
+        uint32_t t;
+        double d;
+        Decide(best, change, hub, mp.new_id, &t, &d);
+        if (kAsync)
+          AsyncEpilogue(v, t, d, mp, st, out);
+        else
+          SyncEpilogue(v, t, d, mp.new_id, st, out);
+      }
     }
-    best = BlockBest<THREADS>(best, sd, sc);
-    if (threadIdx.x == 0) {
-      uint32_t t;
-      double d;
-      Decide(best, change, hub, mp.new_id, &t, &d);
-      if (kAsync)
-        AsyncEpilogue(v, t, d, mp, st, out);
-      else
-        SyncEpilogue(v, t, d, mp.new_id, st, out);
+    __syncthreads();
+    // Back to an empty table.  An overflowing vertex may have stored keys
+    // past the list; then the whole probe range is reset.
+    if (ovf) {
+      for (uint32_t s = threadIdx.x; s < cap; s += THREADS) {
+        keys[s] = kNoneId;
+        vals[s] = 0.0;
+        cnt[s] = 0;
+      }
+    } else {
+      for (uint32_t q = threadIdx.x; q < n_occ; q += THREADS) {
+        const uint32_t s = occ[q];
+        keys[s] = kNoneId;
+        vals[s] = 0.0;
+        cnt[s] = 0;
+      }
     }
+    if (threadIdx.x == 0) nocc = 0;
     __syncthreads();
   }
 }
@@ -1038,6 +1105,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   __shared__ uint32_t sw[kHubThreads / kWave];
   __shared__ uint32_t s_base;
   __shared__ RepairStage<kHubThreads> stg;
+  constexpr uint32_t kList = TABLE * 3 / 4;
+  __shared__ uint16_t occ[kList];
+  __shared__ uint32_t nocc;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
   const int64_t i = hb.item_of[t];
@@ -1056,13 +1126,17 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     vals[s] = 0.0;
     cnt[s] = 0;
   }
+  if (threadIdx.x == 0) nocc = 0;
   __syncthreads();
+  const SlotList sl{occ, &nocc, kList};
   const bool hub = deg > kHubDegree;
   const int32_t scale = kAsync ? 0 : mp.gscale[v];
     const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
   const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
+  // The compaction queue shares the repair stage's LDS (never live together).
   const bool overflow = AccumulateBuffered<kHubThreads, kAsync>(
-      g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part);
+      g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part,
+      reinterpret_cast<uint32_t*>(&stg), sl);
   if (__syncthreads_or(overflow)) {
     if (threadIdx.x == 0) {
       atomicOr(&hb.hflag[i], 2u);
@@ -1073,13 +1147,14 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const uint32_t cv = st.ids[v];
   const int own = PartOf(cv, parts) == part ? HashFind(keys, mask, cv) : -1;
   double T = own >= 0 ? vals[own] : 0.0;
+  const uint32_t n_occ = min(nocc, kList);
   if (!kAsync) {
     bool inexact = false;
-    for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads)
-      inexact = inexact || SlotInexact(cnt[s], thr, mexp);
+    for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads)
+      inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
     if (__syncthreads_or(inexact)) {
       T = RepairInexactSlots<kHubThreads>(g, mp, src, start, deg, hub, keys, vals, cnt, cap,
-                                          parts, part, own, thr, mexp, &stg);
+                                          parts, part, own, thr, mexp, occ, n_occ, &stg);
       if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
     }
   }
@@ -1087,9 +1162,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     hb.tval[i] = T;
     atomicOr(&hb.hflag[i], 1u);
   }
-  // Compact the occupied slots into the entry region.
+  // Copy the occupied slots into the entry region.
   uint32_t mine = 0;
-  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) mine += keys[s] != kNoneId;
+  for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) ++mine;
   uint32_t total = 0;
   uint32_t at = BlockExclusiveScan(mine, sw, &total);
   if (total == 0) {
@@ -1103,10 +1178,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   }
   __syncthreads();
   const int64_t e0 = hb.ent_off[i] + s_base;
-  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
-    const uint32_t c = keys[s];
-    if (c == kNoneId) continue;
-    hb.ekeys[e0 + at] = c;
+  for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) {
+    const uint32_t s = occ[k];
+    hb.ekeys[e0 + at] = keys[s];
     hb.evals[e0 + at] = vals[s];
     ++at;
   }
