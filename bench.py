@@ -28,8 +28,8 @@ BIN_NAMES = ["deg<=4 (4 lanes)", "deg<=8 (8 lanes)", "deg<=16 (16 lanes)", "deg<
              "deg<=64 (64 lanes)", "one-wave LDS hash", "one-workgroup LDS hash",
              "partitioned LDS hash (hub vertices)"]
 BIN_KERNELS = ["k_bm_small<4>", "k_bm_small<8>", "k_bm_small<16>", "k_bm_small<32>",
-               "k_bm_small<64>", "k_bm_hash<64,256> + k_bm_hash<64,1024>",
-               "k_bm_hash<128,2048> + k_bm_hash<256,4096> + k_bm_hash<512,8192>",
+               "k_bm_small<64>", "k_bm_hash<64,256> + k_bm_hash<64,512> + k_bm_hash<128,1024>",
+               "k_bm_hash<256,2048> + k_bm_hash<512,4096> + k_bm_hash<1024,8192>",
                "k_hub_scan + k_hub_gather + k_bm_hub_part + k_bm_hub_cand + k_bm_hub_final"]
 
 
@@ -199,10 +199,11 @@ def measured_traffic(args, dom):
     default = (args.scale == 22 and args.edge_factor == 16.0 and not args.wave_max
                and not args.block_max and not args.hub_part_edges)
     path = os.path.join(ROOT, "profiles", "traffic_rmat22.json")
-    if not default or dom != 7 or not os.path.exists(path):
+    if not default or not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["traffic_bytes_per_round"]
+        grp = json.load(f).get("groups", {}).get(str(dom))
+    return grp["traffic_bytes_per_round"] if grp else None
 
 
 def c4_graphs(pcc_amd, count, seed=4):

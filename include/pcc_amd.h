@@ -288,6 +288,9 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_ASYNC_SUBROUNDS 5 /* async: launches per round, each on 1/k of the frontier (0: automatic) */
 #define PCC_OPT_ASYNC_SINGLETON_RULE 6 /* async: 1 applies the sync singleton rule (default 0) */
 #define PCC_OPT_HUB_PART_EDGES 7 /* edges per hash partition of the largest-degree bin */
+#define PCC_OPT_PULL_FRACTION 8  /* next-frontier expansion pulls when the seeds' (degree + 1)
+                                    sum exceeds value/1024 of (m + n); symmetric graphs only;
+                                    1024 disables (default 512) */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* ------------------------------------------------------------ host data -- */

@@ -329,6 +329,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
         return SetError(PCC_INVALID_ARGUMENT, "hub partition size must lie in [16, 4096]");
       g->hub_part_edges = value;
       return PCC_OK;
+    case PCC_OPT_PULL_FRACTION:
+      if (value < 0 || value > 1024)
+        return SetError(PCC_INVALID_ARGUMENT, "pull fraction must lie in [0, 1024]");
+      g->pull_fraction = (int)value;
+      return PCC_OK;
     default:
       return SetError(PCC_INVALID_ARGUMENT, "unknown option");
   }

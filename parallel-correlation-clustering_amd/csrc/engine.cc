@@ -222,6 +222,7 @@ int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint
                     "CSR has out-of-range neighbour ids or decreasing offsets");
   }
   if (h->gbase.empty()) h->gbase = {0, n};
+  h->symmetric = -1;
   h->finished = true;
   return PCC_OK;
 }
@@ -251,6 +252,7 @@ Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) {
   list2.Reset(al, nn);
   sub_cnt.Reset(al, kBins * 1024);
   sub_cur.Reset(al, kBins * 1024);
+  sub_te.Reset(al, 1024);
   fb_list.Reset(al, nn);
   moves.Reset(al, nn);
   moved.Reset(al, nn);
@@ -396,7 +398,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.pbest_c = c->hb_pbest_c.get();
     c->hb.pbest_d = c->hb_pbest_d.get();
     c->hb.item_of = c->hb_item_of.get();
-    if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
+    if (cfg_.move_method != PCC_ALL_MOVE) {
       c->hubs.Reset(al_, c7);
       c->hubs_part.Reset(al_, c7 + 1);
       c->hubs_ent.Reset(al_, c7 + 1);
@@ -409,6 +411,14 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
       c->hub_count = c7;
       c->hub_te = t7;
     }
+  }
+  c->pull = false;
+  if (cfg_.move_method != PCC_ALL_MOVE && h_->pull_fraction < 1024) {
+    // Contracted levels of a symmetric graph are symmetric (an inter-cluster
+    // edge keeps both directions, an intra-cluster one becomes a self-loop).
+    if (h_->symmetric < 0) h_->symmetric = GraphIsSymmetric(al_, h_->csr.View(), s_) ? 1 : 0;
+    c->pull = h_->symmetric == 1;
+    if (c->pull) c->vol.Reset(al_, 1);
   }
   MoveParams& mp = c->mp;
   mp = MoveParams();
@@ -481,6 +491,9 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   out.moved = ws.moved.get();
   out.moved_clusters = ws.flags.get();
   if (async) PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
+  // The pull expansion reads moved[] of every vertex, not just the frontier's.
+  if (c.pull && cfg_.move_method == PCC_NBHR_MOVE)
+    PCC_HIP_CHECK(hipMemsetAsync(ws.moved.get(), 0, (size_t)n, s_));
   PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 3 * sizeof(unsigned long long), s_));
   const uint64_t salt =
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
@@ -488,12 +501,14 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   // Async rounds run as sub-rounds over disjoint hashed subsets of the
   // frontier, each launch seeing the moves of the previous ones.  The
   // quality of an async round falls as the fraction of the frontier deciding
-  // at once grows; about 1/64 of the frontier per sub-round (and at least 64
-  // vertices) keeps the objective at the level of a sequential run.
+  // at once grows; 1/64 of the frontier per sub-round (one vertex each when
+  // the frontier is smaller than 64) keeps the objective at the level of a
+  // sequential run.  Small frontiers matter most: on contracted levels a few
+  // heavy vertices moving at once can merge into one oversized cluster.
   int subs = 1;
   if (async) {
     subs = h_->async_subrounds > 0 ? h_->async_subrounds
-                                   : (int)std::min<int64_t>(64, std::max<int64_t>(1, (F + 63) / 64));
+                                   : (int)std::min<int64_t>(64, std::max<int64_t>(1, F));
   }
   const uint32_t* lists[kBins];
   int64_t scount[kBins];
@@ -502,16 +517,24 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     scount[b] = counts[b];
   }
   std::vector<uint32_t> hcnt, hoff;
+  std::vector<unsigned long long> hte7;
+  int64_t ste[kBins];
+  for (int b = 0; b < kBins; ++b) ste[b] = te[b];
   if (subs > 1) {
     // Group every bin's list by sub-round so that each launch sees only its
     // own vertices.
     const size_t K = (size_t)kBins * (size_t)subs;
     PCC_HIP_CHECK(hipMemsetAsync(ws.sub_cnt.get(), 0, sizeof(uint32_t) * K, s_));
-    LaunchSubCount(ws.list.get(), c.layout, counts, (uint32_t)subs, ws.sub_cnt.get(), s_);
+    PCC_HIP_CHECK(hipMemsetAsync(ws.sub_te.get(), 0, sizeof(unsigned long long) * (size_t)subs, s_));
+    LaunchSubCount(g, ws.list.get(), c.layout, counts, (uint32_t)subs, ws.sub_cnt.get(),
+                   ws.sub_te.get(), s_);
     hcnt.resize(K);
     hoff.resize(K);
+    hte7.resize((size_t)subs);
     PCC_HIP_CHECK(hipMemcpyAsync(hcnt.data(), ws.sub_cnt.get(), sizeof(uint32_t) * K,
                                  hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipMemcpyAsync(hte7.data(), ws.sub_te.get(),
+                                 sizeof(unsigned long long) * (size_t)subs, hipMemcpyDeviceToHost, s_));
     PCC_HIP_CHECK(hipStreamSynchronize(s_));
     for (int b = 0; b < kBins; ++b) {
       uint32_t run = 0;
@@ -541,8 +564,9 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
         lists[b] = ws.list2.get() + c.layout.base[b] + hoff[k];
         scount[b] = hcnt[k];
       }
+      ste[kBins - 1] = (int64_t)hte7[(size_t)sub];
     }
-    LaunchMoves(c, ws, mp, async, lists, scount, te, out, salt, subs > 1 || sub == 0);
+    LaunchMoves(c, ws, mp, async, lists, scount, ste, out, salt, subs > 1 || sub == 0);
   }
   if (subs > 1) {
     // The frontier expansion walks bin 7 of the whole list.
@@ -577,14 +601,27 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   }
   // Next frontier = neighbours of the seeds (PCC.cc:309-331).
   PCC_HIP_CHECK(hipMemsetAsync(ws.next.get(), 0, (size_t)n, s_));
-  if (cfg_.move_method == PCC_NBHR_CLUSTER_MOVE) {
-    LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.hubs.get(), c.hub_count,
-                         c.hub_te, c.hubs_part.get(), c.hubs_item_of.get(), h_->hub_part_edges,
-                         ws.next.get(), s_);
+  const bool by_cluster = cfg_.move_method == PCC_NBHR_CLUSTER_MOVE;
+  const uint32_t* seed_ids = by_cluster ? st.ids : nullptr;
+  ExpandDir dir;
+  if (c.pull) {
+    // Direction by seed volume: the pull kernels or the push kernels run,
+    // the other side returns at once.
+    PCC_HIP_CHECK(hipMemsetAsync(c.vol.get(), 0, sizeof(unsigned long long), s_));
+    LaunchSeedVolume(g, seed_ids, ws.flags.get(), ws.moved.get(), c.vol.get(), s_);
+    dir.vol = c.vol.get();
+    dir.thr = (unsigned long long)(((double)g.m + (double)n) * h_->pull_fraction / 1024.0);
+  }
+  if (by_cluster) {
+    LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.Hubs(), h_->hub_part_edges,
+                         dir, ws.next.get(), s_);
   } else {
     LaunchExpandMoved(counts, te, c.layout, ws.list.get(), c.hb.part_off, c.hb.item_of,
-                      c.hb.part_edges, g, ws.moved.get(), ws.next.get(), s_);
+                      c.hb.part_edges, g, ws.moved.get(), dir, ws.next.get(), s_);
   }
+  if (c.pull)
+    LaunchPullFrontier(g, seed_ids, ws.flags.get(), ws.moved.get(), h_->th.block_max, c.Hubs(),
+                       h_->hub_part_edges, dir, ws.next.get(), s_);
   LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
   ResetCounters(ws.rc.get(), false);
   LaunchBinFrontier(g, ws.next.get(), h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);

@@ -41,6 +41,11 @@ struct pcc_graph {
   int async_subrounds = 0;  // 0: automatic (see Engine::RoundStep)
   int async_singleton_rule = 0;
   int64_t hub_part_edges = pcc::kHubPartEdges;
+  // Pull expansion when the seed volume exceeds pull_fraction / 1024 of the
+  // graph's (edges + vertices); 1024 disables it.
+  int pull_fraction = 512;
+  // Edge multiset symmetric: -1 unknown, 0 no, 1 yes (set on first use).
+  int symmetric = -1;
   pcc::BinThresholds th;
   int num_graphs() const { return gbase.empty() ? 1 : (int)gbase.size() - 1; }
   ~pcc_graph() {
@@ -107,6 +112,7 @@ class Engine {
   struct Workspace {
     DevBuf<uint32_t> list, list2, fb_list, moves;
     DevBuf<uint32_t> sub_cnt, sub_cur;  // async sub-round segments, kBins * 1024
+    DevBuf<unsigned long long> sub_te;  // bin-7 degree sum per sub-round, 1024
     DevBuf<uint8_t> moved, flags, next, new_flag;
     DevBuf<RoundCounters> rc;
     Workspace(DeviceAllocator* al, int64_t n, hipStream_t s);
@@ -131,11 +137,24 @@ class Engine {
     DevBuf<uint32_t> hb_cbuf, hb_ekeys, hb_ecur, hb_flag, hb_pe_base, hb_pe_cnt, hb_pbest_c,
         hb_item_of;
     DevBuf<double> hb_evals, hb_tval, hb_pbest_d;
-    // Every bin-7 vertex with its partition prefix (NBHR_CLUSTER_MOVE
-    // expansion).
+    // Every bin-7 vertex with its partition prefix (the NBHR_CLUSTER_MOVE
+    // expansion and the pull expansion).
     DevBuf<uint32_t> hubs, hubs_item_of;
     DevBuf<int64_t> hubs_part, hubs_ent;
     int64_t hub_count = 0, hub_te = 0;
+    // Pull expansion allowed (symmetric graph, a neighbourhood move method);
+    // vol receives the seed volume of each round.
+    bool pull = false;
+    DevBuf<unsigned long long> vol;
+    HubList Hubs() const {
+      HubList h;
+      h.list = hubs.get();
+      h.count = hub_count;
+      h.te = hub_te;
+      h.part_off = hubs_part.get();
+      h.item_of = hubs_item_of.get();
+      return h;
+    }
   };
   void PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay, LevelCtx* ctx);
   // Best-move kernels of every bin; lists[b] holds counts[b] vertices of bin

@@ -209,21 +209,32 @@ __device__ __forceinline__ int BinOfItem(const BinCounts& bc, int64_t t, int64_t
   return b;
 }
 
+// Also sums the degrees of each sub-round's bin-7 vertices into te7, which
+// sizes that sub-round's hub launches.
 __global__ __launch_bounds__(kSubThreads) void k_sub_count(const uint32_t* __restrict__ list,
                                                            BinCounts bc, int64_t total, uint32_t S,
-                                                           uint32_t* __restrict__ cnt) {
+                                                           const int64_t* __restrict__ off,
+                                                           uint32_t* __restrict__ cnt,
+                                                           unsigned long long* __restrict__ te7) {
   __shared__ uint32_t h[kBins * kMaxSub];
+  __shared__ unsigned long long h7[kMaxSub];
   for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads) h[k] = 0;
+  for (uint32_t k = threadIdx.x; k < S; k += kSubThreads) h7[k] = 0;
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * kSubThreads + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kSubThreads) {
     int64_t r;
     const int b = BinOfItem(bc, t, &r);
-    atomicAdd(&h[b * S + SubRoundOf(list[bc.base[b] + r], S)], 1u);
+    const uint32_t v = list[bc.base[b] + r];
+    const uint32_t j = SubRoundOf(v, S);
+    atomicAdd(&h[b * S + j], 1u);
+    if (b == kBins - 1) atomicAdd(&h7[j], (unsigned long long)(off[v + 1] - off[v]));
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < kBins * S; k += kSubThreads)
     if (h[k]) atomicAdd(&cnt[k], h[k]);
+  for (uint32_t k = threadIdx.x; k < S; k += kSubThreads)
+    if (h7[k]) atomicAdd(&te7[k], h7[k]);
 }
 
 __global__ __launch_bounds__(kSubThreads) void k_sub_scatter(const uint32_t* __restrict__ list,
@@ -932,7 +943,14 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
 }
 
 // Bins 5 and 6: THREADS cooperating threads and a TABLE-entry LDS table per
-// vertex.
+// vertex.  Each thread holds E = TABLE / (2 THREADS) edges of a vertex in
+// registers (the launch ranges keep deg <= TABLE / 2), and the workgroup runs
+// a two-vertex software pipeline: the adjacency loads of the next vertex are
+// issued before the current vertex's inserts, and its cluster gathers before
+// the current vertex's decision, so both latencies overlap work instead of
+// adding up vertex after vertex.  In sync mode the state is fixed for the
+// round, so reading the next vertex early changes nothing; in async mode it
+// is one more interleaving of the concurrent moves.
 template <int THREADS, int TABLE, bool kAsync>
 __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
                                                      const uint32_t* __restrict__ list,
@@ -942,6 +960,8 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
                                                      RoundCounters* __restrict__ rc,
                                                      uint64_t salt) {
   constexpr int NW = THREADS / kWave;
+  constexpr int E = TABLE / (2 * THREADS);
+  static_assert(E >= 1 && E * THREADS * 2 == TABLE, "table holds two slots per edge slot");
   constexpr uint32_t kList = TABLE * 3 / 4;
   __shared__ uint32_t keys[TABLE];
   __shared__ double vals[TABLE];
@@ -961,29 +981,94 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
   if (threadIdx.x == 0) nocc = 0;
   __syncthreads();
   const SlotList sl{occ, &nocc, kList};
+
+  // The workgroup's next vertex in (dlo, dhi] at or after item i (uniform).
+  // Vertices routed to the ordered kernel are queued on the way.
+  auto next_item = [&](int64_t& i, uint32_t& v, int64_t& start, int64_t& deg) -> bool {
+    for (; i < count; i += gridDim.x) {
+      v = list[PermuteIndex(i, count, salt)];
+      if (kAsync && !InSubRound(mp, v)) continue;
+      start = g.off[v];
+      deg = g.off[v + 1] - start;
+      if (deg <= dlo || deg > dhi) continue;  // another launch's degree range
+      if (deg > (int64_t)E * THREADS || (!kAsync && mp.force_ordered)) {
+        if (threadIdx.x == 0) {
+          if (kAsync)
+            out.moved[v] = 0;
+          else
+            fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
+        }
+        continue;
+      }
+      return true;
+    }
+    return false;
+  };
+  auto load_edges = [&](bool ok, int64_t start, int64_t deg, uint32_t* nb, float* wf) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int64_t q = threadIdx.x + (int64_t)e * THREADS;
+      const bool in = ok && q < deg;
+      nb[e] = in ? g.nbr[start + q] : kNoneId;
+      wf[e] = in ? g.w[start + q] : 0.0f;
+    }
+  };
+  auto gather = [&](uint32_t v, const uint32_t* nb, uint32_t* cc) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) cc[e] = (nb[e] != kNoneId && nb[e] != v) ? st.ids[nb[e]] : kNoneId;
+  };
+
   // Consecutive list items (nearby vertex ids, which share neighbourhoods in
   // RMAT-like graphs) go to workgroups of one XCD.
-  for (int64_t i = XcdItem(blockIdx.x, gridDim.x); i < count; i += gridDim.x) {
-    const uint32_t v = list[PermuteIndex(i, count, salt)];
-    if (kAsync && !InSubRound(mp, v)) continue;  // uniform across the workgroup
-    const int64_t start = g.off[v];
-    const int64_t deg = g.off[v + 1] - start;
-    if (deg <= dlo || deg > dhi) continue;  // another launch's degree range
-    if (!kAsync && mp.force_ordered) {
-      if (threadIdx.x == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
-      continue;
-    }
-    // Probe range sized per vertex (load factor <= 1/2).
+  int64_t i = XcdItem(blockIdx.x, gridDim.x);
+  uint32_t v = 0;
+  int64_t start = 0, deg = 0;
+  bool have = next_item(i, v, start, deg);
+  uint32_t cc[E];
+  float wf[E];
+  {
+    uint32_t nb[E];
+    load_edges(have, start, deg, nb, wf);
+    gather(v, nb, cc);
+  }
+  int64_t i2 = i + gridDim.x;
+  uint32_t v2 = 0;
+  int64_t start2 = 0, deg2 = 0;
+  bool have2 = have && next_item(i2, v2, start2, deg2);
+  while (have) {
+    // Stage 1: the next vertex's adjacency.
+    uint32_t nb2[E];
+    float wf2[E];
+    load_edges(have2, start2, deg2, nb2, wf2);
+    // Stage 2: this vertex's inserts (probe range sized for load <= 1/2).
     const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * deg + 2))));
     const uint32_t mask = cap - 1;
-    const uint32_t cv = st.ids[v];
     const bool hub = deg > kHubDegree;
     const int32_t scale = kAsync ? 0 : mp.gscale[v];
     const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
-    const AdjacencyClusters src{g.nbr + start, st.ids, v};
-    const bool overflow = Accumulate<THREADS, kAsync>(g, mp, src, start, deg, hub, thr, keys, vals,
-                                                      cnt, mask, 1, 0, sl);
+    bool overflow = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (cc[e] == kNoneId) continue;
+      const double x = EdgeValue(wf[e], hub, mp.offset);
+      bool fresh;
+      const int slot = HashSlot(keys, mask, cc[e], &fresh);
+      if (slot < 0 || (fresh && !sl.Add(slot))) {
+        overflow = true;
+        continue;
+      }
+      atomicAdd(&vals[slot], x);
+      if (!kAsync) RecordTerm(cnt, slot, x, thr);
+    }
     const bool ovf = __syncthreads_or(overflow);
+    // Stage 3: the next vertex's cluster gathers and the item after it.
+    uint32_t cc2[E];
+    gather(v2, nb2, cc2);
+    int64_t i3 = i2 + gridDim.x;
+    uint32_t v3 = 0;
+    int64_t start3 = 0, deg3 = 0;
+    const bool have3 = have2 && next_item(i3, v3, start3, deg3);
+    // Stage 4: this vertex's decision.
     const uint32_t n_occ = min(nocc, kList);
     if (ovf) {  // cannot happen at load <= 1/2; kept as a guard
       if (threadIdx.x == 0) {
@@ -993,6 +1078,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
           fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       }
     } else {
+      const uint32_t cv = st.ids[v];
       const int own = HashFind(keys, mask, cv);
       double T = own >= 0 ? vals[own] : 0.0;
       if (!kAsync) {
@@ -1000,6 +1086,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
         for (uint32_t k = threadIdx.x; k < n_occ; k += THREADS)
           inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
         if (__syncthreads_or(inexact)) {
+          const AdjacencyClusters src{g.nbr + start, st.ids, v};
           T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
                                           own, thr, mexp, occ, n_occ, &stg);
           if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
@@ -1011,11 +1098,22 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
       change -= NetWeight(T, k * (st.K[cv] - k), res);
       double mw = 0;
       mw += k;
+      // At most E slots per thread (n_occ <= deg <= E * THREADS): their
+      // cluster weights are loaded together.
+      uint32_t c[E];
+      double Kc[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const uint32_t q = threadIdx.x + (uint32_t)e * THREADS;
+        c[e] = q < n_occ ? keys[occ[q]] : kNoneId;
+        Kc[e] = c[e] != kNoneId ? st.K[c[e]] : 0.0;
+      }
       Cand best{0.0, kNoneId};
-      for (uint32_t q = threadIdx.x; q < n_occ; q += THREADS) {
-        const uint32_t s = occ[q];
-        const uint32_t c = keys[s];
-        const Cand x{change + NetWeight(vals[s], mw * st.K[c], res), c};
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (c[e] == kNoneId) continue;
+        const uint32_t q = threadIdx.x + (uint32_t)e * THREADS;
+        const Cand x{change + NetWeight(vals[occ[q]], mw * Kc[e], res), c[e]};
         best = PickBetter(best, x);
       }
       best = BlockBest<THREADS>(best, sd, sc);
@@ -1048,6 +1146,21 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     }
     if (threadIdx.x == 0) nocc = 0;
     __syncthreads();
+    // Rotate the pipeline.
+    have = have2;
+    v = v2;
+    start = start2;
+    deg = deg2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      cc[e] = cc2[e];
+      wf[e] = wf2[e];
+    }
+    i2 = i3;
+    have2 = have3;
+    v2 = v3;
+    start2 = start3;
+    deg2 = deg3;
   }
 }
 
@@ -1422,8 +1535,9 @@ void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long 
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
-                    uint32_t S, uint32_t* cnt, hipStream_t s) {
+void LaunchSubCount(const CsrView& g, const uint32_t* list, const BinLayout& layout,
+                    const int64_t counts[kBins], uint32_t S, uint32_t* cnt,
+                    unsigned long long* te7, hipStream_t s) {
   BinCounts bc;
   int64_t total = 0;
   for (int b = 0; b < kBins; ++b) {
@@ -1433,7 +1547,7 @@ void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t
   }
   if (total == 0) return;
   hipLaunchKernelGGL(k_sub_count, dim3(GridFor(total, (int)kSubSlab, 1024)), dim3(kSubThreads), 0, s,
-                     list, bc, total, S, cnt);
+                     list, bc, total, S, g.off, cnt, te7);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

@@ -91,6 +91,40 @@ struct EdgeWalk {
   }
 };
 
+// Symmetry fingerprint: sum over directed edges of H(u, v) - H(v, u), H a
+// 64-bit mix of the pair; zero for a symmetric edge multiset.
+__device__ __forceinline__ uint64_t PairHash(uint64_t a, uint64_t b) {
+  uint64_t z = ((a << 32) | b) + 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_sym_fingerprint(CsrView g, int64_t nchunks,
+                                                                  unsigned long long* __restrict__ acc) {
+  unsigned long long x = 0;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t c0 = ch * kEdgeChunk;
+    const int64_t len = min(kEdgeChunk, g.m - c0);
+    const int64_t q0 = (int64_t)threadIdx.x * kEdgeItems;
+    if (q0 >= len) continue;
+    const int64_t q1 = min(q0 + kEdgeItems, len);
+    int64_t u = VertexOfEdge(g.off, 0, g.n, c0 + q0);
+    int64_t uend = g.off[u + 1];
+    for (int64_t q = q0; q < q1; ++q) {
+      const int64_t p = c0 + q;
+      if (p >= uend) {
+        u = VertexOfEdge(g.off, u + 1, g.n, p);
+        uend = g.off[u + 1];
+      }
+      const uint64_t v = g.nbr[p];
+      x += PairHash((uint64_t)u, v) - PairHash(v, (uint64_t)u);
+    }
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) x += __shfl_xor(x, sh);
+  if (LaneId() == 0 && x) atomicAdd(acc, x);
+}
+
 __global__ __launch_bounds__(kEdgeThreads) void k_count_kept(CsrView g,
                                                              const uint32_t* __restrict__ ids,
                                                              GraphMap gm, int64_t nchunks,
@@ -196,8 +230,11 @@ __global__ __launch_bounds__(256) void k_seg_starts(const uint8_t* __restrict__ 
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
-// Segment sums, pbbslib reduce order.  Runs of at most 1024 terms are folded
-// left by one thread; longer runs are queued for k_seg_long.
+// Segment sums, pbbslib reduce order: a run of len terms is folded left when
+// len <= 1024; otherwise it is cut into blocks of bs = max(1024, 4 ceil(sqrt
+// len)) terms, each block is folded left, and the block sums are reduced the
+// same way.  Runs of at most 1024 terms are folded by one thread here; longer
+// runs are queued for the blocked kernels below.
 template <typename V>
 __global__ __launch_bounds__(256) void k_seg_short(const V* __restrict__ vals,
                                                    const int64_t* __restrict__ start, int64_t nseg,
@@ -217,48 +254,107 @@ __global__ __launch_bounds__(256) void k_seg_short(const V* __restrict__ vals,
   }
 }
 
-template <typename V>
-__global__ __launch_bounds__(64) void k_seg_long(const V* __restrict__ vals,
-                                                 const int64_t* __restrict__ start,
-                                                 const uint32_t* __restrict__ longq,
-                                                 const unsigned long long* __restrict__ nlong,
-                                                 V* __restrict__ out, V* __restrict__ scratch,
-                                                 int64_t cap) {
-  const int lane = threadIdx.x;
-  V* bufa = scratch + (size_t)blockIdx.x * cap;
-  V* bufb = bufa + cap / 2;
-  const int64_t count = (int64_t)*nlong;
-  for (int64_t q = blockIdx.x; q < count; q += gridDim.x) {
+__device__ __forceinline__ int64_t PbbsBlockSize(int64_t len) {
+  const int64_t bs = (int64_t)ceil(sqrt((double)len)) * 4;
+  return bs < 1024 ? 1024 : bs;
+}
+
+// Number of level-1 blocks of each queued run.
+__global__ __launch_bounds__(256) void k_seg_nblocks(const int64_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ longq,
+                                                     int64_t nlong, uint32_t* __restrict__ nblk) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nlong;
+       q += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t sgi = longq[q];
-    const V* src = vals + start[sgi];
-    int64_t len = start[sgi + 1] - start[sgi];
-    V* dst = bufa;
-    while (true) {
-      int64_t bs = (int64_t)ceil(sqrt((double)len)) * 4;
-      if (bs < 1024) bs = 1024;
-      const int64_t l = (len + bs - 1) / bs;
-      if (l <= 1) {
-        if (lane == 0) {
-          V r = src[0];
-          for (int64_t i = 1; i < len; ++i) r += src[i];
-          out[sgi] = r;
-        }
-        break;
-      }
-      for (int64_t blk = lane; blk < l; blk += 64) {
-        const int64_t s = blk * bs;
-        const int64_t e = (s + bs < len) ? s + bs : len;
-        V r = src[s];
-        for (int64_t i = s + 1; i < e; ++i) r += src[i];
-        dst[blk] = r;
-      }
-      __syncthreads();
-      src = dst;
-      len = l;
-      dst = (dst == bufa) ? bufb : bufa;
-    }
-    __syncthreads();
+    const int64_t len = start[sgi + 1] - start[sgi];
+    const int64_t bs = PbbsBlockSize(len);
+    nblk[q] = (uint32_t)((len + bs - 1) / bs);
   }
+}
+
+// Ordered left fold of src[0, len) (len >= 1) by one wave; every lane
+// returns the result.
+template <typename V>
+__device__ __forceinline__ V WaveFoldLeft(const V* __restrict__ src, int64_t len) {
+  const int lane = LaneId();
+  V acc = src[0];
+  for (int64_t c0 = 1; c0 < len; c0 += kWave) {
+    const int64_t nk = min((int64_t)kWave, len - c0);
+    const V x = lane < nk ? src[c0 + lane] : (V)0;
+    for (int j = 0; j < nk; ++j) acc += __shfl(x, j);
+  }
+  return acc;
+}
+
+// One wave per level-1 block (item w of the concatenated block lists, boff =
+// exclusive prefix of the block counts): the block's sum in the reference's
+// order.  An unordered sum is used when it is exact in every order -- every
+// term a multiple of 2^g and the sum of magnitudes below 2^(g + p - 1), p the
+// significand width of V -- and an ordered fold otherwise (also for all-zero
+// blocks, whose result sign depends on the order, and non-finite terms).
+template <typename V>
+__global__ __launch_bounds__(256) void k_seg_blocks(const V* __restrict__ vals,
+                                                    const int64_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ longq,
+                                                    const int64_t* __restrict__ boff, int64_t nlong,
+                                                    V* __restrict__ part) {
+  constexpr int kP = sizeof(V) == 4 ? 24 : 53;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (w >= boff[nlong]) return;
+  int64_t lo = 0, hi = nlong - 1;  // largest q with boff[q] <= w
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (boff[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  const uint32_t sgi = longq[lo];
+  const int64_t s = start[sgi], len = start[sgi + 1] - s;
+  const int64_t bs = PbbsBlockSize(len);
+  const int64_t k = w - boff[lo];
+  const int64_t b0 = k * bs, b1 = min(len, b0 + bs);
+  const V* src = vals + s + b0;
+  const int64_t bl = b1 - b0;
+  double sum = 0.0, sabs = 0.0;
+  int gmin = 1 << 20;
+  for (int64_t i = LaneId(); i < bl; i += kWave) {
+    const double x = (double)src[i];
+    sum += x;
+    sabs += fabs(x);
+    gmin = min(gmin, GranExp(x));
+  }
+  sum = WaveSum(sum);
+  sabs = WaveSum(sabs);
+  gmin = WaveMinInt(gmin);
+  const bool exact = sabs > 0.0 && sabs < INFINITY &&
+                     (gmin >= (1 << 19) || sabs < ldexp(1.0, gmin + kP - 1));
+  const V r = exact ? (V)sum : WaveFoldLeft(src, bl);
+  if (LaneId() == 0) part[w] = r;
+}
+
+// One wave per queued run: reduces its level-1 block sums in the reference's
+// order (blocks of max(1024, 4 ceil(sqrt l)) folded left, repeated), in place.
+template <typename V>
+__global__ __launch_bounds__(256) void k_seg_top(const uint32_t* __restrict__ longq,
+                                                 const int64_t* __restrict__ boff, int64_t nlong,
+                                                 V* __restrict__ part, V* __restrict__ out) {
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (q >= nlong) return;
+  V* p = part + boff[q];
+  int64_t len = boff[q + 1] - boff[q];
+  while (len > 1) {
+    const int64_t bs = PbbsBlockSize(len);
+    const int64_t l = (len + bs - 1) / bs;
+    if (l <= 1) break;
+    // Block k's sum lands at index k <= k * bs, after the block was read.
+    for (int64_t k = 0; k < l; ++k) {
+      const int64_t b0 = k * bs;
+      const V r = WaveFoldLeft(p + b0, min(bs, len - b0));
+      if (LaneId() == 0) p[k] = r;
+      __threadfence();  // the next pass reads p[k] from every lane
+    }
+    len = l;
+  }
+  const V r = WaveFoldLeft(p, len);
+  if (LaneId() == 0) out[longq[q]] = r;
 }
 
 __global__ __launch_bounds__(256) void k_emit_compressed(const uint64_t* __restrict__ keys,
@@ -716,14 +812,31 @@ void SegmentSums(DeviceAllocator* al, const V* vals, const int64_t* start, int64
   hipLaunchKernelGGL((k_seg_short<V>), dim3(Grid(nseg, 256, 8192)), dim3(256), 0, s, vals, start,
                      nseg, out, longq.get(), nlong.get());
   PCC_HIP_CHECK(hipGetLastError());
-  if (nvals > 1024) {
-    constexpr int kRegions = 16;
-    const int64_t cap = 2 * (nvals / 1024 + 2);
-    DevBuf<V> scratch(al, cap * kRegions);
-    hipLaunchKernelGGL((k_seg_long<V>), dim3(kRegions), dim3(64), 0, s, vals, start, longq.get(),
-                       nlong.get(), out, scratch.get(), cap);
-    PCC_HIP_CHECK(hipGetLastError());
-  }
+  if (nvals <= 1024) return;
+  unsigned long long hl = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hl, nlong.get(), sizeof(hl), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  const int64_t nl = (int64_t)hl;
+  if (nl == 0) return;
+  DevBuf<uint32_t> nblk(al, nl);
+  DevBuf<int64_t> boff(al, nl + 1);
+  hipLaunchKernelGGL(k_seg_nblocks, dim3(Grid(nl, 256, 4096)), dim3(256), 0, s, start,
+                     longq.get(), nl, nblk.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  ExclusiveScanU32toI64(al, nblk.get(), boff.get(), nl, boff.get() + nl, s);
+  int64_t total = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&total, boff.get() + nl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  // Each run has len > 1024 terms and at most ceil(len / 1024) blocks.
+  if (total <= 0 || total > nvals / 1024 + nl)
+    throw HipError{hipErrorUnknown, "SegmentSums: inconsistent block counts", __FILE__, __LINE__};
+  DevBuf<V> part(al, total);
+  constexpr int kWavesPerBlock = 256 / kWave;
+  hipLaunchKernelGGL((k_seg_blocks<V>), dim3((unsigned)((total + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(256), 0, s, vals, start, longq.get(), boff.get(), nl, part.get());
+  hipLaunchKernelGGL((k_seg_top<V>), dim3((unsigned)((nl + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(256), 0, s, longq.get(), boff.get(), nl, part.get(), out);
+  PCC_HIP_CHECK(hipGetLastError());
 }
 
 // Segment boundaries of sorted keys: returns the number of segments and
@@ -752,6 +865,20 @@ int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<in
 }
 
 }  // namespace
+
+bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
+  if (g.m == 0) return true;
+  const int64_t nchunks = (g.m + kEdgeChunk - 1) / kEdgeChunk;
+  DevBuf<unsigned long long> acc(al, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(acc.get(), 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_sym_fingerprint, dim3(Grid(nchunks, 1, 8192)), dim3(kEdgeThreads), 0, s, g,
+                     nchunks, acc.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  unsigned long long h = 1;
+  PCC_HIP_CHECK(hipMemcpyAsync(&h, acc.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  return h == 0;
+}
 
 void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, uint32_t* gmax,
                        hipStream_t s) {

@@ -137,7 +137,8 @@ __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint8_t* __restrict__ moved,
-                                                      uint8_t* __restrict__ next) {
+                                                      ExpandDir dir, uint8_t* __restrict__ next) {
+  if (dir.Pull()) return;
   for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
     int64_t r = t;
     int b = 0;
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
                                                       const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint8_t* __restrict__ moved,
-                                                      uint8_t* __restrict__ next) {
+                                                      ExpandDir dir, uint8_t* __restrict__ next) {
+  if (dir.Pull()) return;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t = base + threadIdx.x;
@@ -181,8 +183,9 @@ __global__ __launch_bounds__(256) void k_expand_clusters(int64_t n, const int64_
                                                          const uint32_t* __restrict__ nbr,
                                                          const uint32_t* __restrict__ ids,
                                                          const uint8_t* __restrict__ modified,
-                                                         int64_t block_max,
+                                                         int64_t block_max, ExpandDir dir,
                                                          uint8_t* __restrict__ next) {
+  if (dir.Pull()) return;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = base + threadIdx.x;
@@ -208,7 +211,8 @@ __global__ __launch_bounds__(256) void k_expand_hub(const uint32_t* __restrict__
                                                     const uint8_t* __restrict__ moved,
                                                     const uint32_t* __restrict__ ids,
                                                     const uint8_t* __restrict__ modified,
-                                                    uint8_t* __restrict__ next) {
+                                                    ExpandDir dir, uint8_t* __restrict__ next) {
+  if (dir.Pull()) return;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= part_off[count]) return;
   const int64_t i = item_of[t];
@@ -220,6 +224,116 @@ __global__ __launch_bounds__(256) void k_expand_hub(const uint32_t* __restrict__
   const int64_t per = (deg + parts - 1) / parts;
   const int64_t e1 = min(deg, (part + 1) * per);
   for (int64_t p = part * per + threadIdx.x; p < e1; p += blockDim.x) next[nbr[st + p]] = 1;
+}
+
+// Dense (pull) expansion, used when the seeds' edge volume is large: v
+// joins the next frontier iff some neighbour is a seed.  On a symmetric graph
+// this is the same set the push kernels produce (u is a neighbour of v iff v
+// is one of u), reached with an early exit per vertex instead of a write per
+// seed edge.  The seed test is modified[ids[u]] (ids non-null) or moved[u].
+__device__ __forceinline__ bool IsSeed(uint32_t u, const uint32_t* __restrict__ ids,
+                                       const uint8_t* __restrict__ modified,
+                                       const uint8_t* __restrict__ moved) {
+  return ids ? modified[ids[u]] != 0 : moved[u] != 0;
+}
+
+// vol += sum over seeds of (deg + 1), the edgeMap density measure.
+__global__ __launch_bounds__(256) void k_seed_volume(int64_t n, const int64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ ids,
+                                                     const uint8_t* __restrict__ modified,
+                                                     const uint8_t* __restrict__ moved,
+                                                     unsigned long long* __restrict__ vol) {
+  __shared__ unsigned long long sm[256 / kWave];
+  unsigned long long x = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (IsSeed((uint32_t)v, ids, modified, moved)) x += (unsigned long long)(off[v + 1] - off[v]) + 1ull;
+  for (int sh = 32; sh > 0; sh >>= 1) x += __shfl_xor(x, sh);
+  if (LaneId() == 0) sm[threadIdx.x / kWave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < 256 / kWave; ++i) t += sm[i];
+    if (t) atomicAdd(vol, t);
+  }
+}
+
+// Vertices with deg <= block_max: degrees up to 16 by their own thread, larger
+// ones by the whole wave, 64 neighbours per step.
+__global__ __launch_bounds__(256) void k_pull_vertices(int64_t n, const int64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       const uint32_t* __restrict__ ids,
+                                                       const uint8_t* __restrict__ modified,
+                                                       const uint8_t* __restrict__ moved,
+                                                       int64_t block_max, ExpandDir dir,
+                                                       uint8_t* __restrict__ next) {
+  if (!dir.Pull()) return;
+  const int lane = LaneId();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    int64_t st = 0, deg = 0;
+    if (v < n) {
+      st = off[v];
+      deg = off[v + 1] - st;
+      if (deg <= 16) {
+        for (int64_t p = 0; p < deg; ++p)
+          if (IsSeed(nbr[st + p], ids, modified, moved)) {
+            next[v] = 1;
+            break;
+          }
+      }
+    }
+    uint64_t big = __ballot(v < n && deg > 16 && deg <= block_max);
+    const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
+    while (big) {
+      const int l = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int64_t vs = __shfl(st, l), vd = __shfl(deg, l);
+      for (int64_t p0 = 0; p0 < vd; p0 += kWave) {
+        const int64_t p = p0 + lane;
+        const bool hit = p < vd && IsSeed(nbr[vs + p], ids, modified, moved);
+        if (__ballot(hit)) {
+          if (lane == 0) next[wave_v0 + l] = 1;
+          break;
+        }
+      }
+    }
+  }
+}
+
+// Vertices with deg > block_max, one workgroup per partition of the hub list
+// (part_off / item_of as in k_expand_hub); a partition stops at its first hit
+// or when another partition has already marked the hub.
+__global__ __launch_bounds__(256) void k_pull_hub(const uint32_t* __restrict__ hubs, int64_t count,
+                                                  const int64_t* __restrict__ part_off,
+                                                  const uint32_t* __restrict__ item_of,
+                                                  const int64_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ nbr,
+                                                  const uint32_t* __restrict__ ids,
+                                                  const uint8_t* __restrict__ modified,
+                                                  const uint8_t* __restrict__ moved,
+                                                  ExpandDir dir, uint8_t* __restrict__ next) {
+  if (!dir.Pull()) return;
+  const int64_t t = XcdChunkItem(blockIdx.x);
+  if (t >= part_off[count]) return;
+  const int64_t i = item_of[t];
+  const uint32_t v = hubs[i];
+  const int64_t parts = part_off[i + 1] - part_off[i];
+  const int64_t part = t - part_off[i];
+  const int64_t st = off[v], deg = off[v + 1] - st;
+  const int64_t per = (deg + parts - 1) / parts;
+  const int64_t e0 = part * per, e1 = min(deg, e0 + per);
+  for (int64_t p0 = e0; p0 < e1; p0 += blockDim.x) {
+    const int64_t p = p0 + threadIdx.x;
+    const bool hit = p < e1 && IsSeed(nbr[st + p], ids, modified, moved);
+    const bool marked = threadIdx.x == 0 && *(volatile const uint8_t*)&next[v] != 0;
+    // Uniform exit: every thread takes the block-wide result.
+    if (__syncthreads_or(hit || marked)) {
+      if (threadIdx.x == 0) next[v] = 1;
+      return;
+    }
+  }
 }
 
 int Grid(int64_t items, int per_block, int max_blocks) {
@@ -341,8 +455,8 @@ void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
 void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
                        const int64_t* hub_part_off, const uint32_t* hub_item_of,
-                       int64_t part_edges, const CsrView& g, const uint8_t* moved, uint8_t* next,
-                       hipStream_t s) {
+                       int64_t part_edges, const CsrView& g, const uint8_t* moved,
+                       const ExpandDir& dir, uint8_t* next, hipStream_t s) {
   int64_t total;
   const Bins all = MakeBins(counts, layout, &total);
   Bins lo = all, mid = all;
@@ -356,12 +470,12 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
   }
   if (nlo > 0) {
     hipLaunchKernelGGL(k_expand_moved, dim3(Grid(nlo, 256, 8192)), dim3(256), 0, s, lo, nlo, list,
-                       g.off, g.nbr, moved, next);
+                       g.off, g.nbr, moved, dir, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
   if (nmid > 0) {
     hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 16384)), dim3(256), 0, s, mid, nmid, list,
-                       g.off, g.nbr, moved, next);
+                       g.off, g.nbr, moved, dir, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
   const int64_t c7 = counts[kBins - 1];
@@ -370,27 +484,48 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
     items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
     hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s,
                        list + layout.base[kBins - 1], c7, hub_part_off, hub_item_of, g.off, g.nbr,
-                       moved,
-                       (const uint32_t*)nullptr, (const uint8_t*)nullptr, next);
+                       moved, (const uint32_t*)nullptr, (const uint8_t*)nullptr, dir, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
 }
 
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
-                          int64_t block_max, const uint32_t* hubs, int64_t hub_count,
-                          int64_t hub_te, const int64_t* hub_part_off,
-                          const uint32_t* hub_item_of, int64_t part_edges, uint8_t* next,
-                          hipStream_t s) {
+                          int64_t block_max, const HubList& hubs, int64_t part_edges,
+                          const ExpandDir& dir, uint8_t* next, hipStream_t s) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
-                     g.nbr, ids, modified, block_max, next);
+                     g.nbr, ids, modified, block_max, dir, next);
   PCC_HIP_CHECK(hipGetLastError());
-  if (hub_count > 0) {
-    int64_t items = hub_count + hub_te / part_edges + 1;
+  if (hubs.count > 0) {
+    int64_t items = hubs.count + hubs.te / part_edges + 1;
     items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
-    hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s, hubs, hub_count,
-                       hub_part_off, hub_item_of, g.off, g.nbr, (const uint8_t*)nullptr, ids,
-                       modified, next);
+    hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s, hubs.list,
+                       hubs.count, hubs.part_off, hubs.item_of, g.off, g.nbr,
+                       (const uint8_t*)nullptr, ids, modified, dir, next);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+}
+
+void LaunchSeedVolume(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                      const uint8_t* moved, unsigned long long* vol, hipStream_t s) {
+  if (g.n == 0) return;
+  hipLaunchKernelGGL(k_seed_volume, dim3(Grid(g.n, 256, 2048)), dim3(256), 0, s, g.n, g.off, ids,
+                     modified, moved, vol);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchPullFrontier(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                        const uint8_t* moved, int64_t block_max, const HubList& hubs,
+                        int64_t part_edges, const ExpandDir& dir, uint8_t* next, hipStream_t s) {
+  if (g.n == 0) return;
+  hipLaunchKernelGGL(k_pull_vertices, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
+                     g.nbr, ids, modified, moved, block_max, dir, next);
+  PCC_HIP_CHECK(hipGetLastError());
+  if (hubs.count > 0) {
+    int64_t items = hubs.count + hubs.te / part_edges + 1;
+    items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
+    hipLaunchKernelGGL(k_pull_hub, dim3((unsigned)items), dim3(256), 0, s, hubs.list, hubs.count,
+                       hubs.part_off, hubs.item_of, g.off, g.nbr, ids, modified, moved, dir, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
 }

@@ -224,9 +224,11 @@ void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long 
 // Async sub-rounds: cnt[b * S + s] (zeroed by the caller) counts the listed
 // vertices of bin b in sub-round s; with cur[b * S + s] set to the segment
 // starts within each bin region, LaunchSubScatter copies the list into out
-// grouped by (bin, sub-round).  S <= 1024.
-void LaunchSubCount(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
-                    uint32_t S, uint32_t* cnt, hipStream_t s);
+// grouped by (bin, sub-round).  te7[s] (zeroed by the caller) receives the
+// degree sum of sub-round s's bin-7 vertices.  S <= 1024.
+void LaunchSubCount(const CsrView& g, const uint32_t* list, const BinLayout& layout,
+                    const int64_t counts[kBins], uint32_t S, uint32_t* cnt,
+                    unsigned long long* te7, hipStream_t s);
 void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64_t counts[kBins],
                       uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s);
 
@@ -286,6 +288,26 @@ void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
                       hipStream_t s);
 
+// Direction of the next-frontier expansion (the sparse / dense switch of
+// gbbs's edgeMap): with vol null or *vol <= thr the push kernels mark the
+// neighbours of every seed; otherwise the pull kernels test every vertex for a
+// seed neighbour.  Both give the same frontier on a symmetric graph, so the
+// pull side is only enabled for graphs checked to be symmetric.
+struct ExpandDir {
+  const unsigned long long* vol = nullptr;
+  unsigned long long thr = 0;
+  __device__ __forceinline__ bool Pull() const { return vol != nullptr && *vol > thr; }
+};
+
+// Vertices with deg > block_max of a level graph, with partitions of
+// part_edges edges (prefix part_off, owner item_of; see LaunchHubScan).
+struct HubList {
+  const uint32_t* list = nullptr;
+  int64_t count = 0, te = 0;
+  const int64_t* part_off = nullptr;
+  const uint32_t* item_of = nullptr;
+};
+
 // Next frontier = neighbours of the seeds.  Bin 7 of the list is walked in
 // the partitions of hub_part_off (its HubScratch::part_off, partitions of
 // part_edges edges); te[7] = sum of its degrees.
@@ -293,15 +315,25 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
                        const int64_t* hub_part_off, const uint32_t* hub_item_of,
                        int64_t part_edges, const CsrView& g, const uint8_t* moved,
-                       uint8_t* next, hipStream_t s);
+                       const ExpandDir& dir, uint8_t* next, hipStream_t s);
 // Neighbours of every vertex whose cluster is modified.  Vertices with
-// deg > block_max are taken from hubs (hub_count of them, hub_te edges, with
-// partition prefix hub_part_off) instead of the vertex sweep.
+// deg > block_max are taken from hubs instead of the vertex sweep.
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
-                          int64_t block_max, const uint32_t* hubs, int64_t hub_count,
-                          int64_t hub_te, const int64_t* hub_part_off,
-                          const uint32_t* hub_item_of, int64_t part_edges, uint8_t* next,
-                          hipStream_t s);
+                          int64_t block_max, const HubList& hubs, int64_t part_edges,
+                          const ExpandDir& dir, uint8_t* next, hipStream_t s);
+// vol += sum of (deg + 1) over the seeds: modified[ids[v]] if ids is non-null,
+// else moved[v] (which must then hold 0 for every vertex that did not move).
+void LaunchSeedVolume(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                      const uint8_t* moved, unsigned long long* vol, hipStream_t s);
+// Pull side of the expansion (no-op unless dir selects it); hubs must list
+// every vertex with deg > block_max.
+void LaunchPullFrontier(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
+                        const uint8_t* moved, int64_t block_max, const HubList& hubs,
+                        int64_t part_edges, const ExpandDir& dir, uint8_t* next, hipStream_t s);
+// Symmetry fingerprint: true when the multiset of directed edges (u, v)
+// equals that of (v, u), decided by comparing sums of 64-bit hashes of the
+// pairs (a false "symmetric" needs a hash-sum collision, probability ~2^-64).
+bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 
 // ------------------------------------------------------------- primitives --
 

@@ -49,6 +49,7 @@ OPT_BLOCK_MAX_DEG = 4
 OPT_ASYNC_SUBROUNDS = 5
 OPT_ASYNC_SINGLETON_RULE = 6
 OPT_HUB_PART_EDGES = 7
+OPT_PULL_FRACTION = 8
 
 
 class StatusError(RuntimeError):

@@ -94,11 +94,14 @@ def test_best_moves_compressed_graph(pcc, orc):
         assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
 
 
+@pytest.mark.parametrize("pull", [512, 0, 1024])
 @pytest.mark.parametrize("move_method", [0, 1, 2])
 @pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs"])
-def test_round_bit_exact(pcc, orc, name, move_method):
+def test_round_bit_exact(pcc, orc, name, move_method, pull):
+    """pull: the frontier expansion's pull threshold (0: pull whenever any
+    vertex is a seed, 1024: push only, 512: the default switch)."""
     csr = _graphs(pcc)[name]
-    print(f"round {name} move_method={move_method}", flush=True)
+    print(f"round {name} move_method={move_method} pull={pull}", flush=True)
     rng = np.random.default_rng(9)
     n = csr.n
     ids = np.arange(n, dtype=np.uint32)
@@ -109,6 +112,7 @@ def test_round_bit_exact(pcc, orc, name, move_method):
     cfg = pcc.make_config(resolution=0.05, move_method=move_method)
     ocfg = orc.config_from(cfg)
     dg = _dev(pcc, csr)
+    dg.set_option(pcc.OPT_PULL_FRACTION, pull)
     og = orc.G.from_csr(csr)
     ostate = orc.S(ids, sizes, nw, K)
     gstate = (ids, sizes, nw, K)
@@ -127,14 +131,96 @@ def test_round_bit_exact(pcc, orc, name, move_method):
             break
 
 
+def _asymmetric(pcc, csr):
+    """Drops every third edge u -> v with u < v (by a hash of the pair), so
+    some edges have no reverse."""
+    deg = np.diff(csr.offsets)
+    src = np.repeat(np.arange(csr.n, dtype=np.int64), deg)
+    dst = csr.nbrs.astype(np.int64)
+    h = ((src * 1000003 + dst) * 0x9E3779B1) & 0xFFFFFFFF
+    keep = ~((src < dst) & ((h >> 5) % 3 == 0))
+    off = np.zeros(csr.n + 1, np.int64)
+    np.cumsum(np.bincount(src[keep], minlength=csr.n), out=off[1:])
+    return pcc.Csr(csr.n, off, csr.nbrs[keep].copy(), csr.weights[keep].copy())
+
+
+@pytest.mark.parametrize("move_method", [1, 2])
+def test_asymmetric_graph_expands_by_push(pcc, orc, move_method):
+    """On a graph whose edge set is not symmetric, pull and push frontiers
+    differ; the pull side must then stay off, whatever the threshold."""
+    csr = _asymmetric(pcc, _graphs(pcc)["rmat_signed"])
+    print(f"asymmetric round move_method={move_method} n={csr.n} m={csr.m}", flush=True)
+    n = csr.n
+    ids = np.arange(n, dtype=np.uint32)
+    sizes = np.ones(n, np.uint32)
+    nw = np.ones(n)
+    cfg = pcc.make_config(resolution=0.05, move_method=move_method)
+    dg = _dev(pcc, csr)
+    dg.set_option(pcc.OPT_PULL_FRACTION, 0)
+    og = orc.G.from_csr(csr)
+    ostate = orc.S(ids, sizes, nw, nw.copy())
+    gstate = (ids, sizes, nw.copy())
+    ofr = np.ones(n, np.uint8)
+    gfr = ofr.copy()
+    for r in range(3):
+        gi, gs, gk, gfr, gsize = dg.round(cfg, gstate[0], gstate[1], nw, gstate[2], gfr)
+        ofr, osize = orc.round_(og, ostate, orc.config_from(cfg), ofr)
+        assert np.array_equal(gi, ostate.ids), f"ids differ after round {r}"
+        assert np.array_equal(gfr, ofr), f"frontier differs after round {r}"
+        gstate = (gi, gs, gk)
+        if osize == 0:
+            break
+
+
+@pytest.mark.parametrize("pull", [0, 1024])
+@pytest.mark.parametrize("move_method", [1, 2])
+def test_cluster_pull_and_push_agree(pcc, orc, move_method, pull):
+    csr = _graphs(pcc)["rmat_unit_hubs"]
+    print(f"cluster pull={pull} move_method={move_method}", flush=True)
+    cfg = pcc.make_config(resolution=1.0, move_method=move_method, refine=1)
+    dg = _dev(pcc, csr)
+    dg.set_option(pcc.OPT_PULL_FRACTION, pull)
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), True)
+    assert np.array_equal(ids, r.ids)
+
+
+def test_compress_long_runs(pcc, orc):
+    """Two clusters joined by 4200 x 4200 edges with signed float weights: the
+    run of 17.6M terms has more than 1024 level-1 blocks, so the blocked sum
+    recurses; order-sensitive weights force ordered folds in every block."""
+    a = 4200
+    n = 2 * a
+    rng = np.random.default_rng(41)
+    off = np.arange(n + 1, dtype=np.int64) * a
+    left = np.arange(a, dtype=np.uint32)
+    nbrs = np.concatenate([np.tile(left + a, a), np.tile(left, a)])
+    w = rng.uniform(-1.0, 1.0, size=2 * a * a).astype(np.float32)
+    csr = pcc.Csr(n, off, nbrs, w)
+    ids = np.repeat(np.array([0, 1], np.uint32), a)
+    nw = rng.uniform(0.5, 2.0, size=n)
+    print(f"compress long runs n={n} m={csr.m}", flush=True)
+    cg, gnw = _dev(pcc, csr).compress(ids, nw)
+    c = cg.to_csr()
+    og, onw = orc.compress(orc.G.from_csr(csr), ids, nw)
+    assert c.n == og.n and c.m == og.m
+    assert np.array_equal(c.offsets, og.offsets)
+    assert np.array_equal(c.nbrs, og.nbrs)
+    assert np.array_equal(c.weights.view(np.uint32), og.weights.view(np.uint32))
+    assert np.array_equal(gnw.view(np.uint64), onw.view(np.uint64))
+
+
 @pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs"])
 def test_compress_bit_exact(pcc, orc, name):
     csr = _graphs(pcc)[name]
     print(f"compress {name}", flush=True)
     rng = np.random.default_rng(2)
-    for k in (1, 7, csr.n // 10 + 1, csr.n):
+    for k, fine in ((1, False), (7, False), (csr.n // 10 + 1, False), (csr.n, False), (1, True),
+                    (7, True)):
         ids = rng.integers(0, k, size=csr.n).astype(np.uint32)
-        nw = rng.integers(1, 5, size=csr.n).astype(np.float64)
+        # Integer node weights sum exactly in any order; uniform ones do not.
+        nw = rng.uniform(0.0, 1.0, size=csr.n) if fine else \
+            rng.integers(1, 5, size=csr.n).astype(np.float64)
         dg = _dev(pcc, csr)
         cg, gnw = dg.compress(ids, nw)
         c = cg.to_csr()
@@ -143,7 +229,7 @@ def test_compress_bit_exact(pcc, orc, name):
         assert np.array_equal(c.offsets, og.offsets)
         assert np.array_equal(c.nbrs, og.nbrs)
         assert np.array_equal(c.weights.view(np.uint32), og.weights.view(np.uint32))
-        assert np.array_equal(gnw, onw)
+        assert np.array_equal(gnw.view(np.uint64), onw.view(np.uint64))
 
 
 CLUSTER_CASES = [
