@@ -30,7 +30,7 @@ BIN_NAMES = ["deg<=4 (4 lanes)", "deg<=8 (8 lanes)", "deg<=16 (16 lanes)", "deg<
 BIN_KERNELS = ["k_bm_small<4>", "k_bm_small<8>", "k_bm_small<16>", "k_bm_small<32>",
                "k_bm_small<64>", "k_bm_hash<64,256> + k_bm_hash<64,512> + k_bm_hash<128,1024>",
                "k_bm_hash<256,2048> + k_bm_hash<512,4096> + k_bm_hash<1024,8192>",
-               "k_hub_scan + k_hub_gather + k_bm_hub_part + k_bm_hub_cand + k_bm_hub_final"]
+               "k_hub_scan + k_hub_gather + k_bm_hub_part + k_bm_hub_final"]
 
 
 def apply_tuning(g, args, pcc_amd):

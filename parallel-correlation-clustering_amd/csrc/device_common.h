@@ -196,6 +196,27 @@ __device__ __forceinline__ T BlockReduce(T x, T* smem, Op op) {
   return r;
 }
 
+// Rows longer than kLongRow edges are split over workgroups: chunk j of the
+// q-th queued row goes to workgroup (j + 7 q) mod gridDim.x, so one long row
+// does not leave a single wave working while the rest of the grid idles.
+constexpr int64_t kLongRow = 16384;
+constexpr int64_t kLongChunk = 4096;
+template <class F>
+__device__ __forceinline__ void ForLongChunks(const uint32_t* __restrict__ rows, uint32_t nrows,
+                                              const int64_t* __restrict__ off, F f) {
+  const int64_t G = gridDim.x;
+  for (uint32_t q = 0; q < nrows; ++q) {
+    const uint32_t v = rows[q];
+    const int64_t st = off[v], deg = off[v + 1] - st;
+    const int64_t nch = (deg + kLongChunk - 1) / kLongChunk;
+    int64_t j = ((int64_t)blockIdx.x - (7 * (int64_t)q) % G + G) % G;
+    for (; j < nch; j += G) {
+      const int64_t b = j * kLongChunk;
+      f(q, v, st + b, st + (b + kLongChunk < deg ? b + kLongChunk : deg));
+    }
+  }
+}
+
 // Exclusive block scan; `smem` needs blockDim.x / 64 slots.  *total receives
 // the block sum.
 template <typename T>

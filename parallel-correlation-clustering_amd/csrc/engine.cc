@@ -362,7 +362,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   c->hs.vals = c->hvals.get();
   if (hashed) {
     c->gscale.Reset(al_, g.n);
-    LaunchGranThreshold(g, cfg_.edge_weight_offset, c->gscale.get(), s_);
+    LaunchGranThreshold(al_, g, cfg_.edge_weight_offset, c->gscale.get(), s_);
   }
   c->hb = HubScratch();
   const int64_t c7 = c->caps[kBins - 1], t7 = c->caps_te[kBins - 1];
@@ -370,33 +370,24 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb_part.Reset(al_, c7 + 1);
     c->hb_ent.Reset(al_, c7 + 1);
     c->hb_cbuf.Reset(al_, t7);
-    c->hb_ekeys.Reset(al_, t7);
-    c->hb_evals.Reset(al_, t7);
-    c->hb_ecur.Reset(al_, c7);
     c->hb_flag.Reset(al_, c7);
     c->hb_tval.Reset(al_, c7);
     // Partition items of any sub-list: sum ceil(deg / E) < t7 / E + c7.
     const int64_t items = c7 + t7 / h_->hub_part_edges + 1;
-    c->hb_pe_base.Reset(al_, items);
-    c->hb_pe_cnt.Reset(al_, items);
-    c->hb_pbest_c.Reset(al_, items);
-    c->hb_pbest_d.Reset(al_, items);
+    c->hb_px1.Reset(al_, items);
+    c->hb_pc1.Reset(al_, items);
+    c->hb_px2.Reset(al_, items);
     c->hb_item_of.Reset(al_, items);
-    PCC_HIP_CHECK(hipMemsetAsync(c->hb_ecur.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_flag.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     c->hb.part_edges = h_->hub_part_edges;
     c->hb.part_off = c->hb_part.get();
     c->hb.ent_off = c->hb_ent.get();
     c->hb.cbuf = c->hb_cbuf.get();
-    c->hb.ekeys = c->hb_ekeys.get();
-    c->hb.evals = c->hb_evals.get();
-    c->hb.ecur = c->hb_ecur.get();
     c->hb.hflag = c->hb_flag.get();
     c->hb.tval = c->hb_tval.get();
-    c->hb.pe_base = c->hb_pe_base.get();
-    c->hb.pe_cnt = c->hb_pe_cnt.get();
-    c->hb.pbest_c = c->hb_pbest_c.get();
-    c->hb.pbest_d = c->hb_pbest_d.get();
+    c->hb.px1 = c->hb_px1.get();
+    c->hb.pc1 = c->hb_pc1.get();
+    c->hb.px2 = c->hb_px2.get();
     c->hb.item_of = c->hb_item_of.get();
     if (cfg_.move_method != PCC_ALL_MOVE) {
       c->hubs.Reset(al_, c7);

@@ -134,9 +134,8 @@ class Engine {
     // Bin-7 scratch, sized for every bin-7 vertex of the level.
     HubScratch hb;
     DevBuf<int64_t> hb_part, hb_ent;
-    DevBuf<uint32_t> hb_cbuf, hb_ekeys, hb_ecur, hb_flag, hb_pe_base, hb_pe_cnt, hb_pbest_c,
-        hb_item_of;
-    DevBuf<double> hb_evals, hb_tval, hb_pbest_d;
+    DevBuf<uint32_t> hb_cbuf, hb_flag, hb_pc1, hb_item_of;
+    DevBuf<double> hb_tval, hb_px1, hb_px2;
     // Every bin-7 vertex with its partition prefix (the NBHR_CLUSTER_MOVE
     // expansion and the pull expansion).
     DevBuf<uint32_t> hubs, hubs_item_of;

@@ -276,7 +276,9 @@ __global__ __launch_bounds__(kSubThreads) void k_sub_scatter(const uint32_t* __r
 // per such vertex.
 __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t* __restrict__ off,
                                                         const float* __restrict__ w, double offset,
-                                                        int32_t* __restrict__ gscale) {
+                                                        int32_t* __restrict__ gscale,
+                                                        uint32_t* __restrict__ longq,
+                                                        unsigned int* __restrict__ nlong) {
   const double ao = fabs(offset);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -285,8 +287,9 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
     if (v < n) {
       st = off[v];
       deg = off[v + 1] - st;
+      if (deg > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;  // k_gran_long
     }
-    uint64_t big = __ballot(v < n && deg > 64);
+    uint64_t big = __ballot(v < n && deg > 64 && deg <= kLongRow);
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
@@ -316,6 +319,44 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
       if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(ThresholdFor(s), MagExpFor(mx));
     }
   }
+}
+
+// Rows longer than kLongRow: chunk sums and maxima combined with atomics (the
+// bound B only has to hold, not to be reproducible; ThresholdFor leaves a
+// margin for the rounding of any summation order), then packed.
+__global__ __launch_bounds__(256) void k_gran_long(const int64_t* __restrict__ off,
+                                                   const float* __restrict__ w, double offset,
+                                                   const uint32_t* __restrict__ longq,
+                                                   const unsigned int* __restrict__ nlong,
+                                                   double* __restrict__ acc_s,
+                                                   unsigned long long* __restrict__ acc_m) {
+  __shared__ double sm[256 / kWave];
+  const double ao = fabs(offset);
+  ForLongChunks(longq, *nlong, off, [&](uint32_t q, uint32_t, int64_t e0, int64_t e1) {
+    double s = 0, mx = 0;
+    for (int64_t p = e0 + threadIdx.x; p < e1; p += blockDim.x) {
+      const double a = fabs((double)w[p]) + ao;
+      s += a;
+      mx = fmax(mx, a);
+    }
+    s = BlockReduce(s, sm, [](double x, double y) { return x + y; });
+    mx = BlockReduce(mx, sm, [](double x, double y) { return fmax(x, y); });
+    if (threadIdx.x == 0) {
+      atomicAdd(&acc_s[q], s);
+      // Non-negative doubles order like their bit patterns.
+      atomicMax(&acc_m[q], (unsigned long long)__double_as_longlong(mx));
+    }
+  });
+}
+
+__global__ __launch_bounds__(256) void k_gran_long_final(const uint32_t* __restrict__ longq,
+                                                         const unsigned int* __restrict__ nlong,
+                                                         const double* __restrict__ acc_s,
+                                                         const unsigned long long* __restrict__ acc_m,
+                                                         int32_t* __restrict__ gscale) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < *nlong; q += gridDim.x * blockDim.x)
+    gscale[longq[q]] = PackScale(ThresholdFor(acc_s[q]),
+                                 MagExpFor(__longlong_as_double((long long)acc_m[q])));
 }
 
 // Bin-7 prefix sums of a list, one workgroup.
@@ -634,8 +675,18 @@ struct SlotList {
   uint16_t* occ;
   uint32_t* n;
   uint32_t cap_list;
-  __device__ __forceinline__ bool Add(int slot) const {
-    const uint32_t k = atomicAdd(n, 1u);
+  // Appends the fresh slots of a wave with one counter atomic per wave (a
+  // shared counter hit by every inserting lane serializes); called by all
+  // lanes of the wave.  False for a lane whose slot did not fit.
+  __device__ __forceinline__ bool AddWave(bool fresh, int slot) const {
+    const uint64_t fm = __ballot(fresh);
+    if (fm == 0) return true;
+    const int leader = __ffsll((unsigned long long)fm) - 1;
+    uint32_t base = 0;
+    if (LaneId() == leader) base = atomicAdd(n, (uint32_t)__popcll(fm));
+    base = __shfl(base, leader);
+    if (!fresh) return true;
+    const uint32_t k = base + (uint32_t)__popcll(fm & LaneMaskLt());
     if (k >= cap_list) return false;
     occ[k] = (uint16_t)slot;
     return true;
@@ -810,54 +861,6 @@ struct BufferedClusters {
   __device__ __forceinline__ uint32_t Cluster(uint32_t x) const { return x; }
 };
 
-// Adds the neighbours of v whose cluster falls in partition `part` of
-// `parts` into the LDS table (keys / vals / cnt, cap = mask + 1 slots).  Each
-// thread keeps U edges in flight.  Returns true if an insert found the table
-// full.  The value added is EfficientBestMove's: (double)w - offset on the
-// low-degree path, the float w -= offset on the hub path (PCCI.cc:466, 529).
-template <int THREADS, bool kAsync, class Src>
-__device__ __forceinline__ bool Accumulate(const CsrView& g, const MoveParams& mp, Src src,
-                                           int64_t start, int64_t deg, bool hub, int thr,
-                                           uint32_t* keys, double* vals, uint32_t* cnt,
-                                           uint32_t mask, uint32_t parts, uint32_t part,
-                                           const SlotList& sl) {
-  constexpr int U = 4;
-  bool overflow = false;
-  for (int64_t q0 = threadIdx.x; q0 < deg; q0 += (int64_t)THREADS * U) {
-    uint32_t c[U];
-    bool in[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t q = q0 + (int64_t)k * THREADS;
-      in[k] = q < deg;
-      c[k] = in[k] ? src.Neighbour(q) : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) c[k] = in[k] ? src.Cluster(c[k]) : kNoneId;
-    float wf[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t q = q0 + (int64_t)k * THREADS;
-      if (c[k] != kNoneId && parts > 1 && PartOf(c[k], parts) != part) c[k] = kNoneId;
-      wf[k] = c[k] != kNoneId ? g.w[start + q] : 0.0f;
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      if (c[k] == kNoneId) continue;
-      const double x = EdgeValue(wf[k], hub, mp.offset);
-      bool fresh;
-      const int slot = HashSlot(keys, mask, c[k], &fresh);
-      if (slot < 0 || (fresh && !sl.Add(slot))) {
-        overflow = true;
-        continue;
-      }
-      atomicAdd(&vals[slot], x);
-      if (!kAsync) RecordTerm(cnt, slot, x, thr);
-    }
-  }
-  return overflow;
-}
-
 // Accumulate over gathered clusters cb[0, deg) (partition `part` of
 // `parts`).  Only about 1/parts of the entries belong to the partition, so
 // each wave first compacts the positions of its matching entries into a
@@ -873,11 +876,14 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
                                                    uint32_t parts, uint32_t part, uint32_t* wq,
                                                    const SlotList& sl) {
   bool overflow = false;
-  auto insert = [&](uint32_t c, float wf) {
-    const double x = EdgeValue(wf, hub, mp.offset);
-    bool fresh;
-    const int slot = HashSlot(keys, mask, c, &fresh);
-    if (slot < 0 || (fresh && !sl.Add(slot))) {
+  // Called by every lane of the wave; act: this lane has an entry.
+  auto insert = [&](bool act, uint32_t c, float wf) {
+    const double x = act ? EdgeValue(wf, hub, mp.offset) : 0.0;
+    bool fresh = false;
+    const int slot = act ? HashSlot(keys, mask, c, &fresh) : -1;
+    const bool listed = sl.AddWave(fresh, slot);
+    if (!act) return;
+    if (slot < 0 || !listed) {
       overflow = true;
       return;
     }
@@ -905,9 +911,11 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
     for (uint32_t e = 0; e < 4; ++e)
       if (m & (1u << e)) q[at++] = q0 + e;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k < tot; k += kWave) {
-      const uint32_t qq = q[k];
-      insert(cb[qq], wv[qq]);
+    for (uint32_t k0 = 0; k0 < tot; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const bool act = k < tot;
+      const uint32_t qq = act ? q[k] : 0u;
+      insert(act, act ? cb[qq] : 0u, act ? wv[qq] : 0.0f);
     }
     __builtin_amdgcn_wave_barrier();
   };
@@ -942,17 +950,30 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
   return overflow;
 }
 
+// Per-vertex values the hashed kernels read besides the adjacency.
+struct HashVertex {
+  uint32_t v = 0;
+  int64_t start = 0, deg = 0;
+  uint32_t cv = 0;   // current cluster
+  double k = 0;      // node weight
+  int32_t scale = 0; // order-free test scale (sync)
+  double Kcv = 0;    // weight of the current cluster
+};
+
 // Bins 5 and 6: THREADS cooperating threads and a TABLE-entry LDS table per
 // vertex.  Each thread holds E = TABLE / (2 THREADS) edges of a vertex in
 // registers (the launch ranges keep deg <= TABLE / 2), and the workgroup runs
-// a two-vertex software pipeline: the adjacency loads of the next vertex are
-// issued before the current vertex's inserts, and its cluster gathers before
-// the current vertex's decision, so both latencies overlap work instead of
-// adding up vertex after vertex.  In sync mode the state is fixed for the
-// round, so reading the next vertex early changes nothing; in async mode it
-// is one more interleaving of the concurrent moves.
+// a software pipeline over its vertices: the adjacency and per-vertex loads of
+// the next vertex are issued before the current vertex's inserts, its cluster
+// gathers before the current vertex's decision, and (sync) thread 0 finishes
+// a vertex's epilogue while the next one is being inserted, so the memory
+// latencies overlap work instead of adding up vertex after vertex.  In sync
+// mode the state is fixed for the round, so reading ahead changes nothing; in
+// async mode it is one more interleaving of the concurrent moves.
+// At least 4 waves per SIMD (128 VGPRs), so the LDS budget, not registers,
+// sets how many workgroups share a CU.
 template <int THREADS, int TABLE, bool kAsync>
-__global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
+__global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
                                                      const uint32_t* __restrict__ list,
                                                      int64_t count, int64_t dlo, int64_t dhi,
                                                      MoveOut out,
@@ -984,12 +1005,12 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
 
   // The workgroup's next vertex in (dlo, dhi] at or after item i (uniform).
   // Vertices routed to the ordered kernel are queued on the way.
-  auto next_item = [&](int64_t& i, uint32_t& v, int64_t& start, int64_t& deg) -> bool {
+  auto next_item = [&](int64_t& i, HashVertex& h) -> bool {
     for (; i < count; i += gridDim.x) {
-      v = list[PermuteIndex(i, count, salt)];
+      const uint32_t v = list[PermuteIndex(i, count, salt)];
       if (kAsync && !InSubRound(mp, v)) continue;
-      start = g.off[v];
-      deg = g.off[v + 1] - start;
+      const int64_t start = g.off[v];
+      const int64_t deg = g.off[v + 1] - start;
       if (deg <= dlo || deg > dhi) continue;  // another launch's degree range
       if (deg > (int64_t)E * THREADS || (!kAsync && mp.force_ordered)) {
         if (threadIdx.x == 0) {
@@ -1000,60 +1021,79 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
         }
         continue;
       }
+      h.v = v;
+      h.start = start;
+      h.deg = deg;
       return true;
     }
     return false;
   };
-  auto load_edges = [&](bool ok, int64_t start, int64_t deg, uint32_t* nb, float* wf) {
+  auto load_vertex = [&](bool ok, HashVertex& h, uint32_t* nb, float* wf) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int64_t q = threadIdx.x + (int64_t)e * THREADS;
-      const bool in = ok && q < deg;
-      nb[e] = in ? g.nbr[start + q] : kNoneId;
-      wf[e] = in ? g.w[start + q] : 0.0f;
+      const bool in = ok && q < h.deg;
+      nb[e] = in ? g.nbr[h.start + q] : kNoneId;
+      wf[e] = in ? g.w[h.start + q] : 0.0f;
+    }
+    if (ok) {
+      h.cv = st.ids[h.v];
+      h.k = st.nw[h.v];
+      h.scale = kAsync ? 0 : mp.gscale[h.v];
     }
   };
-  auto gather = [&](uint32_t v, const uint32_t* nb, uint32_t* cc) {
+  auto gather = [&](bool ok, HashVertex& h, const uint32_t* nb, uint32_t* cc) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) cc[e] = (nb[e] != kNoneId && nb[e] != v) ? st.ids[nb[e]] : kNoneId;
+    for (int e = 0; e < E; ++e)
+      cc[e] = (nb[e] != kNoneId && nb[e] != h.v) ? st.ids[nb[e]] : kNoneId;
+    if (ok) h.Kcv = st.K[h.cv];
   };
+  // Sync epilogue of a decided vertex, split so that its loads are issued one
+  // stage before their use (SyncEpilogue's rule).
+  bool pend = false;
+  uint32_t pv = 0, pt = 0, pcv = 0, psz_t = 0, psz_c = 0;
+  double pd = 0.0;
 
   // Consecutive list items (nearby vertex ids, which share neighbourhoods in
   // RMAT-like graphs) go to workgroups of one XCD.
   int64_t i = XcdItem(blockIdx.x, gridDim.x);
-  uint32_t v = 0;
-  int64_t start = 0, deg = 0;
-  bool have = next_item(i, v, start, deg);
+  HashVertex cur, nxt;
+  bool have = next_item(i, cur);
   uint32_t cc[E];
   float wf[E];
   {
     uint32_t nb[E];
-    load_edges(have, start, deg, nb, wf);
-    gather(v, nb, cc);
+    load_vertex(have, cur, nb, wf);
+    gather(have, cur, nb, cc);
   }
   int64_t i2 = i + gridDim.x;
-  uint32_t v2 = 0;
-  int64_t start2 = 0, deg2 = 0;
-  bool have2 = have && next_item(i2, v2, start2, deg2);
+  bool have2 = have && next_item(i2, nxt);
   while (have) {
-    // Stage 1: the next vertex's adjacency.
+    // Stage 1: the next vertex's adjacency and values; the pending
+    // epilogue's cluster sizes.
     uint32_t nb2[E];
     float wf2[E];
-    load_edges(have2, start2, deg2, nb2, wf2);
+    load_vertex(have2, nxt, nb2, wf2);
+    if (!kAsync && pend && threadIdx.x == 0) {
+      psz_t = pt != mp.new_id ? st.sizes[pt] : 0u;
+      psz_c = st.sizes[pcv];
+    }
     // Stage 2: this vertex's inserts (probe range sized for load <= 1/2).
+    const int64_t deg = cur.deg;
     const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * deg + 2))));
     const uint32_t mask = cap - 1;
     const bool hub = deg > kHubDegree;
-    const int32_t scale = kAsync ? 0 : mp.gscale[v];
-    const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
+    const int thr = ScaleThr(cur.scale), mexp = ScaleMexp(cur.scale);
     bool overflow = false;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      if (cc[e] == kNoneId) continue;
-      const double x = EdgeValue(wf[e], hub, mp.offset);
-      bool fresh;
-      const int slot = HashSlot(keys, mask, cc[e], &fresh);
-      if (slot < 0 || (fresh && !sl.Add(slot))) {
+      const bool act = cc[e] != kNoneId;
+      const double x = act ? EdgeValue(wf[e], hub, mp.offset) : 0.0;
+      bool fresh = false;
+      const int slot = act ? HashSlot(keys, mask, cc[e], &fresh) : -1;
+      const bool listed = sl.AddWave(fresh, slot);
+      if (!act) continue;
+      if (slot < 0 || !listed) {
         overflow = true;
         continue;
       }
@@ -1061,14 +1101,27 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
       if (!kAsync) RecordTerm(cnt, slot, x, thr);
     }
     const bool ovf = __syncthreads_or(overflow);
-    // Stage 3: the next vertex's cluster gathers and the item after it.
+    // Stage 3: the next vertex's cluster gathers, the item after it, and the
+    // pending epilogue.
     uint32_t cc2[E];
-    gather(v2, nb2, cc2);
+    gather(have2, nxt, nb2, cc2);
     int64_t i3 = i2 + gridDim.x;
-    uint32_t v3 = 0;
-    int64_t start3 = 0, deg3 = 0;
-    const bool have3 = have2 && next_item(i3, v3, start3, deg3);
+    HashVertex nx2;
+    const bool have3 = have2 && next_item(i3, nx2);
+    if (!kAsync && pend && threadIdx.x == 0) {
+      uint32_t t = pt;
+      double d = pd;
+      if (t != mp.new_id && psz_t == 1 && psz_c == 1 && pcv >= t) {
+        t = pcv;
+        d = 0.0;
+      }
+      const bool mv = d > 0;
+      out.moves[pv] = mv ? t : kNoneId;
+      out.moved[pv] = mv ? 1 : 0;
+      pend = false;
+    }
     // Stage 4: this vertex's decision.
+    const uint32_t v = cur.v;
     const uint32_t n_occ = min(nocc, kList);
     if (ovf) {  // cannot happen at load <= 1/2; kept as a guard
       if (threadIdx.x == 0) {
@@ -1078,7 +1131,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
           fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
       }
     } else {
-      const uint32_t cv = st.ids[v];
+      const uint32_t cv = cur.cv;
       const int own = HashFind(keys, mask, cv);
       double T = own >= 0 ? vals[own] : 0.0;
       if (!kAsync) {
@@ -1086,16 +1139,16 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
         for (uint32_t k = threadIdx.x; k < n_occ; k += THREADS)
           inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
         if (__syncthreads_or(inexact)) {
-          const AdjacencyClusters src{g.nbr + start, st.ids, v};
-          T = RepairInexactSlots<THREADS>(g, mp, src, start, deg, hub, keys, vals, cnt, cap, 1, 0,
-                                          own, thr, mexp, occ, n_occ, &stg);
+          const AdjacencyClusters src{g.nbr + cur.start, st.ids, v};
+          T = RepairInexactSlots<THREADS>(g, mp, src, cur.start, deg, hub, keys, vals, cnt, cap, 1,
+                                          0, own, thr, mexp, occ, n_occ, &stg);
           if (threadIdx.x == 0) atomicAdd(&rc->repair_count, 1ull);
         }
       }
-      const double k = st.nw[v];
+      const double k = cur.k;
       const double res = ResOf(mp, v);
       double change = 0;
-      change -= NetWeight(T, k * (st.K[cv] - k), res);
+      change -= NetWeight(T, k * (cur.Kcv - k), res);
       double mw = 0;
       mw += k;
       // At most E slots per thread (n_occ <= deg <= E * THREADS): their
@@ -1121,10 +1174,17 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
         uint32_t t;
         double d;
         Decide(best, change, hub, mp.new_id, &t, &d);
-        if (kAsync)
+        if (kAsync) {
           AsyncEpilogue(v, t, d, mp, st, out);
-        else
-          SyncEpilogue(v, t, d, mp.new_id, st, out);
+        } else {
+          if (out.dbg_target) out.dbg_target[v] = t;
+          if (out.dbg_delta) out.dbg_delta[v] = d;
+          pend = true;
+          pv = v;
+          pt = t;
+          pd = d;
+          pcv = cv;
+        }
       }
     }
     __syncthreads();
@@ -1148,9 +1208,7 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     __syncthreads();
     // Rotate the pipeline.
     have = have2;
-    v = v2;
-    start = start2;
-    deg = deg2;
+    cur = nxt;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       cc[e] = cc2[e];
@@ -1158,10 +1216,10 @@ __global__ __launch_bounds__(THREADS) void k_bm_hash(CsrView g, StateView st, Mo
     }
     i2 = i3;
     have2 = have3;
-    v2 = v3;
-    start2 = start3;
-    deg2 = deg3;
+    nxt = nx2;
   }
+  if (!kAsync && pend && threadIdx.x == 0)
+    SyncEpilogue(pv, pt, pd, mp.new_id, st, out);
 }
 
 // Bin 7, gather pass: the cluster of every neighbour of the listed vertices,
@@ -1201,9 +1259,45 @@ __global__ __launch_bounds__(256) void k_hub_gather(CsrView g, StateView st, Mov
   }
 }
 
+// Top two candidate values (see HubScratch::px1): x1 with the smallest
+// cluster holding it, x2 the largest value strictly below x1.
+struct Top2 {
+  double x1 = -INFINITY;
+  uint32_t c1 = kNoneId;
+  double x2 = -INFINITY;
+};
+__device__ __forceinline__ Top2 MergeTop2(Top2 a, Top2 b) {
+  if (a.c1 == kNoneId) return Top2{b.x1, b.c1, b.x2 > a.x2 ? b.x2 : a.x2};
+  if (b.c1 == kNoneId) return Top2{a.x1, a.c1, a.x2 > b.x2 ? a.x2 : b.x2};
+  if (a.x1 > b.x1) return Top2{a.x1, a.c1, fmax(a.x2, b.x1)};
+  if (b.x1 > a.x1) return Top2{b.x1, b.c1, fmax(b.x2, a.x1)};
+  return Top2{a.x1, min(a.c1, b.c1), fmax(a.x2, b.x2)};
+}
+__device__ __forceinline__ Top2 WaveTop2(Top2 x) {
+  for (int s = kWave >> 1; s > 0; s >>= 1) {
+    Top2 y;
+    y.x1 = __shfl_xor(x.x1, s);
+    y.c1 = __shfl_xor(x.c1, s);
+    y.x2 = __shfl_xor(x.x2, s);
+    x = MergeTop2(x, y);
+  }
+  return x;
+}
+template <int THREADS>
+__device__ __forceinline__ Top2 BlockTop2(Top2 x, Top2* sm) {
+  x = WaveTop2(x);
+  __syncthreads();
+  if (LaneId() == 0) sm[threadIdx.x / kWave] = x;
+  __syncthreads();
+  Top2 r = sm[0];
+  for (int i = 1; i < THREADS / kWave; ++i) r = MergeTop2(r, sm[i]);
+  return r;
+}
+
 // Bin 7, partition pass: work item t (partition p of list item i) builds the
-// LDS table of v's neighbour clusters in partition p and appends its entries
-// to v's entry region; the partition holding c(v) records S_{c(v)}.
+// LDS table of v's neighbour clusters in partition p and reduces them to the
+// partition's top two candidate values; the partition holding c(v) records
+// S_{c(v)}.
 constexpr int kHubThreads = 512;
 
 template <int TABLE, bool kAsync>
@@ -1215,8 +1309,6 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   __shared__ uint32_t keys[TABLE];
   __shared__ double vals[TABLE];
   __shared__ uint32_t cnt[TABLE];
-  __shared__ uint32_t sw[kHubThreads / kWave];
-  __shared__ uint32_t s_base;
   __shared__ RepairStage<kHubThreads> stg;
   constexpr uint32_t kList = TABLE * 3 / 4;
   __shared__ uint16_t occ[kList];
@@ -1251,10 +1343,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
       g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part,
       reinterpret_cast<uint32_t*>(&stg), sl);
   if (__syncthreads_or(overflow)) {
-    if (threadIdx.x == 0) {
-      atomicOr(&hb.hflag[i], 2u);
-      hb.pe_cnt[t] = 0;
-    }
+    if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
     return;
   }
   const uint32_t cv = st.ids[v];
@@ -1275,27 +1364,23 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     hb.tval[i] = T;
     atomicOr(&hb.hflag[i], 1u);
   }
-  // Copy the occupied slots into the entry region.
-  uint32_t mine = 0;
-  for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) ++mine;
-  uint32_t total = 0;
-  uint32_t at = BlockExclusiveScan(mine, sw, &total);
-  if (total == 0) {
-    if (threadIdx.x == 0) hb.pe_cnt[t] = 0;
-    return;
-  }
-  if (threadIdx.x == 0) {
-    s_base = atomicAdd(&hb.ecur[i], total);
-    hb.pe_base[t] = s_base;
-    hb.pe_cnt[t] = total;
-  }
-  __syncthreads();
-  const int64_t e0 = hb.ent_off[i] + s_base;
+  // The partition's candidates, reduced to the top two values of
+  // x_c = NetWeight(S_c, k_v K_c, res); the decision adds the own-cluster
+  // term, which only the partition holding c(v) knows, afterwards.
+  const double res = ResOf(mp, v);
+  double mw = 0;
+  mw += st.nw[v];
+  Top2 top;
   for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) {
     const uint32_t s = occ[k];
-    hb.ekeys[e0 + at] = keys[s];
-    hb.evals[e0 + at] = vals[s];
-    ++at;
+    const uint32_t c = keys[s];
+    top = MergeTop2(top, Top2{NetWeight(vals[s], mw * st.K[c], res), c, -INFINITY});
+  }
+  top = BlockTop2<kHubThreads>(top, reinterpret_cast<Top2*>(&stg));
+  if (threadIdx.x == 0) {
+    hb.px1[t] = top.x1;
+    hb.pc1[t] = top.c1;
+    hb.px2[t] = top.x2;
   }
 }
 
@@ -1311,47 +1396,12 @@ __device__ __forceinline__ double HubChange(const StateView& st, const MoveParam
   return change;
 }
 
-// Bin 7, candidate pass: one workgroup per partition item reduces that
-// partition's entries to its best candidate (pbest_*[t]).
-constexpr int kSelThreads = 256;
-
-template <bool kAsync>
-__global__ __launch_bounds__(kSelThreads) void k_bm_hub_cand(CsrView g, StateView st,
-                                                             MoveParams mp,
-                                                             const uint32_t* __restrict__ list,
-                                                             int64_t count, HubScratch hb) {
-  __shared__ double sd[kSelThreads / kWave];
-  __shared__ uint32_t sc[kSelThreads / kWave];
-  const int64_t t = XcdChunkItem(blockIdx.x);
-  if (t >= hb.part_off[count]) return;
-  const int64_t i = hb.item_of[t];
-  const uint32_t v = list[i];
-  if (kAsync && !InSubRound(mp, v)) return;
-  if (!kAsync && mp.force_ordered) return;
-  const uint32_t flag = hb.hflag[i];
-  if (flag & 2u) return;
-  const double change = HubChange(st, mp, hb, i, v, flag);
-  const double res = ResOf(mp, v);
-  double mw = 0;
-  mw += st.nw[v];
-  const int64_t e0 = hb.ent_off[i] + hb.pe_base[t];
-  const uint32_t ne = hb.pe_cnt[t];
-  Cand best{0.0, kNoneId};
-  for (uint32_t j = threadIdx.x; j < ne; j += kSelThreads) {
-    const uint32_t c = hb.ekeys[e0 + j];
-    const Cand x{change + NetWeight(hb.evals[e0 + j], mw * st.K[c], res), c};
-    best = PickBetter(best, x);
-  }
-  best = BlockBest<kSelThreads>(best, sd, sc);
-  if (threadIdx.x == 0) {
-    hb.pbest_d[t] = best.d;
-    hb.pbest_c[t] = best.c;
-  }
-}
-
-// Bin 7, decision pass: one wave per list item reduces the partition bests to
-// EfficientBestMove's decision, applies the sync or async epilogue (or lists
-// the vertex for the ordered kernel) and resets the item's counters.
+// Bin 7, decision pass: one wave per list item merges the partitions' top two
+// values, adds the own-cluster term and applies EfficientBestMove's decision:
+// the best delta is fl(change + x1) for the smallest cluster with x = x1,
+// unless a smaller x2 rounds to the same delta, in which case a larger id
+// with a smaller x could tie; such vertices (sync) go to the ordered kernel.
+// Then the sync or async epilogue runs and the item's flags are reset.
 template <bool kAsync>
 __global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, MoveParams mp,
                                                       const uint32_t* __restrict__ list,
@@ -1364,10 +1414,7 @@ __global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, M
   for (int64_t i = wave; i < count; i += nwaves) {
     const uint32_t v = list[i];
     const uint32_t flag = hb.hflag[i];
-    if (lane == 0) {
-      hb.hflag[i] = 0;
-      hb.ecur[i] = 0;
-    }
+    if (lane == 0) hb.hflag[i] = 0;
     if (kAsync && !InSubRound(mp, v)) continue;
     if (!kAsync && (mp.force_ordered || (flag & 2u))) {
       if (lane == 0) fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
@@ -1377,13 +1424,23 @@ __global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, M
       if (lane == 0) out.moved[v] = 0;
       continue;
     }
-    Cand best{0.0, kNoneId};
+    Top2 top;
     for (int64_t t = hb.part_off[i] + lane; t < hb.part_off[i + 1]; t += kWave)
-      best = PickBetter(best, Cand{hb.pbest_d[t], hb.pbest_c[t]});
-    best = GroupBest(best, kWave);
+      top = MergeTop2(top, Top2{hb.px1[t], hb.pc1[t], hb.px2[t]});
+    top = WaveTop2(top);
     if (lane == 0) {
       const int64_t deg = g.off[v + 1] - g.off[v];
       const double change = HubChange(st, mp, hb, i, v, flag);
+      Cand best{0.0, kNoneId};
+      bool tied = false;
+      if (top.c1 != kNoneId) {
+        best = Cand{change + top.x1, top.c1};
+        tied = top.x2 > -INFINITY && change + top.x2 == best.d;
+      }
+      if (!kAsync && tied) {
+        fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;
+        continue;
+      }
       uint32_t tt;
       double d;
       Decide(best, change, deg > kHubDegree, mp.new_id, &tt, &d);
@@ -1566,11 +1623,26 @@ void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchGranThreshold(const CsrView& g, double offset, int32_t* gscale, hipStream_t s) {
+void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, int32_t* gscale,
+                         hipStream_t s) {
   if (g.n == 0) return;
+  const int64_t maxlong = g.m / kLongRow + 1;
+  DevBuf<uint32_t> longq(al, maxlong);
+  DevBuf<unsigned int> nlong(al, 1);
+  DevBuf<double> acc_s(al, maxlong);
+  DevBuf<unsigned long long> acc_m(al, maxlong);
+  PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned int), s));
+  PCC_HIP_CHECK(hipMemsetAsync(acc_s.get(), 0, sizeof(double) * (size_t)maxlong, s));
+  PCC_HIP_CHECK(hipMemsetAsync(acc_m.get(), 0, sizeof(unsigned long long) * (size_t)maxlong, s));
   const int grid = GridFor(g.n, 256, 8192);
   hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset,
-                     gscale);
+                     gscale, longq.get(), nlong.get());
+  if (g.m > kLongRow) {
+    hipLaunchKernelGGL(k_gran_long, dim3(1024), dim3(256), 0, s, g.off, g.w, offset, longq.get(),
+                       nlong.get(), acc_s.get(), acc_m.get());
+    hipLaunchKernelGGL(k_gran_long_final, dim3(16), dim3(256), 0, s, longq.get(), nlong.get(),
+                       acc_s.get(), acc_m.get(), gscale);
+  }
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -1705,16 +1777,12 @@ void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
     hipLaunchKernelGGL(k_hub_gather<true>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
                        count, hb);
     LaunchHubPart<true>(items, g, st, mp, list, count, hb, rc, s);
-    hipLaunchKernelGGL(k_bm_hub_cand<true>, dim3((unsigned)items), dim3(kSelThreads), 0, s, g, st,
-                       mp, list, count, hb);
     hipLaunchKernelGGL(k_bm_hub_final<true>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count, hb,
                        out, fb_list, rc);
   } else {
     hipLaunchKernelGGL(k_hub_gather<false>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
                        count, hb);
     LaunchHubPart<false>(items, g, st, mp, list, count, hb, rc, s);
-    hipLaunchKernelGGL(k_bm_hub_cand<false>, dim3((unsigned)items), dim3(kSelThreads), 0, s, g, st,
-                       mp, list, count, hb);
     hipLaunchKernelGGL(k_bm_hub_final<false>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count,
                        hb, out, fb_list, rc);
   }

@@ -573,12 +573,32 @@ __global__ __launch_bounds__(256) void k_degrees(CsrView g, double* __restrict__
     nw[v] = (double)(g.off[v + 1] - g.off[v]);
 }
 
+// Adds x into a[key] with one atomic per distinct key of the wave (large
+// clusters otherwise serialize on a single address).  All lanes call it; act
+// marks the lanes that contribute.
+__device__ __forceinline__ void WaveKeyedAdd(bool act, uint32_t key, double x, double* a) {
+  uint64_t todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const bool mine = act && key == k;
+    const uint64_t grp = __ballot(mine);
+    double y = mine ? x : 0.0;
+    for (int sh = 32; sh > 0; sh >>= 1) y += __shfl_xor(y, sh);
+    if (LaneId() == leader) atomicAdd(&a[k], y);
+    todo &= ~grp;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_cluster_weights(int64_t n, const uint32_t* __restrict__ ids,
                                                          const double* __restrict__ nw,
                                                          double* __restrict__ K) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&K[ids[v]], nw[v]);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    const bool in = v < n;
+    WaveKeyedAdd(in, in ? ids[v] : 0u, in ? nw[v] : 0.0, K);
+  }
 }
 
 // Per-vertex terms of ComputeObjective (PCCI.cc:124-145): the intra-cluster
@@ -627,7 +647,9 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
                                                          const double* __restrict__ nw,
                                                          const double* __restrict__ K,
                                                          double offset, double* __restrict__ e,
-                                                         double* __restrict__ r) {
+                                                         double* __restrict__ r,
+                                                         uint32_t* __restrict__ longq,
+                                                         unsigned int* __restrict__ nlong) {
   const int lane = LaneId();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -640,6 +662,7 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
       b = g.off[v];
       deg = g.off[v + 1] - b;
       r[v] = nw[v] * (K[cv] - nw[v]);
+      if (deg > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;  // k_objective_long
       if (deg <= 16) {
         double tot = 0.0;
         for (int64_t p = 0; p < deg; ++p) {
@@ -649,7 +672,7 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
         e[v] = tot;
       }
     }
-    uint64_t big = __ballot(in && deg > 16);
+    uint64_t big = __ballot(in && deg > 16 && deg <= kLongRow);
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
@@ -697,13 +720,87 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
   }
 }
 
+// Rows longer than kLongRow: the any-order sum and the order-free test's
+// inputs, chunk by chunk over the whole grid (acc: sum, sum of magnitudes;
+// gmin; nonzero count).
+__global__ __launch_bounds__(256) void k_objective_long(CsrView g, const uint32_t* __restrict__ ids,
+                                                        double offset,
+                                                        const uint32_t* __restrict__ longq,
+                                                        const unsigned int* __restrict__ nlong,
+                                                        double* __restrict__ acc,
+                                                        int* __restrict__ gmin,
+                                                        unsigned long long* __restrict__ nz) {
+  __shared__ double sm[256 / kWave];
+  __shared__ int si[256 / kWave];
+  __shared__ unsigned long long su[256 / kWave];
+  ForLongChunks(longq, *nlong, g.off, [&](uint32_t q, uint32_t v, int64_t e0, int64_t e1) {
+    const uint32_t cv = ids[v];
+    double sum = 0.0, sabs = 0.0;
+    int gm = 1 << 20;
+    unsigned long long z = 0;
+    for (int64_t p = e0 + threadIdx.x; p < e1; p += blockDim.x) {
+      const double x = ObjTerm(g, ids, cv, offset, p);
+      sum += x;
+      sabs += fabs(x);
+      gm = min(gm, GranExp(x));
+      z += x != 0.0;
+    }
+    sum = BlockReduce(sum, sm, [](double a, double b) { return a + b; });
+    sabs = BlockReduce(sabs, sm, [](double a, double b) { return a + b; });
+    gm = BlockReduce(gm, si, [](int a, int b) { return a < b ? a : b; });
+    z = BlockReduce(z, su, [](unsigned long long a, unsigned long long b) { return a + b; });
+    if (threadIdx.x == 0) {
+      atomicAdd(&acc[2 * q], sum);
+      atomicAdd(&acc[2 * q + 1], sabs);
+      atomicMin(&gmin[q], gm);
+      atomicAdd(&nz[q], z);
+    }
+  });
+}
+
+// One wave per long row: the any-order sum when it is exact in every order
+// (at most two nonzero terms, or all terms multiples of 2^gmin with the sum of
+// magnitudes below 2^(gmin + 51): a margin of one bit for the rounding of the
+// combined magnitude sum), the pbbs-order fold otherwise.
+__global__ __launch_bounds__(256) void k_objective_long_final(CsrView g,
+                                                              const uint32_t* __restrict__ ids,
+                                                              double offset,
+                                                              const uint32_t* __restrict__ longq,
+                                                              const unsigned int* __restrict__ nlong,
+                                                              const double* __restrict__ acc,
+                                                              const int* __restrict__ gmin,
+                                                              const unsigned long long* __restrict__ nz,
+                                                              double* __restrict__ e) {
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t qq = q; qq < (int64_t)*nlong; qq += nw) {
+    const uint32_t v = longq[qq];
+    const double sum = acc[2 * qq], sabs = acc[2 * qq + 1];
+    const int gm = gmin[qq];
+    double tot = sum;
+    if (nz[qq] > 2 && !(gm >= (1 << 19) || sabs < ldexp(1.0, gm + 51))) {
+      const uint32_t vc = ids[v];
+      const int64_t vb = g.off[v], vd = g.off[v + 1] - vb;
+      int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
+      if (bs < 1024) bs = 1024;
+      bool have = false;
+      for (int64_t s0 = 0; s0 < vd; s0 += bs) {
+        const double blk = ObjFold(g, ids, vc, offset, vb, s0, min(s0 + bs, vd), 0.0, false);
+        tot = have ? tot + blk : blk;
+        have = true;
+      }
+    }
+    if (LaneId() == 0) e[v] = tot;
+  }
+}
+
 // Per-vertex intra-cluster edge count and per-cluster degree sums
 // (PMC.cc:43-61).
 __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint32_t* __restrict__ ids,
                                                           double* __restrict__ intra,
                                                           double* __restrict__ dsum) {
   // Integer counts: any order is exact.  Degrees above 64 are counted by the
-  // whole wave.
+  // whole wave, rows above kLongRow by k_modularity_long.
   const int lane = LaneId();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -715,14 +812,14 @@ __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint3
       cv = ids[v];
       b = g.off[v];
       deg = g.off[v + 1] - b;
-      atomicAdd(&dsum[cv], (double)deg);
       if (deg <= 64) {
         int64_t cnt = 0;
         for (int64_t p = b; p < b + deg; ++p) cnt += ids[g.nbr[p]] == cv ? 1 : 0;
         intra[v] = (double)cnt;
       }
     }
-    uint64_t big = __ballot(in && deg > 64);
+    WaveKeyedAdd(in, cv, (double)deg, dsum);
+    uint64_t big = __ballot(in && deg > 64 && deg <= kLongRow);
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
@@ -735,6 +832,31 @@ __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint3
       if (lane == 0) intra[wave_v0 + l] = (double)cnt;
     }
   }
+}
+
+// Rows longer than kLongRow (intra[v] zeroed beforehand): chunk counts added
+// atomically, exact for integer counts below 2^53.
+__global__ __launch_bounds__(256) void k_modularity_long(CsrView g, const uint32_t* __restrict__ ids,
+                                                         const uint32_t* __restrict__ longq,
+                                                         const unsigned int* __restrict__ nlong,
+                                                         double* __restrict__ intra) {
+  __shared__ unsigned long long su[256 / kWave];
+  ForLongChunks(longq, *nlong, g.off, [&](uint32_t, uint32_t v, int64_t e0, int64_t e1) {
+    const uint32_t cv = ids[v];
+    unsigned long long c = 0;
+    for (int64_t p = e0 + threadIdx.x; p < e1; p += blockDim.x) c += ids[g.nbr[p]] == cv ? 1ull : 0ull;
+    c = BlockReduce(c, su, [](unsigned long long a, unsigned long long b) { return a + b; });
+    if (threadIdx.x == 0 && c) atomicAdd(&intra[v], (double)c);
+  });
+}
+
+// Rows longer than kLongRow, listed for the long kernels.
+__global__ __launch_bounds__(256) void k_list_long_rows(int64_t n, const int64_t* __restrict__ off,
+                                                        uint32_t* __restrict__ longq,
+                                                        unsigned int* __restrict__ nlong) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (off[v + 1] - off[v] > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;
 }
 
 __device__ __forceinline__ int GraphOf(const int64_t* gbase, int G, int64_t v) {
@@ -757,6 +879,27 @@ __global__ __launch_bounds__(256) void k_modularity_penalty(CsrView g, const dou
     const double total = (double)(g.off[gbase[gi + 1]] - g.off[gbase[gi]]);
     const double d = dsum[c];
     t[c] = (gamma[gi] * d * d) / total;
+  }
+}
+
+// One graph: per-workgroup partial sums of a[] and b[] over fixed vertex
+// ranges (pa / pb), summed in workgroup order by k_graph_sums.
+__global__ __launch_bounds__(256) void k_partial_sums(int64_t n, const double* __restrict__ a,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ pa,
+                                                      double* __restrict__ pb) {
+  __shared__ double sm[4];
+  double x = 0.0, y = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    x += a[v];
+    y += b[v];
+  }
+  x = BlockReduce(x, sm, [](double p, double q) { return p + q; });
+  y = BlockReduce(y, sm, [](double p, double q) { return p + q; });
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = x;
+    pb[blockIdx.x] = y;
   }
 }
 
@@ -1034,6 +1177,22 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s) {
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+// Per-graph sums of a[] and b[] (fixed order): one graph spreads over many
+// workgroups, a batch takes one workgroup per graph.
+void GraphSums(DeviceAllocator* al, int64_t n, const double* a, const double* b, int G,
+               const int64_t* gbase, double* sa, double* sb, hipStream_t s) {
+  if (G == 1) {
+    constexpr int kParts = 1024;
+    DevBuf<double> pa(al, kParts), pb(al, kParts);
+    hipLaunchKernelGGL(k_partial_sums, dim3(kParts), dim3(256), 0, s, n, a, b, pa.get(), pb.get());
+    hipLaunchKernelGGL(k_partial_sums, dim3(1), dim3(256), 0, s, (int64_t)kParts, pa.get(),
+                       pb.get(), sa, sb);
+  } else {
+    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, a, b, gbase, sa, sb);
+  }
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
                       int G, const int64_t* gbase, const double* res, double offset,
                       double* out_dev, hipStream_t s) {
@@ -1044,14 +1203,29 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
   PCC_HIP_CHECK(hipMemsetAsync(e.get(), 0, sizeof(double) * (size_t)nn, s));
   PCC_HIP_CHECK(hipMemsetAsync(r.get(), 0, sizeof(double) * (size_t)nn, s));
   if (n > 0) {
+    const int64_t maxlong = g.m / kLongRow + 1;
+    DevBuf<uint32_t> longq(al, maxlong);
+    DevBuf<unsigned int> nlong(al, 1);
+    DevBuf<double> acc(al, 2 * maxlong);
+    DevBuf<int> gmin(al, maxlong);
+    DevBuf<unsigned long long> nz(al, maxlong);
+    PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned int), s));
+    PCC_HIP_CHECK(hipMemsetAsync(acc.get(), 0, sizeof(double) * 2 * (size_t)maxlong, s));
+    PCC_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)gmin.get(), 1 << 20, (size_t)maxlong, s));
+    PCC_HIP_CHECK(hipMemsetAsync(nz.get(), 0, sizeof(unsigned long long) * (size_t)maxlong, s));
     const int grid = Grid(n, 256, 8192);
     hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, K.get());
     hipLaunchKernelGGL(k_objective_terms, dim3(grid), dim3(256), 0, s, g, ids, nw, K.get(), offset,
-                       e.get(), r.get());
+                       e.get(), r.get(), longq.get(), nlong.get());
+    if (g.m > kLongRow) {
+      hipLaunchKernelGGL(k_objective_long, dim3(1024), dim3(256), 0, s, g, ids, offset, longq.get(),
+                         nlong.get(), acc.get(), gmin.get(), nz.get());
+      hipLaunchKernelGGL(k_objective_long_final, dim3(64), dim3(256), 0, s, g, ids, offset,
+                         longq.get(), nlong.get(), acc.get(), gmin.get(), nz.get(), e.get());
+    }
     PCC_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, e.get(), r.get(), gbase, se.get(),
-                     sr.get());
+  GraphSums(al, n, e.get(), r.get(), G, gbase, se.get(), sr.get(), s);
   hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(256), 0, s, G, se.get(), sr.get(), res,
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());
@@ -1069,12 +1243,21 @@ void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* id
     const int grid = Grid(n, 256, 8192);
     hipLaunchKernelGGL(k_modularity_terms, dim3(grid), dim3(256), 0, s, g, ids, intra.get(),
                        dsum.get());
+    if (g.m > kLongRow) {
+      const int64_t maxlong = g.m / kLongRow + 1;
+      DevBuf<uint32_t> longq(al, maxlong);
+      DevBuf<unsigned int> nlong(al, 1);
+      PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned int), s));
+      hipLaunchKernelGGL(k_list_long_rows, dim3(grid), dim3(256), 0, s, n, g.off, longq.get(),
+                         nlong.get());
+      hipLaunchKernelGGL(k_modularity_long, dim3(1024), dim3(256), 0, s, g, ids, longq.get(),
+                         nlong.get(), intra.get());
+    }
     hipLaunchKernelGGL(k_modularity_penalty, dim3(grid), dim3(256), 0, s, g, dsum.get(), G, gbase,
                        gamma, t.get());
     PCC_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, intra.get(), t.get(), gbase, si.get(),
-                     st.get());
+  GraphSums(al, n, intra.get(), t.get(), G, gbase, si.get(), st.get(), s);
   hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, g, G, gbase, si.get(), st.get(),
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());

@@ -186,18 +186,15 @@ struct HubScratch {
   int64_t* part_off = nullptr;
   uint32_t* item_of = nullptr;  // list item of each partition
   int64_t* ent_off = nullptr;
-  uint32_t* cbuf = nullptr;  // neighbour clusters, at ent_off like the entries
-  uint32_t* ekeys = nullptr;
-  double* evals = nullptr;
-  uint32_t* ecur = nullptr;
+  uint32_t* cbuf = nullptr;  // neighbour clusters of list item i at ent_off[i]
   uint32_t* hflag = nullptr;
   double* tval = nullptr;
-  // Per partition item t: its entries ent_off[i] + pe_base[t] (pe_cnt[t]
-  // of them) and its best candidate.
-  uint32_t* pe_base = nullptr;
-  uint32_t* pe_cnt = nullptr;
-  double* pbest_d = nullptr;
-  uint32_t* pbest_c = nullptr;
+  // Per partition item t, over its candidate clusters c with
+  // x_c = NetWeight(S_c, k_v K_c, res): the largest x (px1), the smallest
+  // cluster with that value (pc1), and the largest x below it (px2).
+  double* px1 = nullptr;
+  uint32_t* pc1 = nullptr;
+  double* px2 = nullptr;
 };
 
 // Scratch for the ordered fallback path.
@@ -233,7 +230,8 @@ void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64
                       uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s);
 
 // gscale[v] for every vertex with deg > 64 (see MoveParams::gscale).
-void LaunchGranThreshold(const CsrView& g, double offset, int32_t* gscale, hipStream_t s);
+void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, int32_t* gscale,
+                         hipStream_t s);
 
 // Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
 // partitions of part_edges edges, and the list item of every partition

@@ -370,3 +370,41 @@ def test_cluster_hub_partitions(pcc, orc, case):
     ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION)
     r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), modularity)
     assert np.array_equal(ids, r.ids)
+
+
+@pytest.mark.parametrize("weights", ["unit", "signed"])
+def test_objective_long_rows(pcc, orc, weights):
+    """A vertex of degree 40000 (above the split-row threshold) next to a
+    ring: the objective and modularity of its long row are summed over many
+    workgroups (order-free sum or the ordered fold) and must match the oracle."""
+    n = 40001
+    rng = np.random.default_rng(43)
+    hub = np.zeros(n - 1, np.int64)
+    leaves = np.arange(1, n, dtype=np.int64)
+    ring_a = leaves
+    ring_b = np.roll(leaves, 1)
+    src = np.concatenate([hub, leaves, ring_a, ring_b])
+    dst = np.concatenate([leaves, hub, ring_b, ring_a])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=off[1:])
+    if weights == "unit":
+        w = np.ones(len(src), np.float32)
+    else:
+        a, b = np.minimum(src, dst), np.maximum(src, dst)
+        w = np.sin((a * 7919 + b * 104729) % 100003).astype(np.float32)  # symmetric, signed
+    csr = pcc.Csr(n, off, dst.astype(np.uint32), w)
+    ids = rng.integers(0, 5, size=n).astype(np.uint32)
+    nw = rng.integers(1, 4, size=n).astype(np.float64)
+    print(f"objective long rows {weights} n={n} m={csr.m}", flush=True)
+    dg = _dev(pcc, csr)
+    g = orc.G.from_csr(csr)
+    for res, offv in [(0.01, 0.0), (0.3, 0.25)]:
+        cfg = pcc.make_config(resolution=res, edge_weight_offset=offv)
+        got = dg.objective(cfg, ids, nw)
+        ref = orc.objective(g, ids, nw, res, offv)
+        assert got == pytest.approx(ref, rel=1e-12, abs=1e-9)
+    for gamma in (0.5, 1.0):
+        assert dg.modularity(gamma, ids) == pytest.approx(orc.modularity(g, ids, gamma), rel=1e-12,
+                                                         abs=1e-12)
