@@ -57,8 +57,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads for the CPU baseline (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gather-ceiling", action="store_true",
+                    help="skip the random-gather ceiling (a child process using torch)")
     ap.add_argument("--workload", default="rmat22",
-                    choices=["rmat22", "batched", "modularity-async", "serving", "copy"],
+                    choices=["rmat22", "batched", "modularity-async", "serving", "copy", "gather"],
                     help="rmat22: the headline line; the others are secondary measurements")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of SBM graphs")
     ap.add_argument("--batch-edges", type=int, default=1 << 30,
@@ -89,6 +91,8 @@ def main() -> int:
         return bench_serving(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "gather":
+        return bench_gather(args)
 
     import numpy as np
     import pcc_amd
@@ -133,6 +137,9 @@ def main() -> int:
     dom_gbs = dom_bytes / bin_s[dom] / 1e9 if bin_s[dom] > 0 else 0.0
     all_gbs = sum(s.bestmove_bytes for s in stats) / bm_s / 1e9 if bm_s > 0 else 0.0
 
+    # torch and this library bring separate HIP runtimes, so the ceiling is
+    # measured in a child process.
+    gceil = None if args.no_gather_ceiling else gather_ceiling_child(args.scale)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(csr, cfg, args.cpu_threads)
@@ -185,6 +192,18 @@ def main() -> int:
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(args, dom),
             "algorithmic_bytes_per_launch": round(dom_bytes / max(bin_l[dom], 1)),
+        },
+        # The dominant kernels wait on random lookups rather than on bytes:
+        # one ids[u] gather per scanned edge (plus K[c] per distinct cluster,
+        # not counted here), against the measured random-gather rate.
+        "gather_roofline": None if not gceil else {
+            "bound": "random 4-byte gathers",
+            "achieved": round(bin_e[dom] / bin_s[dom] / 1e9, 2) if bin_s[dom] > 0 else 0.0,
+            "peak": round(gceil / 1e9, 2),
+            "unit": "G gathers/s",
+            "frac": round(bin_e[dom] / bin_s[dom] / gceil, 4) if bin_s[dom] > 0 else 0.0,
+            "peak_source": f"torch.index_select of 45M random indices into {csr.n} u32 entries "
+                           "(bench.py --workload gather)",
         },
         "cpu_baseline": cpu,
     }
@@ -424,6 +443,54 @@ def bench_copy(args):
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"copy of {4 * n} bytes (read {4 * n} + write {4 * n} per step)"},
             "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def gather_ceiling(n_ids, gathers=45_000_000, reps=10):
+    """Random 4-byte gathers per second from an array of n_ids entries
+    (torch.index_select with uniform random indices): the access pattern of
+    the best-move kernels' ids[u] lookups, measured on this device."""
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    src = torch.randint(0, 1 << 30, (n_ids,), device="cuda", dtype=torch.int32, generator=g)
+    idx = torch.randint(0, n_ids, (gathers,), device="cuda", dtype=torch.int64, generator=g)
+    out = torch.empty(gathers, device="cuda", dtype=torch.int32)
+    torch.index_select(src, 0, idx, out=out)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        torch.index_select(src, 0, idx, out=out)
+    t1.record()
+    torch.cuda.synchronize()
+    return gathers * reps / (t0.elapsed_time(t1) * 1e-3)
+
+
+def gather_ceiling_child(scale):
+    """gather_ceiling in a child process (bench.py --workload gather)."""
+    import subprocess
+
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", "gather",
+                        "--scale", str(scale)], capture_output=True, text=True, timeout=300)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        print(f"# gather ceiling unavailable (rc={r.returncode})", file=sys.stderr, flush=True)
+        return None
+    return json.loads(lines[-1])["value"] * 1e9
+
+
+def bench_gather(args):
+    """Random-gather ceiling for the level-0 id array of the headline workload."""
+    n_ids = 1 << args.scale
+    rate = gather_ceiling(n_ids)
+    line = {"metric": "random 4-byte gathers (ceiling of the ids[u] lookups)", "value": round(rate / 1e9, 2),
+            "unit": "G gathers/s", "n_gpus": 1, "steps": 10, "warmup": 1, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic", "config": {"workload": f"45M uniform random gathers from {n_ids} u32 entries"}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -861,6 +861,33 @@ struct BufferedClusters {
   __device__ __forceinline__ uint32_t Cluster(uint32_t x) const { return x; }
 };
 
+// Accumulate over one partition's bucket of (cluster, weight bits) pairs:
+// every pair belongs to the partition, so no filtering is needed.
+template <int THREADS, bool kAsync>
+__device__ __forceinline__ bool AccumulateBucket(const MoveParams& mp, const uint2* __restrict__ bk,
+                                                 uint32_t nb, bool hub, int thr, uint32_t* keys,
+                                                 double* vals, uint32_t* cnt, uint32_t mask,
+                                                 const SlotList& sl) {
+  bool overflow = false;
+  for (uint32_t k0 = 0; k0 < nb; k0 += THREADS) {
+    const uint32_t k = k0 + threadIdx.x;
+    const bool act = k < nb;
+    const uint2 e = act ? bk[k] : make_uint2(kNoneId, 0u);
+    const double x = act ? EdgeValue(__uint_as_float(e.y), hub, mp.offset) : 0.0;
+    bool fresh = false;
+    const int slot = act ? HashSlot(keys, mask, e.x, &fresh) : -1;
+    const bool listed = sl.AddWave(fresh, slot);
+    if (!act) continue;
+    if (slot < 0 || !listed) {
+      overflow = true;
+      continue;
+    }
+    atomicAdd(&vals[slot], x);
+    if (!kAsync) RecordTerm(cnt, slot, x, thr);
+  }
+  return overflow;
+}
+
 // Accumulate over gathered clusters cb[0, deg) (partition `part` of
 // `parts`).  Only about 1/parts of the entries belong to the partition, so
 // each wave first compacts the positions of its matching entries into a
@@ -1224,38 +1251,78 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
 
 // Bin 7, gather pass: the cluster of every neighbour of the listed vertices,
 // cbuf[ent_off[i] + q] (kNoneId for self-loops), one workgroup per partition
-// of edges, so the partition pass reads clusters without gathers.
+// of edges, so the partition pass reads clusters without gathers.  The same
+// (cluster, weight) pairs are also appended to the bucket of the partition
+// their cluster hashes to (HubScratch::bkt): counted per bucket in LDS, then
+// one global reservation per bucket and workgroup.
+constexpr int kGatherThreads = 256;
+constexpr int kGatherU = 8;
+constexpr int kGatherMaxBuckets = 4096;
+
 template <bool kAsync>
-__global__ __launch_bounds__(256) void k_hub_gather(CsrView g, StateView st, MoveParams mp,
-                                                    const uint32_t* __restrict__ list,
-                                                    int64_t count, HubScratch hb) {
+__global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateView st,
+                                                               MoveParams mp,
+                                                               const uint32_t* __restrict__ list,
+                                                               int64_t count, HubScratch hb) {
+  __shared__ uint32_t lh[kGatherMaxBuckets];
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
   const int64_t i = hb.item_of[t];
   const uint32_t v = list[i];
   if (kAsync && !InSubRound(mp, v)) return;
   if (!kAsync && mp.force_ordered) return;
-  const int64_t parts = hb.part_off[i + 1] - hb.part_off[i];
-  const int64_t part = t - hb.part_off[i];
+  const int64_t p0 = hb.part_off[i];
+  const int64_t parts = hb.part_off[i + 1] - p0;
+  const int64_t part = t - p0;
   const int64_t start = g.off[v], deg = g.off[v + 1] - start;
   const int64_t per = (deg + parts - 1) / parts;
   const int64_t e0 = part * per, e1 = min(deg, (part + 1) * per);
+  const bool local = parts <= kGatherMaxBuckets;
   uint32_t* out = hb.cbuf + hb.ent_off[i];
-  constexpr int U = 8;
-  for (int64_t q0 = e0 + threadIdx.x; q0 < e1; q0 += (int64_t)blockDim.x * U) {
-    uint32_t u[U];
+  constexpr int U = kGatherU;
+  for (int64_t q0 = e0; q0 < e1; q0 += (int64_t)kGatherThreads * U) {
+    uint32_t c[U];
+    float w[U];
+    uint32_t b[U], pos[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const int64_t q = q0 + (int64_t)k * blockDim.x;
-      u[k] = q < e1 ? g.nbr[start + q] : v;
+      const int64_t q = q0 + threadIdx.x + (int64_t)k * kGatherThreads;
+      c[k] = q < e1 ? g.nbr[start + q] : v;
+      w[k] = q < e1 ? g.w[start + q] : 0.0f;
     }
 #pragma unroll
-    for (int k = 0; k < U; ++k) u[k] = u[k] != v ? st.ids[u[k]] : kNoneId;
+    for (int k = 0; k < U; ++k) c[k] = c[k] != v ? st.ids[c[k]] : kNoneId;
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const int64_t q = q0 + (int64_t)k * blockDim.x;
-      if (q < e1) out[q] = u[k];
+      const int64_t q = q0 + threadIdx.x + (int64_t)k * kGatherThreads;
+      if (q < e1) out[q] = c[k];
+      b[k] = c[k] != kNoneId ? PartOf(c[k], (uint32_t)parts) : 0u;
     }
+    if (local) {
+      __syncthreads();
+      for (int64_t j = threadIdx.x; j < parts; j += kGatherThreads) lh[j] = 0;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (c[k] != kNoneId) pos[k] = atomicAdd(&lh[b[k]], 1u);
+      __syncthreads();
+      for (int64_t j = threadIdx.x; j < parts; j += kGatherThreads) {
+        const uint32_t n = lh[j];
+        lh[j] = n ? atomicAdd(&hb.bcur[p0 + j], n) : 0u;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (c[k] != kNoneId) pos[k] += lh[b[k]];
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (c[k] != kNoneId) pos[k] = atomicAdd(&hb.bcur[p0 + b[k]], 1u);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (c[k] != kNoneId && pos[k] < (uint32_t)hb.bcap)
+        hb.bkt[(p0 + b[k]) * hb.bcap + pos[k]] = make_uint2(c[k], __float_as_uint(w[k]));
   }
 }
 
@@ -1338,11 +1405,21 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const int32_t scale = kAsync ? 0 : mp.gscale[v];
     const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
   const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
-  // The compaction queue shares the repair stage's LDS (never live together).
-  const bool overflow = AccumulateBuffered<kHubThreads, kAsync>(
-      g, mp, src.c, start, deg, hub, thr, keys, vals, cnt, mask, parts, part,
-      reinterpret_cast<uint32_t*>(&stg), sl);
-  if (__syncthreads_or(overflow)) {
+  const uint32_t nb = hb.bcur[t];
+  // The partition's bucket when it did not overflow; otherwise the whole
+  // cluster buffer, filtered (the compaction queue shares the repair stage's
+  // LDS, never live together).
+  const bool overflow =
+      nb <= (uint32_t)hb.bcap
+          ? AccumulateBucket<kHubThreads, kAsync>(mp, hb.bkt + t * hb.bcap, nb, hub, thr, keys,
+                                                  vals, cnt, mask, sl)
+          : AccumulateBuffered<kHubThreads, kAsync>(g, mp, src.c, start, deg, hub, thr, keys, vals,
+                                                    cnt, mask, parts, part,
+                                                    reinterpret_cast<uint32_t*>(&stg), sl);
+  const bool ovf = __syncthreads_or(overflow);
+  // Every thread has read bcur[t] before the barrier.
+  if (threadIdx.x == 0) hb.bcur[t] = 0;
+  if (ovf) {
     if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
     return;
   }

@@ -195,7 +195,19 @@ struct HubScratch {
   double* px1 = nullptr;
   uint32_t* pc1 = nullptr;
   double* px2 = nullptr;
+  // The same clusters bucketed by partition: item t's (cluster, weight bits)
+  // pairs at bkt[t * bcap ...], bcur[t] of them; bcur[t] > bcap means the
+  // bucket overflowed and the partition filters cbuf instead.  The partition
+  // pass resets bcur[t] to 0.
+  uint2* bkt = nullptr;
+  uint32_t* bcur = nullptr;
+  int64_t bcap = 0;
 };
+
+// Bucket capacity per partition item: a partition's share of a vertex's
+// edges is at most part_edges when clusters hash evenly; heavy clusters can
+// overflow it.
+inline int64_t HubBucketCap(int64_t part_edges) { return part_edges + part_edges / 2 + 64; }
 
 // Scratch for the ordered fallback path.
 struct HashScratch {
