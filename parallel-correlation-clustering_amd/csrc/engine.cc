@@ -434,6 +434,10 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   } else {
     mp.new_id = (uint32_t)g.n;
   }
+  // Candidate pruning needs every resolution and node weight >= 0.
+  bool nonneg = lay.batch ? AllNonNegative(al_, lay.gres.get(), lay.G, s_) : mp.res >= 0.0;
+  nonneg = nonneg && AllNonNegative(al_, st.nw, g.n, s_);
+  mp.prune = nonneg ? 1 : 0;
 }
 
 void Engine::BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],

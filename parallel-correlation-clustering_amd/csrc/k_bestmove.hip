@@ -580,7 +580,9 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
     double mw = 0;
     mw += k;
     Cand cand{0.0, kNoneId};
-    if (first) {
+    // (These vertices are not hubs, so a pruned candidate is the same as none:
+    // both decide "new cluster" with delta = change.)
+    if (first && !(mp.prune && S <= 0.0)) {
       cand.c = c;
       cand.d = change + NetWeight(S, mw * st.K[c], res);
     }
@@ -1186,6 +1188,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
       for (int e = 0; e < E; ++e) {
         const uint32_t q = threadIdx.x + (uint32_t)e * THREADS;
         c[e] = q < n_occ ? keys[occ[q]] : kNoneId;
+        if (c[e] != kNoneId && mp.prune && vals[occ[q]] <= 0.0) c[e] = kNoneId;  // cannot win
         Kc[e] = c[e] != kNoneId ? st.K[c[e]] : 0.0;
       }
       Cand best{0.0, kNoneId};
@@ -1198,6 +1201,8 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
       }
       best = BlockBest<THREADS>(best, sd, sc);
       if (threadIdx.x == 0) {
+        // Only pruned candidates: a candidate exists, none beats staying.
+        if (best.c == kNoneId && n_occ > 0) best = Cand{-INFINITY, 0u};
         uint32_t t;
         double d;
         Decide(best, change, hub, mp.new_id, &t, &d);
@@ -1451,6 +1456,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) {
     const uint32_t s = occ[k];
     const uint32_t c = keys[s];
+    if (mp.prune && vals[s] <= 0.0) continue;  // cannot win (see MoveParams::prune)
     top = MergeTop2(top, Top2{NetWeight(vals[s], mw * st.K[c], res), c, -INFINITY});
   }
   top = BlockTop2<kHubThreads>(top, reinterpret_cast<Top2*>(&stg));
@@ -1458,6 +1464,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     hb.px1[t] = top.x1;
     hb.pc1[t] = top.c1;
     hb.px2[t] = top.x2;
+    if (n_occ > 0) atomicOr(&hb.hflag[i], 4u);  // a candidate exists
   }
 }
 
@@ -1513,6 +1520,8 @@ __global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, M
       if (top.c1 != kNoneId) {
         best = Cand{change + top.x1, top.c1};
         tied = top.x2 > -INFINITY && change + top.x2 == best.d;
+      } else if (flag & 4u) {
+        best = Cand{-INFINITY, 0u};  // only pruned candidates: none beats staying
       }
       if (!kAsync && tied) {
         fb_list[atomicAdd(&rc->fb_count, 1ull)] = v;

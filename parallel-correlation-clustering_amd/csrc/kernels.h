@@ -163,6 +163,10 @@ struct MoveParams {
   // Per-vertex scale of the order-free test of the hashed bins (deg > 64):
   // PackScale(threshold, magnitude exponent), see SlotInexact.
   const int32_t* gscale = nullptr;
+  // 1 when every resolution and node weight is >= 0: then a candidate c with
+  // S_c <= 0 has delta <= change + S_c <= change and can never be chosen over
+  // staying, so its cluster weight K_c is not looked up.
+  int prune = 0;
 };
 
 struct MoveOut {
@@ -436,6 +440,8 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s);
 
 // Fixed-order double sum.
 void SumF64(DeviceAllocator* al, const double* in, int64_t n, double* out, hipStream_t s);
+// True when every in[i] >= 0 (synchronizes the stream).
+bool AllNonNegative(DeviceAllocator* al, const double* in, int64_t n, hipStream_t s);
 
 // ComputeObjective (parallel-correlation-clusterer-internal.cc:118-149) of
 // each of the G graphs [gbase[g], gbase[g + 1]) with resolution res[g]; ids
