@@ -15,6 +15,7 @@
 //     way; n' = 1 + max cluster id, so unused ids become isolated vertices.
 
 #include <cmath>
+#include <vector>
 
 #include "device_common.h"
 #include "kernels.h"
@@ -882,24 +883,26 @@ __global__ __launch_bounds__(256) void k_modularity_penalty(CsrView g, const dou
   }
 }
 
-// One graph: per-workgroup partial sums of a[] and b[] over fixed vertex
-// ranges (pa / pb), summed in workgroup order by k_graph_sums.
-__global__ __launch_bounds__(256) void k_partial_sums(int64_t n, const double* __restrict__ a,
-                                                      const double* __restrict__ b,
-                                                      double* __restrict__ pa,
-                                                      double* __restrict__ pb) {
+// Per-chunk sums of a[] and b[]: chunk j covers [cs[j], cs[j + 1]) (chunks
+// start at their graph's first vertex, so a graph's partials do not depend on
+// what else is in the batch).
+__global__ __launch_bounds__(256) void k_chunk_sums(const int64_t* __restrict__ cs,
+                                                    const double* __restrict__ a,
+                                                    const double* __restrict__ b,
+                                                    double* __restrict__ pa,
+                                                    double* __restrict__ pb) {
   __shared__ double sm[4];
+  const int64_t j = blockIdx.x;
   double x = 0.0, y = 0.0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = cs[j] + threadIdx.x; v < cs[j + 1]; v += blockDim.x) {
     x += a[v];
     y += b[v];
   }
   x = BlockReduce(x, sm, [](double p, double q) { return p + q; });
   y = BlockReduce(y, sm, [](double p, double q) { return p + q; });
   if (threadIdx.x == 0) {
-    pa[blockIdx.x] = x;
-    pb[blockIdx.x] = y;
+    pa[j] = x;
+    pb[j] = y;
   }
 }
 
@@ -1177,20 +1180,40 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s) {
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-// Per-graph sums of a[] and b[] (fixed order): one graph spreads over many
-// workgroups, a batch takes one workgroup per graph.
-void GraphSums(DeviceAllocator* al, int64_t n, const double* a, const double* b, int G,
+// Per-graph sums of a[] and b[] in a fixed order that depends only on the
+// graph: chunks of 4096 vertices from the graph's first vertex, summed per
+// workgroup, then the chunk sums per graph.  A graph in a batch therefore gets
+// the same bits as the same graph alone.
+void GraphSums(DeviceAllocator* al, const double* a, const double* b, int G,
                const int64_t* gbase, double* sa, double* sb, hipStream_t s) {
-  if (G == 1) {
-    constexpr int kParts = 1024;
-    DevBuf<double> pa(al, kParts), pb(al, kParts);
-    hipLaunchKernelGGL(k_partial_sums, dim3(kParts), dim3(256), 0, s, n, a, b, pa.get(), pb.get());
-    hipLaunchKernelGGL(k_partial_sums, dim3(1), dim3(256), 0, s, (int64_t)kParts, pa.get(),
-                       pb.get(), sa, sb);
-  } else {
-    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, a, b, gbase, sa, sb);
+  constexpr int64_t kChunk = 4096;
+  std::vector<int64_t> hb((size_t)G + 1);
+  PCC_HIP_CHECK(hipMemcpyAsync(hb.data(), gbase, sizeof(int64_t) * hb.size(), hipMemcpyDeviceToHost,
+                               s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  // Chunk starts; graphs are contiguous, so each chunk ends where the next
+  // one (or the last graph) ends.
+  std::vector<int64_t> cs, coff((size_t)G + 1);
+  for (int gi = 0; gi < G; ++gi) {
+    coff[(size_t)gi] = (int64_t)cs.size();
+    for (int64_t v = hb[(size_t)gi]; v < hb[(size_t)gi + 1]; v += kChunk) cs.push_back(v);
   }
+  coff[(size_t)G] = (int64_t)cs.size();
+  const int64_t nch = (int64_t)cs.size();
+  cs.push_back(hb[(size_t)G]);
+  DevBuf<int64_t> dcs(al, nch + 1), dcoff(al, (int64_t)G + 1);
+  DevBuf<double> pa(al, nch > 0 ? nch : 1), pb(al, nch > 0 ? nch : 1);
+  PCC_HIP_CHECK(hipMemcpyAsync(dcs.get(), cs.data(), sizeof(int64_t) * cs.size(),
+                               hipMemcpyHostToDevice, s));
+  PCC_HIP_CHECK(hipMemcpyAsync(dcoff.get(), coff.data(), sizeof(int64_t) * coff.size(),
+                               hipMemcpyHostToDevice, s));
+  if (nch > 0)
+    hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)nch), dim3(256), 0, s, dcs.get(), a, b,
+                       pa.get(), pb.get());
+  hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, pa.get(), pb.get(), dcoff.get(), sa,
+                     sb);
   PCC_HIP_CHECK(hipGetLastError());
+  PCC_HIP_CHECK(hipStreamSynchronize(s));  // cs / coff stay alive until the copies ran
 }
 
 void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
@@ -1225,7 +1248,7 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     }
     PCC_HIP_CHECK(hipGetLastError());
   }
-  GraphSums(al, n, e.get(), r.get(), G, gbase, se.get(), sr.get(), s);
+  GraphSums(al, e.get(), r.get(), G, gbase, se.get(), sr.get(), s);
   hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(256), 0, s, G, se.get(), sr.get(), res,
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());
@@ -1257,7 +1280,7 @@ void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* id
                        gamma, t.get());
     PCC_HIP_CHECK(hipGetLastError());
   }
-  GraphSums(al, n, intra.get(), t.get(), G, gbase, si.get(), st.get(), s);
+  GraphSums(al, intra.get(), t.get(), G, gbase, si.get(), st.get(), s);
   hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, g, G, gbase, si.get(), st.get(),
                      out_dev);
   PCC_HIP_CHECK(hipGetLastError());

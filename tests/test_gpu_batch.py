@@ -45,7 +45,9 @@ def test_batch_equals_single_runs(pcc, orc, case):
         c1 = pcc.make_config(resolution=float(res[i]), **kw)
         single, sst = pcc.DeviceGraph.from_csr(g).cluster(c1)
         assert np.array_equal(ids[i], single), f"graph {i} differs from its single run"
-        assert obj[i] == pytest.approx(sst.objective, rel=1e-12, abs=1e-9)
+        # The per-graph objective sums follow the graph's own chunking, so a
+        # graph in a batch gets the same bits as the graph alone.
+        assert obj[i] == sst.objective
         r = orc.cluster(orc.G.from_csr(g), orc.config_from(c1))
         assert np.array_equal(ids[i], r.ids), f"graph {i} differs from the oracle"
 
