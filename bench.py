@@ -43,6 +43,8 @@ def apply_tuning(g, args, pcc_amd):
         g.set_option(pcc_amd.OPT_HUB_PART_EDGES, args.hub_part_edges)
     if args.async_subrounds is not None:
         g.set_option(pcc_amd.OPT_ASYNC_SUBROUNDS, args.async_subrounds)
+    if args.pull_fraction is not None:
+        g.set_option(pcc_amd.OPT_PULL_FRACTION, args.pull_fraction)
 
 
 def main() -> int:
@@ -73,6 +75,8 @@ def main() -> int:
     ap.add_argument("--block-max", type=int, default=0, help="upper degree of the workgroup bin")
     ap.add_argument("--hub-part-edges", type=int, default=0, help="edges per hub partition")
     ap.add_argument("--async-subrounds", type=int, default=None, help="async sub-rounds (0: auto)")
+    ap.add_argument("--pull-fraction", type=int, default=None,
+                    help="frontier pull threshold in 1/1024 of m + n (1024: push only)")
     args = ap.parse_args()
     big = args.workload == "modularity-async"
     if args.scale is None:

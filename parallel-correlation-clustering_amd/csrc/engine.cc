@@ -493,8 +493,13 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   out.moved = ws.moved.get();
   out.moved_clusters = ws.flags.get();
   if (async) PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
+  // Pull can only win when the seeds' volume may exceed the threshold.  With
+  // NBHR_MOVE the seeds are movers, a subset of the frontier, so a frontier
+  // of volume TE + F below the threshold rules pull out for this round.
+  const double pull_thr = ((double)g.m + (double)n) * h_->pull_fraction / 1024.0;
+  const bool pull = c.pull && (cfg_.move_method != PCC_NBHR_MOVE || (double)(TE + F) > pull_thr);
   // The pull expansion reads moved[] of every vertex, not just the frontier's.
-  if (c.pull && cfg_.move_method == PCC_NBHR_MOVE)
+  if (pull && cfg_.move_method == PCC_NBHR_MOVE)
     PCC_HIP_CHECK(hipMemsetAsync(ws.moved.get(), 0, (size_t)n, s_));
   PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 3 * sizeof(unsigned long long), s_));
   const uint64_t salt =
@@ -606,13 +611,13 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   const bool by_cluster = cfg_.move_method == PCC_NBHR_CLUSTER_MOVE;
   const uint32_t* seed_ids = by_cluster ? st.ids : nullptr;
   ExpandDir dir;
-  if (c.pull) {
+  if (pull) {
     // Direction by seed volume: the pull kernels or the push kernels run,
     // the other side returns at once.
     PCC_HIP_CHECK(hipMemsetAsync(c.vol.get(), 0, sizeof(unsigned long long), s_));
     LaunchSeedVolume(g, seed_ids, ws.flags.get(), ws.moved.get(), c.vol.get(), s_);
     dir.vol = c.vol.get();
-    dir.thr = (unsigned long long)(((double)g.m + (double)n) * h_->pull_fraction / 1024.0);
+    dir.thr = (unsigned long long)pull_thr;
   }
   if (by_cluster) {
     LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.Hubs(), h_->hub_part_edges,
@@ -621,7 +626,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     LaunchExpandMoved(counts, te, c.layout, ws.list.get(), c.hb.part_off, c.hb.item_of,
                       c.hb.part_edges, g, ws.moved.get(), dir, ws.next.get(), s_);
   }
-  if (c.pull)
+  if (pull)
     LaunchPullFrontier(g, seed_ids, ws.flags.get(), ws.moved.get(), h_->th.block_max, c.Hubs(),
                        h_->hub_part_edges, dir, ws.next.get(), s_);
   LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
@@ -640,8 +645,17 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
                                               int64_t inner) {
   const int64_t n = L.g.n;
   const int G = L.lay.G;
-  LevelCtx c;
-  PrepareLevel(L.g, L.State(), L.lay, &c);
+  LevelCtx* cp = nullptr;
+  for (auto& e : ctx_cache_)
+    if (e.first == &L) cp = e.second.get();
+  std::unique_ptr<LevelCtx> fresh;
+  if (!cp) {
+    fresh = std::make_unique<LevelCtx>();
+    PrepareLevel(L.g, L.State(), L.lay, fresh.get());
+    cp = fresh.get();
+    if (cfg_.refine) ctx_cache_.emplace_back(&L, std::move(fresh));
+  }
+  LevelCtx& c = *cp;
   Workspace ws(al_, n, s_);
   DevBuf<uint8_t> gmoved(al_, G);
   PCC_HIP_CHECK(hipMemsetAsync(gmoved.get(), 0, (size_t)G, s_));

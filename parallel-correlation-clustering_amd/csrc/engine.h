@@ -196,6 +196,10 @@ class Engine {
   int64_t bin_vertices_[kBins] = {0};
   int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
   double bestmove_bytes_ = 0;
+  // With refinement every level's graph is visited twice; its best-move
+  // context (bins, thresholds, hub lists and scratch) depends only on the
+  // graph and its node weights, so the second visit reuses it.
+  std::vector<std::pair<const Level*, std::unique_ptr<LevelCtx>>> ctx_cache_;
 };
 
 // Resolution used for the objective: for modularity runs res' = gamma / m_g

@@ -474,7 +474,7 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
     PCC_HIP_CHECK(hipGetLastError());
   }
   if (nmid > 0) {
-    hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 16384)), dim3(256), 0, s, mid, nmid, list,
+    hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid, nmid, list,
                        g.off, g.nbr, moved, dir, next);
     PCC_HIP_CHECK(hipGetLastError());
   }
@@ -493,7 +493,7 @@ void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* 
                           int64_t block_max, const HubList& hubs, int64_t part_edges,
                           const ExpandDir& dir, uint8_t* next, hipStream_t s) {
   if (g.n == 0) return;
-  hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
+  hipLaunchKernelGGL(k_expand_clusters, dim3(Grid(g.n, 256, 4096)), dim3(256), 0, s, g.n, g.off,
                      g.nbr, ids, modified, block_max, dir, next);
   PCC_HIP_CHECK(hipGetLastError());
   if (hubs.count > 0) {
@@ -518,7 +518,7 @@ void LaunchPullFrontier(const CsrView& g, const uint32_t* ids, const uint8_t* mo
                         const uint8_t* moved, int64_t block_max, const HubList& hubs,
                         int64_t part_edges, const ExpandDir& dir, uint8_t* next, hipStream_t s) {
   if (g.n == 0) return;
-  hipLaunchKernelGGL(k_pull_vertices, dim3(Grid(g.n, 256, 8192)), dim3(256), 0, s, g.n, g.off,
+  hipLaunchKernelGGL(k_pull_vertices, dim3(Grid(g.n, 256, 4096)), dim3(256), 0, s, g.n, g.off,
                      g.nbr, ids, modified, moved, block_max, dir, next);
   PCC_HIP_CHECK(hipGetLastError());
   if (hubs.count > 0) {
