@@ -83,6 +83,7 @@ def lib() -> ctypes.CDLL:
         l.oracle_set_threads.argtypes = [ctypes.c_int]
         l.oracle_get_threads.restype = ctypes.c_int
         l.oracle_best_move.argtypes = [P, P, P, ctypes.c_uint32, P, P]
+        l.oracle_best_move_set.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         l.oracle_sync_decisions.argtypes = [P, P, P, P, P, P, P, P]
         l.oracle_apply_moves.argtypes = [ctypes.c_int64, P, P, P]
         l.oracle_round.argtypes = [P, P, P, P, ctypes.c_uint64]
@@ -177,6 +178,15 @@ def best_move(g: G, s: S, cfg: OConfig, v: int) -> Tuple[int, float]:
     d = ctypes.c_double()
     lib().oracle_best_move(ctypes.byref(g.view), ctypes.byref(s.view), ctypes.byref(cfg), v,
                            ctypes.byref(t), ctypes.byref(d))
+    return int(t.value), float(d.value)
+
+
+def best_move_set(g: G, s: S, cfg: OConfig, members) -> Tuple[int, float]:
+    mem = np.ascontiguousarray(members, dtype=np.uint32)
+    t = ctypes.c_uint32()
+    d = ctypes.c_double()
+    lib().oracle_best_move_set(ctypes.byref(g.view), ctypes.byref(s.view), ctypes.byref(cfg),
+                               _p(mem), mem.shape[0], ctypes.byref(t), ctypes.byref(d))
     return int(t.value), float(d.value)
 
 

@@ -263,6 +263,155 @@ bool AsyncMove(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
   }
 }
 
+// ClusteringHelper::BestMove(graph, moving_nodes), PCCI.cc:644-707, with
+// BestMoveFromStats, UTIL.h:70-113.  members is the moving set in the order
+// the reference visits it (ascending vertex ids: the groups come from
+// OutputIndicesById, PSO.h:183-209); flag is an all-zero scratch of n bytes.
+void BestMoveSet(const oracle_graph& g, const oracle_state& s, const oracle_config& cfg,
+                 const std::vector<uint32_t>& members, std::vector<uint8_t>& flag,
+                 uint32_t* out_t, double* out_d) {
+  const double offset = cfg.edge_weight_offset, res = cfg.resolution;
+  for (uint32_t u : members) flag[u] = 1;
+  // cluster_moving_weights (PCCI.cc:657): a handful of entries.
+  std::vector<std::pair<uint32_t, double>> cmw;
+  auto cmw_at = [&](uint32_t c) -> double& {
+    for (auto& e : cmw)
+      if (e.first == c) return e.second;
+    cmw.emplace_back(c, 0.0);
+    return cmw.back().second;
+  };
+  int64_t total_deg = 0;
+  for (uint32_t u : members) total_deg += g.offsets[u + 1] - g.offsets[u];
+  ClusterMap& after = t_map;  // class_1_together_after
+  after.Reset((size_t)total_deg);
+  double class2 = 0.0, class1_together = 0.0, moving_w = 0.0;
+  for (uint32_t node : members) {  // PCCI.cc:665-688
+    const uint32_t nc = s.cluster_ids[node];
+    cmw_at(nc) += s.node_weights[node];
+    moving_w += s.node_weights[node];
+    for (int64_t p = g.offsets[node]; p < g.offsets[(int64_t)node + 1]; ++p) {
+      const uint32_t nb = g.nbrs[p];
+      // The lambda takes the weight as float: the offset is subtracted in
+      // double and the result rounded back to float (PCCI.cc:668-669).
+      const float w = (float)((double)g.weights[p] - offset);
+      const uint32_t ncl = s.cluster_ids[nb];
+      if (flag[nb]) {
+        if (nc != ncl) class2 += (double)w;
+      } else {
+        if (nc == ncl) class1_together += (double)w;
+        after.At(ncl) += (double)w;
+      }
+    }
+  }
+  for (uint32_t u : members) flag[u] = 0;
+  class2 /= 2;  // RemoveDoubleCounting, UTIL.h:32
+  // BestMoveFromStats, UTIL.h:79-111.  `max_edges += a * b` is one fused
+  // multiply-add under GCC's default contraction, as NetWeight is.
+  double change = 0.0;
+  double max_edges = moving_w * moving_w / 2;
+  for (const auto& e : cmw) max_edges -= e.second * e.second / 2;
+  change += NetWeight(class2, max_edges, res);
+  max_edges = 0.0;
+  for (const auto& e : cmw)
+    max_edges = std::fma(e.second, s.cluster_weights[e.first] - e.second, max_edges);
+  change -= NetWeight(class1_together, max_edges, res);
+  bool have = false;
+  uint32_t best_c = 0;
+  double best_d = change;
+  for (uint32_t slot : after.used) {
+    const uint32_t c = after.keys[slot];
+    double moving_in_c = 0.0;
+    for (const auto& e : cmw)
+      if (e.first == c) moving_in_c = e.second;
+    const double me = moving_w * (s.cluster_weights[c] - moving_in_c);
+    const double overall = change + NetWeight(after.vals[slot], me, res);
+    if (overall > best_d || (overall == best_d && have && c < best_c)) {
+      best_c = c;
+      best_d = overall;
+      have = true;
+    }
+  }
+  *out_t = have ? best_c : (uint32_t)g.n;
+  *out_d = best_d;
+}
+
+// ClusteringHelper::AsyncMove(graph, moving_nodes), PCCI.cc:405-445
+// (sequential execution).
+bool AsyncMoveSet(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+                  const std::vector<uint32_t>& members, std::vector<uint8_t>& flag) {
+  if (members.empty()) return false;
+  if (members.size() == 1) return AsyncMove(g, s, cfg, members[0]);
+  uint32_t t;
+  double d;
+  BestMoveSet(g, s, cfg, members, flag, &t, &d);
+  if (d <= 0) return false;
+  const uint32_t n = (uint32_t)g.n;
+  const uint32_t cur = s.cluster_ids[members[0]];
+  const uint32_t cnt = (uint32_t)members.size();
+  s.cluster_sizes[cur] -= cnt;
+  // The moved weight is a float reduce of the node weights (PCCI.cc:416-419).
+  std::vector<float> fw(members.size());
+  for (size_t i = 0; i < members.size(); ++i) fw[i] = (float)s.node_weights[members[i]];
+  const float total = PbbsReduce(fw.data(), fw.size(), 0.0f, [](float a, float b) { return a + b; });
+  s.cluster_weights[cur] += (double)(-1 * total);
+  if (t != n) {
+    s.cluster_sizes[t] += cnt;
+    s.cluster_weights[t] += (double)total;
+    for (uint32_t u : members) s.cluster_ids[u] = t;
+    return true;
+  }
+  size_t i = cur;
+  while (true) {
+    if (s.cluster_sizes[i] == 0) {
+      s.cluster_sizes[i] = cnt;
+      s.cluster_weights[i] += (double)total;
+      for (uint32_t u : members) s.cluster_ids[u] = (uint32_t)i;
+      return true;
+    }
+    i = (i + 1) % n;
+  }
+}
+
+// The whole-cluster phase of a DEFAULT_CLUSTER_MOVES round, PCC.cc:167-296
+// (subclustering NONE): every non-empty cluster, as grouped after the vertex
+// phase, tries to move as a whole.  moved_clusters is or-ed with the clusters
+// it modifies.
+void ClusterMovesPhase(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+                       std::vector<uint8_t>& moved_clusters) {
+  const int64_t n = g.n;
+  std::vector<std::vector<uint32_t>> groups((size_t)n);
+  for (int64_t v = 0; v < n; ++v) groups[s.cluster_ids[v]].push_back((uint32_t)v);
+  std::vector<uint8_t> flag((size_t)n, 0);
+  std::vector<uint8_t> additional((size_t)n, 0);
+  if (!cfg.async) {
+    std::vector<uint32_t> moves((size_t)n, kNone);
+    for (int64_t c = 0; c < n; ++c) {  // PCC.cc:263-280
+      const auto& grp = groups[(size_t)c];
+      if (grp.empty()) continue;
+      uint32_t t;
+      double d;
+      BestMoveSet(g, s, cfg, grp, flag, &t, &d);
+      const uint32_t cur = s.cluster_ids[grp.front()];
+      if (t < (uint32_t)n && cur >= t) {
+        t = cur;
+        d = 0;
+      }
+      if (d > 0)
+        for (uint32_t u : grp) moves[u] = t;
+    }
+    oracle_apply_moves(n, &s, moves.data(), additional.data());  // PCC.cc:285
+  } else {
+    for (int64_t c = 0; c < n; ++c) {  // PCC.cc:254-262
+      const auto& grp = groups[(size_t)c];
+      if (grp.empty()) continue;
+      // The reference marks the entries indexed by the members' vertex ids.
+      if (AsyncMoveSet(g, s, cfg, grp, flag))
+        for (uint32_t u : grp) additional[u] = 1;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) moved_clusters[(size_t)i] |= additional[(size_t)i];
+}
+
 uint64_t SplitMix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -409,6 +558,14 @@ void oracle_best_move(const oracle_graph* g, const oracle_state* s,
   BestMove(*g, *s, cfg->resolution, cfg->edge_weight_offset, v, out_target, out_delta);
 }
 
+void oracle_best_move_set(const oracle_graph* g, const oracle_state* s,
+                          const oracle_config* cfg, const uint32_t* members, int64_t count,
+                          uint32_t* out_target, double* out_delta) {
+  std::vector<uint32_t> set(members, members + count);
+  std::vector<uint8_t> flag((size_t)g->n, 0);
+  BestMoveSet(*g, *s, *cfg, set, flag, out_target, out_delta);
+}
+
 void oracle_sync_decisions(const oracle_graph* g, const oracle_state* s,
                            const oracle_config* cfg, const uint8_t* frontier,
                            uint32_t* out_moves, uint8_t* out_moved,
@@ -516,6 +673,8 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
                           nullptr);
     oracle_apply_moves(n, s, moves.data(), moved_clusters.data());  // PCC.cc:163-165
   }
+  if (cfg->clustering_moves_method == 0)  // DEFAULT_CLUSTER_MOVES, PCC.cc:167-297
+    ClusterMovesPhase(*g, *s, *cfg, moved_clusters);
   if (cfg->move_method == 1) {  // ALL_MOVE, PCC.cc:299-307
     std::memset(frontier, 1, (size_t)n);
     return n;
@@ -657,8 +816,11 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
   if (cfg.clustering_moves_method == 1) {  // PCC.cc:493-500
     num_iterations = cfg.louvain_num_iterations > 0 ? cfg.louvain_num_iterations : 10;
     num_inner = cfg.louvain_num_inner_iterations > 0 ? cfg.louvain_num_inner_iterations : 10;
+  } else if (cfg.clustering_moves_method == 0) {  // DEFAULT_CLUSTER_MOVES, PCC.cc:488-492
+    num_iterations = 1;
+    num_inner = cfg.num_iterations > 0 ? cfg.num_iterations : 20;
   } else {
-    return 12;  // DEFAULT_CLUSTER_MOVES is not on this path
+    return 12;  // PCC.cc:501-505
   }
   int64_t rounds = 0;
   g_edges_scanned = 0;

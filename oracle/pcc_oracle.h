@@ -66,6 +66,13 @@ void oracle_best_move(const oracle_graph* g, const oracle_state* s,
                       const oracle_config* cfg, uint32_t v,
                       uint32_t* out_target, double* out_delta);
 
+/* BestMove(graph, moving_nodes), PCCI.cc:644-707 with BestMoveFromStats,
+ * UTIL.h:70-113: the best cluster for the whole set members[0..count) (in
+ * the reference's visiting order).  target == n means "a new cluster". */
+void oracle_best_move_set(const oracle_graph* g, const oracle_state* s,
+                          const oracle_config* cfg, const uint32_t* members,
+                          int64_t count, uint32_t* out_target, double* out_delta);
+
 /* The sync vertex map of BestMovesForVertexSubset, PCC.cc:132-160: best move,
  * singleton rule, then moves[v] iff delta > 0.  frontier == NULL means all
  * vertices.  out_moves[v] = target or ORACLE_NO_MOVE; out_moved[v] in {0,1};
@@ -120,6 +127,8 @@ double oracle_modularity(const oracle_graph* g, const uint32_t* cluster_ids,
  * otherwise it is RefineClusters with that clustering (ids in [0, n)).
  * out_ids receives the final per-vertex cluster ids exactly as the reference's
  * cluster_ids vector before OutputIndicesById (PCC.cc:606-616).
+ * clustering_moves_method 0 (DEFAULT_CLUSTER_MOVES) runs one level of
+ * rounds that move vertices and then whole clusters (PCC.cc:167-297, 488-492).
  * Returns 0, or 12 (UNIMPLEMENTED) for an unsupported moves method. */
 typedef struct oracle_result {
   double objective;          /* PCC.cc:619-624 */
