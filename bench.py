@@ -47,6 +47,13 @@ def apply_tuning(g, args, pcc_amd):
         g.set_option(pcc_amd.OPT_PULL_FRACTION, args.pull_fraction)
 
 
+def set_tuned(g, t, pcc_amd):
+    """Applies an autotune result (pcc_autotune) to another handle."""
+    g.set_option(pcc_amd.OPT_WAVE_MAX_DEG, int(t["wave_max_deg"]))
+    g.set_option(pcc_amd.OPT_BLOCK_MAX_DEG, int(t["block_max_deg"]))
+    g.set_option(pcc_amd.OPT_HUB_PART_EDGES, int(t["hub_part_edges"]))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +82,8 @@ def main() -> int:
     ap.add_argument("--block-max", type=int, default=0, help="upper degree of the workgroup bin")
     ap.add_argument("--hub-part-edges", type=int, default=0, help="edges per hub partition")
     ap.add_argument("--async-subrounds", type=int, default=None, help="async sub-rounds (0: auto)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched: keep the default degree-bin thresholds instead of pcc_autotune")
     ap.add_argument("--pull-fraction", type=int, default=None,
                     help="frontier pull threshold in 1/1024 of m + n (1024: push only)")
     args = ap.parse_args()
@@ -285,6 +294,17 @@ def bench_batched(args):
     obj_b = [None] * len(graphs)
     rounds = 0
     te_b = 0
+    tune = None
+    if not args.no_autotune:
+        # Degree-bin thresholds and hub partition size chosen on the first
+        # batch by pcc_autotune (results do not depend on them), then used
+        # for every batch and for the plain runs.
+        t0 = time.perf_counter()
+        tune = batches[0].autotune(cfg, resolutions=res[groups[0]], max_trials=8)
+        for b in batches[1:]:
+            set_tuned(b, tune, pcc_amd)
+        tune["tuning_seconds"] = round(time.perf_counter() - t0, 3)
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         for grp, b in zip(groups, batches):
             b.cluster_batch(cfg, res[grp])
@@ -306,6 +326,8 @@ def bench_batched(args):
     same = 0
     for i in range(plain_n):
         g = pcc_amd.DeviceGraph.from_csr(graphs[i])
+        if tune:
+            set_tuned(g, tune, pcc_amd)
         c1 = pcc_amd.make_config(resolution=float(res[i]), move_method=pcc_amd.NBHR_MOVE, refine=1)
         t1 = time.perf_counter()
         ids, sst = g.cluster(c1)
@@ -327,6 +349,7 @@ def bench_batched(args):
         "speedup_vs_plain": round(batch_gps / plain_gps, 2),
         "identical_results": f"{same}/{plain_n}",
         "batches": len(groups), "batch_rounds": rounds, "batch_edges_scanned": te_b,
+        "autotune": tune,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -143,6 +143,9 @@ typedef struct pcc_stats {
   int64_t bin_edges[8];        /* edges scanned */
   int64_t bin_vertices[8];     /* vertices processed */
   int64_t bin_launches[8];     /* launches (timing on) */
+  /* Whole-cluster phase (clustering_moves_method DEFAULT_CLUSTER_MOVES). */
+  double cluster_move_seconds; /* kernel time (timing on) */
+  int64_t cluster_moves;       /* clusters moved */
 } pcc_stats;
 
 /* ParallelCorrelationClusterer::Cluster
@@ -291,7 +294,31 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_PULL_FRACTION 8  /* next-frontier expansion pulls when the seeds' (degree + 1)
                                     sum exceeds value/1024 of (m + n); symmetric graphs only;
                                     1024 disables (default 512) */
+#define PCC_OPT_CLUSTER_SUBROUNDS 9 /* async whole-cluster moves: sub-rounds per round
+                                       (default 16, range [1, 1024]) */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
+
+/* Autotuner of the degree-bin thresholds (PCC_OPT_WAVE_MAX_DEG,
+ * PCC_OPT_BLOCK_MAX_DEG) and the hub partition size (PCC_OPT_HUB_PART_EDGES)
+ * for one workload: runs whole Cluster calls on the handle (a single graph or
+ * a batch; resolutions per graph as in pcc_batch_cluster, may be NULL) for
+ * candidate settings by coordinate descent from the current ones, timing the
+ * device span of each call, and leaves the fastest setting on the handle.
+ * These settings only choose which kernel handles a vertex: the clustering
+ * itself does not depend on them.  max_trials <= 0 means the default budget
+ * (one sweep over the three settings).  No counterpart in the reference (its
+ * CPU path has no kernel shapes to choose); it serves the batched and serving
+ * forms. */
+typedef struct pcc_tune_result {
+  int64_t wave_max_deg;      /* chosen settings */
+  int64_t block_max_deg;
+  int64_t hub_part_edges;
+  int64_t trials;            /* Cluster calls made */
+  double initial_seconds;    /* device span with the settings found on entry */
+  double best_seconds;       /* device span with the chosen settings */
+} pcc_tune_result;
+int pcc_autotune(pcc_graph* g, const pcc_config* cfg, int objective,
+                 const double* resolutions, int max_trials, pcc_tune_result* out);
 
 /* ------------------------------------------------------------ host data -- */
 

@@ -181,6 +181,26 @@ __device__ __forceinline__ int WaveMaxInt(int x) {
   return x;
 }
 
+// Source vertex of edge p (p < m): the largest u with off[u] <= p, searched in
+// [lo, n).
+__device__ __forceinline__ int64_t VertexOfEdge(const int64_t* __restrict__ off, int64_t lo,
+                                                int64_t n, int64_t p) {
+  // The next few vertices first (the common case), then bisection.
+  for (int k = 0; k < 4 && lo + 1 < n; ++k) {
+    if (off[lo + 1] > p) return lo;
+    ++lo;
+  }
+  int64_t hi = n;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo < n ? lo : n - 1;  // off[n] == m > p keeps lo < n; guard anyway
+}
+
 // Block-wide helpers; `smem` needs blockDim.x / 64 slots.
 template <typename T, typename Op>
 __device__ __forceinline__ T BlockReduce(T x, T* smem, Op op) {

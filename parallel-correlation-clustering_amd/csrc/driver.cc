@@ -296,6 +296,80 @@ void pcc_graph_destroy(pcc_graph* g) {
   delete g;
 }
 
+int pcc_autotune(pcc_graph* g, const pcc_config* cfg, int objective, const double* resolutions,
+                 int max_trials, pcc_tune_result* out) {
+  ClearError();
+  return Guard("pcc_autotune", [&]() -> int {
+    PCC_RETURN_IF_ERROR(CheckGraph(g));
+    if (!cfg || !out) return SetError(PCC_INVALID_ARGUMENT, "null argument");
+    struct Setting {
+      int64_t wave, block, part;
+    };
+    const int64_t n = g->csr.n;
+    std::vector<uint32_t> ids((size_t)std::max<int64_t>(n, 1));
+    const bool timing0 = g->timing;
+    g->timing = false;  // per-launch events would distort the comparison
+    const int G = g->num_graphs();
+    int trials = 0;
+    const int budget = max_trials > 0 ? max_trials : 16;
+    // Device span of one Cluster call with setting x.
+    auto run = [&](const Setting& x, double* sec) -> int {
+      g->th.wave_max = x.wave;
+      g->th.block_max = x.block;
+      g->hub_part_edges = x.part;
+      pcc_stats st;
+      ++trials;
+      PCC_RETURN_IF_ERROR(ClusterImpl(g, cfg, objective, nullptr, G > 1 ? resolutions : nullptr,
+                                      nullptr, ids.data(), nullptr, nullptr, &st));
+      *sec = st.cluster_seconds;
+      return PCC_OK;
+    };
+    Setting best{g->th.wave_max, g->th.block_max, g->hub_part_edges};
+    double best_s = 0;
+    int rc = run(best, &best_s);
+    if (rc != PCC_OK) {
+      g->timing = timing0;
+      return rc;
+    }
+    out->initial_seconds = best_s;
+    // One coordinate at a time: workgroup bin bound, wave bin bound, hub
+    // partition size.
+    const int64_t blocks[] = {1024, 2048, 4096};
+    const int64_t waves[] = {128, 256, 512};
+    const int64_t parts[] = {1024, 2048, 4096};
+    for (int coord = 0; coord < 3 && rc == PCC_OK; ++coord) {
+      const int64_t* cand = coord == 0 ? blocks : coord == 1 ? waves : parts;
+      const Setting base = best;
+      for (int i = 0; i < 3 && trials < budget; ++i) {
+        Setting x = base;
+        if (coord == 0) x.block = cand[i];
+        if (coord == 1) x.wave = cand[i];
+        if (coord == 2) x.part = cand[i];
+        if (x.wave > x.block) continue;
+        if (x.wave == base.wave && x.block == base.block && x.part == base.part) continue;
+        double sec = 0;
+        rc = run(x, &sec);
+        if (rc != PCC_OK) break;
+        if (sec < best_s) {
+          best_s = sec;
+          best = x;
+        }
+      }
+    }
+    g->timing = timing0;
+    g->th.wave_max = best.wave;
+    g->th.block_max = best.block;
+    g->hub_part_edges = best.part;
+    if (rc != PCC_OK) return rc;
+    out->wave_max_deg = best.wave;
+    out->block_max_deg = best.block;
+    out->hub_part_edges = best.part;
+    out->trials = trials;
+    out->best_seconds = best_s;
+    return PCC_OK;
+  });
+}
+
 int pcc_set_option(pcc_graph* g, int key, int64_t value) {
   ClearError();
   if (!g) return SetError(PCC_INVALID_ARGUMENT, "null graph");
@@ -328,6 +402,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
       if (value < 16 || value > 4096)
         return SetError(PCC_INVALID_ARGUMENT, "hub partition size must lie in [16, 4096]");
       g->hub_part_edges = value;
+      return PCC_OK;
+    case PCC_OPT_CLUSTER_SUBROUNDS:
+      if (value < 1 || value > 1024)
+        return SetError(PCC_INVALID_ARGUMENT, "cluster sub-rounds must lie in [1, 1024]");
+      g->cluster_subrounds = (int)value;
       return PCC_OK;
     case PCC_OPT_PULL_FRACTION:
       if (value < 0 || value > 1024)

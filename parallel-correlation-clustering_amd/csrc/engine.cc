@@ -598,6 +598,18 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     RunNewClusterAssign(al_, st, (uint32_t)n, c.mp.vgraph, c.mp.gbase, ws.new_flag.get(),
                         ws.flags.get(), ws.rc.get(), s_);
   }
+  StopTimer(ta, 9);
+  if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
+    // Whole-cluster moves after the vertex moves (PCC.cc:167-297); the
+    // clusters they modify join the vertex phase's in ws.flags.
+    Timer tm = StartTimer();
+    cluster_moves_ += RunClusterMoves(al_, g, st, c.mp, G, counts, c.layout, ws.list.get(), async,
+                                      (uint32_t)h_->cluster_subrounds,
+                                      (uint32_t)rounds_ * 0x9E3779B9u,
+                                      ws.flags.get(), ws.new_flag.get(), ws.rc.get(), s_);
+    StopTimer(tm, 12);
+  }
+  ta = StartTimer();
   if (cfg_.move_method == PCC_ALL_MOVE) {  // PCC.cc:299-307: frontier unchanged
     StopTimer(ta, 9);
     RoundCounters hrc;
@@ -704,9 +716,9 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
   if (cfg_.clustering_moves_method == PCC_LOUVAIN) {  // PCC.cc:493-500
     num_iterations = cfg_.louvain_num_iterations > 0 ? cfg_.louvain_num_iterations : 10;
     num_inner = cfg_.louvain_num_inner_iterations > 0 ? cfg_.louvain_num_inner_iterations : 10;
-  } else if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
-    return SetError(PCC_UNIMPLEMENTED,
-                    "DEFAULT_CLUSTER_MOVES (whole-cluster moves) is not on this path; use LOUVAIN");
+  } else if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {  // PCC.cc:488-492
+    num_iterations = 1;
+    num_inner = cfg_.num_iterations > 0 ? cfg_.num_iterations : 20;
   } else {
     return SetError(PCC_UNIMPLEMENTED,
                     "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
@@ -925,6 +937,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       stats_->bin_launches[b] = kind_launches_[b];
     }
     stats_->ordered_seconds = kind_seconds_[8];
+    stats_->cluster_move_seconds = kind_seconds_[12];
+    stats_->cluster_moves = cluster_moves_;
   }
   return PCC_OK;
 }

@@ -40,6 +40,7 @@ struct pcc_graph {
   bool force_ordered = false;
   int async_subrounds = 0;  // 0: automatic (see Engine::RoundStep)
   int async_singleton_rule = 0;
+  int cluster_subrounds = (int)pcc::kClusterSubRounds;  // async whole-cluster moves
   int64_t hub_part_edges = pcc::kHubPartEdges;
   // Pull expansion when the seed volume exceeds pull_fraction / 1024 of the
   // graph's (edges + vertices); 1024 disables it.
@@ -195,6 +196,7 @@ class Engine {
   int64_t bin_edges_[kBins] = {0};
   int64_t bin_vertices_[kBins] = {0};
   int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
+  int64_t cluster_moves_ = 0;
   double bestmove_bytes_ = 0;
   // With refinement every level's graph is visited twice; its best-move
   // context (bins, thresholds, hub lists and scratch) depends only on the

@@ -57,26 +57,6 @@ constexpr int kEdgeItems = 16;
 constexpr int kEdgeThreads = 256;
 constexpr int64_t kEdgeChunk = (int64_t)kEdgeItems * kEdgeThreads;
 
-// Source vertex of edge p (p < m): the largest u with off[u] <= p, searched in
-// [lo, n).
-__device__ __forceinline__ int64_t VertexOfEdge(const int64_t* __restrict__ off, int64_t lo,
-                                                int64_t n, int64_t p) {
-  // The next few vertices first (the common case), then bisection.
-  for (int k = 0; k < 4 && lo + 1 < n; ++k) {
-    if (off[lo + 1] > p) return lo;
-    ++lo;
-  }
-  int64_t hi = n;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (off[mid] <= p)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo < n ? lo : n - 1;  // off[n] == m > p keeps lo < n; guard anyway
-}
-
 struct EdgeWalk {
   int64_t u = -1, uend = 0;
   uint32_t cu = 0;
@@ -985,6 +965,8 @@ void SegmentSums(DeviceAllocator* al, const V* vals, const int64_t* start, int64
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+}  // namespace
+
 // Segment boundaries of sorted keys: returns the number of segments and
 // fills start[0..nseg] (start[nseg] = n).
 int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<int64_t>* start,
@@ -1009,8 +991,6 @@ int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<in
   PCC_HIP_CHECK(hipGetLastError());
   return (int64_t)nseg;
 }
-
-}  // namespace
 
 bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
   if (g.m == 0) return true;

@@ -290,6 +290,21 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
                          uint8_t* modified, RoundCounters* rc, hipStream_t s);
 
+// The whole-cluster phase of a DEFAULT_CLUSTER_MOVES round
+// (parallel-correlation-clusterer.cc:167-297) over the clusters of the graphs
+// whose frontier list (counts / layout / list) is non-empty.  Sync: the moves
+// are applied as MoveNodesToCluster, or-ing modified clusters into modified.
+// Async: sub_rounds sub-rounds over a hash of the cluster id (mixed with
+// salt); modified is marked at the members' vertex ids as the reference does.
+// Returns the number of cluster moves.  Synchronizes the stream.
+int64_t RunClusterMoves(DeviceAllocator* al, const CsrView& g, const StateView& st,
+                        const MoveParams& mp, int G, const int64_t counts[kBins],
+                        const BinLayout& layout, const uint32_t* list, bool async,
+                        uint32_t sub_rounds, uint32_t salt, uint8_t* modified, uint8_t* new_flag,
+                        RoundCounters* rc, hipStream_t s);
+// Async sub-rounds of the whole-cluster phase.
+constexpr uint32_t kClusterSubRounds = 16;
+
 // Empty cluster ids at the start of an async round (MoveParams::free_*).
 struct FreeIds {
   DevBuf<uint32_t> ids;
@@ -369,6 +384,11 @@ void RadixSortPairsF32(DeviceAllocator* al, uint64_t* keys, float* vals, int64_t
                        int end_bit, hipStream_t s);
 void RadixSortPairsF64(DeviceAllocator* al, uint64_t* keys, double* vals, int64_t n,
                        int begin_bit, int end_bit, hipStream_t s);
+
+// Segment boundaries of sorted keys: returns the number of runs of equal keys
+// and fills start[0..nseg] (start[nseg] = n).  Synchronizes the stream.
+int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<int64_t>* start,
+                 hipStream_t s);
 
 // ------------------------------------------------------------ contraction --
 

@@ -50,6 +50,7 @@ OPT_ASYNC_SUBROUNDS = 5
 OPT_ASYNC_SINGLETON_RULE = 6
 OPT_HUB_PART_EDGES = 7
 OPT_PULL_FRACTION = 8
+OPT_CLUSTER_SUBROUNDS = 9
 
 
 class StatusError(RuntimeError):
@@ -106,6 +107,8 @@ class Stats(ctypes.Structure):
         ("bin_edges", ctypes.c_int64 * 8),
         ("bin_vertices", ctypes.c_int64 * 8),
         ("bin_launches", ctypes.c_int64 * 8),
+        ("cluster_move_seconds", ctypes.c_double),
+        ("cluster_moves", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -114,6 +117,20 @@ class Stats(ctypes.Structure):
             v = getattr(self, name)
             d[name] = list(v) if hasattr(v, "__len__") else v
         return d
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("wave_max_deg", ctypes.c_int64),
+        ("block_max_deg", ctypes.c_int64),
+        ("hub_part_edges", ctypes.c_int64),
+        ("trials", ctypes.c_int64),
+        ("initial_seconds", ctypes.c_double),
+        ("best_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ServerOptions(ctypes.Structure):
@@ -174,6 +191,8 @@ _SIGS = {
     "pcc_round": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, _P, _P, _P, ctypes.POINTER(_I64)]),
     "pcc_compress": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(_P), _P]),
     "pcc_set_option": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
+    "pcc_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.c_int,
+                                    ctypes.POINTER(TuneResult)]),
     "pcc_batch_from_csr": (ctypes.c_int, [_I64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "pcc_batch_num_graphs": (_I64, [_P]),
     "pcc_batch_cluster": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, _P, _P, _P,
@@ -359,6 +378,16 @@ class DeviceGraph:
 
     def set_option(self, key: int, value: int) -> None:
         _check(lib().pcc_set_option(self._h, key, value))
+
+    def autotune(self, cfg: Config, objective: int = OBJECTIVE_CORRELATION, resolutions=None,
+                 max_trials: int = 0) -> dict:
+        """pcc_autotune: picks the degree-bin thresholds and hub partition
+        size for this workload and leaves them set on the handle."""
+        res = None if resolutions is None else np.ascontiguousarray(resolutions, dtype=np.float64)
+        out = TuneResult()
+        _check(lib().pcc_autotune(self._h, ctypes.byref(cfg), objective, _ptr(res), max_trials,
+                                  ctypes.byref(out)))
+        return out.as_dict()
 
     def to_csr(self) -> Csr:
         n, m = self.n, self.m

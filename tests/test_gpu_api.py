@@ -81,7 +81,7 @@ def test_error_statuses(pcc):
     csr = pcc.gen_sbm(50, 2, 0.3, 0.0, seed=24)
     g = pcc.DeviceGraph.from_csr(csr)
     with pytest.raises(pcc.StatusError) as e:
-        g.cluster(pcc.make_config(clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES))
+        g.cluster(pcc.make_config(clustering_moves_method=5))  # PCC.cc:501-505
     assert e.value.code == pcc.UNIMPLEMENTED
     with pytest.raises(pcc.StatusError) as e:
         g.cluster(pcc.make_config(subclustering_method=1))
@@ -133,3 +133,31 @@ def test_cli_clone(pcc, orc, tmp_path):
     bad = subprocess.run([CLI, "--clusterer_name=Nope", "--clusterer_config=", f"--input_graph={path}"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode != 0 and "UNIMPLEMENTED" in bad.stderr
+
+
+def test_autotune_keeps_results(pcc):
+    csr = pcc.gen_rmat(14, 16, w_lo=-1.0, w_hi=1.0, seed=77)
+    cfg = pcc.make_config(resolution=0.05, move_method=2, refine=1)
+    g = pcc.DeviceGraph.from_csr(csr)
+    ref, _ = g.cluster(cfg)
+    t = g.autotune(cfg, max_trials=6)
+    assert 2 <= t["trials"] <= 6
+    assert t["wave_max_deg"] in (128, 256, 512)
+    assert t["block_max_deg"] in (1024, 2048, 4096)
+    assert t["hub_part_edges"] in (1024, 2048, 4096)
+    assert 0 < t["best_seconds"] <= t["initial_seconds"]
+    ids, _ = g.cluster(cfg)
+    assert np.array_equal(ids, ref)
+
+
+def test_autotune_batch(pcc):
+    graphs = [pcc.gen_rmat(s, 8, seed=90 + s) for s in (8, 11, 9, 12)]
+    res = np.array([0.01, 0.05, 0.02, 0.08])
+    cfg = pcc.make_config(move_method=0)
+    b = pcc.DeviceBatch.from_csrs(graphs)
+    ref, _, _, _ = b.cluster_batch(cfg, res)
+    t = b.autotune(cfg, resolutions=res)
+    assert t["trials"] >= 2
+    ids, _, _, _ = b.cluster_batch(cfg, res)
+    for a, r in zip(ids, ref):
+        assert np.array_equal(a, r)

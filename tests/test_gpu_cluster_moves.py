@@ -1,0 +1,179 @@
+"""Whole-cluster moves (clustering_moves_method DEFAULT_CLUSTER_MOVES) on the
+GPU against the oracle.
+
+Sync mode is bit-exact: state after each round (ids, sizes, FP64 cluster
+weights, next frontier), final ids of whole runs and of batches.  Async mode
+is racy in the reference (parallel-correlation-clusterer.cc:254-262), so it is
+checked statistically against the oracle's async runs over eight processing
+orders, like the vertex-move async tests.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import random_state
+from test_oracle_cluster_moves import two_stuck_cliques
+from test_oracle import to_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def _graphs(pcc):
+    return {
+        "sbm_unit": pcc.gen_sbm(600, 12, 0.15, 0.01, seed=11),
+        "rmat_signed": pcc.gen_rmat(12, 8, w_lo=-1.0, w_hi=1.0, seed=12),
+        "rmat_hubs": pcc.gen_rmat(15, 16, w_lo=-1.0, w_hi=1.0, seed=13),
+        "rmat_unit_hubs": pcc.gen_rmat(15, 16, seed=14),
+    }
+
+
+@pytest.mark.parametrize("state_kind", ["singletons", "blocks"])
+@pytest.mark.parametrize("move_method", [0, 1, 2])
+@pytest.mark.parametrize("name", ["sbm_unit", "rmat_signed", "rmat_hubs"])
+def test_cluster_move_rounds_bit_exact(pcc, orc, name, move_method, state_kind):
+    """blocks: a random state of large clusters, so runs of inter-cluster
+    edges are long (the blocked / ordered sums) and many clusters move."""
+    csr = _graphs(pcc)[name]
+    print(f"cluster-move rounds {name} mm={move_method} {state_kind}", flush=True)
+    n = csr.n
+    rng = np.random.default_rng(5)
+    if state_kind == "singletons":
+        ids, sizes, nw = np.arange(n, dtype=np.uint32), np.ones(n, np.uint32), np.ones(n)
+        K = nw.copy()
+    else:
+        ids, sizes, nw, K = random_state(rng, n, max(2, n // 200))
+    for res, off in [(0.05, 0.0), (0.02, 0.1)]:
+        cfg = pcc.make_config(resolution=res, edge_weight_offset=off, move_method=move_method,
+                              clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES)
+        ocfg = orc.config_from(cfg)
+        dg = pcc.DeviceGraph.from_csr(csr)
+        og = orc.G.from_csr(csr)
+        ostate = orc.S(ids, sizes, nw, K)
+        gstate = (ids, sizes, K)
+        gfr = np.ones(n, np.uint8)
+        ofr = gfr.copy()
+        for r in range(4):
+            gi, gs, gk, gfr, gsize = dg.round(cfg, gstate[0], gstate[1], nw, gstate[2], gfr)
+            ofr, osize = orc.round_(og, ostate, ocfg, ofr)
+            assert np.array_equal(gi, ostate.ids), f"ids differ after round {r} (res {res})"
+            assert np.array_equal(gs, ostate.sizes), f"sizes differ after round {r}"
+            assert np.array_equal(gk.view(np.uint64), ostate.K.view(np.uint64)), \
+                f"cluster weights differ after round {r}"
+            assert np.array_equal(gfr, ofr), f"frontier differs after round {r}"
+            assert gsize == osize
+            gstate = (gi, gs, gk)
+            if osize == 0:
+                break
+
+
+CASES = [
+    ("sbm_unit", dict(resolution=0.01, move_method=1, num_iterations=8), False),
+    ("rmat_signed", dict(resolution=0.05, move_method=2), False),
+    ("rmat_signed", dict(resolution=0.05, move_method=0, edge_weight_offset=0.1), False),
+    ("rmat_hubs", dict(resolution=0.05, move_method=0), False),
+    ("rmat_unit_hubs", dict(resolution=1.0, move_method=0), True),
+    ("sbm_unit", dict(resolution=0.01, move_method=2, refine=1), False),  # identity quirk
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_default_cluster_moves_bit_exact(pcc, orc, case):
+    name, kw, modularity = CASES[case]
+    csr = _graphs(pcc)[name]
+    print(f"default-cluster-moves {name} {kw} modularity={modularity}", flush=True)
+    cfg = pcc.make_config(clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES, **kw)
+    dg = pcc.DeviceGraph.from_csr(csr)
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), modularity)
+    assert np.array_equal(ids, r.ids)
+    assert st.levels == r.levels == 1 and st.rounds == r.rounds
+    assert st.objective == pytest.approx(r.objective, rel=1e-9, abs=1e-6)
+    if not kw.get("refine"):
+        assert st.cluster_moves > 0
+
+
+def test_stuck_cliques_merge(pcc, orc):
+    A, W = two_stuck_cliques()
+    off, nb, w = to_csr(A, W)
+    csr = pcc.Csr(8, off, nb, w)
+    init = np.array([0] * 4 + [4] * 4, np.uint32)
+    cfg = pcc.make_config(resolution=0.1, clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES)
+    out, st = pcc.DeviceGraph.from_csr(csr).refine(cfg, init)
+    assert np.array_equal(out, np.full(8, 4, np.uint32))
+    assert st.cluster_moves == 1
+
+
+def _batch_graphs(pcc, seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(20), np.log(3000))))
+        if i % 3 == 2:
+            out.append(pcc.gen_rmat(max(4, int(np.log2(n))), float(rng.uniform(2, 10)),
+                                    w_lo=-1.0, w_hi=1.0, seed=seed * 1000 + i))
+        else:
+            out.append(pcc.gen_sbm(n, max(1, n // int(rng.integers(10, 60))),
+                                   float(rng.uniform(0.05, 0.4)), 0.002, 0.0, 1.0,
+                                   seed=seed * 1000 + i))
+    return out
+
+
+@pytest.mark.parametrize("move_method", [0, 2])
+def test_batch_default_cluster_moves(pcc, orc, move_method):
+    graphs = _batch_graphs(pcc, 70 + move_method, 16)
+    rng = np.random.default_rng(move_method)
+    res = rng.uniform(0.001, 0.1, size=len(graphs))
+    kw = dict(move_method=move_method, clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES,
+              num_iterations=12)
+    ids, obj, _, st = pcc.DeviceBatch.from_csrs(graphs).cluster_batch(pcc.make_config(**kw), res)
+    for i, g in enumerate(graphs):
+        c1 = pcc.make_config(resolution=float(res[i]), **kw)
+        single, sst = pcc.DeviceGraph.from_csr(g).cluster(c1)
+        assert np.array_equal(ids[i], single), f"graph {i} differs from its single run"
+        assert obj[i] == sst.objective
+        r = orc.cluster(orc.G.from_csr(g), orc.config_from(c1))
+        assert np.array_equal(ids[i], r.ids), f"graph {i} differs from the oracle"
+
+
+ASYNC_CASES = [
+    # (graph, config kwargs, modularity)
+    (dict(kind="sbm", n=4000, blocks=40, seed=32), dict(resolution=0.01, move_method=2), False),
+    (dict(kind="rmat", scale=14, seed=35, w_lo=-1.0, w_hi=1.0),
+     dict(resolution=0.05, move_method=1, num_iterations=10), False),
+    (dict(kind="rmat", scale=14, seed=31, w_lo=1.0, w_hi=1.0),
+     dict(resolution=1.0, move_method=1, num_iterations=10), True),
+    (dict(kind="rmat", scale=14, seed=35, w_lo=-1.0, w_hi=1.0),
+     dict(resolution=0.05, move_method=2, num_iterations=10), False),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ASYNC_CASES)))
+def test_async_default_cluster_moves(pcc, orc, case):
+    spec, kw, modularity = ASYNC_CASES[case]
+    if spec["kind"] == "sbm":
+        csr = pcc.gen_sbm(spec["n"], spec["blocks"], 0.1, 0.002, seed=spec["seed"])
+    else:
+        csr = pcc.gen_rmat(spec["scale"], 16, w_lo=spec["w_lo"], w_hi=spec["w_hi"],
+                           seed=spec["seed"])
+    print(f"async cluster moves {spec} {kw}", flush=True)
+    obj = pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION
+    og = orc.G.from_csr(csr)
+    vals = []
+    for seed in range(1, 9):
+        c = pcc.make_config(clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES, async_=1,
+                            permute=1, permute_seed=seed, **kw)
+        r = orc.cluster(og, orc.config_from(c), modularity)
+        vals.append(r.modularity if modularity else r.objective)
+    cfg = pcc.make_config(clustering_moves_method=pcc.DEFAULT_CLUSTER_MOVES, async_=1, **kw)
+    ids, st = pcc.DeviceGraph.from_csr(csr).cluster(cfg, obj)
+    got = st.modularity if modularity else st.objective
+    # The reported value is the oracle's evaluation of the returned ids.
+    if modularity:
+        assert got == pytest.approx(orc.modularity(og, ids, kw["resolution"]), abs=1e-9)
+    else:
+        assert got == pytest.approx(orc.objective(og, ids, np.ones(csr.n), kw["resolution"]),
+                                    rel=1e-9)
+    lo = min(vals)
+    print(f"  gpu {got:.6f} oracle mean {np.mean(vals):.6f} range [{lo:.6f}, {max(vals):.6f}]",
+          flush=True)
+    assert got >= lo - 0.005 * abs(np.mean(vals))
