@@ -69,7 +69,8 @@ def main() -> int:
     ap.add_argument("--no-gather-ceiling", action="store_true",
                     help="skip the random-gather ceiling (a child process using torch)")
     ap.add_argument("--workload", default="rmat22",
-                    choices=["rmat22", "batched", "modularity-async", "serving", "copy", "gather"],
+                    choices=["rmat22", "batched", "modularity-async", "serving", "copy", "gather",
+                             "cluster-moves"],
                     help="rmat22: the headline line; the others are secondary measurements")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of SBM graphs")
     ap.add_argument("--batch-edges", type=int, default=1 << 30,
@@ -84,6 +85,8 @@ def main() -> int:
     ap.add_argument("--async-subrounds", type=int, default=None, help="async sub-rounds (0: auto)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: keep the default degree-bin thresholds instead of pcc_autotune")
+    ap.add_argument("--autotune", action="store_true",
+                    help="rmat22: run pcc_autotune on the workload first and report its choice")
     ap.add_argument("--pull-fraction", type=int, default=None,
                     help="frontier pull threshold in 1/1024 of m + n (1024: push only)")
     args = ap.parse_args()
@@ -106,6 +109,8 @@ def main() -> int:
         return bench_copy(args)
     if args.workload == "gather":
         return bench_gather(args)
+    if args.workload == "cluster-moves":
+        return bench_cluster_moves(args)
 
     import numpy as np
     import pcc_amd
@@ -117,10 +122,14 @@ def main() -> int:
     t0 = time.time()
     g = pcc_amd.DeviceGraph.from_csr(csr)
     upload_s = time.time() - t0
-    g.set_option(pcc_amd.OPT_TIMING, 1)
     apply_tuning(g, args, pcc_amd)
     print(f"# graph n={csr.n} m={csr.m} gen={gen_s:.1f}s upload={upload_s:.2f}s", file=sys.stderr,
           flush=True)
+    tune = None
+    if args.autotune:
+        tune = g.autotune(cfg, max_trials=8)
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
+    g.set_option(pcc_amd.OPT_TIMING, 1)
 
     for i in range(args.warmup):
         ids, st = g.cluster(cfg)
@@ -218,6 +227,72 @@ def main() -> int:
             "peak_source": f"torch.index_select of 45M random indices into {csr.n} u32 entries "
                            "(bench.py --workload gather)",
         },
+        "cpu_baseline": cpu,
+    }
+    if tune:
+        line["autotune"] = tune
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_cluster_moves(args):
+    """Whole-cluster moves (DEFAULT_CLUSTER_MOVES) on the RMAT graph of the
+    default workload: one level of up to 20 sync NBHR_MOVE rounds, each a
+    vertex phase and a whole-cluster phase.  The CPU figure is the oracle's
+    first round (both phases) from singletons on the same graph."""
+    import numpy as np
+    import pcc_amd
+
+    csr = pcc_amd.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
+    cfg = pcc_amd.make_config(resolution=0.05, move_method=pcc_amd.NBHR_MOVE,
+                              clustering_moves_method=pcc_amd.DEFAULT_CLUSTER_MOVES)
+    g = pcc_amd.DeviceGraph.from_csr(csr)
+    g.set_option(pcc_amd.OPT_TIMING, 1)
+    for _ in range(args.warmup):
+        g.cluster(cfg)
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ids, st = g.cluster(cfg)
+        stats.append(st)
+        print(f"# step {i}: {st.cluster_seconds*1e3:.1f} ms rounds={st.rounds} "
+              f"cluster moves={st.cluster_moves} clusters={st.num_clusters}", file=sys.stderr,
+              flush=True)
+    wall = (time.perf_counter() - t0) / max(args.steps, 1)
+    st = stats[-1]
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        og = oracle.G.from_csr(csr)
+        s = oracle.S.singletons(csr.n)
+        t1 = time.perf_counter()
+        oracle.round_(og, s, oracle.config_from(cfg), np.ones(csr.n, np.uint8))
+        cpu_s = time.perf_counter() - t1
+        oracle.set_threads(1)
+        cpu = {"value": round(cpu_s * 1e3, 3), "unit": "ms per round", "cores": threads,
+               "kind": "port",
+               "sample": "round 0 (vertex and whole-cluster phases) from singletons, same graph; "
+                         "the whole-cluster phase of the oracle is single-threaded"}
+    line = {
+        "metric": "whole-cluster-move Cluster time (DEFAULT_CLUSTER_MOVES, sync NBHR_MOVE)",
+        "value": round(st.cluster_seconds * 1e3, 3), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (move deltas) / f32 (edge weights)",
+        "data": "synthetic (seeded RMAT generated on the host, resident in HBM before timing)",
+        "config": {"workload": f"RMAT scale {args.scale} edge factor {args.edge_factor:g}, "
+                               "U[-1,1] weights, seed 2; sync NBHR_MOVE resolution 0.05, "
+                               "DEFAULT_CLUSTER_MOVES (one level, up to 20 rounds)",
+                   "n": csr.n, "m_directed": csr.m, "parallelism": "single GPU"},
+        "rounds": st.rounds, "cluster_moves": st.cluster_moves,
+        "cluster_phase_ms": round(st.cluster_move_seconds * 1e3, 3),
+        "bestmove_ms": round(st.bestmove_seconds * 1e3, 3),
+        "gpu_ms_per_round": round(st.cluster_seconds * 1e3 / max(st.rounds, 1), 3),
+        "objective": st.objective, "num_clusters": st.num_clusters,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

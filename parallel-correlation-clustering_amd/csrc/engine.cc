@@ -492,16 +492,19 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   out.moves = ws.moves.get();
   out.moved = ws.moved.get();
   out.moved_clusters = ws.flags.get();
-  if (async) PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
   // Pull can only win when the seeds' volume may exceed the threshold.  With
   // NBHR_MOVE the seeds are movers, a subset of the frontier, so a frontier
   // of volume TE + F below the threshold rules pull out for this round.
   const double pull_thr = ((double)g.m + (double)n) * h_->pull_fraction / 1024.0;
   const bool pull = c.pull && (cfg_.move_method != PCC_NBHR_MOVE || (double)(TE + F) > pull_thr);
-  // The pull expansion reads moved[] of every vertex, not just the frontier's.
-  if (pull && cfg_.move_method == PCC_NBHR_MOVE)
-    PCC_HIP_CHECK(hipMemsetAsync(ws.moved.get(), 0, (size_t)n, s_));
-  PCC_HIP_CHECK(hipMemsetAsync(&ws.rc.get()->fb_count, 0, 3 * sizeof(unsigned long long), s_));
+  // One launch clears what the round accumulates into: the modified /
+  // destination clusters (only async best moves write them before the
+  // apply), the next frontier, the movers when the pull expansion of
+  // NBHR_MOVE reads every vertex's flag, the round counters (the host has
+  // read this frontier's bin counts already) and the seed volume.
+  LaunchRoundReset(n, ws.flags.get(), cfg_.move_method != PCC_ALL_MOVE ? ws.next.get() : nullptr,
+                   (pull && cfg_.move_method == PCC_NBHR_MOVE) ? ws.moved.get() : nullptr,
+                   ws.rc.get(), 3, pull ? c.vol.get() : nullptr, s_);
   const uint64_t salt =
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
   Timer all = StartTimer();
@@ -592,7 +595,6 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   Timer ta = StartTimer();
   if (!async) {
     // MoveNodesToCluster (PCCI.cc:256-369).
-    PCC_HIP_CHECK(hipMemsetAsync(ws.flags.get(), 0, (size_t)n, s_));
     LaunchApply(counts, c.layout, ws.list.get(), st, c.mp.new_id, ws.moves.get(), ws.flags.get(),
                 ws.new_flag.get(), ws.rc.get(), s_);
     RunNewClusterAssign(al_, st, (uint32_t)n, c.mp.vgraph, c.mp.gbase, ws.new_flag.get(),
@@ -611,22 +613,18 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   }
   ta = StartTimer();
   if (cfg_.move_method == PCC_ALL_MOVE) {  // PCC.cc:299-307: frontier unchanged
+    // The next round's bins are this round's, so nothing is read back: the
+    // host keeps launching while the device works.
     StopTimer(ta, 9);
-    RoundCounters hrc;
-    int64_t c2[kBins], t2[kBins];
-    ReadCounters(ws.rc.get(), c2, t2, &hrc);
-    fallback_ += (int64_t)(hrc.fb_count + hrc.repair_count);
     return F;
   }
   // Next frontier = neighbours of the seeds (PCC.cc:309-331).
-  PCC_HIP_CHECK(hipMemsetAsync(ws.next.get(), 0, (size_t)n, s_));
   const bool by_cluster = cfg_.move_method == PCC_NBHR_CLUSTER_MOVE;
   const uint32_t* seed_ids = by_cluster ? st.ids : nullptr;
   ExpandDir dir;
   if (pull) {
     // Direction by seed volume: the pull kernels or the push kernels run,
     // the other side returns at once.
-    PCC_HIP_CHECK(hipMemsetAsync(c.vol.get(), 0, sizeof(unsigned long long), s_));
     LaunchSeedVolume(g, seed_ids, ws.flags.get(), ws.moved.get(), c.vol.get(), s_);
     dir.vol = c.vol.get();
     dir.thr = (unsigned long long)pull_thr;
@@ -642,12 +640,9 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     LaunchPullFrontier(g, seed_ids, ws.flags.get(), ws.moved.get(), h_->th.block_max, c.Hubs(),
                        h_->hub_part_edges, dir, ws.next.get(), s_);
   LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
-  ResetCounters(ws.rc.get(), false);
   LaunchBinFrontier(g, ws.next.get(), h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
   StopTimer(ta, 9);
-  RoundCounters hrc;
-  ReadCounters(ws.rc.get(), counts, te, &hrc);
-  fallback_ += (int64_t)(hrc.fb_count + hrc.repair_count);
+  ReadCounters(ws.rc.get(), counts, te, nullptr);
   int64_t f = 0;
   for (int b = 0; b < kBins; ++b) f += counts[b];
   return f;
@@ -691,6 +686,13 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
     const int64_t fsize = RoundStep(c, ws, L.lay.vgraph.get(), G, gmoved.get(), counts, te);
     local_moved = fsize != 0;
     ++rounds;
+  }
+  {
+    // Ordered-path and repaired vertices of every round of this call.
+    RoundCounters hrc;
+    int64_t c2[kBins], t2[kBins];
+    ReadCounters(ws.rc.get(), c2, t2, &hrc);
+    fallback_ += (int64_t)(hrc.fb_total + hrc.fb_count + hrc.repair_count);
   }
   std::vector<uint8_t> moved((size_t)G, 0);
   if (cfg_.move_method == PCC_ALL_MOVE) {

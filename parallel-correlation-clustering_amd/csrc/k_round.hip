@@ -112,6 +112,94 @@ __global__ __launch_bounds__(256) void k_new_assign(uint32_t n, StateView st,
   }
 }
 
+// Levels of at most kSmallAssignMax vertices run the whole empty-id
+// assignment in one workgroup (one launch instead of a dozen; the same
+// result): thread t owns the contiguous ids [t * per, (t + 1) * per).
+constexpr int kSmallAssignThreads = 1024;
+constexpr uint32_t kSmallAssignMax = 1u << 18;
+__global__ __launch_bounds__(kSmallAssignThreads) void k_new_assign_small(
+    uint32_t n, StateView st, const uint32_t* __restrict__ vgraph,
+    const int64_t* __restrict__ gbase, uint8_t* __restrict__ new_flag,
+    uint32_t* __restrict__ zero_list, uint32_t* __restrict__ zrank, uint32_t* __restrict__ nrank,
+    uint8_t* __restrict__ modified, const unsigned long long* __restrict__ gate) {
+  if (*gate == 0) return;
+  __shared__ uint32_t sw[kSmallAssignThreads / kWave];
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t lo = min(n, threadIdx.x * per), hi = min(n, lo + per);
+  uint32_t z = 0, m = 0;
+  for (uint32_t i = lo; i < hi; ++i) {
+    z += st.sizes[i] == 0 ? 1u : 0u;
+    m += new_flag[i] ? 1u : 0u;
+  }
+  uint32_t zt = 0, mt = 0;
+  uint32_t zr = BlockExclusiveScan(z, sw, &zt);
+  uint32_t mr = BlockExclusiveScan(m, sw, &mt);
+  for (uint32_t i = lo; i < hi; ++i) {
+    zrank[i] = zr;
+    nrank[i] = mr;
+    if (st.sizes[i] == 0) zero_list[zr++] = i;
+    if (new_flag[i]) ++mr;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (uint32_t v = lo; v < hi; ++v) {
+    if (!new_flag[v]) continue;
+    int64_t base = 0, end = n;
+    if (vgraph) {
+      const uint32_t g = vgraph[v];
+      base = gbase[g];
+      end = gbase[g + 1];
+    }
+    const uint32_t idx = zrank[base] + (nrank[v] - nrank[base]);
+    new_flag[v] = 0;
+    if (idx >= zt) continue;  // inconsistent caller state: no write out of range
+    const uint32_t c = zero_list[idx];
+    if ((int64_t)c < base || (int64_t)c >= end) continue;
+    st.ids[v] = c;
+    st.sizes[c] = 1;
+    modified[c] = 1;
+    st.K[c] = st.nw[v];
+  }
+}
+
+// Per-round clears in one launch: up to three byte arrays of n entries, the
+// round counters selected by `counters` (bit 0: cnt / te, bit 1: fb_count /
+// new_count / repair_count, folded into fb_total first) and one 64-bit
+// accumulator.
+__global__ __launch_bounds__(256) void k_round_reset(int64_t n, uint8_t* __restrict__ a,
+                                                     uint8_t* __restrict__ b,
+                                                     uint8_t* __restrict__ c,
+                                                     RoundCounters* __restrict__ rc, int counters,
+                                                     unsigned long long* __restrict__ acc) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n16 = n / 16;
+  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  for (int64_t i = tid; i < n16; i += stride) {
+    if (a) reinterpret_cast<uint4*>(a)[i] = zero;
+    if (b) reinterpret_cast<uint4*>(b)[i] = zero;
+    if (c) reinterpret_cast<uint4*>(c)[i] = zero;
+  }
+  for (int64_t i = n16 * 16 + tid; i < n; i += stride) {
+    if (a) a[i] = 0;
+    if (b) b[i] = 0;
+    if (c) c[i] = 0;
+  }
+  if (tid < kBins && (counters & 1)) {
+    rc->cnt[tid] = 0;
+    rc->te[tid] = 0;
+  }
+  if (tid == 0) {
+    if (counters & 2) {
+      rc->fb_total += rc->fb_count + rc->repair_count;
+      rc->fb_count = 0;
+      rc->new_count = 0;
+      rc->repair_count = 0;
+    }
+    if (acc) *acc = 0;
+  }
+}
+
 // 16 lanes per seed vertex mark its neighbours.
 // Marks the neighbours of the selected vertices.  Each thread owns one
 // vertex; degrees up to 16 are walked by the thread itself, larger ones by
@@ -375,6 +463,14 @@ __global__ __launch_bounds__(256) void k_graph_flags(int64_t n, const uint8_t* _
     if (next[v]) flags[vgraph ? vgraph[v] : 0] = 1;
 }
 
+void LaunchRoundReset(int64_t n, uint8_t* a, uint8_t* b, uint8_t* c, RoundCounters* rc,
+                      int counters, unsigned long long* acc, hipStream_t s) {
+  const int64_t work = (a || b || c) ? (n + 15) / 16 : 1;
+  hipLaunchKernelGGL(k_round_reset, dim3(Grid(work, 256, 2048)), dim3(256), 0, s, n, a, b, c, rc,
+                     counters, acc);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
                       hipStream_t s) {
   if (n <= 0) return;
@@ -387,8 +483,16 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
                          uint8_t* modified, RoundCounters* rc, hipStream_t s) {
   const unsigned long long* gate = &rc->new_count;
-  DevBuf<uint8_t> z(al, n);
+  if (n == 0) return;
   DevBuf<uint32_t> zrank(al, n), zero_list(al, n), nrank(al, n);
+  if (n <= kSmallAssignMax) {
+    hipLaunchKernelGGL(k_new_assign_small, dim3(1), dim3(kSmallAssignThreads), 0, s, n, st,
+                       vgraph, gbase, new_flag, zero_list.get(), zrank.get(), nrank.get(),
+                       modified, gate);
+    PCC_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  DevBuf<uint8_t> z(al, n);
   const int grid = Grid(n, 256, 4096);
   PCC_HIP_CHECK(hipMemsetAsync(zero_list.get(), 0xFF, sizeof(uint32_t) * (size_t)n, s));
   hipLaunchKernelGGL(k_zero_flags, dim3(grid), dim3(256), 0, s, n, st.sizes, z.get(), gate);

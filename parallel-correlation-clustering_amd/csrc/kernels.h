@@ -128,6 +128,9 @@ struct RoundCounters {
   unsigned long long new_count;
   unsigned long long repair_count;  // hash tables with order-repaired slots
   unsigned long long moved_count;
+  // fb_count + repair_count of earlier rounds, folded in by the round reset
+  // (statistics read once per IterateBestMoves call).
+  unsigned long long fb_total;
 };
 
 struct BinLayout {
@@ -312,6 +315,14 @@ struct FreeIds {
 };
 void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
                     const int64_t* gbase, FreeIds* f, hipStream_t s);
+
+// Clears of one round in a single launch: byte arrays a, b, c (each may be
+// null) of n entries, the counters selected by `counters` (bit 0: cnt / te,
+// bit 1: fb_count / new_count / repair_count, after adding fb_count and
+// repair_count to fb_total) and *acc (may be null).  Byte
+// arrays must start 16-byte aligned (device allocations are).
+void LaunchRoundReset(int64_t n, uint8_t* a, uint8_t* b, uint8_t* c, RoundCounters* rc,
+                      int counters, unsigned long long* acc, hipStream_t s);
 
 // flags[graph of v] = 1 for every v with next[v] != 0.
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
