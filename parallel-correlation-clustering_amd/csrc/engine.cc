@@ -130,15 +130,6 @@ __global__ void k_iota(uint32_t* p, int64_t n) {
     p[i] = (uint32_t)i;
 }
 
-__global__ void k_max_degree(int64_t n, const int64_t* off, unsigned long long* out) {
-  unsigned long long m = 0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x)
-    m = max(m, (unsigned long long)(off[v + 1] - off[v]));
-  for (int s = 32; s > 0; s >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, s));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
-}
-
 // vgraph[v] = g with base[g] <= v < base[g + 1].
 __global__ void k_fill_vgraph(int64_t n, const int64_t* __restrict__ base, int G,
                               uint32_t* __restrict__ vgraph) {
@@ -316,26 +307,16 @@ void Engine::CollectTimers() {
   pending_.clear();
 }
 
-int64_t Engine::MaxDegree(const CsrView& g) {
-  DevBuf<unsigned long long> d(al_, 1);
-  PCC_HIP_CHECK(hipMemsetAsync(d.get(), 0, sizeof(unsigned long long), s_));
-  if (g.n > 0)
-    hipLaunchKernelGGL(k_max_degree, dim3((unsigned)std::min<int64_t>((g.n + 255) / 256, 4096)),
-                       dim3(256), 0, s_, g.n, g.off, d.get());
-  unsigned long long hd = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&hd, d.get(), sizeof(hd), hipMemcpyDeviceToHost, s_));
-  PCC_HIP_CHECK(hipStreamSynchronize(s_));
-  return (int64_t)hd;
-}
-
 void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& lay,
                           LevelCtx* c) {
   c->g = g;
   c->st = st;
-  DevBuf<unsigned long long> caps(al_, 2 * kBins);
-  PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * 2 * kBins, s_));
-  LaunchBinCapacity(g, h_->th, caps.get(), s_);
-  unsigned long long hcaps[2 * kBins];
+  // Bin capacities, the largest degree and the sign check of the node
+  // weights and resolutions, read back together.
+  DevBuf<unsigned long long> caps(al_, kLevelStats);
+  PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * kLevelStats, s_));
+  LaunchBinCapacity(g, h_->th, st.nw, lay.batch ? lay.gres.get() : nullptr, lay.G, caps.get(), s_);
+  unsigned long long hcaps[kLevelStats];
   PCC_HIP_CHECK(hipMemcpyAsync(hcaps, caps.get(), sizeof(hcaps), hipMemcpyDeviceToHost, s_));
   PCC_HIP_CHECK(hipStreamSynchronize(s_));
   int64_t run = 0;
@@ -347,7 +328,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   }
   const bool hashed = c->caps[5] + c->caps[6] + c->caps[7] > 0;
   // Ordered fallback scratch, regions sized for the largest degree.
-  const int64_t maxdeg = hashed ? MaxDegree(g) : 0;
+  const int64_t maxdeg = hashed ? (int64_t)hcaps[2 * kBins] : 0;
   if (maxdeg >= (int64_t)1 << 30)
     throw HipError{hipErrorInvalidValue, "vertex degree >= 2^30 is not supported", __FILE__,
                    __LINE__};
@@ -435,8 +416,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     mp.new_id = (uint32_t)g.n;
   }
   // Candidate pruning needs every resolution and node weight >= 0.
-  bool nonneg = lay.batch ? AllNonNegative(al_, lay.gres.get(), lay.G, s_) : mp.res >= 0.0;
-  nonneg = nonneg && AllNonNegative(al_, st.nw, g.n, s_);
+  const bool nonneg = hcaps[2 * kBins + 1] == 0 && (lay.batch || mp.res >= 0.0);
   mp.prune = nonneg ? 1 : 0;
 }
 

@@ -175,7 +175,6 @@ class Engine {
                     int64_t counts[kBins], int64_t te[kBins]);
   void BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],
                    int64_t te[kBins]);
-  int64_t MaxDegree(const CsrView& g);
 
  private:
   void ResetCounters(RoundCounters* rc, bool all);

@@ -139,11 +139,14 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
 }
 
 __global__ __launch_bounds__(256) void k_bin_capacity(int64_t n, const int64_t* __restrict__ off,
-                                                      BinThresholds th,
+                                                      BinThresholds th, const double* __restrict__ nw,
+                                                      const double* __restrict__ gres, int64_t G,
                                                       unsigned long long* __restrict__ caps) {
-  __shared__ unsigned long long h[2 * kBins];
-  if (threadIdx.x < 2 * kBins) h[threadIdx.x] = 0;
+  __shared__ unsigned long long h[kLevelStats];
+  if (threadIdx.x < kLevelStats) h[threadIdx.x] = 0;
   __syncthreads();
+  unsigned long long dmax = 0;
+  bool neg = false;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = base + threadIdx.x;
@@ -152,6 +155,8 @@ __global__ __launch_bounds__(256) void k_bin_capacity(int64_t n, const int64_t* 
     if (v < n) {
       deg = (unsigned long long)(off[v + 1] - off[v]);
       b = BinOf((int64_t)deg, th.wave_max, th.block_max);
+      dmax = deg > dmax ? deg : dmax;
+      if (nw) neg = neg || !(nw[v] >= 0.0);
     }
     uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
     while (present) {
@@ -165,8 +170,21 @@ __global__ __launch_bounds__(256) void k_bin_capacity(int64_t n, const int64_t* 
       }
     }
   }
+  if (gres && blockIdx.x == 0)
+    for (int64_t i = threadIdx.x; i < G; i += blockDim.x) neg = neg || !(gres[i] >= 0.0);
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    const unsigned long long o = __shfl_xor(dmax, sh);
+    dmax = o > dmax ? o : dmax;
+  }
+  const bool any_neg = __ballot(neg) != 0;
+  if (LaneId() == 0) {
+    if (dmax) atomicMax(&h[2 * kBins], dmax);
+    if (any_neg) h[2 * kBins + 1] = 1;
+  }
   __syncthreads();
   if (threadIdx.x < 2 * kBins && h[threadIdx.x]) atomicAdd(&caps[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 2 * kBins && h[threadIdx.x]) atomicMax(&caps[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 2 * kBins + 1 && h[threadIdx.x]) caps[threadIdx.x] = 1;
 }
 
 // Every bin-7 vertex appended to list (order unspecified).
@@ -1670,11 +1688,12 @@ void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThres
   PCC_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
-                       hipStream_t s) {
-  if (g.n == 0) return;
+void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, const double* nw,
+                       const double* gres, int64_t G, unsigned long long* caps, hipStream_t s) {
+  if (g.n == 0 && !gres) return;
   const int grid = GridFor(g.n, 256 * 16, 1024);
-  hipLaunchKernelGGL(k_bin_capacity, dim3(grid), dim3(256), 0, s, g.n, g.off, th, caps);
+  hipLaunchKernelGGL(k_bin_capacity, dim3(grid), dim3(256), 0, s, g.n, g.off, th, nw, gres, G,
+                     caps);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

@@ -120,16 +120,6 @@ __global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ in
   if (LaneId() == 0) atomicMax(out, m);
 }
 
-// out = 1 if some in[i] < 0 (NaN counts as negative), else left as is.
-__global__ __launch_bounds__(256) void k_any_negative(const double* __restrict__ in, int64_t n,
-                                                      unsigned int* __restrict__ out) {
-  bool neg = false;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    neg = neg || !(in[i] >= 0.0);
-  if (__ballot(neg) && LaneId() == 0) atomicOr(out, 1u);
-}
-
 // Fixed-order sum: per-block sums in a fixed tree, then one block folds them.
 __global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ in, int64_t n,
                                                      double* __restrict__ part) {
@@ -341,19 +331,6 @@ void ReduceMaxU32(DeviceAllocator* al, const uint32_t* in, int64_t n, uint32_t* 
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 2048);
   hipLaunchKernelGGL(k_max_u32, dim3(grid), dim3(256), 0, s, in, n, out);
   PCC_HIP_CHECK(hipGetLastError());
-}
-
-bool AllNonNegative(DeviceAllocator* al, const double* in, int64_t n, hipStream_t s) {
-  if (n <= 0) return true;
-  DevBuf<unsigned int> flag(al, 1);
-  PCC_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(unsigned int), s));
-  const int grid = (int)std::min<int64_t>((n + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_any_negative, dim3(grid), dim3(256), 0, s, in, n, flag.get());
-  PCC_HIP_CHECK(hipGetLastError());
-  unsigned int h = 1;
-  PCC_HIP_CHECK(hipMemcpyAsync(&h, flag.get(), sizeof(h), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
-  return h == 0;
 }
 
 void SumF64(DeviceAllocator* al, const double* in, int64_t n, double* out, hipStream_t s) {

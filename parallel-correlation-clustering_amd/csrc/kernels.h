@@ -232,10 +232,14 @@ void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThres
                        const BinLayout& layout, uint32_t* list, RoundCounters* rc,
                        hipStream_t s);
 
-// Histogram of bins over all vertices: caps[b] vertices and caps[kBins + b]
-// edges of bin b (capacity of each list region).
-void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, unsigned long long* caps,
-                       hipStream_t s);
+// Per-level statistics in one pass (caps zeroed by the caller, kLevelStats
+// entries): caps[b] vertices and caps[kBins + b] edges of bin b (capacity of
+// each list region), caps[2 kBins] the largest degree, caps[2 kBins + 1] = 1
+// if a node weight nw[v] (nw may be null) or a resolution gres[g] (gres may
+// be null, G entries) is negative or NaN.
+constexpr int kLevelStats = 2 * kBins + 2;
+void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, const double* nw,
+                       const double* gres, int64_t G, unsigned long long* caps, hipStream_t s);
 
 // Async sub-rounds: cnt[b * S + s] (zeroed by the caller) counts the listed
 // vertices of bin b in sub-round s; with cur[b * S + s] set to the segment
@@ -471,8 +475,6 @@ void LaunchDegrees(const CsrView& g, double* nw, hipStream_t s);
 
 // Fixed-order double sum.
 void SumF64(DeviceAllocator* al, const double* in, int64_t n, double* out, hipStream_t s);
-// True when every in[i] >= 0 (synchronizes the stream).
-bool AllNonNegative(DeviceAllocator* al, const double* in, int64_t n, hipStream_t s);
 
 // ComputeObjective (parallel-correlation-clusterer-internal.cc:118-149) of
 // each of the G graphs [gbase[g], gbase[g + 1]) with resolution res[g]; ids
