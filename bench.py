@@ -316,11 +316,15 @@ def measured_traffic(args, dom):
 def c4_graphs(pcc_amd, count, seed=4):
     """Batched workload: SBM graphs, n ~ logU[200, 2e5], mean degree ~ U[8, 64],
     power-law (exponent 2) community sizes, p_out / p_in = 0.01, weights
-    U[0, 1] rounded to float32, resolution ~ U[0.001, 0.1]."""
+    U[0, 1] rounded to float32, resolution ~ U[0.001, 0.1].  The parameters
+    are drawn in order from one seeded stream; the graphs are generated by a
+    pool of host threads (the generator releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     rng = np.random.default_rng(seed)
-    graphs, res = [], []
+    params, res = [], []
     for i in range(count):
         n = int(np.exp(rng.uniform(np.log(200), np.log(200000))))
         d = rng.uniform(8, 64)
@@ -337,8 +341,21 @@ def c4_graphs(pcc_amd, count, seed=4):
         # p_in so that the expected degree is d, with p_out = 0.01 p_in.
         denom = float((sizes * (sizes - 1)).sum() + 0.01 * (sizes * (n - sizes)).sum()) / n
         p_in = min(1.0, d / max(denom, 1e-9))
-        graphs.append(pcc_amd.gen_sbm_blocks(sizes, p_in, 0.01 * p_in, 0.0, 1.0, seed=seed * 100003 + i))
+        params.append((sizes, p_in, seed * 100003 + i))
         res.append(rng.uniform(0.001, 0.1))
+    graphs = [None] * count
+    step = max(1, count // 10)
+    workers = min(16, os.cpu_count() or 1)
+
+    def make(i):
+        sizes, p_in, gseed = params[i]
+        graphs[i] = pcc_amd.gen_sbm_blocks(sizes, p_in, 0.01 * p_in, 0.0, 1.0, seed=gseed)
+        return i
+
+    with ThreadPoolExecutor(workers) as ex:
+        for done, _ in enumerate(ex.map(make, range(count)), 1):
+            if count >= 512 and done % step == 0:
+                print(f"# generated {done}/{count} graphs", file=sys.stderr, flush=True)
     return graphs, np.array(res)
 
 

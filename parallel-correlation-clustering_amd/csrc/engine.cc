@@ -6,7 +6,8 @@
 // device-resident CSR and cluster state, for one graph or a batch of graphs.
 // One HIP stream per handle; the host synchronizes once per best-move round to
 // read the next frontier's size (the loop condition of IterateBestMoves) and
-// the per-bin counts that size the next launches.
+// the per-bin counts that size the next launches (ALL_MOVE rounds, whose
+// frontier never changes, read nothing back).
 
 #include "engine.h"
 
@@ -16,47 +17,106 @@
 #include <cstddef>
 #include <cstring>
 #include <new>
+#include <set>
 
 namespace pcc {
 
 // ------------------------------------------------------------- allocator --
 
 namespace {
+// Block sizes: powers of two up to 1 MiB, then four classes per octave, so a
+// large buffer wastes at most a quarter of its size.
 size_t RoundSize(size_t b) {
   size_t r = 256;
-  while (r < b) r <<= 1;
-  return r;
+  while (r < b && r < ((size_t)1 << 20)) r <<= 1;
+  if (r >= b) return r;
+  while ((r << 1) < b) r <<= 1;  // r < b <= 2r
+  const size_t q = r / 4;
+  return r + ((b - r + q - 1) / q) * q;
 }
 }  // namespace
 
-DeviceAllocator::~DeviceAllocator() { ReleaseAll(); }
+namespace {
+// Every allocator of the process, for the out-of-memory release.
+std::mutex g_alloc_mu;
+std::set<DeviceAllocator*>& Allocators() {
+  static std::set<DeviceAllocator*> s;
+  return s;
+}
+}  // namespace
+
+DeviceAllocator::DeviceAllocator() {
+  std::lock_guard<std::mutex> l(g_alloc_mu);
+  Allocators().insert(this);
+}
+
+DeviceAllocator::~DeviceAllocator() {
+  {
+    std::lock_guard<std::mutex> l(g_alloc_mu);
+    Allocators().erase(this);
+  }
+  ReleaseAll();
+}
+
+size_t DeviceAllocator::BytesInUse() const {
+  std::lock_guard<std::mutex> l(mu_);
+  return in_use_;
+}
+
+size_t DeviceAllocator::ReleaseFree() {
+  std::lock_guard<std::mutex> l(mu_);
+  size_t bytes = 0;
+  for (auto& kv : free_)
+    for (void* q : kv.second) {
+      (void)hipFree(q);
+      bytes += kv.first;
+    }
+  free_.clear();
+  return bytes;
+}
 
 void* DeviceAllocator::Alloc(size_t bytes) {
   const size_t sz = RoundSize(bytes);
-  auto it = free_.find(sz);
   void* p = nullptr;
-  if (it != free_.end() && !it->second.empty()) {
-    p = it->second.back();
-    it->second.pop_back();
-  } else {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    // The smallest cached block of at least sz bytes, if it is less than
+    // twice as large.
+    for (auto it = free_.lower_bound(sz); it != free_.end() && it->first < 2 * sz; ++it) {
+      if (it->second.empty()) continue;
+      p = it->second.back();
+      it->second.pop_back();
+      live_[p] = it->first;
+      in_use_ += it->first;
+      return p;
+    }
+  }
+  // No lock is held here: the release below locks other allocators (always
+  // registry first, then one allocator), never the other way round.
+  if (hipMalloc(&p, sz) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)ReleaseFree();
     if (hipMalloc(&p, sz) != hipSuccess) {
       (void)hipGetLastError();
-      // Return cached blocks to the device and try once more.
-      for (auto& kv : free_)
-        for (void* q : kv.second) (void)hipFree(q);
-      free_.clear();
+      {
+        std::lock_guard<std::mutex> l(g_alloc_mu);
+        for (DeviceAllocator* a : Allocators())
+          if (a != this) (void)a->ReleaseFree();
+      }
       if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
         throw AllocError{bytes};
       }
     }
   }
+  std::lock_guard<std::mutex> l(mu_);
   live_[p] = sz;
   in_use_ += sz;
   return p;
 }
 
 void DeviceAllocator::Free(void* p) {
+  std::lock_guard<std::mutex> l(mu_);
   auto it = live_.find(p);
   if (it == live_.end()) return;
   in_use_ -= it->second;
@@ -65,6 +125,7 @@ void DeviceAllocator::Free(void* p) {
 }
 
 void DeviceAllocator::ReleaseAll() {
+  std::lock_guard<std::mutex> l(mu_);
   for (auto& kv : free_)
     for (void* q : kv.second) (void)hipFree(q);
   free_.clear();

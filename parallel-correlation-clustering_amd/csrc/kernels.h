@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,16 +18,25 @@ namespace pcc {
 
 // Caching device allocator: freed blocks go to per-size free lists and are
 // reused without hipFree, so a clustering call never synchronizes on memory
-// management after its first level.
+// management after its first level.  When the device is out of memory, the
+// cached free blocks of this allocator and then of every other allocator in
+// the process (each graph handle has one) are returned to the device before
+// the allocation fails.  Thread-safe.
 class DeviceAllocator {
  public:
+  DeviceAllocator();
   ~DeviceAllocator();
+  DeviceAllocator(const DeviceAllocator&) = delete;
+  DeviceAllocator& operator=(const DeviceAllocator&) = delete;
   void* Alloc(size_t bytes);  // throws std::bad_alloc-like AllocError
   void Free(void* p);
   void ReleaseAll();
-  size_t BytesInUse() const { return in_use_; }
+  // Returns the cached free blocks to the device; the bytes released.
+  size_t ReleaseFree();
+  size_t BytesInUse() const;
 
  private:
+  mutable std::mutex mu_;
   std::map<size_t, std::vector<void*>> free_;
   std::map<void*, size_t> live_;
   size_t in_use_ = 0;
