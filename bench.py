@@ -635,6 +635,15 @@ def cpu_baseline(csr, cfg, threads):
         if el > 10.0 or reps >= 20:
             break
     gteps = reps * csr.m / el / 1e9
+    # The paper's sequential baseline (SEQ-CC): one sequential round over all
+    # vertices from singletons, single-threaded.
+    oracle.set_threads(1)
+    scfg = oracle.config_from(cfg)
+    scfg.sequential = 1
+    s = oracle.S.singletons(csr.n)
+    t1 = time.perf_counter()
+    oracle.round_(g, s, scfg, np.ones(csr.n, np.uint8))
+    seq_s = time.perf_counter() - t1
     return {
         "value": round(gteps, 4),
         "unit": "GTEPS",
@@ -642,6 +651,11 @@ def cpu_baseline(csr, cfg, threads):
         "kind": "port",
         "sample": f"{reps} x best-move round 0 of level 0 (all {csr.n} vertices, {csr.m} edges) "
                   f"of the same graph, {el:.1f} s",
+        "sequential": {
+            "value": round(csr.m / seq_s / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"round 0 of level 0 of the sequential clusterer (SEQ-CC, vertices moving "
+                      f"one after another) on the same graph, {seq_s:.1f} s",
+        },
     }
 
 

@@ -32,6 +32,8 @@ class OConfig(ctypes.Structure):
         ("async_", ctypes.c_int32),
         ("permute", ctypes.c_int32),
         ("permute_seed", ctypes.c_uint64),
+        ("sequential", ctypes.c_int32),
+        ("all_iter", ctypes.c_int32),
     ]
 
 
@@ -159,10 +161,11 @@ class S:
 
 def config(resolution=0.0, edge_weight_offset=0.0, louvain_num_iterations=0,
            louvain_num_inner_iterations=0, clustering_moves_method=1, move_method=0, refine=False,
-           async_=False, permute=False, permute_seed=0x5EED, num_iterations=0) -> OConfig:
+           async_=False, permute=False, permute_seed=0x5EED, num_iterations=0,
+           sequential=False, all_iter=False) -> OConfig:
     return OConfig(resolution, edge_weight_offset, num_iterations, louvain_num_iterations,
                    louvain_num_inner_iterations, clustering_moves_method, move_method, int(refine),
-                   int(async_), int(permute), permute_seed)
+                   int(async_), int(permute), permute_seed, int(sequential), int(all_iter))
 
 
 def config_from(pcfg) -> OConfig:

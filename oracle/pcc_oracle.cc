@@ -11,6 +11,9 @@
 //   clustering/clusterers/parallel-modularity-clusterer.cc         (PMC.cc)
 //   parallel/parallel-graph-utils.cc                                (PGU.cc)
 //   parallel/parallel-sequence-ops.h                                (PSO.h)
+//   clustering/clusterers/correlation-clusterer.cc, correlation-clusterer-
+//   internal.cc, modularity-clusterer.cc: the sequential clusterer (SEQ-CC /
+//   SEQ-MOD, oracle_config.sequential), the paper's own CPU baseline
 //
 // Floating-point policy (pinned identically in the HIP kernels):
 //   * EdgeSum::NetWeight (UTIL.h:36-39) computes weight_ - res * x.  The
@@ -115,9 +118,11 @@ struct ClusterMap {
 
 thread_local ClusterMap t_map;
 
-// ClusteringHelper::EfficientBestMove, PCCI.cc:448-583.
+// ClusteringHelper::EfficientBestMove, PCCI.cc:448-583.  flat: every degree
+// takes the adjacency-order path, as SeqClusteringHelper::EfficientBestMove
+// (correlation-clusterer-internal.cc:116-174) does.
 void BestMove(const oracle_graph& g, const oracle_state& s, double res,
-              double offset, uint32_t v, uint32_t* out_t, double* out_d) {
+              double offset, uint32_t v, uint32_t* out_t, double* out_d, bool flat = false) {
   const int64_t b = g.offsets[v], e = g.offsets[(int64_t)v + 1];
   const int64_t deg = e - b;
   const uint32_t n = (uint32_t)g.n;
@@ -126,7 +131,7 @@ void BestMove(const oracle_graph& g, const oracle_state& s, double res,
   ClusterMap& after = t_map;
   after.Reset((size_t)deg);
 
-  if (deg <= kHubDegree) {
+  if (flat || deg <= kHubDegree) {
     // PCCI.cc:457-511
     double together = 0.0;  // class_1_currently_together (EdgeSum starts at 0)
     double moving_nodes_weight = 0;
@@ -433,6 +438,50 @@ int64_t ExpandFrontier(const oracle_graph& g, const std::vector<uint8_t>& seed,
   return cnt;
 }
 
+// BestMovesForVertexSubset of the sequential clusterer,
+// correlation-clusterer.cc:96-152: the listed vertices move one after another
+// in ascending order with SeqClusteringHelper::AsyncMove
+// (correlation-clusterer-internal.cc:87-114), whose new cluster is the first
+// empty id counted from 0; a mover marks its destination cluster.
+int64_t SeqRound(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+                 uint8_t* frontier) {
+  const int64_t n = g.n;
+  std::vector<uint8_t> moved_vertex((size_t)n, 0), moved_clusters((size_t)n, 0);
+  for (int64_t v = 0; v < n; ++v) {
+    if (!frontier[v]) continue;
+    g_edges_scanned += g.offsets[v + 1] - g.offsets[v];
+    uint32_t t;
+    double d;
+    BestMove(g, s, cfg.resolution, cfg.edge_weight_offset, (uint32_t)v, &t, &d, true);
+    if (d <= 0) continue;
+    const uint32_t cur = s.cluster_ids[v];
+    const double k = s.node_weights[v];
+    s.cluster_sizes[cur] -= 1;
+    s.cluster_weights[cur] -= k;
+    if (t != (uint32_t)n) {
+      s.cluster_sizes[t] += 1;
+      s.cluster_weights[t] += k;
+      s.cluster_ids[v] = t;
+    } else {
+      size_t i = 0;
+      while (s.cluster_sizes[i] != 0) i = (i + 1) % (size_t)n;
+      s.cluster_sizes[i] = 1;
+      s.cluster_weights[i] += k;
+      s.cluster_ids[v] = (uint32_t)i;
+    }
+    moved_vertex[(size_t)v] = 1;
+    moved_clusters[s.cluster_ids[v]] = 1;
+  }
+  if (cfg.move_method == 1) {  // ALL_MOVE, correlation-clusterer.cc:111-118
+    std::memset(frontier, 1, (size_t)n);
+    return n;
+  }
+  std::vector<uint8_t> seed((size_t)n);
+  for (int64_t v = 0; v < n; ++v)  // correlation-clusterer.cc:125-134
+    seed[(size_t)v] = cfg.move_method == 0 ? moved_clusters[s.cluster_ids[v]] : moved_vertex[v];
+  return ExpandFrontier(g, seed, frontier);
+}
+
 struct OwnedCsr {
   int64_t n = 0, m = 0;
   std::vector<int64_t> offsets;
@@ -447,8 +496,14 @@ struct OwnedCsr {
 // CompressGraph, PCCI.cc:709-782, with ComputeInterClusterEdgesSort,
 // PGU.cc:156-231, RetrieveInterClusterEdges, PGU.cc:70-122, GetOffsets,
 // PGU.cc:33-62.
+// seq: SeqCompressGraph (correlation-clusterer-internal.cc:178-279) instead:
+// each run is a float left fold from 0 in CSR order (all_edges[cu][cv] =
+// aggregate(element, w), :195-198) and node weights are summed from 0 in
+// vertex order (:273-275).  The sequential clusterer lists a compressed
+// vertex's neighbours in hash-map order (:223-228), which nothing fixes; this
+// restatement lists them by ascending id.
 void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
-              OwnedCsr* out) {
+              OwnedCsr* out, bool seq = false) {
   uint32_t max_id = 0;
   for (int64_t i = 0; i < g.n; ++i) max_id = std::max(max_id, ids[i]);
   const int64_t n2 = 1 + (int64_t)max_id;  // PCCI.cc:713-717
@@ -482,9 +537,14 @@ void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
       ++j;
     }
     // PGU.cc:206-217: float aggregate through a (double, double) -> double lambda.
-    const float w = PbbsReduce(seg.data(), seg.size(), 0.0f, [](float a, float b) {
-      return (float)((double)a + (double)b);
-    });
+    float w = 0.0f;
+    if (seq) {
+      for (float x : seg) w = (float)((double)w + (double)x);
+    } else {
+      w = PbbsReduce(seg.data(), seg.size(), 0.0f, [](float a, float b) {
+        return (float)((double)a + (double)b);
+      });
+    }
     out->nbrs.push_back(edges[i].cv);
     out->weights.push_back(w);
     counts[edges[i].cu] += 1;
@@ -507,8 +567,14 @@ void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
       grp.push_back(nw[order[j]]);
       ++j;
     }
-    out->node_weights[ids[order[i]]] =
-        PbbsReduce(grp.data(), grp.size(), 0.0, [](double a, double b) { return a + b; });
+    if (seq) {
+      double acc = 0.0;
+      for (double x : grp) acc += x;
+      out->node_weights[ids[order[i]]] = acc;
+    } else {
+      out->node_weights[ids[order[i]]] =
+          PbbsReduce(grp.data(), grp.size(), 0.0, [](double a, double b) { return a + b; });
+    }
     i = j;
   }
 }
@@ -645,6 +711,7 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
                      const oracle_config* cfg, uint8_t* frontier,
                      uint64_t round_seed) {
   // BestMovesForVertexSubset, PCC.cc:115-338.
+  if (cfg->sequential) return SeqRound(*g, *s, *cfg, frontier);
   const int64_t n = g->n;
   for (int64_t v = 0; v < n; ++v)
     if (frontier[v]) g_edges_scanned += g->offsets[v + 1] - g->offsets[v];
@@ -692,7 +759,10 @@ int oracle_iterate_best_moves(const oracle_graph* g, oracle_state* s,
   std::vector<uint8_t> frontier((size_t)g->n, 1);
   bool moved = false, local_moved = true;
   int64_t it = 0;
-  for (it = 0; it < inner && local_moved; ++it) {
+  // all_iter (sequential clusterer): until nothing moves,
+  // correlation-clusterer.cc:173-183.
+  const bool until_done = cfg->sequential && cfg->all_iter;
+  for (it = 0; (until_done || it < inner) && local_moved; ++it) {
     const int64_t sz = oracle_round(g, s, cfg, frontier.data(), seed * 1315423911ull + (uint64_t)it);
     local_moved = sz != 0;
     moved |= local_moved;
@@ -822,10 +892,11 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
   } else {
     return 12;  // PCC.cc:501-505
   }
+  if (cfg.sequential && cfg.all_iter) num_iterations = INT64_MAX;  // correlation-clusterer.cc:266-267
   int64_t rounds = 0;
   g_edges_scanned = 0;
   std::vector<Level> levels;
-  levels.reserve((size_t)num_iterations + 1);
+  levels.reserve((size_t)std::min<int64_t>(num_iterations, 64) + 1);
   levels.emplace_back();
   {
     Level& L = levels.back();
@@ -869,7 +940,7 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
     levels.emplace_back();                  // CompressGraph, PCC.cc:569-571
     Level& cur = levels[(size_t)iter];
     Level& nxt = levels.back();
-    Compress(cg, cur.ids.data(), cur.nw.data(), &nxt.csr);
+    Compress(cg, cur.ids.data(), cur.nw.data(), &nxt.csr, cfg.sequential != 0);
     const int64_t n2 = nxt.csr.n;
     nxt.nw = nxt.csr.node_weights;  // singletons with aggregated weights, PCC.cc:583-585
     nxt.ids.resize((size_t)n2);

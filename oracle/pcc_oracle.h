@@ -37,6 +37,11 @@ typedef struct oracle_config {
   int32_t async;
   int32_t permute;
   uint64_t permute_seed;              /* processing-order seed when permute */
+  /* 1: the sequential CorrelationClusterer / ModularityClusterer instead
+   * (clustering/clusterers/correlation-clusterer.cc, -internal.cc): vertices
+   * move one after another in ascending order, no whole-cluster phase. */
+  int32_t sequential;
+  int32_t all_iter;                   /* field 17 (sequential clusterer only) */
 } oracle_config;
 
 /* Read-only CSR view: offsets[n+1], nbrs[m], weights[m] (float, as the
