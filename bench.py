@@ -45,6 +45,8 @@ def apply_tuning(g, args, pcc_amd):
         g.set_option(pcc_amd.OPT_ASYNC_SUBROUNDS, args.async_subrounds)
     if args.pull_fraction is not None:
         g.set_option(pcc_amd.OPT_PULL_FRACTION, args.pull_fraction)
+    if args.hash_waves is not None:
+        g.set_option(pcc_amd.OPT_HASH_WAVES, args.hash_waves)
 
 
 def set_tuned(g, t, pcc_amd):
@@ -87,6 +89,8 @@ def main() -> int:
                     help="batched: keep the default degree-bin thresholds instead of pcc_autotune")
     ap.add_argument("--autotune", action="store_true",
                     help="rmat22: run pcc_autotune on the workload first and report its choice")
+    ap.add_argument("--hash-waves", type=int, default=None,
+                    help="hash-bin workgroups per launch, in chip-fulls (0: one per vertex)")
     ap.add_argument("--pull-fraction", type=int, default=None,
                     help="frontier pull threshold in 1/1024 of m + n (1024: push only)")
     args = ap.parse_args()

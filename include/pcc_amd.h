@@ -294,6 +294,9 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_PULL_FRACTION 8  /* next-frontier expansion pulls when the seeds' (degree + 1)
                                     sum exceeds value/1024 of (m + n); symmetric graphs only;
                                     1024 disables (default 512) */
+#define PCC_OPT_HASH_WAVES 10      /* workgroups per hash-bin launch as a multiple of those
+                                       resident at once (default 1; 0: one per vertex up to
+                                       8192) */
 #define PCC_OPT_CLUSTER_SUBROUNDS 9 /* async whole-cluster moves: sub-rounds per round
                                        (default 16, range [1, 1024]) */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);

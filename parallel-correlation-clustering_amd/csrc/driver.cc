@@ -403,6 +403,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
         return SetError(PCC_INVALID_ARGUMENT, "hub partition size must lie in [16, 4096]");
       g->hub_part_edges = value;
       return PCC_OK;
+    case PCC_OPT_HASH_WAVES:
+      if (value < 0 || value > 64)
+        return SetError(PCC_INVALID_ARGUMENT, "hash-bin waves must lie in [0, 64]");
+      g->th.hash_waves = value;
+      return PCC_OK;
     case PCC_OPT_CLUSTER_SUBROUNDS:
       if (value < 1 || value > 1024)
         return SetError(PCC_INVALID_ARGUMENT, "cluster sub-rounds must lie in [1, 1024]");

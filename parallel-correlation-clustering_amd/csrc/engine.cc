@@ -387,7 +387,8 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->caps_te[b] = (int64_t)hcaps[kBins + b];
     run += (int64_t)hcaps[b];
   }
-  const bool hashed = c->caps[5] + c->caps[6] + c->caps[7] > 0;
+  bool hashed = false;
+  for (int b = kWaveBin0; b < kBins; ++b) hashed = hashed || c->caps[b] > 0;
   // Ordered fallback scratch, regions sized for the largest degree.
   const int64_t maxdeg = hashed ? (int64_t)hcaps[2 * kBins] : 0;
   if (maxdeg >= (int64_t)1 << 30)
@@ -492,12 +493,21 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
                          const uint32_t* const lists[kBins], const int64_t counts[kBins],
                          const int64_t te[kBins], const MoveOut& out, uint64_t salt,
                          bool scan_hubs) {
-  for (int b = 0; b + 1 < kBins; ++b) {
-    if (counts[b] == 0) continue;
-    Timer tb = StartTimer();
-    LaunchBestMoveBin(b, counts[b], async, h_->th, c.g, c.st, mp, lists[b], out, ws.fb_list.get(),
-                      ws.rc.get(), salt, s_);
-    StopTimer(tb, b);
+  // One timer per reported group (the three ranges of a hash group together).
+  for (int b0 = 0; b0 < kHubBin;) {
+    const int grp = StatGroupOf(b0);
+    int b1 = b0;
+    bool any = false;
+    for (; b1 < kHubBin && StatGroupOf(b1) == grp; ++b1) any = any || counts[b1] > 0;
+    if (any) {
+      Timer tb = StartTimer();
+      for (int b = b0; b < b1; ++b)
+        if (counts[b] > 0)
+          LaunchBestMoveBin(b, counts[b], async, h_->th, c.g, c.st, mp, lists[b], out,
+                            ws.fb_list.get(), ws.rc.get(), salt, s_);
+      StopTimer(tb, grp);
+    }
+    b0 = b1;
   }
   const int64_t c7 = counts[kBins - 1];
   if (c7 > 0) {
@@ -507,7 +517,7 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
       LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
     LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_);
-    StopTimer(tb, kBins - 1);
+    StopTimer(tb, StatGroupOf(kHubBin));
   }
 }
 
@@ -522,8 +532,8 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   for (int b = 0; b < kBins; ++b) {
     F += counts[b];
     TE += te[b];
-    bin_edges_[b] += te[b];
-    bin_vertices_[b] += counts[b];
+    bin_edges_[StatGroupOf(b)] += te[b];
+    bin_vertices_[StatGroupOf(b)] += counts[b];
   }
   edges_scanned_ += TE;
   frontier_vertices_ += F;
@@ -626,7 +636,9 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
       LaunchHubScan(g, ws.list.get() + c.layout.base[kBins - 1], counts[kBins - 1],
                     c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
   }
-  if (!async && counts[5] + counts[6] + counts[7] > 0) {
+  int64_t hashed_count = 0;
+  for (int b = kWaveBin0; b < kBins; ++b) hashed_count += counts[b];
+  if (!async && hashed_count > 0) {
     Timer tf = StartTimer();
     LaunchBestMoveOrdered(g, st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s_);
     StopTimer(tf, 8);
@@ -973,7 +985,7 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     for (int b = 0; b <= 8; ++b) stats_->bestmove_launches += kind_launches_[b];
     stats_->apply_seconds = kind_seconds_[9];
     stats_->contract_seconds = kind_seconds_[10];
-    for (int b = 0; b < kBins; ++b) {
+    for (int b = 0; b < kStatGroups; ++b) {
       stats_->bin_seconds[b] = kind_seconds_[b];
       stats_->bin_edges[b] = bin_edges_[b];
       stats_->bin_vertices[b] = bin_vertices_[b];

@@ -192,8 +192,8 @@ class Engine {
   std::vector<std::pair<Timer, int>> pending_;
   double kind_seconds_[16] = {0};
   int64_t kind_launches_[16] = {0};
-  int64_t bin_edges_[kBins] = {0};
-  int64_t bin_vertices_[kBins] = {0};
+  int64_t bin_edges_[kStatGroups] = {0};
+  int64_t bin_vertices_[kStatGroups] = {0};
   int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
   int64_t cluster_moves_ = 0;
   double bestmove_bytes_ = 0;

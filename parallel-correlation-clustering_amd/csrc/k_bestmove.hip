@@ -43,15 +43,40 @@ namespace pcc {
 
 namespace {
 
+// Upper degrees of the hash bins' ranges (a range can be empty when the
+// thresholds are set below its nominal bound).
+struct HashBounds {
+  int64_t b[6];  // bins 5..10: deg <= b[bin - 5]
+};
+__host__ __device__ __forceinline__ HashBounds HashBoundsOf(int64_t wave_max, int64_t block_max) {
+  HashBounds h;
+  const int64_t w1 = wave_max < 127 ? wave_max : 127;
+  int64_t w2 = wave_max < 255 ? wave_max : 255;
+  w2 = w2 > w1 ? w2 : w1;
+  int64_t k1 = block_max < 1023 ? block_max : 1023;
+  k1 = k1 > wave_max ? k1 : wave_max;
+  int64_t k2 = block_max < 2047 ? block_max : 2047;
+  k2 = k2 > k1 ? k2 : k1;
+  h.b[0] = w1;
+  h.b[1] = w2;
+  h.b[2] = wave_max;
+  h.b[3] = k1;
+  h.b[4] = k2;
+  h.b[5] = block_max;
+  return h;
+}
+
 __device__ __forceinline__ int BinOf(int64_t deg, int64_t wave_max, int64_t block_max) {
   if (deg <= 4) return 0;
   if (deg <= 8) return 1;
   if (deg <= 16) return 2;
   if (deg <= 32) return 3;
   if (deg <= 64) return 4;
-  if (deg <= wave_max) return 5;
-  if (deg <= block_max) return 6;
-  return 7;
+  if (deg > block_max) return kHubBin;
+  const HashBounds h = HashBoundsOf(wave_max, block_max);
+  int b = kWaveBin0;
+  while (deg > h.b[b - kWaveBin0]) ++b;
+  return b;
 }
 
 __device__ __forceinline__ uint64_t LaneMaskLt() {
@@ -1775,40 +1800,46 @@ static void LaunchSmall(int64_t count, const CsrView& g, const StateView& st, co
                      salt);
 }
 
-// One launch per degree range (dlo, dhi] of a hashed bin, each with the
-// table size that range needs at load factor <= 1/2: smaller tables leave
-// LDS for more workgroups per CU.  The workgroups stride over the bin's list
-// and skip other ranges' vertices.
+// One hash bin: the kernel whose table fits the bin's degree range (dlo, dhi]
+// at load factor <= 1/2 (smaller tables leave LDS for more workgroups per CU);
+// one workgroup per listed vertex up to 8192, striding beyond.
 template <int THREADS, int TABLE, bool A>
-static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, const CsrView& g,
-                       const StateView& st, const MoveParams& mp, const uint32_t* list,
-                       const MoveOut& out, uint32_t* fb_list, RoundCounters* rc, uint64_t salt,
-                       bool whole_bin, hipStream_t s) {
-  if (dhi <= dlo) return;
-  const int grid = GridFor(count, 1, whole_bin ? (1 << 20) : 8192);
+static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_waves,
+                       const CsrView& g, const StateView& st, const MoveParams& mp,
+                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
+                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
+  if (dhi <= dlo || count <= 0) return;
+  int max_grid = 8192;
+  if (hash_waves > 0) {
+    // Workgroups resident at once: 256 CUs, each limited by its 160 KB of
+    // LDS (the table: 16 bytes a slot) and its 32 wave slots.
+    constexpr int64_t kLdsPerGroup = (int64_t)TABLE * 16 + 4096;
+    int64_t per_cu = std::min<int64_t>((160 * 1024) / kLdsPerGroup, 32 / (THREADS / 64));
+    if (per_cu < 1) per_cu = 1;
+    max_grid = (int)std::min<int64_t>(8192, 256 * per_cu * hash_waves);
+  }
+  const int grid = GridFor(count, 1, max_grid);
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
                      list, count, dlo, dhi, out, fb_list, rc, salt);
 }
 
 template <bool A>
-static void LaunchHashBins(int bin, int64_t count, const BinThresholds& th, const CsrView& g,
-                           const StateView& st, const MoveParams& mp, const uint32_t* list,
-                           const MoveOut& out, uint32_t* fb_list, RoundCounters* rc,
-                           uint64_t salt, hipStream_t s) {
-  if (bin == 5) {
-    const int64_t lo = 64, hi = th.wave_max;
-    const int64_t c1 = std::min<int64_t>(127, hi);
-    const int64_t c2 = std::max(c1, std::min<int64_t>(255, hi));
-    LaunchHash<64, 256, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<64, 512, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<128, 1024, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
-  } else {
-    const int64_t lo = th.wave_max, hi = th.block_max;
-    const int64_t c1 = std::max(lo, std::min<int64_t>(1023, hi));
-    const int64_t c2 = std::max(c1, std::min<int64_t>(2047, hi));
-    LaunchHash<256, 2048, A>(count, lo, c1, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<512, 4096, A>(count, c1, c2, g, st, mp, list, out, fb_list, rc, salt, false, s);
-    LaunchHash<1024, 8192, A>(count, c2, hi, g, st, mp, list, out, fb_list, rc, salt, false, s);
+static void LaunchHashBin(int bin, int64_t count, const BinThresholds& th, const CsrView& g,
+                          const StateView& st, const MoveParams& mp, const uint32_t* list,
+                          const MoveOut& out, uint32_t* fb_list, RoundCounters* rc,
+                          uint64_t salt, hipStream_t s) {
+  const HashBounds h = HashBoundsOf(th.wave_max, th.block_max);
+  const int k = bin - kWaveBin0;
+  const int64_t lo = k == 0 ? 64 : h.b[k - 1], hi = h.b[k];
+  const int64_t hw = th.hash_waves;
+  switch (bin) {
+    case 5: LaunchHash<64, 256, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    case 6: LaunchHash<64, 512, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    case 7: LaunchHash<128, 1024, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    case 8: LaunchHash<256, 2048, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    case 9: LaunchHash<512, 4096, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    default:
+      LaunchHash<1024, 8192, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s);
   }
 }
 
@@ -1840,11 +1871,15 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
       break;
     case 5:
     case 6:
-      async ? LaunchHashBins<true>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s)
-            : LaunchHashBins<false>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s);
+    case 7:
+    case 8:
+    case 9:
+    case 10:
+      async ? LaunchHashBin<true>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s)
+            : LaunchHashBin<false>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s);
       break;
     default:
-      throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: bin 7 goes through LaunchBestMoveHub",
+      throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: the hub bin goes through LaunchBestMoveHub",
                      __FILE__, __LINE__};
   }
   PCC_HIP_CHECK(hipGetLastError());

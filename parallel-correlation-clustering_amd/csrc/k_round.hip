@@ -566,7 +566,7 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
   Bins lo = all, mid = all;
   int64_t nlo = 0, nmid = 0;
   for (int b = 0; b < kBins; ++b) {
-    const bool is_lo = b <= 4, is_mid = b == 5 || b == 6;
+    const bool is_lo = b < kWaveBin0, is_mid = b >= kWaveBin0 && b < kHubBin;
     lo.count[b] = is_lo ? all.count[b] : 0;
     mid.count[b] = is_mid ? all.count[b] : 0;
     nlo += lo.count[b];

@@ -118,16 +118,33 @@ struct StateView {
   double* K = nullptr;
 };
 
-// Degree bins of the best-move kernels.
-constexpr int kBins = 8;
-// 0..4: deg <= 4, 8, 16, 32, 64 -> 4..64 lanes per vertex, ordered shuffles
-// 5: deg <= wave_max   -> one wave, LDS hash
-// 6: deg <= block_max  -> one workgroup, LDS hash
-// 7: larger            -> several workgroups per vertex, each an LDS hash over
-//                         one hash partition of the neighbour clusters
+// Degree bins of the best-move kernels: one frontier list and one kernel
+// shape per bin.
+//   0..4   deg <= 4, 8, 16, 32, 64 -> 4..64 lanes per vertex, ordered shuffles
+//   5..7   deg <= 127, 255, wave_max         -> one wave, LDS hash
+//   8..10  deg <= 1023, 2047, block_max      -> one workgroup, LDS hash
+//   11     larger -> several workgroups per vertex, each an LDS hash over one
+//          hash partition of the neighbour clusters
+// Each hash table size gets its own list, so a launch only visits vertices of
+// its range.  Statistics and timers report 8 groups: bins 0..4, the wave
+// hash bins, the workgroup hash bins, the partitioned bin.
+constexpr int kBins = 12;
+constexpr int kWaveBin0 = 5;   // first one-wave hash bin
+constexpr int kBlockBin0 = 8;  // first one-workgroup hash bin
+constexpr int kHubBin = 11;
+constexpr int kStatGroups = 8;
+inline int StatGroupOf(int b) {
+  return b < kWaveBin0 ? b : (b < kBlockBin0 ? 5 : (b < kHubBin ? 6 : 7));
+}
 struct BinThresholds {
   int64_t wave_max = 512;
   int64_t block_max = 4096;
+  // Workgroups per hash-bin launch, as a multiple of the workgroups that fit
+  // on the chip at once (0: one per listed vertex, up to 8192).  Fewer
+  // workgroups each run the software pipeline over several vertices (one
+  // chip-full measured fastest on the RMAT scale-22 workload: 37.1 vs 34.3
+  // best-move GTEPS with one workgroup per vertex).
+  int64_t hash_waves = 1;
 };
 
 // Device-resident per-round counters.

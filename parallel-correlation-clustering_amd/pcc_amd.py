@@ -51,6 +51,7 @@ OPT_ASYNC_SINGLETON_RULE = 6
 OPT_HUB_PART_EDGES = 7
 OPT_PULL_FRACTION = 8
 OPT_CLUSTER_SUBROUNDS = 9
+OPT_HASH_WAVES = 10
 
 
 class StatusError(RuntimeError):
