@@ -91,27 +91,43 @@ __device__ __forceinline__ unsigned long long WaveSumU64(unsigned long long x) {
 
 // ------------------------------------------------------------- binning --
 
-// Each workgroup bins a chunk of kBinChunk vertices: counts per bin in LDS,
-// one global reservation per bin and chunk, then the list writes.
+// Each workgroup bins a chunk of kBinChunk vertices: every thread counts its
+// kBinItems vertices per bin in registers (counts packed four to a 64-bit
+// word, 16 bits each: a chunk holds at most 4096 vertices), one block scan
+// of the packed words ranks the threads within each bin, one global
+// reservation per bin and chunk places the chunk, and the threads write
+// their vertices (in thread order within a bin; nothing depends on the order
+// of a bin's list).
 constexpr int kBinItems = 16;
 constexpr int kBinThreads = 256;
 constexpr int64_t kBinChunk = (int64_t)kBinItems * kBinThreads;
+static_assert(kBins <= 12 && kBinChunk < 65536, "three 64-bit words of 16-bit counts");
+
+__device__ __forceinline__ uint64_t PackedField(uint64_t w0, uint64_t w1, uint64_t w2, int b) {
+  const uint64_t w = b < 4 ? w0 : (b < 8 ? w1 : w2);
+  return (w >> (16 * (b & 3))) & 0xFFFFull;
+}
+__device__ __forceinline__ void PackedAdd(uint64_t& w0, uint64_t& w1, uint64_t& w2, int b) {
+  const uint64_t inc = 1ull << (16 * (b & 3));
+  w0 += b < 4 ? inc : 0ull;
+  w1 += (b >= 4 && b < 8) ? inc : 0ull;
+  w2 += b >= 8 ? inc : 0ull;
+}
 
 __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
     int64_t n, const int64_t* __restrict__ off, const uint8_t* __restrict__ frontier,
     BinThresholds th, BinLayout layout, uint32_t* __restrict__ list,
     RoundCounters* __restrict__ rc) {
-  __shared__ unsigned long long s_cnt[kBins], s_te[kBins], s_base[kBins];
-  __shared__ uint32_t s_pos[kBins];
+  __shared__ unsigned long long s_scan[kBinThreads / kWave];
+  __shared__ unsigned long long s_te[kBins], s_base[kBins];
   for (int64_t c0 = (int64_t)blockIdx.x * kBinChunk; c0 < n;
        c0 += (int64_t)gridDim.x * kBinChunk) {
-    if (threadIdx.x < kBins) {
-      s_cnt[threadIdx.x] = 0;
-      s_te[threadIdx.x] = 0;
-      s_pos[threadIdx.x] = 0;
-    }
-    __syncthreads();
+    if (threadIdx.x < kBins) s_te[threadIdx.x] = 0;
     int8_t bin[kBinItems];
+    uint64_t p0 = 0, p1 = 0, p2 = 0;
+    unsigned long long te[kBins];
+#pragma unroll
+    for (int j = 0; j < kBins; ++j) te[j] = 0;
 #pragma unroll
     for (int k = 0; k < kBinItems; ++k) {
       const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
@@ -122,42 +138,37 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
         b = BinOf((int64_t)deg, th.wave_max, th.block_max);
       }
       bin[k] = (int8_t)b;
-      uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
-      while (present) {
-        const int bb = __ffs(present) - 1;
-        present &= present - 1;
-        const uint64_t mask = __ballot(b == bb);
-        const unsigned long long te = WaveSumU64(b == bb ? deg : 0ull);
-        if (LaneId() == 0) {
-          atomicAdd(&s_cnt[bb], (unsigned long long)__popcll(mask));
-          atomicAdd(&s_te[bb], te);
-        }
-      }
+      if (b >= 0) PackedAdd(p0, p1, p2, b);
+#pragma unroll
+      for (int j = 0; j < kBins; ++j) te[j] += b == j ? deg : 0ull;
+    }
+    // Thread ranks within each bin (the scans synchronize first, so s_te is
+    // clear before the sums below).
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    uint64_t r0 = BlockExclusiveScan<unsigned long long>(p0, s_scan, &t0);
+    uint64_t r1 = BlockExclusiveScan<unsigned long long>(p1, s_scan, &t1);
+    uint64_t r2 = BlockExclusiveScan<unsigned long long>(p2, s_scan, &t2);
+#pragma unroll
+    for (int j = 0; j < kBins; ++j) {
+      const unsigned long long x = WaveSumU64(te[j]);
+      if (LaneId() == 0 && x) atomicAdd(&s_te[j], x);
     }
     __syncthreads();
-    if (threadIdx.x < kBins && s_cnt[threadIdx.x] > 0) {
-      s_base[threadIdx.x] = atomicAdd(&rc->cnt[threadIdx.x], s_cnt[threadIdx.x]);
-      atomicAdd(&rc->te[threadIdx.x], s_te[threadIdx.x]);
+    if (threadIdx.x < kBins) {
+      const int j = threadIdx.x;
+      const uint64_t cnt = PackedField(t0, t1, t2, j);
+      s_base[j] = cnt ? atomicAdd(&rc->cnt[j], (unsigned long long)cnt) : 0ull;
+      if (s_te[j]) atomicAdd(&rc->te[j], s_te[j]);
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kBinItems; ++k) {
-      const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
       const int b = bin[k];
-      uint32_t present = WaveOr(b >= 0 ? (1u << b) : 0u);
-      while (present) {
-        const int bb = __ffs(present) - 1;
-        present &= present - 1;
-        const uint64_t mask = __ballot(b == bb);
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t pos = 0;
-        if (LaneId() == leader) pos = atomicAdd(&s_pos[bb], (uint32_t)__popcll(mask));
-        pos = __shfl(pos, leader);
-        if (b == bb) {
-          const uint32_t r = (uint32_t)__popcll(mask & LaneMaskLt());
-          list[layout.base[bb] + (int64_t)s_base[bb] + pos + r] = (uint32_t)v;
-        }
-      }
+      if (b < 0) continue;
+      const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
+      const uint64_t pos = PackedField(r0, r1, r2, b);
+      list[layout.base[b] + (int64_t)s_base[b] + (int64_t)pos] = (uint32_t)v;
+      PackedAdd(r0, r1, r2, b);
     }
     __syncthreads();
   }

@@ -6,7 +6,7 @@ validated by tests/test_oracle.py.  Each .npz holds the input CSR (as the
 seeded generators produce it), the config, and the oracle's final cluster ids,
 objective and modularity.
 
-Run: python tests/golden/make_golden.py
+Run: python tests/golden/make_golden.py [case ...]   (all cases by default)
 """
 
 import json
@@ -35,6 +35,11 @@ CASES = {
     # RMAT, unit weights, modularity, sync NBHR_CLUSTER_MOVE, refine.
     "rmat11_modularity": (dict(kind="rmat", scale=11, ef=8, lo=1.0, hi=1.0, seed=3),
                           "correlation_clusterer_config { resolution: 1.0 refine: true }", 1),
+    # RMAT, signed FP weights, whole-cluster moves (DEFAULT_CLUSTER_MOVES),
+    # sync NBHR_MOVE, res 0.05, 12 rounds.
+    "rmat10_cluster_moves": (dict(kind="rmat", scale=10, ef=16, lo=-1.0, hi=1.0, seed=4),
+                             "correlation_clusterer_config { resolution: 0.05 move_method: NBHR_MOVE "
+                             "clustering_moves_method: DEFAULT_CLUSTER_MOVES num_iterations: 12 }", 0),
 }
 
 
@@ -45,8 +50,10 @@ def make_graph(spec):
                             seed=spec["seed"])
 
 
-def main():
+def main(names):
     for name, (spec, text, modularity) in CASES.items():
+        if names and name not in names:
+            continue
         csr = make_graph(spec)
         cfg = pcc_amd.parse_config(text)
         r = oracle.cluster(oracle.G.from_csr(csr), oracle.config_from(cfg), bool(modularity))
@@ -58,4 +65,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -28,7 +28,7 @@ def _graph(pcc, spec):
 
 
 def test_fixtures_present():
-    assert len(FILES) >= 3
+    assert len(FILES) >= 4
 
 
 @pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
