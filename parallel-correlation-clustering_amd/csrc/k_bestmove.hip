@@ -1506,13 +1506,27 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const double res = ResOf(mp, v);
   double mw = 0;
   mw += st.nw[v];
-  Top2 top;
-  for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads) {
-    const uint32_t s = occ[k];
-    const uint32_t c = keys[s];
-    if (mp.prune && vals[s] <= 0.0) continue;  // cannot win (see MoveParams::prune)
-    top = MergeTop2(top, Top2{NetWeight(vals[s], mw * st.K[c], res), c, -INFINITY});
+  // A thread's (at most KS) candidate clusters have their weights loaded
+  // together, not one dependent load per loop trip.
+  constexpr int KS = (int)((kList + kHubThreads - 1) / kHubThreads);
+  uint32_t cc[KS];
+  double sv[KS], kc[KS];
+#pragma unroll
+  for (int e = 0; e < KS; ++e) {
+    const uint32_t k = threadIdx.x + (uint32_t)e * kHubThreads;
+    cc[e] = kNoneId;
+    sv[e] = 0.0;
+    if (k < n_occ) {
+      const uint32_t s = occ[k];
+      sv[e] = vals[s];
+      if (!(mp.prune && sv[e] <= 0.0)) cc[e] = keys[s];  // cannot win (see MoveParams::prune)
+    }
+    kc[e] = cc[e] != kNoneId ? st.K[cc[e]] : 0.0;
   }
+  Top2 top;
+#pragma unroll
+  for (int e = 0; e < KS; ++e)
+    if (cc[e] != kNoneId) top = MergeTop2(top, Top2{NetWeight(sv[e], mw * kc[e], res), cc[e], -INFINITY});
   top = BlockTop2<kHubThreads>(top, reinterpret_cast<Top2*>(&stg));
   if (threadIdx.x == 0) {
     hb.px1[t] = top.x1;
