@@ -34,6 +34,7 @@ class OConfig(ctypes.Structure):
         ("permute_seed", ctypes.c_uint64),
         ("sequential", ctypes.c_int32),
         ("all_iter", ctypes.c_int32),
+        ("subclustering_method", ctypes.c_int32),
     ]
 
 
@@ -104,6 +105,10 @@ def lib() -> ctypes.CDLL:
         l.oracle_cluster.argtypes = [P, P, ctypes.c_int, P, P, P, P]
         l.oracle_cluster.restype = ctypes.c_int
         l.oracle_flatten.argtypes = [ctypes.c_int64, P, P, P]
+        l.oracle_subclusters.argtypes = [P, P, ctypes.c_int, P, P]
+        l.oracle_subclusters.restype = ctypes.c_int64
+        l.oracle_triangle_score.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
+        l.oracle_triangle_score.restype = ctypes.c_double
         l.oracle_rank_relabel.argtypes = [ctypes.c_int64, P, P]
         _lib = l
     return _lib
@@ -162,10 +167,11 @@ class S:
 def config(resolution=0.0, edge_weight_offset=0.0, louvain_num_iterations=0,
            louvain_num_inner_iterations=0, clustering_moves_method=1, move_method=0, refine=False,
            async_=False, permute=False, permute_seed=0x5EED, num_iterations=0,
-           sequential=False, all_iter=False) -> OConfig:
+           sequential=False, all_iter=False, subclustering_method=0) -> OConfig:
     return OConfig(resolution, edge_weight_offset, num_iterations, louvain_num_iterations,
                    louvain_num_inner_iterations, clustering_moves_method, move_method, int(refine),
-                   int(async_), int(permute), permute_seed, int(sequential), int(all_iter))
+                   int(async_), int(permute), permute_seed, int(sequential), int(all_iter),
+                   subclustering_method)
 
 
 def config_from(pcfg) -> OConfig:
@@ -173,7 +179,8 @@ def config_from(pcfg) -> OConfig:
     return config(pcfg.resolution, pcfg.edge_weight_offset, pcfg.louvain_num_iterations,
                   pcfg.louvain_num_inner_iterations, pcfg.clustering_moves_method,
                   pcfg.move_method, bool(pcfg.refine), bool(pcfg.async_), bool(pcfg.permute),
-                  pcfg.permute_seed, pcfg.num_iterations)
+                  pcfg.permute_seed, pcfg.num_iterations,
+                  subclustering_method=pcfg.subclustering_method)
 
 
 def best_move(g: G, s: S, cfg: OConfig, v: int) -> Tuple[int, float]:
@@ -218,6 +225,14 @@ def round_(g: G, s: S, cfg: OConfig, frontier, seed: int = 0):
     size = lib().oracle_round(ctypes.byref(g.view), ctypes.byref(s.view), ctypes.byref(cfg),
                               _p(fr), seed)
     return fr, int(size)
+
+
+def iterate_best_moves(g: G, s: S, cfg: OConfig, num_inner: int, seed: int = 0) -> int:
+    """IterateBestMoves on one level; 1 if anything moved."""
+    rounds = ctypes.c_int64(0)
+    return int(lib().oracle_iterate_best_moves(ctypes.byref(g.view), ctypes.byref(s.view),
+                                               ctypes.byref(cfg), num_inner, seed,
+                                               ctypes.byref(rounds)))
 
 
 def compress(g: G, ids, nw):
@@ -271,6 +286,21 @@ def cluster(g: G, cfg: OConfig, modularity_objective: bool = False, node_weights
         raise RuntimeError(f"oracle_cluster returned {rc}")
     return Result(out[: g.n], r.objective, r.modularity, r.levels, r.rounds, r.edges_scanned,
                   r.num_clusters, r.seconds)
+
+
+def subclusters(g: G, ids, method: int) -> Tuple[np.ndarray, np.ndarray, int]:
+    """CompressSubclusters' numbering: (sub ids, parent rank per id, total)."""
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    sub = np.zeros(g.n, np.uint32)
+    parent = np.zeros(g.n, np.uint32)
+    total = lib().oracle_subclusters(ctypes.byref(g.view), _p(ids), method, _p(sub), _p(parent))
+    if total < 0:
+        raise ValueError(f"subclustering method {method} is not restated")
+    return sub, parent, int(total)
+
+
+def triangle_score(g: G, u: int, v: int) -> float:
+    return float(lib().oracle_triangle_score(ctypes.byref(g.view), u, v))
 
 
 def flatten(ids, compressed) -> np.ndarray:

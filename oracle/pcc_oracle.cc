@@ -600,8 +600,96 @@ void ResetFromIds(int64_t n, oracle_state& s, const std::vector<uint32_t>& flat)
   }
 }
 
+// ---------------------------------------------------------- subclustering --
+
+// CountTriangles, PCCI.cc:786-799, for one edge (u, v): GBBS's vertex
+// intersect counts the common entries of the two adjacency lists by a merge
+// (it assumes ascending lists), divided by the sum of the two degrees.
+double TriangleScore(const oracle_graph& g, uint32_t u, uint32_t v) {
+  int64_t i = g.offsets[u], j = g.offsets[v];
+  const int64_t ie = g.offsets[(int64_t)u + 1], je = g.offsets[(int64_t)v + 1];
+  size_t common = 0;
+  while (i < ie && j < je) {
+    if (g.nbrs[i] == g.nbrs[j]) {
+      ++i;
+      ++j;
+      ++common;
+    } else if (g.nbrs[i] < g.nbrs[j]) {
+      ++i;
+    } else {
+      ++j;
+    }
+  }
+  return (double)common / (double)((ie - g.offsets[u]) + (je - g.offsets[v]));
+}
+
+// The numbering of CompressSubclusters (PCC.cc:436-474) with
+// ComputeSubcluster (PCCI.cc:1101-1120).  Clusters are taken in ascending id
+// order with their members ascending (OutputIndicesById,
+// parallel-sequence-ops.h:184-209).  Within cluster r, member i gets label
+// root(i) from a union-find over the cluster's members:
+//   method 2 CONNECTIVITY (PCCI.cc:846-895): every edge between two members;
+//   method 1 TRIANGLE (PCCI.cc:1015-1083): edges with
+//     TriangleScore > 0.25 between two members.
+// GBBS's SimpleUnionAsyncStruct links the larger root under the smaller one,
+// so root(i) is the smallest member index of i's component.  (TRIANGLE's
+// workefficient_cc::CC labels come from a randomized decomposition; the
+// smallest index is used for them too.)  A cluster spans size (CONNECTIVITY)
+// or 1 + its largest label (TRIANGLE) ids; the spans are laid out in cluster
+// order, sub[v] = span start + label, and every id j of cluster r's span
+// starts the next level in cluster r (PCC.cc:457-462, PCCI.cc:60-77).
+// Returns the sum of the spans; parent[j] is filled for j < min(sum, n).
+int64_t SubclusterIds(const oracle_graph& g, const uint32_t* ids, int method,
+                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent) {
+  const int64_t n = g.n;
+  std::vector<std::vector<uint32_t>> members((size_t)n);
+  for (int64_t v = 0; v < n; ++v) members[ids[v]].push_back((uint32_t)v);
+  std::vector<int64_t> index((size_t)n, 0);
+  sub->assign((size_t)n, 0);
+  parent->assign((size_t)n, 0);
+  std::vector<uint32_t> uf;
+  auto find = [&](uint32_t x) {
+    while (uf[x] != x) x = uf[x];
+    return x;
+  };
+  int64_t next = 0, rank = 0;
+  for (int64_t c = 0; c < n; ++c) {
+    const auto& cl = members[(size_t)c];
+    if (cl.empty()) continue;
+    const uint32_t size = (uint32_t)cl.size();
+    for (uint32_t i = 0; i < size; ++i) index[cl[i]] = i;
+    uf.resize(size);
+    std::iota(uf.begin(), uf.end(), 0u);
+    for (uint32_t i = 0; i < size; ++i) {
+      const uint32_t u = cl[i];
+      for (int64_t p = g.offsets[u]; p < g.offsets[(int64_t)u + 1]; ++p) {
+        const uint32_t v = g.nbrs[p];
+        if (ids[v] != (uint32_t)c) continue;
+        if (method == 1 && !(TriangleScore(g, u, v) > 0.25)) continue;
+        uint32_t a = find(i), b = find((uint32_t)index[v]);
+        if (a == b) continue;
+        if (a < b) std::swap(a, b);
+        uf[a] = b;
+      }
+    }
+    uint32_t max_label = 0;
+    for (uint32_t i = 0; i < size; ++i) {
+      const uint32_t r = find(i);
+      (*sub)[cl[i]] = (uint32_t)(next + r);
+      max_label = std::max(max_label, r);
+    }
+    const int64_t span = method == 2 ? (int64_t)size : (int64_t)max_label + 1;
+    for (int64_t j = next; j < next + span && j < n; ++j) (*parent)[(size_t)j] = (uint32_t)rank;
+    next += span;
+    ++rank;
+  }
+  return next;
+}
+
 struct Level {
   OwnedCsr csr;  // unused for level 0 (the input graph)
+  // With subclustering: the next level's vertex of every vertex of this one.
+  std::vector<uint32_t> map;
   std::vector<uint32_t> ids, sizes;
   std::vector<double> nw, K;
   oracle_state State() {
@@ -848,6 +936,20 @@ double oracle_modularity(const oracle_graph* g, const uint32_t* ids, double reso
   return modularity;
 }
 
+int64_t oracle_subclusters(const oracle_graph* g, const uint32_t* ids, int method, uint32_t* sub,
+                           uint32_t* parent) {
+  if (method != 1 && method != 2) return -1;
+  std::vector<uint32_t> s, p;
+  const int64_t total = SubclusterIds(*g, ids, method, &s, &p);
+  std::copy(s.begin(), s.end(), sub);
+  std::copy(p.begin(), p.end(), parent);
+  return total;
+}
+
+double oracle_triangle_score(const oracle_graph* g, uint32_t u, uint32_t v) {
+  return TriangleScore(*g, u, v);
+}
+
 void oracle_flatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out) {
   for (int64_t i = 0; i < n; ++i)
     out[i] = ids[i] == kNone ? kNone : compressed[ids[i]];  // PGU.cc:237-241
@@ -893,6 +995,9 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
     return 12;  // PCC.cc:501-505
   }
   if (cfg.sequential && cfg.all_iter) num_iterations = INT64_MAX;  // correlation-clusterer.cc:266-267
+  const bool subclust = cfg.subclustering_method != 0 && !cfg.sequential;
+  if (subclust && (cfg.subclustering_method > 2 || cfg.clustering_moves_method != 1))
+    return 12;  // SLOW_CONNECTIVITY and the in-round subcluster moves are not restated
   int64_t rounds = 0;
   g_edges_scanned = 0;
   std::vector<Level> levels;
@@ -928,27 +1033,61 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
     const int moved = oracle_iterate_best_moves(&cg, &st, &cfg, num_inner,
                                                 cfg.permute_seed + (uint64_t)iter, &rounds);
     if (!moved) {  // PCC.cc:543-547
+      // With subclustering the level started from its parent clustering,
+      // which is then the result.
+      if (subclust && !cfg.refine && iter > 0) {
+        std::vector<uint32_t> flat((size_t)n);
+        oracle_flatten(n, cluster_ids.data(), levels[(size_t)iter].ids.data(), flat.data());
+        cluster_ids.swap(flat);
+      }
       iter--;
       break;
     }
-    if (!cfg.refine) {  // PCC.cc:550
+    const bool last = iter == num_iterations - 1;
+    if (!cfg.refine && (!subclust || last)) {  // PCC.cc:550
       std::vector<uint32_t> flat((size_t)n);
       oracle_flatten(n, cluster_ids.data(), levels[(size_t)iter].ids.data(), flat.data());
       cluster_ids.swap(flat);
     }
-    if (iter == num_iterations - 1) break;  // PCC.cc:556
-    levels.emplace_back();                  // CompressGraph, PCC.cc:569-571
+    if (last) break;          // PCC.cc:556
+    levels.emplace_back();    // CompressGraph, PCC.cc:569-571
     Level& cur = levels[(size_t)iter];
     Level& nxt = levels.back();
-    Compress(cg, cur.ids.data(), cur.nw.data(), &nxt.csr, cfg.sequential != 0);
-    const int64_t n2 = nxt.csr.n;
-    nxt.nw = nxt.csr.node_weights;  // singletons with aggregated weights, PCC.cc:583-585
-    nxt.ids.resize((size_t)n2);
-    nxt.sizes.assign((size_t)n2, 1);
-    nxt.K.resize((size_t)n2);
-    for (int64_t i = 0; i < n2; ++i) {
-      nxt.ids[(size_t)i] = (uint32_t)i;
-      nxt.K[(size_t)i] = nxt.nw[(size_t)i];
+    if (subclust) {
+      // CompressSubclusters (PCC.cc:564-567): the subclusters become the next
+      // level's vertices and start in their parent cluster.  The next level's
+      // vertex of a vertex is its subcluster id, which is what the flatten
+      // steps compose (the reference composes cluster ids there, PCC.cc:550
+      // and 594-595, which index the next level by the wrong vertex).
+      std::vector<uint32_t> parent;
+      SubclusterIds(cg, cur.ids.data(), cfg.subclustering_method, &cur.map, &parent);
+      Compress(cg, cur.map.data(), cur.nw.data(), &nxt.csr);
+      const int64_t n2 = nxt.csr.n;
+      nxt.nw = nxt.csr.node_weights;
+      nxt.ids.resize((size_t)n2);
+      nxt.sizes.assign((size_t)n2, 0);
+      nxt.K.assign((size_t)n2, 0.0);
+      for (int64_t j = 0; j < n2; ++j) {  // SetClustering, PCCI.cc:68-75
+        nxt.ids[(size_t)j] = parent[(size_t)j];
+        nxt.sizes[parent[(size_t)j]] += 1;
+        nxt.K[parent[(size_t)j]] += nxt.nw[(size_t)j];
+      }
+      if (!cfg.refine) {
+        std::vector<uint32_t> flat((size_t)n);
+        oracle_flatten(n, cluster_ids.data(), cur.map.data(), flat.data());
+        cluster_ids.swap(flat);
+      }
+    } else {
+      Compress(cg, cur.ids.data(), cur.nw.data(), &nxt.csr, cfg.sequential != 0);
+      const int64_t n2 = nxt.csr.n;
+      nxt.nw = nxt.csr.node_weights;  // singletons with aggregated weights, PCC.cc:583-585
+      nxt.ids.resize((size_t)n2);
+      nxt.sizes.assign((size_t)n2, 1);
+      nxt.K.resize((size_t)n2);
+      for (int64_t i = 0; i < n2; ++i) {
+        nxt.ids[(size_t)i] = (uint32_t)i;
+        nxt.K[(size_t)i] = nxt.nw[(size_t)i];
+      }
     }
     if (!cfg.refine) {
       // Without refinement only the current level is needed.
@@ -962,7 +1101,8 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
       Level& L = levels[(size_t)i];
       const int64_t ni = cg.n;
       std::vector<uint32_t> flat((size_t)ni);
-      oracle_flatten(ni, L.ids.data(), levels[(size_t)i + 1].ids.data(), flat.data());
+      oracle_flatten(ni, subclust ? L.map.data() : L.ids.data(), levels[(size_t)i + 1].ids.data(),
+                     flat.data());
       oracle_state st = L.State();
       ResetFromIds(ni, st, flat);
       oracle_iterate_best_moves(&cg, &st, &cfg, num_inner, cfg.permute_seed + 7919ull * (uint64_t)(i + 1),

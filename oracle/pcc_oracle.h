@@ -42,6 +42,8 @@ typedef struct oracle_config {
    * move one after another in ascending order, no whole-cluster phase. */
   int32_t sequential;
   int32_t all_iter;                   /* field 17 (sequential clusterer only) */
+  /* field 10: 0 NONE, 1 TRIANGLE, 2 CONNECTIVITY (LOUVAIN only) */
+  int32_t subclustering_method;
 } oracle_config;
 
 /* Read-only CSR view: offsets[n+1], nbrs[m], weights[m] (float, as the
@@ -149,6 +151,17 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg,
                    int modularity, const double* node_weights,
                    const uint32_t* initial_ids, uint32_t* out_ids,
                    oracle_result* out);
+
+/* CompressSubclusters' numbering (PCC.cc:436-474, PCCI.cc:846-1120) of the
+ * clustering ids: method 1 TRIANGLE, 2 CONNECTIVITY.  sub[v] = the subcluster
+ * id of v; parent[j] (length n) = rank of the cluster whose span holds id j,
+ * for j below min(total, n).  Returns total = the sum of the spans, or -1
+ * for another method. */
+int64_t oracle_subclusters(const oracle_graph* g, const uint32_t* ids, int method,
+                           uint32_t* sub, uint32_t* parent);
+
+/* CountTriangles' per-edge score, PCCI.cc:786-799. */
+double oracle_triangle_score(const oracle_graph* g, uint32_t u, uint32_t v);
 
 /* FlattenClustering, PGU.cc:233-243. */
 void oracle_flatten(int64_t n, const uint32_t* ids, const uint32_t* compressed,

@@ -778,8 +778,19 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     return SetError(PCC_UNIMPLEMENTED,
                     "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
   }
-  if (cfg_.subclustering_method != 0)
-    return SetError(PCC_UNIMPLEMENTED, "subclustering_method other than NONE is not on this path");
+  // CompressSubclusters (PCC.cc:564-567) runs between LOUVAIN levels; the
+  // subcluster moves inside DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249) and
+  // SLOW_CONNECTIVITY (PCCI.cc:897-1013) are not on this path.
+  const int subcl = cfg_.subclustering_method;
+  if (subcl != 0) {
+    if (subcl != 1 && subcl != 2)
+      return SetError(PCC_UNIMPLEMENTED, "SLOW_CONNECTIVITY_SUBCLUSTERING is not on this path");
+    if (cfg_.clustering_moves_method != PCC_LOUVAIN)
+      return SetError(PCC_UNIMPLEMENTED,
+                      "subclustering inside DEFAULT_CLUSTER_MOVES rounds is not on this path");
+    if (h_->num_graphs() > 1)
+      return SetError(PCC_UNIMPLEMENTED, "subclustering of batched graphs is not on this path");
+  }
 
   hipEvent_t e0, e1;
   PCC_HIP_CHECK(hipEventCreate(&e0));
@@ -841,11 +852,58 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
         any = true;
       }
     }
-    if (!cfg_.refine && any) {  // PCC.cc:550
+    const bool last = iter == num_iterations - 1;
+    if (!cfg_.refine && any && (!subcl || last)) {  // PCC.cc:550
       DevBuf<uint8_t> dsel = Upload(al_, sel_o, s_);
       LaunchFlattenShift(n, cluster_ids.get(), vgraph0, dsel.get(), nullptr, L.ids.get(), s_);
     }
-    if (iter == num_iterations - 1 || !any) break;  // PCC.cc:556
+    // With subclustering a level starts from its parent clustering, which is
+    // the result when nothing moves there.
+    if (subcl && !cfg_.refine && !any && iter > 0)
+      LaunchFlatten(n, cluster_ids.get(), L.ids.get(), cluster_ids.get(), s_);
+    if (last || !any) break;  // PCC.cc:556
+    if (subcl) {
+      // CompressSubclusters (PCC.cc:436-474, 564-567): the subclusters become
+      // the next level's vertices and start in their parent cluster.  The
+      // next level's vertex of a vertex is its subcluster id, and that is what
+      // the flattens compose (the reference composes the cluster ids there,
+      // PCC.cc:550 and 594-595, indexing the next level by the wrong vertex).
+      Timer tc = StartTimer();
+      DevBuf<uint32_t> parent;
+      const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent, s_);
+      if (n2 < 0) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return SetError(PCC_INVALID_ARGUMENT,
+                        "TRIANGLE_SUBCLUSTERING needs strictly ascending adjacency lists");
+      }
+      auto next = std::make_unique<Level>();
+      next->lay.G = 1;
+      next->lay.batch = false;
+      next->lay.base = {0, n2};
+      next->lay.orig.push_back(0);
+      CompressGraph(al_, L.g, L.map.get(), L.nw.get(), nullptr, nullptr, nullptr, n2, &next->own,
+                    &next->nw, s_);
+      next->g = next->own.View();
+      next->ids.Reset(al_, n2);
+      next->sizes.Reset(al_, n2);
+      next->K.Reset(al_, n2);
+      LaunchSetClustering(n2, parent.get(), next->State(), s_);
+      StopTimer(tc, 10);
+      if (!cfg_.refine) {
+        LaunchFlatten(n, cluster_ids.get(), L.map.get(), cluster_ids.get(), s_);
+        if (iter > 0) {
+          L.own = DevCsr();
+          L.ids.Release();
+          L.sizes.Release();
+          L.K.Release();
+          L.nw.Release();
+        }
+        L.map.Release();
+      }
+      levels.push_back(std::move(next));
+      continue;
+    }
     // CompressGraph (PCC.cc:569-571) of the graphs that moved, and singleton
     // helpers with aggregated node weights (PCC.cc:583-585).
     Timer tc = StartTimer();
@@ -935,6 +993,9 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
         }
       }
       if (!anysel) continue;
+      if (subcl)  // the next level's vertex of each vertex is its subcluster
+        PCC_HIP_CHECK(hipMemcpyAsync(Li.ids.get(), Li.map.get(), 4 * (size_t)Li.g.n,
+                                     hipMemcpyDeviceToDevice, s_));
       if (batch) {
         DevBuf<uint8_t> dsel = Upload(al_, sel, s_);
         DevBuf<int64_t> dnb = Upload(al_, nbase, s_);

@@ -79,6 +79,9 @@ struct Layout {
 struct Level {
   DevCsr own;   // contracted graphs (level > 0)
   CsrView g;    // this level's graph
+  // With subclustering: the next level's vertex (subcluster id) of every
+  // vertex of this one, kept for the refinement flatten.
+  DevBuf<uint32_t> map;
   DevBuf<uint32_t> ids, sizes;
   DevBuf<double> nw, K;
   Layout lay;

@@ -1,0 +1,292 @@
+// k_subcluster.hip -- the subcluster numbering of CompressSubclusters
+// (clustering/clusterers/parallel-correlation-clusterer.cc:436-474) with
+// ComputeSubcluster (parallel-correlation-clusterer-internal.cc:1101-1120):
+// every cluster of a level is split into the components of a subgraph of its
+// own edges, and the components become the next level's vertices.
+//
+//   CONNECTIVITY (PCCI.cc:846-895): all edges between two members;
+//   TRIANGLE (PCCI.cc:1015-1083, scores from CountTriangles, PCCI.cc:786-799):
+//     edges (u, v) between two members with |N(u) & N(v)| / (deg u + deg v)
+//     > 0.25, i.e. 4 |N(u) & N(v)| > deg u + deg v (the quotient of two
+//     integers below 2^32 cannot round onto 1/4 from above).
+//
+// The reference numbers a cluster's members 0..size-1 in ascending vertex
+// order and labels each member with the union-find root of its component;
+// GBBS's SimpleUnionAsyncStruct hooks the larger root under the smaller, so
+// the label is the position of the component's smallest vertex.  This file
+// runs the same union-find on vertex ids, lock-free: a root is hooked only by
+// a CAS from itself to a smaller root and finds halve paths by CAS, so parent
+// pointers only decrease and every component ends rooted at its smallest
+// vertex, whatever the order.  A cluster spans size ids (CONNECTIVITY) or
+// 1 + its largest label (TRIANGLE); spans are laid out in ascending cluster
+// id order, sub[v] = span start + label, and id j of the r-th non-empty
+// cluster's span starts the next level in cluster r.
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace pcc {
+
+namespace {
+
+int ScGrid(int64_t items, int per_block, int max_blocks) {
+  int64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+__device__ __forceinline__ uint32_t UfLoad(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint32_t UfFind(uint32_t* uf, uint32_t x) {
+  uint32_t px = UfLoad(uf + x);
+  while (px != x) {
+    const uint32_t ppx = UfLoad(uf + px);
+    if (ppx != px) atomicCAS(uf + x, px, ppx);
+    x = px;
+    px = ppx;
+  }
+  return x;
+}
+
+__device__ void UfUnite(uint32_t* uf, uint32_t a, uint32_t b) {
+  while (true) {
+    a = UfFind(uf, a);
+    b = UfFind(uf, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(uf + a, a, b) == a) return;
+  }
+}
+
+// Common entries of two strictly ascending adjacency lists: each entry of the
+// shorter list is searched in the rest of the longer one.
+__device__ uint32_t CommonNeighbours(const uint32_t* __restrict__ nbr, int64_t a0, int64_t a1,
+                                     int64_t b0, int64_t b1) {
+  if (a1 - a0 > b1 - b0) {
+    int64_t t = a0;
+    a0 = b0;
+    b0 = t;
+    t = a1;
+    a1 = b1;
+    b1 = t;
+  }
+  uint32_t common = 0;
+  int64_t lo = b0;
+  for (int64_t i = a0; i < a1 && lo < b1; ++i) {
+    const uint32_t x = nbr[i];
+    int64_t hi = b1;
+    while (lo < hi) {  // first position >= x
+      const int64_t mid = (lo + hi) >> 1;
+      if (nbr[mid] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo < b1 && nbr[lo] == x) {
+      ++common;
+      ++lo;
+    }
+  }
+  return common;
+}
+
+__global__ __launch_bounds__(256) void k_sc_init(int64_t n, const uint32_t* __restrict__ ids,
+                                                 uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ uf,
+                                                 uint64_t* __restrict__ keys) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = ids[v];
+    uf[v] = (uint32_t)v;
+    atomicAdd(&cnt[c], 1u);
+    keys[v] = ((uint64_t)c << 32) | (uint64_t)v;
+  }
+}
+
+// Rows must be strictly ascending for the triangle scores (GBBS's merge
+// intersect assumes it; contracted levels are built that way).
+__global__ __launch_bounds__(256) void k_sc_sorted(CsrView g, uint32_t* __restrict__ bad) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p + 1 < g.m;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    if (g.nbr[p] >= g.nbr[p + 1]) {
+      // p and p + 1 in one row unless a row ends at p + 1.
+      const int64_t u = VertexOfEdge(g.off, 0, g.n, p);
+      if (g.off[u + 1] > p + 1) atomicOr(bad, 1u);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sc_rank(int64_t n, const uint64_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ start,
+                                                 uint32_t* __restrict__ pos) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    pos[(uint32_t)k] = (uint32_t)(i - (int64_t)start[k >> 32]);
+  }
+}
+
+// Unions over the cluster-internal edges, kScItems consecutive CSR edges per
+// thread.
+constexpr int kScItems = 16;
+__global__ __launch_bounds__(256) void k_sc_union(CsrView g, const uint32_t* __restrict__ ids,
+                                                  int method, uint32_t* __restrict__ uf) {
+  const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
+  for (int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ch < nchunks;
+       ch += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p0 = ch * kScItems;
+    const int64_t p1 = min(p0 + (int64_t)kScItems, g.m);
+    int64_t u = VertexOfEdge(g.off, 0, g.n, p0);
+    int64_t uend = g.off[u + 1];
+    for (int64_t p = p0; p < p1; ++p) {
+      while (p >= uend) {
+        ++u;
+        uend = g.off[u + 1];
+      }
+      const uint32_t v = g.nbr[p];
+      if (v == (uint32_t)u || ids[v] != ids[u]) continue;
+      if (method == 1) {
+        const int64_t du = uend - g.off[u];
+        const int64_t dv = g.off[(int64_t)v + 1] - g.off[v];
+        const uint32_t common = CommonNeighbours(g.nbr, g.off[u], uend, g.off[v], g.off[(int64_t)v + 1]);
+        if (!(4 * (int64_t)common > du + dv)) continue;
+      }
+      UfUnite(uf, (uint32_t)u, v);
+    }
+  }
+}
+
+// Labels: position of the component's root; TRIANGLE spans = 1 + the largest
+// label of the cluster.
+__global__ __launch_bounds__(256) void k_sc_label(int64_t n, const uint32_t* __restrict__ ids,
+                                                  uint32_t* __restrict__ uf,
+                                                  const uint32_t* __restrict__ pos,
+                                                  uint32_t* __restrict__ lab,
+                                                  uint32_t* __restrict__ span) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = UfFind(uf, (uint32_t)v);
+    const uint32_t l = pos[r];
+    lab[v] = l;
+    if (span) atomicMax(&span[ids[v]], l + 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sc_sub(int64_t n, const uint32_t* __restrict__ ids,
+                                                const uint32_t* __restrict__ lab,
+                                                const uint32_t* __restrict__ offset,
+                                                uint32_t* __restrict__ sub) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    sub[v] = offset[ids[v]] + lab[v];
+}
+
+// mark[span start of every non-empty cluster] = 1 (starts are < n2: a
+// non-empty cluster holds an id at or after its start).
+__global__ __launch_bounds__(256) void k_sc_mark(int64_t n, const uint32_t* __restrict__ cnt,
+                                                 const uint32_t* __restrict__ offset,
+                                                 uint8_t* __restrict__ mark) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x)
+    if (cnt[c] > 0) mark[offset[c]] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_sc_parent(int64_t n2, const uint8_t* __restrict__ mark,
+                                                   const uint32_t* __restrict__ ex,
+                                                   uint32_t* __restrict__ parent) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n2;
+       j += (int64_t)gridDim.x * blockDim.x)
+    parent[j] = ex[j] + mark[j] - 1u;
+}
+
+int ScBits(uint64_t maxval) {
+  int b = 0;
+  while (b < 63 && (maxval >> b) != 0) ++b;
+  return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s) {
+  const int64_t n = g.n;
+  if (n <= 0) return 0;
+  const int grid = ScGrid(n, 256, 8192);
+  if (method == 1 && g.m > 1) {
+    DevBuf<uint32_t> bad(al, 1);
+    PCC_HIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
+    hipLaunchKernelGGL(k_sc_sorted, dim3(ScGrid(g.m, 256, 8192)), dim3(256), 0, s, g, bad.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    uint32_t hbad = 0;
+    PCC_HIP_CHECK(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+    PCC_HIP_CHECK(hipStreamSynchronize(s));
+    if (hbad) return -1;
+  }
+  DevBuf<uint32_t> cnt(al, n), start(al, n), uf(al, n), pos(al, n), lab(al, n);
+  DevBuf<uint64_t> keys(al, n);
+  DevBuf<float> dummy(al, n);
+  PCC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, 4 * (size_t)n, s));
+  hipLaunchKernelGGL(k_sc_init, dim3(grid), dim3(256), 0, s, n, ids, cnt.get(), uf.get(),
+                     keys.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  ExclusiveScanU32toU32(al, cnt.get(), start.get(), n, nullptr, s);
+  // Cluster ids are vertex-range ids (< n): sort on the id bits only; the low
+  // word (the vertex) is already ascending within every id.
+  RadixSortPairsF32(al, keys.get(), dummy.get(), n, 32, 32 + ScBits((uint64_t)(n - 1)), s);
+  dummy.Release();
+  hipLaunchKernelGGL(k_sc_rank, dim3(grid), dim3(256), 0, s, n, keys.get(), start.get(),
+                     pos.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  keys.Release();
+  if (g.m > 0) {
+    const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
+    hipLaunchKernelGGL(k_sc_union, dim3(ScGrid(nchunks, 256, 8192)), dim3(256), 0, s, g, ids,
+                       method, uf.get());
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  DevBuf<uint32_t> span;
+  const uint32_t* offset = start.get();  // CONNECTIVITY: span = size
+  DevBuf<uint32_t> toff;
+  if (method == 1) {
+    span.Reset(al, n);
+    toff.Reset(al, n);
+    PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
+  }
+  hipLaunchKernelGGL(k_sc_label, dim3(grid), dim3(256), 0, s, n, ids, uf.get(), pos.get(),
+                     lab.get(), span.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  if (method == 1) {
+    ExclusiveScanU32toU32(al, span.get(), toff.get(), n, nullptr, s);
+    offset = toff.get();
+  }
+  sub->Reset(al, n);
+  hipLaunchKernelGGL(k_sc_sub, dim3(grid), dim3(256), 0, s, n, ids, lab.get(), offset,
+                     sub->get());
+  PCC_HIP_CHECK(hipGetLastError());
+  DevBuf<uint32_t> dmax(al, 1);
+  ReduceMaxU32(al, sub->get(), n, dmax.get(), s);
+  uint32_t hmax = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hmax, dmax.get(), 4, hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  const int64_t n2 = (int64_t)hmax + 1;
+  DevBuf<uint8_t> mark(al, n2);
+  DevBuf<uint32_t> ex(al, n2);
+  PCC_HIP_CHECK(hipMemsetAsync(mark.get(), 0, (size_t)n2, s));
+  hipLaunchKernelGGL(k_sc_mark, dim3(grid), dim3(256), 0, s, n, cnt.get(), offset, mark.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  ExclusiveScanU8toU32(al, mark.get(), ex.get(), n2, nullptr, s);
+  parent->Reset(al, n2);
+  hipLaunchKernelGGL(k_sc_parent, dim3(ScGrid(n2, 256, 8192)), dim3(256), 0, s, n2, mark.get(),
+                     ex.get(), parent->get());
+  PCC_HIP_CHECK(hipGetLastError());
+  return n2;
+}
+
+}  // namespace pcc

@@ -465,6 +465,15 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
                    const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
                    int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s);
 
+// The subcluster numbering of CompressSubclusters
+// (parallel-correlation-clusterer.cc:436-474) of one graph's clustering ids:
+// method 1 TRIANGLE, 2 CONNECTIVITY.  Fills sub (n entries: the next level's
+// vertex of every vertex) and parent (n2 entries: the cluster every next-level
+// vertex starts in) and returns n2 = 1 + max sub, or -1 when method 1 meets an
+// adjacency row that is not strictly ascending.  Synchronizes the stream.
+int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s);
+
 // FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,
                    hipStream_t s);

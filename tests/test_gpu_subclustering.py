@@ -1,0 +1,79 @@
+"""Subclustering between LOUVAIN levels (CompressSubclusters,
+clustering/clusterers/parallel-correlation-clusterer.cc:436-474, 564-567):
+the HIP path against the oracle, bit-exact after the level-0 rounds, the
+subcluster numbering (union-find components, triangle scores), the
+compression by subcluster and the next level started from the parent
+clusters, with and without refinement."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GRAPHS = {
+    "sbm_dense": lambda pcc: pcc.gen_sbm(600, 12, 0.7, 0.01, seed=31),
+    "sbm_sparse": lambda pcc: pcc.gen_sbm(2000, 20, 0.08, 0.002, seed=32),
+    "rmat_unit": lambda pcc: pcc.gen_rmat(12, 8, seed=33),
+    "rmat_signed": lambda pcc: pcc.gen_rmat(12, 8, w_lo=-1.0, w_hi=1.0, seed=34),
+    "rmat_hubs": lambda pcc: pcc.gen_rmat(15, 16, seed=35),
+}
+
+CASES = []
+for name in GRAPHS:
+    for method in (1, 2):
+        CASES.append((name, method, dict(resolution=0.05), False))
+CASES += [
+    ("sbm_dense", 1, dict(resolution=0.05, refine=1), False),
+    ("sbm_sparse", 2, dict(resolution=0.01, refine=1), False),
+    ("rmat_hubs", 2, dict(resolution=1.0, refine=1), True),
+    ("rmat_hubs", 1, dict(resolution=1.0), True),
+    ("rmat_signed", 2, dict(resolution=0.1, edge_weight_offset=0.25, move_method=2), False),
+    ("sbm_sparse", 1, dict(resolution=0.01, move_method=0, louvain_num_iterations=3), False),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_subclustering_bit_exact(pcc, orc, case):
+    name, method, kw, modularity = CASES[case]
+    csr = GRAPHS[name](pcc)
+    print(f"subclustering {name} method={method} {kw} modularity={modularity}", flush=True)
+    cfg = pcc.make_config(subclustering_method=method, **kw)
+    dg = pcc.DeviceGraph.from_csr(csr)
+    ids, st = dg.cluster(cfg, pcc.OBJECTIVE_MODULARITY if modularity else pcc.OBJECTIVE_CORRELATION)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), modularity)
+    assert np.array_equal(ids, r.ids)
+    assert st.levels == r.levels
+    assert st.num_clusters == r.num_clusters
+    assert st.objective == pytest.approx(r.objective, rel=1e-9, abs=1e-6)
+
+
+def test_subclustering_changes_the_levels(pcc, orc):
+    """The subcluster path is not the plain one: on dense blocks merged at
+    level 0, TRIANGLE splits the clusters and the runs differ."""
+    csr = GRAPHS["sbm_dense"](pcc)
+    dg = pcc.DeviceGraph.from_csr(csr)
+    plain, _ = dg.cluster(pcc.make_config(resolution=0.05))
+    tri, _ = dg.cluster(pcc.make_config(resolution=0.05, subclustering_method=1))
+    assert not np.array_equal(plain, tri)
+
+
+def test_triangle_needs_ascending_rows(pcc):
+    csr = GRAPHS["sbm_dense"](pcc)
+    nbrs = csr.nbrs.copy()
+    a, b = int(csr.offsets[5]), int(csr.offsets[6])
+    nbrs[a:b] = nbrs[a:b][::-1].copy()  # one row descending
+    bad = pcc.Csr(csr.n, csr.offsets, nbrs, csr.weights)
+    dg = pcc.DeviceGraph.from_csr(bad)
+    with pytest.raises(pcc.StatusError) as e:
+        dg.cluster(pcc.make_config(resolution=0.05, subclustering_method=1))
+    assert e.value.code == pcc.INVALID_ARGUMENT
+    ids, _ = dg.cluster(pcc.make_config(resolution=0.05, subclustering_method=2))
+    assert ids.shape[0] == csr.n
+
+
+def test_subclustering_rejects_batches(pcc):
+    graphs = [pcc.gen_sbm(200, 4, 0.3, 0.02, seed=s) for s in (1, 2)]
+    with pytest.raises(pcc.StatusError) as e:
+        pcc.DeviceBatch.from_csrs(graphs).cluster_batch(
+            pcc.make_config(resolution=0.05, subclustering_method=2))
+    assert e.value.code == pcc.UNIMPLEMENTED

@@ -1,0 +1,170 @@
+"""Pins the oracle's subclustering (CompressSubclusters between LOUVAIN
+levels): clustering/clusterers/parallel-correlation-clusterer.cc:436-474,
+564-567 and parallel-correlation-clusterer-internal.cc:786-1120.
+
+Checks:
+  * the per-edge triangle score equals (A @ A)[u, v] / (deg u + deg v) from a
+    dense adjacency matrix (self-loops included as diagonal entries);
+  * the subcluster numbering equals a breadth-first search over each
+    cluster's members written here from the definition: components of the
+    cluster's own edges (TRIANGLE: edges scoring above 0.25), labelled by the
+    position of their smallest member, spans laid out in cluster order;
+  * a two-level run (with and without refinement) equals the composition of
+    the oracle's own primitives: level-0 rounds, the numbering, compression by
+    subcluster, the next level started from the parent clusters, and the
+    result read through the subcluster ids.
+"""
+
+from collections import deque
+
+import numpy as np
+import pytest
+
+from test_oracle import to_csr
+
+
+def random_graph(rng, n, p, self_loops=False):
+    A = np.triu(rng.random((n, n)) < p, 1)
+    A = A | A.T
+    if self_loops:
+        A[np.diag_indices(n)] = rng.random(n) < 0.3
+    W = np.where(A, rng.integers(1, 4, size=(n, n)), 0).astype(np.float32)
+    W = np.triu(W) + np.triu(W, 1).T
+    return A, W
+
+
+def dense_scores(A):
+    Ai = A.astype(np.int64)
+    common = Ai @ Ai
+    deg = Ai.sum(1)
+    with np.errstate(invalid="ignore"):  # 0 / 0 between isolated vertices
+        return common / (deg[:, None] + deg[None, :]).astype(np.float64)
+
+
+def bfs_subclusters(A, ids, method):
+    n = A.shape[0]
+    S = dense_scores(A) if method == 1 else None
+    sub = np.zeros(n, np.int64)
+    parent = []
+    nxt = 0
+    for r, c in enumerate(sorted(set(ids.tolist()))):
+        mem = [v for v in range(n) if ids[v] == c]
+        where = {v: i for i, v in enumerate(mem)}
+        label = {}
+        for v in mem:
+            if v in label:
+                continue
+            comp, q = [v], deque([v])
+            label[v] = -1
+            while q:
+                u = q.popleft()
+                for x in np.flatnonzero(A[u]):
+                    x = int(x)
+                    if x not in where or x in label:
+                        continue
+                    if method == 1 and not S[u, x] > 0.25:
+                        continue
+                    label[x] = -1
+                    comp.append(x)
+                    q.append(x)
+            lo = min(where[x] for x in comp)
+            for x in comp:
+                label[x] = lo
+        for v in mem:
+            sub[v] = nxt + label[v]
+        span = len(mem) if method == 2 else max(label.values()) + 1
+        parent += [r] * span
+        nxt += span
+    return sub, np.array(parent, np.int64), nxt
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_triangle_score_matches_dense(orc, seed):
+    rng = np.random.default_rng(900 + seed)
+    n = 24
+    A, W = random_graph(rng, n, 0.3, self_loops=True)
+    g = orc.G(*to_csr(A, W))
+    S = dense_scores(A)
+    for u in range(n):
+        for v in np.flatnonzero(A[u]):
+            assert orc.triangle_score(g, u, int(v)) == S[u, v]
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("seed", range(6))
+def test_subclusters_match_bfs(orc, method, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = 30
+    A, W = random_graph(rng, n, 0.12 + 0.05 * (seed % 3), self_loops=seed % 2 == 1)
+    g = orc.G(*to_csr(A, W))
+    # Clusters with unused ids in between, so spans and ranks differ from ids.
+    ids = (rng.integers(0, 6, size=n) * 3 + 1).astype(np.uint32)
+    sub, parent, total = orc.subclusters(g, ids, method)
+    want_sub, want_parent, want_total = bfs_subclusters(A, ids.astype(np.int64), method)
+    assert total == want_total
+    assert np.array_equal(sub, want_sub)
+    k = min(total, n)
+    assert np.array_equal(parent[:k], want_parent[:k])
+
+
+def test_subclusters_split_disconnected_cluster(orc):
+    # Cluster 0 = {0, 1, 2, 3}: edges 0-1 and 2-3 only -> two subclusters.
+    n = 5
+    A = np.zeros((n, n), bool)
+    for u, v in [(0, 1), (2, 3), (3, 4)]:
+        A[u, v] = A[v, u] = True
+    W = A.astype(np.float32)
+    g = orc.G(*to_csr(A, W))
+    ids = np.array([0, 0, 0, 0, 4], np.uint32)
+    sub, parent, total = orc.subclusters(g, ids, 2)
+    assert list(sub) == [0, 0, 2, 2, 4] and total == 5
+    assert list(parent) == [0, 0, 0, 0, 1]
+    # Triangle: no edge lies on a triangle -> every member alone.
+    sub, parent, total = orc.subclusters(g, ids, 1)
+    assert list(sub) == [0, 1, 2, 3, 4] and total == 5
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("refine", [False, True])
+def test_two_levels_compose(orc, pcc, method, refine):
+    # Dense blocks for TRIANGLE (edges need common neighbours), so that the
+    # level-0 clusters (several blocks each) split into subclusters.
+    csr = pcc.gen_sbm(300, 10, 0.7 if method == 1 else 0.3, 0.02, seed=5 + method)
+    g = orc.G.from_csr(csr)
+    n = csr.n
+    res = 0.05 if method == 1 else 0.02
+    cfg = orc.config(resolution=res, louvain_num_iterations=2, refine=refine,
+                     subclustering_method=method)
+    got = orc.cluster(g, cfg)
+    # Level 0 from singletons.
+    plain = orc.config(resolution=res, louvain_num_iterations=2, refine=refine)
+    s0 = orc.S.singletons(n)
+    assert orc.iterate_best_moves(g, s0, plain, 10, plain.permute_seed) == 1
+    sub, parent, total = orc.subclusters(g, s0.ids, method)
+    assert len(set(sub.tolist())) > len(set(s0.ids.tolist()))  # some cluster splits
+    g1, nw1 = orc.compress(g, sub, np.ones(n))
+    n1 = g1.n
+    assert n1 == int(sub.max()) + 1
+    s1 = orc.S.from_ids(parent[:n1], nw1)
+    assert np.array_equal(s1.sizes[: int(parent[:n1].max()) + 1],
+                          np.bincount(parent[:n1]))
+    moved1 = orc.iterate_best_moves(g1, s1, plain, 10, plain.permute_seed + 1)
+    if not refine:
+        want = s1.ids[sub]
+    elif moved1:
+        # Refinement of level 0 from the level-1 clusters read through the
+        # subcluster ids.
+        s0r = orc.S.from_ids(orc.rank_relabel(s1.ids[sub]), np.ones(n))
+        orc.iterate_best_moves(g, s0r, plain, 10, plain.permute_seed + 7919)
+        want = s0r.ids
+    else:
+        want = np.arange(n)  # refinement with one level keeps the identity
+    assert np.array_equal(orc.rank_relabel(got.ids), orc.rank_relabel(want))
+
+
+def test_subclustering_unsupported_combinations(orc, pcc):
+    g = orc.G.from_csr(pcc.gen_sbm(100, 4, 0.3, 0.02, seed=1))
+    for cfg in [orc.config(resolution=0.05, subclustering_method=3),
+                orc.config(resolution=0.05, subclustering_method=2, clustering_moves_method=0)]:
+        with pytest.raises(RuntimeError):
+            orc.cluster(g, cfg)
