@@ -377,6 +377,59 @@ bool AsyncMoveSet(const oracle_graph& g, oracle_state& s, const oracle_config& c
   }
 }
 
+// Sync set moves of the given vertex groups, members ascending (PCC.cc:220-241
+// for subclusters, 263-285 for clusters): every group decides on the same
+// state, moves only to a larger id than the cluster of its first member, iff
+// delta > 0, and the moves are applied as MoveNodesToCluster.  moved_clusters
+// is or-ed with the clusters modified.
+void GroupMovesSync(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+                    const std::vector<std::vector<uint32_t>>& groups,
+                    std::vector<uint8_t>& moved_clusters) {
+  const int64_t n = g.n;
+  std::vector<uint8_t> flag((size_t)n, 0);
+  std::vector<uint8_t> additional((size_t)n, 0);
+  std::vector<uint32_t> moves((size_t)n, kNone);
+  for (const auto& grp : groups) {
+    if (grp.empty()) continue;
+    uint32_t t;
+    double d;
+    BestMoveSet(g, s, cfg, grp, flag, &t, &d);
+    const uint32_t cur = s.cluster_ids[grp.front()];
+    if (t < (uint32_t)n && cur >= t) {
+      t = cur;
+      d = 0;
+    }
+    if (d > 0)
+      for (uint32_t u : grp) moves[u] = t;
+  }
+  oracle_apply_moves(n, &s, moves.data(), additional.data());
+  for (int64_t i = 0; i < n; ++i) moved_clusters[(size_t)i] |= additional[(size_t)i];
+}
+
+// Vertex groups by key (key[v] < n), members ascending (OutputIndicesById).
+std::vector<std::vector<uint32_t>> GroupsOf(int64_t n, const uint32_t* key) {
+  std::vector<std::vector<uint32_t>> groups((size_t)n);
+  for (int64_t v = 0; v < n; ++v) groups[key[v]].push_back((uint32_t)v);
+  return groups;
+}
+
+int64_t SubclusterIds(const oracle_graph& g, const uint32_t* ids, int method,
+                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent);
+
+// Sync DEFAULT_CLUSTER_MOVES with subclustering, PCC.cc:175-297: the clusters
+// after the vertex phase are split into subclusters, the subclusters move as
+// sets, then the clusters as they were before the subcluster moves (which may
+// span several clusters by now) move as sets.
+void SubclusterMovesPhase(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
+                          std::vector<uint8_t>& moved_clusters) {
+  const int64_t n = g.n;
+  const auto clusters = GroupsOf(n, s.cluster_ids);
+  std::vector<uint32_t> sub, parent;
+  SubclusterIds(g, s.cluster_ids, cfg.subclustering_method, &sub, &parent);
+  GroupMovesSync(g, s, cfg, GroupsOf(n, sub.data()), moved_clusters);
+  GroupMovesSync(g, s, cfg, clusters, moved_clusters);
+}
+
 // The whole-cluster phase of a DEFAULT_CLUSTER_MOVES round, PCC.cc:167-296
 // (subclustering NONE): every non-empty cluster, as grouped after the vertex
 // phase, tries to move as a whole.  moved_clusters is or-ed with the clusters
@@ -384,35 +437,19 @@ bool AsyncMoveSet(const oracle_graph& g, oracle_state& s, const oracle_config& c
 void ClusterMovesPhase(const oracle_graph& g, oracle_state& s, const oracle_config& cfg,
                        std::vector<uint8_t>& moved_clusters) {
   const int64_t n = g.n;
-  std::vector<std::vector<uint32_t>> groups((size_t)n);
-  for (int64_t v = 0; v < n; ++v) groups[s.cluster_ids[v]].push_back((uint32_t)v);
+  const std::vector<std::vector<uint32_t>> groups = GroupsOf(n, s.cluster_ids);
+  if (!cfg.async) {
+    GroupMovesSync(g, s, cfg, groups, moved_clusters);  // PCC.cc:263-285
+    return;
+  }
   std::vector<uint8_t> flag((size_t)n, 0);
   std::vector<uint8_t> additional((size_t)n, 0);
-  if (!cfg.async) {
-    std::vector<uint32_t> moves((size_t)n, kNone);
-    for (int64_t c = 0; c < n; ++c) {  // PCC.cc:263-280
-      const auto& grp = groups[(size_t)c];
-      if (grp.empty()) continue;
-      uint32_t t;
-      double d;
-      BestMoveSet(g, s, cfg, grp, flag, &t, &d);
-      const uint32_t cur = s.cluster_ids[grp.front()];
-      if (t < (uint32_t)n && cur >= t) {
-        t = cur;
-        d = 0;
-      }
-      if (d > 0)
-        for (uint32_t u : grp) moves[u] = t;
-    }
-    oracle_apply_moves(n, &s, moves.data(), additional.data());  // PCC.cc:285
-  } else {
-    for (int64_t c = 0; c < n; ++c) {  // PCC.cc:254-262
-      const auto& grp = groups[(size_t)c];
-      if (grp.empty()) continue;
-      // The reference marks the entries indexed by the members' vertex ids.
-      if (AsyncMoveSet(g, s, cfg, grp, flag))
-        for (uint32_t u : grp) additional[u] = 1;
-    }
+  for (int64_t c = 0; c < n; ++c) {  // PCC.cc:254-262
+    const auto& grp = groups[(size_t)c];
+    if (grp.empty()) continue;
+    // The reference marks the entries indexed by the members' vertex ids.
+    if (AsyncMoveSet(g, s, cfg, grp, flag))
+      for (uint32_t u : grp) additional[u] = 1;
   }
   for (int64_t i = 0; i < n; ++i) moved_clusters[(size_t)i] |= additional[(size_t)i];
 }
@@ -828,8 +865,13 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
                           nullptr);
     oracle_apply_moves(n, s, moves.data(), moved_clusters.data());  // PCC.cc:163-165
   }
-  if (cfg->clustering_moves_method == 0)  // DEFAULT_CLUSTER_MOVES, PCC.cc:167-297
-    ClusterMovesPhase(*g, *s, *cfg, moved_clusters);
+  if (cfg->clustering_moves_method == 0) {  // DEFAULT_CLUSTER_MOVES, PCC.cc:167-297
+    // Subcluster moves are restated for sync rounds only.
+    if ((cfg->subclustering_method == 1 || cfg->subclustering_method == 2) && !cfg->async)
+      SubclusterMovesPhase(*g, *s, *cfg, moved_clusters);
+    else
+      ClusterMovesPhase(*g, *s, *cfg, moved_clusters);
+  }
   if (cfg->move_method == 1) {  // ALL_MOVE, PCC.cc:299-307
     std::memset(frontier, 1, (size_t)n);
     return n;
@@ -996,8 +1038,8 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
   }
   if (cfg.sequential && cfg.all_iter) num_iterations = INT64_MAX;  // correlation-clusterer.cc:266-267
   const bool subclust = cfg.subclustering_method != 0 && !cfg.sequential;
-  if (subclust && (cfg.subclustering_method > 2 || cfg.clustering_moves_method != 1))
-    return 12;  // SLOW_CONNECTIVITY and the in-round subcluster moves are not restated
+  if (subclust && (cfg.subclustering_method > 2 || (cfg.clustering_moves_method == 0 && cfg.async)))
+    return 12;  // SLOW_CONNECTIVITY and async subcluster moves are not restated
   int64_t rounds = 0;
   g_edges_scanned = 0;
   std::vector<Level> levels;

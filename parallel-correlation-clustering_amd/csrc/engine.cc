@@ -654,7 +654,21 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
                         ws.flags.get(), ws.rc.get(), s_);
   }
   StopTimer(ta, 9);
-  if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
+  if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES && cfg_.subclustering_method != 0) {
+    // Subcluster moves, then moves of the clusters as they were before them
+    // (PCC.cc:175-297, sync): both are set moves of vertex groups.
+    Timer tm = StartTimer();
+    DevBuf<uint32_t> pre(al_, n), sub, parent;
+    PCC_HIP_CHECK(hipMemcpyAsync(pre.get(), st.ids, 4 * (size_t)n, hipMemcpyDeviceToDevice, s_));
+    (void)SubclusterIds(al_, g, st.ids, cfg_.subclustering_method, &sub, &parent, s_, false);
+    parent.Release();
+    cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, sub.get(), ws.flags.get(),
+                                    ws.new_flag.get(), ws.rc.get(), s_);
+    sub.Release();
+    cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, pre.get(), ws.flags.get(),
+                                    ws.new_flag.get(), ws.rc.get(), s_);
+    StopTimer(tm, 12);
+  } else if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
     // Whole-cluster moves after the vertex moves (PCC.cc:167-297); the
     // clusters they modify join the vertex phase's in ws.flags.
     Timer tm = StartTimer();
@@ -778,18 +792,24 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     return SetError(PCC_UNIMPLEMENTED,
                     "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
   }
-  // CompressSubclusters (PCC.cc:564-567) runs between LOUVAIN levels; the
-  // subcluster moves inside DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249) and
-  // SLOW_CONNECTIVITY (PCCI.cc:897-1013) are not on this path.
+  // CompressSubclusters (PCC.cc:564-567) runs between LOUVAIN levels and the
+  // subcluster moves inside sync DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249);
+  // their async form and SLOW_CONNECTIVITY (PCCI.cc:897-1013) are not on this
+  // path.
   const int subcl = cfg_.subclustering_method;
   if (subcl != 0) {
     if (subcl != 1 && subcl != 2)
       return SetError(PCC_UNIMPLEMENTED, "SLOW_CONNECTIVITY_SUBCLUSTERING is not on this path");
-    if (cfg_.clustering_moves_method != PCC_LOUVAIN)
+    if (cfg_.clustering_moves_method != PCC_LOUVAIN && cfg_.async)
       return SetError(PCC_UNIMPLEMENTED,
-                      "subclustering inside DEFAULT_CLUSTER_MOVES rounds is not on this path");
+                      "async subcluster moves inside DEFAULT_CLUSTER_MOVES rounds are not on this path");
     if (h_->num_graphs() > 1)
       return SetError(PCC_UNIMPLEMENTED, "subclustering of batched graphs is not on this path");
+    // Triangle scores merge adjacency lists (GBBS's intersect assumes them
+    // ascending); contracted levels are built ascending.
+    if (subcl == 1 && !RowsAscending(al_, g0, s_))
+      return SetError(PCC_INVALID_ARGUMENT,
+                      "TRIANGLE_SUBCLUSTERING needs strictly ascending adjacency lists");
   }
 
   hipEvent_t e0, e1;
@@ -870,13 +890,7 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       // PCC.cc:550 and 594-595, indexing the next level by the wrong vertex).
       Timer tc = StartTimer();
       DevBuf<uint32_t> parent;
-      const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent, s_);
-      if (n2 < 0) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return SetError(PCC_INVALID_ARGUMENT,
-                        "TRIANGLE_SUBCLUSTERING needs strictly ascending adjacency lists");
-      }
+      const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent, s_, false);
       auto next = std::make_unique<Level>();
       next->lay.G = 1;
       next->lay.batch = false;

@@ -352,8 +352,10 @@ __global__ __launch_bounds__(256) void k_cm_decide(int64_t n, StateView st, Move
 
 // Sync apply over every vertex: MoveNodesToCluster with moves[v] =
 // cmove[c(v)] (same update rules as the vertex phase's apply).
+// grp non-null: moves are per group, cmove[grp[v]].
 __global__ __launch_bounds__(256) void k_cm_apply_sync(int64_t n, StateView st, uint32_t new_id,
                                                        const uint32_t* __restrict__ cmove,
+                                                       const uint32_t* __restrict__ grp,
                                                        uint8_t* __restrict__ modified,
                                                        uint8_t* __restrict__ new_flag,
                                                        RoundCounters* __restrict__ rc) {
@@ -361,7 +363,7 @@ __global__ __launch_bounds__(256) void k_cm_apply_sync(int64_t n, StateView st, 
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t cur = st.ids[v];
-    const uint32_t t = cmove[cur];
+    const uint32_t t = cmove[grp ? grp[v] : cur];
     if (t == kNoneId || t == cur) continue;
     const double k = st.nw[v];
     atomicSub(&st.sizes[cur], 1u);
@@ -466,7 +468,400 @@ __global__ __launch_bounds__(256) void k_cm_async_members(int64_t n, uint32_t* _
   }
 }
 
+// ----------------------------------------------------------- group moves --
+// BestMove(graph, moving_nodes) for arbitrary vertex groups (subclusters, or
+// clusters as they were before the subcluster moves, which may span several
+// clusters by now).  For group S with moving weight W and W_c = its weight
+// in cluster c (PCCI.cc:655-688, UTIL.h:79-111):
+//   change  = NetWeight(class2 / 2, W^2/2 - sum_c W_c^2/2)
+//             - NetWeight(class1_together, sum_c W_c (K_c - W_c))
+//   delta_d = change + NetWeight(S_d, W (K_d - W_d))
+// class2 = edges inside S between different clusters, class1_together =
+// edges leaving S inside their own cluster, S_d = edges leaving S into d.
+// Edge runs are keyed (group, code): code d < n for S_d, n for
+// class1_together, n + 1 for class2; each is a left fold in CSR order, i.e.
+// over the members in ascending order, as the reference's member loop.
+// W, W_c and the max_edges sums are exact for integer node weights, as the
+// cluster weights are.
+
+__device__ __forceinline__ uint32_t GmRecords(uint32_t gu, uint32_t gv, uint32_t cu, uint32_t cv) {
+  if (gu == gv) return cu != cv ? 1u : 0u;
+  return cu == cv ? 2u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_gm_vert(int64_t n, StateView st,
+                                                 const uint32_t* __restrict__ grp, int vbits,
+                                                 double* __restrict__ wsum,
+                                                 uint32_t* __restrict__ front,
+                                                 uint64_t* __restrict__ vkeys,
+                                                 double* __restrict__ vvals) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t gv = grp[v];
+    atomicAdd(&wsum[gv], st.nw[v]);
+    atomicMin(&front[gv], (uint32_t)v);
+    vkeys[v] = ((uint64_t)gv << vbits) | (uint64_t)st.ids[v];
+    vvals[v] = st.nw[v];
+  }
+}
+
+// W_{g,c} of every (group, cluster) run, and the group sums of W_c^2 / 2 and
+// W_c (K_c - W_c).
+__global__ __launch_bounds__(256) void k_gm_cmw(const uint64_t* __restrict__ vkeys,
+                                                const double* __restrict__ vvals,
+                                                const int64_t* __restrict__ vstart, int64_t nv,
+                                                int vbits, StateView st,
+                                                uint64_t* __restrict__ ukeys,
+                                                double* __restrict__ cw, double* __restrict__ hsq,
+                                                double* __restrict__ fm) {
+  const uint64_t vmask = (1ull << vbits) - 1ull;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s0 = vstart[r], s1 = vstart[r + 1];
+    double acc = 0.0;
+    for (int64_t i = s0; i < s1; ++i) acc += vvals[i];
+    const uint64_t k = vkeys[s0];
+    const uint32_t gg = (uint32_t)(k >> vbits), c = (uint32_t)(k & vmask);
+    ukeys[r] = k;
+    cw[r] = acc;
+    atomicAdd(&hsq[gg], acc * acc / 2);
+    atomicAdd(&fm[gg], acc * (st.K[c] - acc));
+  }
+}
+
+// One pass over kCmItems consecutive CSR edges of a thread: the number of
+// records (count) or the records themselves (emit, at out).
+template <bool kEmit>
+__device__ __forceinline__ uint32_t GmEdges(const CsrView& g, const uint32_t* __restrict__ ids,
+                                            const uint32_t* __restrict__ grp, int64_t c0,
+                                            int64_t q0, int64_t len, int cbits, double offset,
+                                            uint64_t* __restrict__ keys,
+                                            float* __restrict__ vals, int64_t out) {
+  uint32_t c = 0;
+  if (q0 >= len) return 0;
+  const int64_t q1 = min(q0 + (int64_t)kCmItems, len);
+  const uint64_t n = (uint64_t)g.n;
+  int64_t u = -1, uend = 0;
+  uint32_t gu = 0, cu = 0;
+  for (int64_t q = q0; q < q1; ++q) {
+    const int64_t p = c0 + q;
+    if (p >= uend) {
+      u = VertexOfEdge(g.off, u < 0 ? 0 : u + 1, g.n, p);
+      uend = g.off[u + 1];
+      gu = grp[u];
+      cu = ids[u];
+    }
+    const uint32_t v = g.nbr[p];
+    const uint32_t gv = grp[v], cv = ids[v];
+    if (!kEmit) {
+      c += GmRecords(gu, gv, cu, cv);
+      continue;
+    }
+    // The reference's lambda parameter is a float: w - offset rounded back
+    // to float (PCCI.cc:668-669).
+    const float w = (float)((double)g.w[p] - offset);
+    const uint64_t gk = (uint64_t)gu << cbits;
+    if (gu == gv) {
+      if (cu != cv) {
+        keys[out + c] = gk | (n + 1);
+        vals[out + c] = w;
+        ++c;
+      }
+    } else {
+      keys[out + c] = gk | (uint64_t)cv;
+      vals[out + c] = w;
+      ++c;
+      if (cu == cv) {
+        keys[out + c] = gk | n;
+        vals[out + c] = w;
+        ++c;
+      }
+    }
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(kCmThreads) void k_gm_count(CsrView g, const uint32_t* __restrict__ ids,
+                                                         const uint32_t* __restrict__ grp,
+                                                         int64_t nchunks,
+                                                         uint32_t* __restrict__ chunk_cnt) {
+  __shared__ uint32_t sw[kCmThreads / kWave];
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t c0 = ch * kCmChunk;
+    const int64_t len = min(kCmChunk, g.m - c0);
+    const uint32_t c = GmEdges<false>(g, ids, grp, c0, (int64_t)threadIdx.x * kCmItems, len, 0,
+                                      0.0, nullptr, nullptr, 0);
+    uint32_t total = 0;
+    (void)BlockExclusiveScan(c, sw, &total);
+    if (threadIdx.x == 0) chunk_cnt[ch] = total;
+  }
+}
+
+__global__ __launch_bounds__(kCmThreads) void k_gm_emit(CsrView g, const uint32_t* __restrict__ ids,
+                                                        const uint32_t* __restrict__ grp,
+                                                        int64_t nchunks,
+                                                        const int64_t* __restrict__ chunk_off,
+                                                        int cbits, double offset,
+                                                        uint64_t* __restrict__ keys,
+                                                        float* __restrict__ vals) {
+  __shared__ uint32_t sw[kCmThreads / kWave];
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t c0 = ch * kCmChunk;
+    const int64_t len = min(kCmChunk, g.m - c0);
+    const int64_t q0 = (int64_t)threadIdx.x * kCmItems;
+    const uint32_t c = GmEdges<false>(g, ids, grp, c0, q0, len, 0, 0.0, nullptr, nullptr, 0);
+    uint32_t total = 0;
+    const uint32_t at = BlockExclusiveScan(c, sw, &total);
+    (void)GmEdges<true>(g, ids, grp, c0, q0, len, cbits, offset, keys, vals,
+                        chunk_off[ch] + (int64_t)at);
+  }
+}
+
+// class1_together and class2 sums of every group.
+__global__ __launch_bounds__(256) void k_gm_special(const uint64_t* __restrict__ keys,
+                                                    const int64_t* __restrict__ start,
+                                                    const double* __restrict__ sums, int64_t nseg,
+                                                    int cbits, uint64_t n,
+                                                    double* __restrict__ c1,
+                                                    double* __restrict__ c2) {
+  const uint64_t cmask = (1ull << cbits) - 1ull;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nseg;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[start[r]];
+    const uint64_t code = k & cmask;
+    if (code == n) c1[k >> cbits] = sums[r];
+    if (code == n + 1) c2[k >> cbits] = sums[r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gm_change(int64_t n, MoveParams mp,
+                                                   const uint32_t* __restrict__ front,
+                                                   const double* __restrict__ wsum,
+                                                   const double* __restrict__ hsq,
+                                                   const double* __restrict__ fm,
+                                                   const double* __restrict__ c1,
+                                                   const double* __restrict__ c2,
+                                                   double* __restrict__ chg) {
+  for (int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < n;
+       gi += (int64_t)gridDim.x * blockDim.x) {
+    if (front[gi] == kNoneId) continue;
+    const double W = wsum[gi];
+    const double me2 = W * W / 2 - hsq[gi];
+    double change = 0.0;
+    change += NetWeight(c2[gi] / 2, me2, mp.res);  // RemoveDoubleCounting, UTIL.h:32
+    change -= NetWeight(c1[gi], fm[gi], mp.res);
+    chg[gi] = change;
+  }
+}
+
+// W_{g,d} by binary search over the sorted (group, cluster) keys; 0 if S has
+// no member in d.
+__device__ __forceinline__ double MovingIn(const uint64_t* __restrict__ ukeys,
+                                           const double* __restrict__ cw, int64_t nv,
+                                           uint64_t key) {
+  int64_t lo = 0, hi = nv;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ukeys[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < nv && ukeys[lo] == key ? cw[lo] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_gm_delta(const uint64_t* __restrict__ keys,
+                                                  const int64_t* __restrict__ start,
+                                                  const double* __restrict__ sums, int64_t nseg,
+                                                  int cbits, int vbits, uint64_t n, StateView st,
+                                                  MoveParams mp, const double* __restrict__ wsum,
+                                                  const double* __restrict__ chg,
+                                                  const uint64_t* __restrict__ ukeys,
+                                                  const double* __restrict__ cw, int64_t nv,
+                                                  double* __restrict__ dlt,
+                                                  unsigned long long* __restrict__ gmax) {
+  const uint64_t cmask = (1ull << cbits) - 1ull;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nseg;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[start[r]];
+    const uint64_t d = k & cmask;
+    if (d >= n) continue;
+    const uint32_t gg = (uint32_t)(k >> cbits);
+    const double W = wsum[gg];
+    const double wd = MovingIn(ukeys, cw, nv, ((uint64_t)gg << vbits) | d);
+    const double x = chg[gg] + NetWeight(sums[r], W * (st.K[d] - wd), mp.res);
+    dlt[r] = x;
+    atomicMax(&gmax[gg], OrdBits(x));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gm_pick(const uint64_t* __restrict__ keys,
+                                                 const int64_t* __restrict__ start, int64_t nseg,
+                                                 int cbits, uint64_t n,
+                                                 const double* __restrict__ dlt,
+                                                 const unsigned long long* __restrict__ gmax,
+                                                 uint32_t* __restrict__ gtgt) {
+  const uint64_t cmask = (1ull << cbits) - 1ull;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nseg;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[start[r]];
+    const uint64_t d = k & cmask;
+    if (d >= n) continue;
+    const uint32_t gg = (uint32_t)(k >> cbits);
+    if (dlt[r] == FromOrd(gmax[gg])) atomicMin(&gtgt[gg], (uint32_t)d);
+  }
+}
+
+// Best move of every group, then the sync rule against the cluster of the
+// group's first member (PCC.cc:224-233, 270-279) and delta > 0.
+__global__ __launch_bounds__(256) void k_gm_decide(int64_t n, StateView st, MoveParams mp,
+                                                   const uint32_t* __restrict__ front,
+                                                   const double* __restrict__ chg,
+                                                   const unsigned long long* __restrict__ gmax,
+                                                   const uint32_t* __restrict__ gtgt,
+                                                   uint32_t* __restrict__ gmove,
+                                                   unsigned long long* __restrict__ nmoves) {
+  unsigned long long moved = 0;
+  for (int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < n;
+       gi += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t out = kNoneId;
+    if (front[gi] != kNoneId) {
+      uint32_t t = mp.new_id;
+      double d = chg[gi];
+      if (gmax[gi] != 0ull) {
+        const double mx = FromOrd(gmax[gi]);
+        if (mx > d) {
+          t = gtgt[gi];
+          d = mx;
+        }
+      }
+      if (t != mp.new_id && st.ids[front[gi]] >= t) d = 0;
+      if (d > 0) {
+        out = t;
+        ++moved;
+      }
+    }
+    gmove[gi] = out;
+  }
+  for (int s = 32; s > 0; s >>= 1) moved += __shfl_xor(moved, s);
+  if (LaneId() == 0 && moved) atomicAdd(nmoves, moved);
+}
+
 }  // namespace
+
+int64_t RunGroupMoves(DeviceAllocator* al, const CsrView& g, const StateView& st,
+                      const MoveParams& mp, const uint32_t* grp, uint8_t* modified,
+                      uint8_t* new_flag, RoundCounters* rc, hipStream_t s) {
+  const int64_t n = g.n;
+  if (n == 0) return 0;
+  const int vbits = CmBits((uint64_t)(n > 1 ? n - 1 : 1));
+  const int cbits = CmBits((uint64_t)n + 1);
+  const int grid_n = CmGrid(n, 256, 8192);
+  DevBuf<double> wsum(al, n), hsq(al, n), fm(al, n), c1(al, n), c2(al, n), chg(al, n);
+  DevBuf<uint32_t> front(al, n), gtgt(al, n), gmove(al, n);
+  DevBuf<unsigned long long> gmax(al, n), nmoves(al, 1);
+  for (double* p : {wsum.get(), hsq.get(), fm.get(), c1.get(), c2.get()})
+    PCC_HIP_CHECK(hipMemsetAsync(p, 0, sizeof(double) * (size_t)n, s));
+  PCC_HIP_CHECK(hipMemsetAsync(front.get(), 0xFF, sizeof(uint32_t) * (size_t)n, s));
+  PCC_HIP_CHECK(hipMemsetAsync(gtgt.get(), 0xFF, sizeof(uint32_t) * (size_t)n, s));
+  PCC_HIP_CHECK(hipMemsetAsync(gmax.get(), 0, sizeof(unsigned long long) * (size_t)n, s));
+  PCC_HIP_CHECK(hipMemsetAsync(nmoves.get(), 0, sizeof(unsigned long long), s));
+  // Moving weight per (group, cluster).
+  DevBuf<uint64_t> vkeys(al, n);
+  DevBuf<double> vvals(al, n);
+  hipLaunchKernelGGL(k_gm_vert, dim3(grid_n), dim3(256), 0, s, n, st, grp, vbits, wsum.get(),
+                     front.get(), vkeys.get(), vvals.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  RadixSortPairsF64(al, vkeys.get(), vvals.get(), n, 0, 2 * vbits, s);
+  DevBuf<int64_t> vstart;
+  const int64_t nv = Segments(al, vkeys.get(), n, &vstart, s);
+  DevBuf<uint64_t> ukeys(al, nv);
+  DevBuf<double> cw(al, nv);
+  hipLaunchKernelGGL(k_gm_cmw, dim3(CmGrid(nv, 256, 8192)), dim3(256), 0, s, vkeys.get(),
+                     vvals.get(), vstart.get(), nv, vbits, st, ukeys.get(), cw.get(), hsq.get(),
+                     fm.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  vkeys.Release();
+  vvals.Release();
+  // Edge runs keyed (group, code).
+  const int64_t nchunks = (g.m + kCmChunk - 1) / kCmChunk;
+  const int cgrid = (int)std::min<int64_t>(nchunks > 0 ? nchunks : 1, 1 << 16);
+  int64_t me = 0;
+  DevBuf<int64_t> coff(al, nchunks + 1);
+  if (nchunks > 0) {
+    DevBuf<uint32_t> ccnt(al, nchunks);
+    hipLaunchKernelGGL(k_gm_count, dim3(cgrid), dim3(kCmThreads), 0, s, g, st.ids, grp, nchunks,
+                       ccnt.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    ExclusiveScanU32toI64(al, ccnt.get(), coff.get(), nchunks, coff.get() + nchunks, s);
+    PCC_HIP_CHECK(hipMemcpyAsync(&me, coff.get() + nchunks, sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, s));
+    PCC_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  DevBuf<uint64_t> keys;
+  DevBuf<int64_t> start;
+  DevBuf<double> sums, dlt;
+  int64_t nseg = 0;
+  if (me > 0) {
+    keys.Reset(al, me);
+    DevBuf<float> vals(al, me);
+    hipLaunchKernelGGL(k_gm_emit, dim3(cgrid), dim3(kCmThreads), 0, s, g, st.ids, grp, nchunks,
+                       coff.get(), cbits, mp.offset, keys.get(), vals.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    RadixSortPairsF32(al, keys.get(), vals.get(), me, 0, vbits + cbits, s);
+    nseg = Segments(al, keys.get(), me, &start, s);
+    sums.Reset(al, nseg);
+    dlt.Reset(al, nseg);
+    DevBuf<uint32_t> longq(al, nseg);
+    DevBuf<unsigned long long> nlong(al, 1);
+    PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned long long), s));
+    const int grid_r = CmGrid(nseg, 256, 8192);
+    hipLaunchKernelGGL(k_cm_short_sums, dim3(grid_r), dim3(256), 0, s, vals.get(), start.get(),
+                       nseg, sums.get(), longq.get(), nlong.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    if (me > kCmShortRun) {
+      unsigned long long hl = 0;
+      PCC_HIP_CHECK(hipMemcpyAsync(&hl, nlong.get(), sizeof(hl), hipMemcpyDeviceToHost, s));
+      PCC_HIP_CHECK(hipStreamSynchronize(s));
+      if (hl > 0) {
+        constexpr int kWavesPerBlock = 256 / kWave;
+        hipLaunchKernelGGL(k_cm_long_sums,
+                           dim3((unsigned)((hl + kWavesPerBlock - 1) / kWavesPerBlock)),
+                           dim3(256), 0, s, vals.get(), start.get(), longq.get(), (int64_t)hl,
+                           sums.get());
+        PCC_HIP_CHECK(hipGetLastError());
+      }
+    }
+    hipLaunchKernelGGL(k_gm_special, dim3(grid_r), dim3(256), 0, s, keys.get(), start.get(),
+                       sums.get(), nseg, cbits, (uint64_t)n, c1.get(), c2.get());
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_gm_change, dim3(grid_n), dim3(256), 0, s, n, mp, front.get(), wsum.get(),
+                     hsq.get(), fm.get(), c1.get(), c2.get(), chg.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  if (nseg > 0) {
+    const int grid_r = CmGrid(nseg, 256, 8192);
+    hipLaunchKernelGGL(k_gm_delta, dim3(grid_r), dim3(256), 0, s, keys.get(), start.get(),
+                       sums.get(), nseg, cbits, vbits, (uint64_t)n, st, mp, wsum.get(), chg.get(),
+                       ukeys.get(), cw.get(), nv, dlt.get(), gmax.get());
+    hipLaunchKernelGGL(k_gm_pick, dim3(grid_r), dim3(256), 0, s, keys.get(), start.get(), nseg,
+                       cbits, (uint64_t)n, dlt.get(), gmax.get(), gtgt.get());
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_gm_decide, dim3(grid_n), dim3(256), 0, s, n, st, mp, front.get(),
+                     chg.get(), gmax.get(), gtgt.get(), gmove.get(), nmoves.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  // MoveNodesToCluster(moves) (PCC.cc:241, 285).
+  PCC_HIP_CHECK(hipMemsetAsync(&rc->new_count, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_cm_apply_sync, dim3(grid_n), dim3(256), 0, s, n, st, mp.new_id,
+                     gmove.get(), grp, modified, new_flag, rc);
+  PCC_HIP_CHECK(hipGetLastError());
+  RunNewClusterAssign(al, st, (uint32_t)n, mp.vgraph, mp.gbase, new_flag, modified, rc, s);
+  unsigned long long hm = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hm, nmoves.get(), sizeof(hm), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  return (int64_t)hm;
+}
 
 int64_t RunClusterMoves(DeviceAllocator* al, const CsrView& g, const StateView& st,
                         const MoveParams& mp, int G, const int64_t counts[kBins],
@@ -579,7 +974,7 @@ int64_t RunClusterMoves(DeviceAllocator* al, const CsrView& g, const StateView& 
       // counted afresh for the empty-id assignment.
       PCC_HIP_CHECK(hipMemsetAsync(&rc->new_count, 0, sizeof(unsigned long long), s));
       hipLaunchKernelGGL(k_cm_apply_sync, dim3(grid_n), dim3(256), 0, s, n, st, mp.new_id,
-                         cmove.get(), modified, new_flag, rc);
+                         cmove.get(), nullptr, modified, new_flag, rc);
       PCC_HIP_CHECK(hipGetLastError());
       RunNewClusterAssign(al, st, (uint32_t)n, mp.vgraph, mp.gbase, new_flag, modified, rc, s);
     } else {

@@ -214,21 +214,25 @@ int ScBits(uint64_t maxval) {
 
 }  // namespace
 
+bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
+  if (g.m <= 1) return true;
+  DevBuf<uint32_t> bad(al, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
+  hipLaunchKernelGGL(k_sc_sorted, dim3(ScGrid(g.m, 256, 8192)), dim3(256), 0, s, g, bad.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  uint32_t hbad = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  return hbad == 0;
+}
+
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
-                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s) {
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s,
+                      bool check_rows) {
   const int64_t n = g.n;
   if (n <= 0) return 0;
   const int grid = ScGrid(n, 256, 8192);
-  if (method == 1 && g.m > 1) {
-    DevBuf<uint32_t> bad(al, 1);
-    PCC_HIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
-    hipLaunchKernelGGL(k_sc_sorted, dim3(ScGrid(g.m, 256, 8192)), dim3(256), 0, s, g, bad.get());
-    PCC_HIP_CHECK(hipGetLastError());
-    uint32_t hbad = 0;
-    PCC_HIP_CHECK(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
-    PCC_HIP_CHECK(hipStreamSynchronize(s));
-    if (hbad) return -1;
-  }
+  if (method == 1 && check_rows && !RowsAscending(al, g, s)) return -1;
   DevBuf<uint32_t> cnt(al, n), start(al, n), uf(al, n), pos(al, n), lab(al, n);
   DevBuf<uint64_t> keys(al, n);
   DevBuf<float> dummy(al, n);

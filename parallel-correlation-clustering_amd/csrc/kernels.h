@@ -324,6 +324,17 @@ void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
                          uint8_t* modified, RoundCounters* rc, hipStream_t s);
 
+// Sync set moves of arbitrary vertex groups (grp[v] < n; one graph): every
+// group with members gets BestMove(graph, members)
+// (parallel-correlation-clusterer-internal.cc:644-707), the sync rule against
+// the cluster of its smallest member (parallel-correlation-clusterer.cc:224-233,
+// 270-279) and moves iff delta > 0; the moves are applied as
+// MoveNodesToCluster, or-ing modified clusters into modified.  Returns the
+// number of groups that move.  Synchronizes the stream.
+int64_t RunGroupMoves(DeviceAllocator* al, const CsrView& g, const StateView& st,
+                      const MoveParams& mp, const uint32_t* grp, uint8_t* modified,
+                      uint8_t* new_flag, RoundCounters* rc, hipStream_t s);
+
 // The whole-cluster phase of a DEFAULT_CLUSTER_MOVES round
 // (parallel-correlation-clusterer.cc:167-297) over the clusters of the graphs
 // whose frontier list (counts / layout / list) is non-empty.  Sync: the moves
@@ -471,8 +482,11 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
 // vertex of every vertex) and parent (n2 entries: the cluster every next-level
 // vertex starts in) and returns n2 = 1 + max sub, or -1 when method 1 meets an
 // adjacency row that is not strictly ascending.  Synchronizes the stream.
+// check_rows false skips the row-order check (the caller made it already).
+bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
-                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s);
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s,
+                      bool check_rows = true);
 
 // FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,

@@ -1,9 +1,11 @@
-"""Subclustering between LOUVAIN levels (CompressSubclusters,
-clustering/clusterers/parallel-correlation-clusterer.cc:436-474, 564-567):
-the HIP path against the oracle, bit-exact after the level-0 rounds, the
-subcluster numbering (union-find components, triangle scores), the
-compression by subcluster and the next level started from the parent
-clusters, with and without refinement."""
+"""Subclustering against the oracle, bit-exact:
+  * between LOUVAIN levels (CompressSubclusters,
+    clustering/clusterers/parallel-correlation-clusterer.cc:436-474, 564-567):
+    the subcluster numbering (union-find components, triangle scores), the
+    compression by subcluster and the next level started from the parent
+    clusters, with and without refinement;
+  * inside sync DEFAULT_CLUSTER_MOVES rounds (PCC.cc:175-297): subcluster set
+    moves, then set moves of the clusters as they were before them."""
 
 import numpy as np
 import pytest
@@ -21,7 +23,8 @@ GRAPHS = {
 CASES = []
 for name in GRAPHS:
     for method in (1, 2):
-        CASES.append((name, method, dict(resolution=0.05), False))
+        if (name, method) != ("rmat_hubs", 1):  # covered below at resolution 1
+            CASES.append((name, method, dict(resolution=0.05), False))
 CASES += [
     ("sbm_dense", 1, dict(resolution=0.05, refine=1), False),
     ("sbm_sparse", 2, dict(resolution=0.01, refine=1), False),
@@ -30,6 +33,18 @@ CASES += [
     ("rmat_signed", 2, dict(resolution=0.1, edge_weight_offset=0.25, move_method=2), False),
     ("sbm_sparse", 1, dict(resolution=0.01, move_method=0, louvain_num_iterations=3), False),
 ]
+# DEFAULT_CLUSTER_MOVES rounds (one level).
+for name, method, kw, modularity in [
+        ("sbm_dense", 1, dict(resolution=0.05), False),
+        ("sbm_dense", 2, dict(resolution=0.05, move_method=1), False),
+        ("sbm_sparse", 2, dict(resolution=0.01), False),
+        ("sbm_sparse", 1, dict(resolution=0.01, move_method=2), False),
+        ("rmat_unit", 2, dict(resolution=0.05), False),
+        ("rmat_signed", 1, dict(resolution=0.1, edge_weight_offset=0.25), False),
+        ("rmat_signed", 2, dict(resolution=0.1, move_method=1, num_iterations=5), False),
+        ("rmat_hubs", 2, dict(resolution=1.0), True),
+        ("rmat_hubs", 1, dict(resolution=1.0, move_method=2), True)]:
+    CASES.append((name, method, dict(clustering_moves_method=0, **kw), modularity))
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
@@ -69,6 +84,14 @@ def test_triangle_needs_ascending_rows(pcc):
     assert e.value.code == pcc.INVALID_ARGUMENT
     ids, _ = dg.cluster(pcc.make_config(resolution=0.05, subclustering_method=2))
     assert ids.shape[0] == csr.n
+
+
+def test_async_subcluster_moves_unimplemented(pcc):
+    dg = pcc.DeviceGraph.from_csr(GRAPHS["sbm_dense"](pcc))
+    with pytest.raises(pcc.StatusError) as e:
+        dg.cluster(pcc.make_config(resolution=0.05, clustering_moves_method=0, async_=1,
+                                   subclustering_method=2))
+    assert e.value.code == pcc.UNIMPLEMENTED
 
 
 def test_subclustering_rejects_batches(pcc):

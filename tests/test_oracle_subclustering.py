@@ -162,9 +162,64 @@ def test_two_levels_compose(orc, pcc, method, refine):
     assert np.array_equal(orc.rank_relabel(got.ids), orc.rank_relabel(want))
 
 
+def _group_moves(orc, g, s, cfg, key):
+    """Sync set moves of the groups of key, written from PCC.cc:220-241 and
+    263-285 with the oracle's own set move (pinned by brute force in
+    test_oracle_cluster_moves.py)."""
+    n = g.n
+    moves = np.full(n, 0xFFFFFFFF, np.uint32)
+    moved = 0
+    for kv in np.unique(key):
+        members = np.flatnonzero(key == kv)
+        t, d = orc.best_move_set(g, s, cfg, members)
+        cur = int(s.ids[members[0]])
+        if t < n and cur >= t:
+            d = 0.0
+        if d > 0:
+            moves[members] = t
+            moved += 1
+    orc.apply_moves(s, moves)
+    return moved
+
+
+def _round_case(orc, method, seed):
+    """One DEFAULT_CLUSTER_MOVES round with subclustering (PCC.cc:121-297):
+    vertex moves, then the subclusters of the resulting clusters move as sets,
+    then the clusters as they were before the subcluster moves.  Returns the
+    number of subclusters that moved."""
+    rng = np.random.default_rng(1200 + seed)
+    n = 36
+    A, W = random_graph(rng, n, 0.3 if method == 1 else 0.12)
+    g = orc.G(*to_csr(A, W))
+    ids = rng.integers(0, 5, size=n).astype(np.uint32) * 7
+    k = rng.integers(1, 3, size=n).astype(np.float64)
+    cfg = orc.config(resolution=0.08, clustering_moves_method=0, move_method=1,
+                     subclustering_method=method)
+    got = orc.S.from_ids(ids, k)
+    orc.round_(g, got, cfg, np.ones(n, np.uint8))
+    want = orc.S.from_ids(ids, k)
+    vmoves, _, _, _ = orc.sync_decisions(g, want, cfg)
+    orc.apply_moves(want, vmoves)
+    before = want.ids.copy()
+    sub, _, _ = orc.subclusters(g, want.ids, method)
+    sub_moves = _group_moves(orc, g, want, cfg, sub)
+    _group_moves(orc, g, want, cfg, before)
+    assert np.array_equal(got.ids, want.ids)
+    assert np.array_equal(got.sizes, want.sizes)
+    assert np.array_equal(got.K, want.K)
+    return sub_moves
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_cluster_move_round_with_subclusters(orc, method):
+    moved = sum(_round_case(orc, method, seed) for seed in range(5))
+    assert moved > 0  # the subcluster phase is exercised
+
+
 def test_subclustering_unsupported_combinations(orc, pcc):
     g = orc.G.from_csr(pcc.gen_sbm(100, 4, 0.3, 0.02, seed=1))
     for cfg in [orc.config(resolution=0.05, subclustering_method=3),
-                orc.config(resolution=0.05, subclustering_method=2, clustering_moves_method=0)]:
+                orc.config(resolution=0.05, subclustering_method=2, clustering_moves_method=0,
+                           async_=True)]:
         with pytest.raises(RuntimeError):
             orc.cluster(g, cfg)
