@@ -105,7 +105,8 @@ def lib() -> ctypes.CDLL:
         l.oracle_cluster.argtypes = [P, P, ctypes.c_int, P, P, P, P]
         l.oracle_cluster.restype = ctypes.c_int
         l.oracle_flatten.argtypes = [ctypes.c_int64, P, P, P]
-        l.oracle_subclusters.argtypes = [P, P, ctypes.c_int, P, P]
+        l.oracle_subclusters.argtypes = [P, P, ctypes.c_int, P, P, ctypes.c_double,
+                                         ctypes.c_double]
         l.oracle_subclusters.restype = ctypes.c_int64
         l.oracle_triangle_score.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
         l.oracle_triangle_score.restype = ctypes.c_double
@@ -288,12 +289,14 @@ def cluster(g: G, cfg: OConfig, modularity_objective: bool = False, node_weights
                   r.num_clusters, r.seconds)
 
 
-def subclusters(g: G, ids, method: int) -> Tuple[np.ndarray, np.ndarray, int]:
+def subclusters(g: G, ids, method: int, offset: float = 0.0,
+                resolution: float = 0.0) -> Tuple[np.ndarray, np.ndarray, int]:
     """CompressSubclusters' numbering: (sub ids, parent rank per id, total)."""
     ids = np.ascontiguousarray(ids, dtype=np.uint32)
     sub = np.zeros(g.n, np.uint32)
     parent = np.zeros(g.n, np.uint32)
-    total = lib().oracle_subclusters(ctypes.byref(g.view), _p(ids), method, _p(sub), _p(parent))
+    total = lib().oracle_subclusters(ctypes.byref(g.view), _p(ids), method, _p(sub), _p(parent),
+                                     offset, resolution)
     if total < 0:
         raise ValueError(f"subclustering method {method} is not restated")
     return sub, parent, int(total)

@@ -414,7 +414,8 @@ std::vector<std::vector<uint32_t>> GroupsOf(int64_t n, const uint32_t* key) {
 }
 
 int64_t SubclusterIds(const oracle_graph& g, const uint32_t* ids, int method,
-                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent);
+                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent, double offset,
+                      double res);
 
 // Sync DEFAULT_CLUSTER_MOVES with subclustering, PCC.cc:175-297: the clusters
 // after the vertex phase are split into subclusters, the subclusters move as
@@ -425,7 +426,8 @@ void SubclusterMovesPhase(const oracle_graph& g, oracle_state& s, const oracle_c
   const int64_t n = g.n;
   const auto clusters = GroupsOf(n, s.cluster_ids);
   std::vector<uint32_t> sub, parent;
-  SubclusterIds(g, s.cluster_ids, cfg.subclustering_method, &sub, &parent);
+  SubclusterIds(g, s.cluster_ids, cfg.subclustering_method, &sub, &parent,
+                cfg.edge_weight_offset, cfg.resolution);
   GroupMovesSync(g, s, cfg, GroupsOf(n, sub.data()), moved_clusters);
   GroupMovesSync(g, s, cfg, clusters, moved_clusters);
 }
@@ -676,9 +678,93 @@ double TriangleScore(const oracle_graph& g, uint32_t u, uint32_t v) {
 // order, sub[v] = span start + label, and every id j of cluster r's span
 // starts the next level in cluster r (PCC.cc:457-462, PCCI.cc:60-77).
 // Returns the sum of the spans; parent[j] is filled for j < min(sum, n).
+// ComputeSubclusterConnectivitySlow, PCCI.cc:897-1013, for one cluster cl
+// (members ascending), restated step by step: members are visited in order;
+// a member still alone, with at least |cl| - 1 edges leaving the cluster,
+// looks at the current groups (OutputIndicesById of tmp: ascending value,
+// members ascending) other than its own; a group is valid if its edges
+// leaving the cluster number at least |group| (|cl| - |group|) and the
+// member's edges into it at least offset * |group| + resolution * |group|
+// (one fused multiply-add under GCC's contraction); the member joins the
+// last valid group with the largest surplus.  fast marks the cluster (1) and
+// the group in hand (2), as the reference's fast_intersect; it is all zero on
+// entry and on exit.  Returns the labels tmp (indexed like cl).
+std::vector<uint32_t> SlowConnectivityLabels(const oracle_graph& g, const std::vector<uint32_t>& cl,
+                                             double offset, double res,
+                                             std::vector<uint8_t>& fast) {
+  const size_t size = cl.size();
+  std::vector<uint32_t> tmp(size);
+  std::iota(tmp.begin(), tmp.end(), 0u);
+  std::vector<uint8_t> singletons(size, 0);
+  for (uint32_t v : cl) fast[v] = 1;
+  auto edges_to = [&](uint32_t v, uint8_t mark) {
+    int count = 0;
+    for (int64_t p = g.offsets[v]; p < g.offsets[(int64_t)v + 1]; ++p)
+      if (fast[g.nbrs[p]] == mark) ++count;
+    return count;
+  };
+  for (size_t i = 0; i < size; ++i) {
+    if (singletons[i] != 0) continue;
+    const uint32_t v = cl[i];
+    const int curr_num_edges = edges_to(v, 0);
+    if ((size_t)curr_num_edges < size - 1) continue;
+    std::vector<std::vector<uint32_t>> groups(size);
+    for (size_t j = 0; j < size; ++j) groups[tmp[j]].push_back((uint32_t)j);
+    groups.erase(std::remove_if(groups.begin(), groups.end(),
+                                [](const std::vector<uint32_t>& x) { return x.empty(); }),
+                 groups.end());
+    std::vector<uint8_t> invalid(groups.size(), 0);
+    size_t num_valid = groups.size();
+    for (size_t j = 0; j < groups.size(); ++j) {
+      const auto& curr = groups[j];
+      if (curr[0] == i) {
+        invalid[j] = 1;
+        --num_valid;
+        continue;
+      }
+      for (uint32_t k : curr) fast[cl[k]] = 2;
+      int num_edges = 0;
+      for (uint32_t k : curr) num_edges += edges_to(cl[k], 0);
+      for (uint32_t k : curr) fast[cl[k]] = 1;
+      if ((size_t)num_edges < curr.size() * (size - curr.size())) {
+        invalid[j] = 1;
+        --num_valid;
+      }
+    }
+    if (num_valid == 0) continue;
+    double max_max = 0;
+    size_t idx = 0;
+    for (size_t j = 0; j < groups.size(); ++j) {
+      if (invalid[j]) continue;
+      const auto& curr = groups[j];
+      for (uint32_t k : curr) fast[cl[k]] = 2;
+      const double weight = (double)edges_to(v, 2);
+      const double cs = (double)curr.size();
+      const double off = std::fma(offset, cs, res * cs);
+      for (uint32_t k : curr) fast[cl[k]] = 1;
+      if (weight < off) {
+        invalid[j] = 1;
+        --num_valid;
+      } else if (weight - off >= max_max) {
+        max_max = weight - off;
+        idx = j;
+      }
+    }
+    if (num_valid == 0) continue;
+    const auto& curr = groups[idx];
+    tmp[i] = tmp[curr[0]];
+    singletons[i] = 1;
+    singletons[curr[0]] = 1;
+  }
+  for (uint32_t v : cl) fast[v] = 0;
+  return tmp;
+}
+
 int64_t SubclusterIds(const oracle_graph& g, const uint32_t* ids, int method,
-                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent) {
+                      std::vector<uint32_t>* sub, std::vector<uint32_t>* parent, double offset,
+                      double res) {
   const int64_t n = g.n;
+  std::vector<uint8_t> fast(method == 3 ? (size_t)n : 0, 0);
   std::vector<std::vector<uint32_t>> members((size_t)n);
   for (int64_t v = 0; v < n; ++v) members[ids[v]].push_back((uint32_t)v);
   std::vector<int64_t> index((size_t)n, 0);
@@ -694,6 +780,19 @@ int64_t SubclusterIds(const oracle_graph& g, const uint32_t* ids, int method,
     const auto& cl = members[(size_t)c];
     if (cl.empty()) continue;
     const uint32_t size = (uint32_t)cl.size();
+    if (method == 3) {
+      const std::vector<uint32_t> tmp = SlowConnectivityLabels(g, cl, offset, res, fast);
+      uint32_t max_label = 0;
+      for (uint32_t i = 0; i < size; ++i) {
+        (*sub)[cl[i]] = (uint32_t)(next + tmp[i]);
+        max_label = std::max(max_label, tmp[i]);
+      }
+      const int64_t span = (int64_t)max_label + 1;  // PCCI.cc:1006-1012
+      for (int64_t j = next; j < next + span && j < n; ++j) (*parent)[(size_t)j] = (uint32_t)rank;
+      next += span;
+      ++rank;
+      continue;
+    }
     for (uint32_t i = 0; i < size; ++i) index[cl[i]] = i;
     uf.resize(size);
     std::iota(uf.begin(), uf.end(), 0u);
@@ -867,7 +966,7 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
   }
   if (cfg->clustering_moves_method == 0) {  // DEFAULT_CLUSTER_MOVES, PCC.cc:167-297
     // Subcluster moves are restated for sync rounds only.
-    if ((cfg->subclustering_method == 1 || cfg->subclustering_method == 2) && !cfg->async)
+    if (cfg->subclustering_method >= 1 && cfg->subclustering_method <= 3 && !cfg->async)
       SubclusterMovesPhase(*g, *s, *cfg, moved_clusters);
     else
       ClusterMovesPhase(*g, *s, *cfg, moved_clusters);
@@ -979,10 +1078,10 @@ double oracle_modularity(const oracle_graph* g, const uint32_t* ids, double reso
 }
 
 int64_t oracle_subclusters(const oracle_graph* g, const uint32_t* ids, int method, uint32_t* sub,
-                           uint32_t* parent) {
-  if (method != 1 && method != 2) return -1;
+                           uint32_t* parent, double offset, double res) {
+  if (method < 1 || method > 3) return -1;
   std::vector<uint32_t> s, p;
-  const int64_t total = SubclusterIds(*g, ids, method, &s, &p);
+  const int64_t total = SubclusterIds(*g, ids, method, &s, &p, offset, res);
   std::copy(s.begin(), s.end(), sub);
   std::copy(p.begin(), p.end(), parent);
   return total;
@@ -1038,8 +1137,8 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
   }
   if (cfg.sequential && cfg.all_iter) num_iterations = INT64_MAX;  // correlation-clusterer.cc:266-267
   const bool subclust = cfg.subclustering_method != 0 && !cfg.sequential;
-  if (subclust && (cfg.subclustering_method > 2 || (cfg.clustering_moves_method == 0 && cfg.async)))
-    return 12;  // SLOW_CONNECTIVITY and async subcluster moves are not restated
+  if (subclust && (cfg.subclustering_method > 3 || (cfg.clustering_moves_method == 0 && cfg.async)))
+    return 12;  // async subcluster moves are not restated
   int64_t rounds = 0;
   g_edges_scanned = 0;
   std::vector<Level> levels;
@@ -1102,7 +1201,8 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
       // steps compose (the reference composes cluster ids there, PCC.cc:550
       // and 594-595, which index the next level by the wrong vertex).
       std::vector<uint32_t> parent;
-      SubclusterIds(cg, cur.ids.data(), cfg.subclustering_method, &cur.map, &parent);
+      SubclusterIds(cg, cur.ids.data(), cfg.subclustering_method, &cur.map, &parent,
+                    cfg.edge_weight_offset, cfg.resolution);
       Compress(cg, cur.map.data(), cur.nw.data(), &nxt.csr);
       const int64_t n2 = nxt.csr.n;
       nxt.nw = nxt.csr.node_weights;

@@ -42,7 +42,7 @@ typedef struct oracle_config {
    * move one after another in ascending order, no whole-cluster phase. */
   int32_t sequential;
   int32_t all_iter;                   /* field 17 (sequential clusterer only) */
-  /* field 10: 0 NONE, 1 TRIANGLE, 2 CONNECTIVITY (LOUVAIN only) */
+  /* field 10: 0 NONE, 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY */
   int32_t subclustering_method;
 } oracle_config;
 
@@ -153,12 +153,13 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg,
                    oracle_result* out);
 
 /* CompressSubclusters' numbering (PCC.cc:436-474, PCCI.cc:846-1120) of the
- * clustering ids: method 1 TRIANGLE, 2 CONNECTIVITY.  sub[v] = the subcluster
+ * clustering ids: method 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY
+ * (which reads edge_weight_offset and resolution).  sub[v] = the subcluster
  * id of v; parent[j] (length n) = rank of the cluster whose span holds id j,
  * for j below min(total, n).  Returns total = the sum of the spans, or -1
  * for another method. */
 int64_t oracle_subclusters(const oracle_graph* g, const uint32_t* ids, int method,
-                           uint32_t* sub, uint32_t* parent);
+                           uint32_t* sub, uint32_t* parent, double offset, double res);
 
 /* CountTriangles' per-edge score, PCCI.cc:786-799. */
 double oracle_triangle_score(const oracle_graph* g, uint32_t u, uint32_t v);

@@ -660,7 +660,8 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     Timer tm = StartTimer();
     DevBuf<uint32_t> pre(al_, n), sub, parent;
     PCC_HIP_CHECK(hipMemcpyAsync(pre.get(), st.ids, 4 * (size_t)n, hipMemcpyDeviceToDevice, s_));
-    (void)SubclusterIds(al_, g, st.ids, cfg_.subclustering_method, &sub, &parent, s_, false);
+    (void)SubclusterIds(al_, g, st.ids, cfg_.subclustering_method, &sub, &parent,
+                        cfg_.edge_weight_offset, cfg_.resolution, s_, false);
     parent.Release();
     cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, sub.get(), ws.flags.get(),
                                     ws.new_flag.get(), ws.rc.get(), s_);
@@ -794,12 +795,11 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
   }
   // CompressSubclusters (PCC.cc:564-567) runs between LOUVAIN levels and the
   // subcluster moves inside sync DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249);
-  // their async form and SLOW_CONNECTIVITY (PCCI.cc:897-1013) are not on this
-  // path.
+  // their async form is not on this path.
   const int subcl = cfg_.subclustering_method;
   if (subcl != 0) {
-    if (subcl != 1 && subcl != 2)
-      return SetError(PCC_UNIMPLEMENTED, "SLOW_CONNECTIVITY_SUBCLUSTERING is not on this path");
+    if (subcl < 1 || subcl > 3)
+      return SetError(PCC_INVALID_ARGUMENT, "unknown subclustering_method");
     if (cfg_.clustering_moves_method != PCC_LOUVAIN && cfg_.async)
       return SetError(PCC_UNIMPLEMENTED,
                       "async subcluster moves inside DEFAULT_CLUSTER_MOVES rounds are not on this path");
@@ -890,7 +890,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       // PCC.cc:550 and 594-595, indexing the next level by the wrong vertex).
       Timer tc = StartTimer();
       DevBuf<uint32_t> parent;
-      const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent, s_, false);
+      const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent,
+                                       cfg_.edge_weight_offset, cfg_.resolution, s_, false);
       auto next = std::make_unique<Level>();
       next->lay.G = 1;
       next->lay.batch = false;

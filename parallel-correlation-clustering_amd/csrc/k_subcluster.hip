@@ -206,6 +206,103 @@ __global__ __launch_bounds__(256) void k_sc_parent(int64_t n2, const uint8_t* __
     parent[j] = ex[j] + mark[j] - 1u;
 }
 
+// Edges of every vertex that leave its cluster.
+__global__ __launch_bounds__(256) void k_sc_outdeg(CsrView g, const uint32_t* __restrict__ ids,
+                                                   uint32_t* __restrict__ outdeg) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = ids[v];
+    uint32_t k = 0;
+    for (int64_t p = g.off[v]; p < g.off[v + 1]; ++p) k += ids[g.nbr[p]] != c ? 1u : 0u;
+    outdeg[v] = k;
+  }
+}
+
+// SLOW_CONNECTIVITY (PCCI.cc:897-1013), one thread per cluster, in the
+// reference's member order.  The reference rebuilds the groups for every
+// member; here each group value t keeps its size, its edges leaving the
+// cluster (the sum over its members) and its smallest member, updated when a
+// member joins.  Member i (still alone, at least size - 1 edges leaving the
+// cluster) counts its edges into each group, then takes the last group t !=
+// its own, in ascending t, with leaving(t) >= |t| (size - |t|) and
+// weight - fma(offset, |t|, res |t|) >= 0 and maximal.  Scratch arrays are
+// indexed base + local index (base = the cluster's start in the sorted
+// member list), so clusters never share entries.
+__global__ __launch_bounds__(64) void k_sc_slow(int64_t n, CsrView g, const uint32_t* __restrict__ ids,
+                                                const uint32_t* __restrict__ cnt,
+                                                const uint32_t* __restrict__ start,
+                                                const uint64_t* __restrict__ members,
+                                                const uint32_t* __restrict__ pos,
+                                                const uint32_t* __restrict__ outdeg, double offset,
+                                                double res, uint32_t* __restrict__ tmp,
+                                                uint32_t* __restrict__ gsize,
+                                                uint64_t* __restrict__ gout,
+                                                uint32_t* __restrict__ gfirst,
+                                                uint8_t* __restrict__ single,
+                                                uint32_t* __restrict__ wcnt,
+                                                uint32_t* __restrict__ lab,
+                                                uint32_t* __restrict__ span) {
+  for (int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ci < n;
+       ci += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t size = cnt[ci];
+    if (size == 0) continue;
+    const int64_t base = start[ci];
+    for (uint32_t j = 0; j < size; ++j) {
+      tmp[base + j] = j;
+      gsize[base + j] = 1;
+      gout[base + j] = outdeg[(uint32_t)members[base + j]];
+      gfirst[base + j] = j;
+      single[base + j] = 0;
+      wcnt[base + j] = 0;
+    }
+    for (uint32_t i = 0; size > 1 && i < size; ++i) {
+      if (single[base + i]) continue;
+      const uint32_t v = (uint32_t)members[base + i];
+      if (outdeg[v] < size - 1) continue;
+      for (int64_t p = g.off[v]; p < g.off[(int64_t)v + 1]; ++p) {
+        const uint32_t u = g.nbr[p];
+        if (ids[u] == (uint32_t)ci) ++wcnt[base + tmp[base + pos[u]]];
+      }
+      double best = 0.0;
+      uint32_t bt = kNoneId;
+      for (uint32_t t = 0; t < size; ++t) {
+        const uint32_t gs = gsize[base + t];
+        if (gs == 0 || t == i) continue;
+        if (gout[base + t] < (uint64_t)gs * (uint64_t)(size - gs)) continue;
+        const double w = (double)wcnt[base + t];
+        const double cs = (double)gs;
+        const double off = __fma_rn(offset, cs, __dmul_rn(res, cs));
+        if (w < off) continue;
+        const double surplus = __dsub_rn(w, off);
+        if (surplus >= best) {
+          best = surplus;
+          bt = t;
+        }
+      }
+      for (int64_t p = g.off[v]; p < g.off[(int64_t)v + 1]; ++p) {
+        const uint32_t u = g.nbr[p];
+        if (ids[u] == (uint32_t)ci) wcnt[base + tmp[base + pos[u]]] = 0;
+      }
+      if (bt == kNoneId) continue;
+      const uint32_t f = gfirst[base + bt];
+      single[base + i] = 1;
+      single[base + f] = 1;
+      tmp[base + i] = bt;
+      gsize[base + i] = 0;
+      gsize[base + bt] += 1;
+      gout[base + bt] += outdeg[v];
+      gfirst[base + bt] = f < i ? f : i;
+    }
+    uint32_t mx = 0;
+    for (uint32_t j = 0; j < size; ++j) {
+      const uint32_t t = tmp[base + j];
+      lab[(uint32_t)members[base + j]] = t;
+      mx = t > mx ? t : mx;
+    }
+    span[ci] = mx + 1;
+  }
+}
+
 int ScBits(uint64_t maxval) {
   int b = 0;
   while (b < 63 && (maxval >> b) != 0) ++b;
@@ -227,8 +324,8 @@ bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
 }
 
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
-                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s,
-                      bool check_rows) {
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, double offset,
+                      double res, hipStream_t s, bool check_rows) {
   const int64_t n = g.n;
   if (n <= 0) return 0;
   const int grid = ScGrid(n, 256, 8192);
@@ -248,30 +345,48 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
   hipLaunchKernelGGL(k_sc_rank, dim3(grid), dim3(256), 0, s, n, keys.get(), start.get(),
                      pos.get());
   PCC_HIP_CHECK(hipGetLastError());
+  DevBuf<uint32_t> span;
+  DevBuf<uint32_t> toff;
+  const uint32_t* offset_of = start.get();  // CONNECTIVITY: span = size
+  if (method == 3) {
+    span.Reset(al, n);
+    toff.Reset(al, n);
+    PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
+    DevBuf<uint32_t> outdeg(al, n), tmp(al, n), gsize(al, n), gfirst(al, n), wcnt(al, n);
+    DevBuf<uint64_t> gout(al, n);
+    DevBuf<uint8_t> single(al, n);
+    hipLaunchKernelGGL(k_sc_outdeg, dim3(grid), dim3(256), 0, s, g, ids, outdeg.get());
+    hipLaunchKernelGGL(k_sc_slow, dim3(ScGrid(n, 64, 1 << 16)), dim3(64), 0, s, n, g, ids,
+                       cnt.get(), start.get(), keys.get(), pos.get(), outdeg.get(), offset, res,
+                       tmp.get(), gsize.get(), gout.get(), gfirst.get(), single.get(), wcnt.get(),
+                       lab.get(), span.get());
+    PCC_HIP_CHECK(hipGetLastError());
+    ExclusiveScanU32toU32(al, span.get(), toff.get(), n, nullptr, s);
+    offset_of = toff.get();
+  }
   keys.Release();
-  if (g.m > 0) {
+  if (method != 3 && g.m > 0) {
     const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
     hipLaunchKernelGGL(k_sc_union, dim3(ScGrid(nchunks, 256, 8192)), dim3(256), 0, s, g, ids,
                        method, uf.get());
     PCC_HIP_CHECK(hipGetLastError());
   }
-  DevBuf<uint32_t> span;
-  const uint32_t* offset = start.get();  // CONNECTIVITY: span = size
-  DevBuf<uint32_t> toff;
   if (method == 1) {
     span.Reset(al, n);
     toff.Reset(al, n);
     PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
   }
-  hipLaunchKernelGGL(k_sc_label, dim3(grid), dim3(256), 0, s, n, ids, uf.get(), pos.get(),
-                     lab.get(), span.get());
-  PCC_HIP_CHECK(hipGetLastError());
+  if (method != 3) {
+    hipLaunchKernelGGL(k_sc_label, dim3(grid), dim3(256), 0, s, n, ids, uf.get(), pos.get(),
+                       lab.get(), span.get());
+    PCC_HIP_CHECK(hipGetLastError());
+  }
   if (method == 1) {
     ExclusiveScanU32toU32(al, span.get(), toff.get(), n, nullptr, s);
-    offset = toff.get();
+    offset_of = toff.get();
   }
   sub->Reset(al, n);
-  hipLaunchKernelGGL(k_sc_sub, dim3(grid), dim3(256), 0, s, n, ids, lab.get(), offset,
+  hipLaunchKernelGGL(k_sc_sub, dim3(grid), dim3(256), 0, s, n, ids, lab.get(), offset_of,
                      sub->get());
   PCC_HIP_CHECK(hipGetLastError());
   DevBuf<uint32_t> dmax(al, 1);
@@ -283,7 +398,7 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
   DevBuf<uint8_t> mark(al, n2);
   DevBuf<uint32_t> ex(al, n2);
   PCC_HIP_CHECK(hipMemsetAsync(mark.get(), 0, (size_t)n2, s));
-  hipLaunchKernelGGL(k_sc_mark, dim3(grid), dim3(256), 0, s, n, cnt.get(), offset, mark.get());
+  hipLaunchKernelGGL(k_sc_mark, dim3(grid), dim3(256), 0, s, n, cnt.get(), offset_of, mark.get());
   PCC_HIP_CHECK(hipGetLastError());
   ExclusiveScanU8toU32(al, mark.get(), ex.get(), n2, nullptr, s);
   parent->Reset(al, n2);

@@ -478,15 +478,16 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
 
 // The subcluster numbering of CompressSubclusters
 // (parallel-correlation-clusterer.cc:436-474) of one graph's clustering ids:
-// method 1 TRIANGLE, 2 CONNECTIVITY.  Fills sub (n entries: the next level's
+// method 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY (which reads offset
+// and res: the edge weight offset and resolution).  Fills sub (n entries: the next level's
 // vertex of every vertex) and parent (n2 entries: the cluster every next-level
 // vertex starts in) and returns n2 = 1 + max sub, or -1 when method 1 meets an
 // adjacency row that is not strictly ascending.  Synchronizes the stream.
 // check_rows false skips the row-order check (the caller made it already).
 bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
-                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, hipStream_t s,
-                      bool check_rows = true);
+                      DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, double offset,
+                      double res, hipStream_t s, bool check_rows = true);
 
 // FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,

@@ -83,11 +83,12 @@ def test_error_statuses(pcc):
     with pytest.raises(pcc.StatusError) as e:
         g.cluster(pcc.make_config(clustering_moves_method=5))  # PCC.cc:501-505
     assert e.value.code == pcc.UNIMPLEMENTED
-    for kw in [dict(subclustering_method=3),  # SLOW_CONNECTIVITY
-               dict(subclustering_method=2, clustering_moves_method=0, async_=1)]:
-        with pytest.raises(pcc.StatusError) as e:
-            g.cluster(pcc.make_config(**kw))
-        assert e.value.code == pcc.UNIMPLEMENTED
+    with pytest.raises(pcc.StatusError) as e:
+        g.cluster(pcc.make_config(subclustering_method=2, clustering_moves_method=0, async_=1))
+    assert e.value.code == pcc.UNIMPLEMENTED
+    with pytest.raises(pcc.StatusError) as e:
+        g.cluster(pcc.make_config(subclustering_method=7))
+    assert e.value.code == pcc.INVALID_ARGUMENT
     bad = pcc.Csr(csr.n, csr.offsets, csr.nbrs.copy(), csr.weights)
     bad.nbrs[3] = csr.n + 7
     with pytest.raises(pcc.StatusError) as e:

@@ -22,7 +22,7 @@ GRAPHS = {
 
 CASES = []
 for name in GRAPHS:
-    for method in (1, 2):
+    for method in (1, 2, 3):
         if (name, method) != ("rmat_hubs", 1):  # covered below at resolution 1
             CASES.append((name, method, dict(resolution=0.05), False))
 CASES += [
@@ -43,7 +43,10 @@ for name, method, kw, modularity in [
         ("rmat_signed", 1, dict(resolution=0.1, edge_weight_offset=0.25), False),
         ("rmat_signed", 2, dict(resolution=0.1, move_method=1, num_iterations=5), False),
         ("rmat_hubs", 2, dict(resolution=1.0), True),
-        ("rmat_hubs", 1, dict(resolution=1.0, move_method=2), True)]:
+        ("rmat_hubs", 1, dict(resolution=1.0, move_method=2), True),
+        ("sbm_dense", 3, dict(resolution=0.05), False),
+        ("rmat_signed", 3, dict(resolution=0.1, edge_weight_offset=0.25), False),
+        ("rmat_hubs", 3, dict(resolution=1.0), True)]:
     CASES.append((name, method, dict(clustering_moves_method=0, **kw), modularity))
 
 

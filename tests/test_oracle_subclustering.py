@@ -124,7 +124,7 @@ def test_subclusters_split_disconnected_cluster(orc):
     assert list(sub) == [0, 1, 2, 3, 4] and total == 5
 
 
-@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("method", [1, 2, 3])
 @pytest.mark.parametrize("refine", [False, True])
 def test_two_levels_compose(orc, pcc, method, refine):
     # Dense blocks for TRIANGLE (edges need common neighbours), so that the
@@ -140,7 +140,7 @@ def test_two_levels_compose(orc, pcc, method, refine):
     plain = orc.config(resolution=res, louvain_num_iterations=2, refine=refine)
     s0 = orc.S.singletons(n)
     assert orc.iterate_best_moves(g, s0, plain, 10, plain.permute_seed) == 1
-    sub, parent, total = orc.subclusters(g, s0.ids, method)
+    sub, parent, total = orc.subclusters(g, s0.ids, method, 0.0, res)
     assert len(set(sub.tolist())) > len(set(s0.ids.tolist()))  # some cluster splits
     g1, nw1 = orc.compress(g, sub, np.ones(n))
     n1 = g1.n
@@ -189,7 +189,7 @@ def _round_case(orc, method, seed):
     number of subclusters that moved."""
     rng = np.random.default_rng(1200 + seed)
     n = 36
-    A, W = random_graph(rng, n, 0.3 if method == 1 else 0.12)
+    A, W = random_graph(rng, n, {1: 0.3, 2: 0.12, 3: 0.5}[method])
     g = orc.G(*to_csr(A, W))
     ids = rng.integers(0, 5, size=n).astype(np.uint32) * 7
     k = rng.integers(1, 3, size=n).astype(np.float64)
@@ -201,7 +201,7 @@ def _round_case(orc, method, seed):
     vmoves, _, _, _ = orc.sync_decisions(g, want, cfg)
     orc.apply_moves(want, vmoves)
     before = want.ids.copy()
-    sub, _, _ = orc.subclusters(g, want.ids, method)
+    sub, _, _ = orc.subclusters(g, want.ids, method, 0.0, 0.08)
     sub_moves = _group_moves(orc, g, want, cfg, sub)
     _group_moves(orc, g, want, cfg, before)
     assert np.array_equal(got.ids, want.ids)
@@ -210,15 +210,99 @@ def _round_case(orc, method, seed):
     return sub_moves
 
 
-@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("method", [1, 2, 3])
 def test_cluster_move_round_with_subclusters(orc, method):
     moved = sum(_round_case(orc, method, seed) for seed in range(5))
     assert moved > 0  # the subcluster phase is exercised
 
 
+def slow_labels(A, cl, offset, res):
+    """ComputeSubclusterConnectivitySlow (PCCI.cc:897-1013) for one cluster,
+    written here from the reference's text with sets instead of marks."""
+    size = len(cl)
+    inside = set(cl)
+    tmp = list(range(size))
+    single = [False] * size
+    nb = [set(int(x) for x in np.flatnonzero(A[v])) for v in range(A.shape[0])]
+    for i in range(size):
+        if single[i]:
+            continue
+        v = cl[i]
+        if len(nb[v] - inside) < size - 1:
+            continue
+        best, pick = 0.0, None
+        for t in sorted(set(tmp)):
+            grp = [j for j in range(size) if tmp[j] == t]
+            if grp[0] == i:
+                continue
+            leaving = sum(len(nb[cl[j]] - inside) for j in grp)
+            if leaving < len(grp) * (size - len(grp)):
+                continue
+            weight = float(len(nb[v] & set(cl[j] for j in grp)))
+            # The test's offsets and resolutions are dyadic: the reference's
+            # fused multiply-add and this sum agree exactly.
+            off = offset * len(grp) + res * len(grp)
+            if weight < off:
+                continue
+            if weight - off >= best:
+                best, pick = weight - off, grp
+        if pick is None:
+            continue
+        tmp[i] = tmp[pick[0]]
+        single[i] = True
+        single[pick[0]] = True
+    return tmp
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_slow_connectivity_matches_restatement(orc, seed):
+    rng = np.random.default_rng(1400 + seed)
+    n = 32
+    A, W = random_graph(rng, n, 0.5, self_loops=seed % 2 == 1)
+    g = orc.G(*to_csr(A, W))
+    ids = (rng.integers(0, 10, size=n) * 2).astype(np.uint32)  # clusters of ~3
+    res, off = [(0.375, 0.0), (0.5, 0.25), (1.0, 0.0), (0.0625, 0.0)][seed % 4]
+    sub, parent, total = orc.subclusters(g, ids, 3, off, res)
+    want = np.zeros(n, np.int64)
+    nxt = 0
+    merged = 0
+    for r, c in enumerate(sorted(set(ids.tolist()))):
+        cl = [v for v in range(n) if ids[v] == c]
+        tmp = slow_labels(A, cl, off, res)
+        merged += len(cl) - len(set(tmp))
+        for j, v in enumerate(cl):
+            want[v] = nxt + tmp[j]
+        span = max(tmp) + 1
+        assert np.all(parent[nxt:nxt + span] == r)
+        nxt += span
+    assert total == nxt
+    assert np.array_equal(sub, want)
+    if seed % 4 != 2:
+        assert merged > 0  # members do join groups
+
+
+def test_slow_connectivity_hand_case(orc):
+    # Cluster {0, 1, 2}; vertices 3..8 outside.  0-1 adjacent, 2 alone; each
+    # of 0, 1, 2 has three edges leaving the cluster (>= size - 1 = 2).
+    # i = 0: groups {1} (leaving 3 >= 1 * 2, weight 1 >= res 0.5) and {2}
+    # (weight 0 < 0.5): 0 joins 1's group, label 1.  i = 1 is no longer alone,
+    # i = 2: group {0, 1} needs 6 >= 2 * 1 leaving edges and weight 0 >= 1.0:
+    # no.  Labels 1, 1, 2.
+    n = 9
+    A = np.zeros((n, n), bool)
+    edges = [(0, 1), (0, 3), (0, 4), (0, 5), (1, 6), (1, 7), (1, 8), (2, 3), (2, 6), (2, 8)]
+    for u, v in edges:
+        A[u, v] = A[v, u] = True
+    g = orc.G(*to_csr(A, A.astype(np.float32)))
+    ids = np.array([0, 0, 0, 3, 4, 5, 6, 7, 8], np.uint32)
+    sub, parent, total = orc.subclusters(g, ids, 3, 0.0, 0.5)
+    assert list(sub[:3]) == [1, 1, 2]
+    assert list(sub[3:]) == [3, 4, 5, 6, 7, 8] and total == 9
+
+
 def test_subclustering_unsupported_combinations(orc, pcc):
     g = orc.G.from_csr(pcc.gen_sbm(100, 4, 0.3, 0.02, seed=1))
-    for cfg in [orc.config(resolution=0.05, subclustering_method=3),
+    for cfg in [orc.config(resolution=0.05, subclustering_method=4),
                 orc.config(resolution=0.05, subclustering_method=2, clustering_moves_method=0,
                            async_=True)]:
         with pytest.raises(RuntimeError):
