@@ -661,7 +661,8 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     DevBuf<uint32_t> pre(al_, n), sub, parent;
     PCC_HIP_CHECK(hipMemcpyAsync(pre.get(), st.ids, 4 * (size_t)n, hipMemcpyDeviceToDevice, s_));
     (void)SubclusterIds(al_, g, st.ids, cfg_.subclustering_method, &sub, &parent,
-                        cfg_.edge_weight_offset, cfg_.resolution, s_, false);
+                        cfg_.edge_weight_offset, cfg_.resolution, s_, false,
+                        h_->symmetric == 1);
     parent.Release();
     cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, sub.get(), ws.flags.get(),
                                     ws.new_flag.get(), ws.rc.get(), s_);
@@ -891,7 +892,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       Timer tc = StartTimer();
       DevBuf<uint32_t> parent;
       const int64_t n2 = SubclusterIds(al_, L.g, L.ids.get(), subcl, &L.map, &parent,
-                                       cfg_.edge_weight_offset, cfg_.resolution, s_, false);
+                                       cfg_.edge_weight_offset, cfg_.resolution, s_, false,
+                                       h_->symmetric == 1);
       auto next = std::make_unique<Level>();
       next->lay.G = 1;
       next->lay.batch = false;

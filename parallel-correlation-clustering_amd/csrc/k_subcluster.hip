@@ -134,10 +134,16 @@ __global__ __launch_bounds__(256) void k_sc_rank(int64_t n, const uint64_t* __re
 }
 
 // Unions over the cluster-internal edges, kScItems consecutive CSR edges per
-// thread.
+// thread.  half: the graph is symmetric, so each edge is taken once (u < v).
+// TRIANGLE scores an edge in the thread when its shorter row has at most
+// kScShortRow entries; longer ones are queued (count only when longq is
+// null) for k_sc_tri_long, one wave per edge.
 constexpr int kScItems = 16;
+constexpr int64_t kScShortRow = 16;
 __global__ __launch_bounds__(256) void k_sc_union(CsrView g, const uint32_t* __restrict__ ids,
-                                                  int method, uint32_t* __restrict__ uf) {
+                                                  int method, int half, uint32_t* __restrict__ uf,
+                                                  int64_t* __restrict__ longq,
+                                                  unsigned long long* __restrict__ nlong) {
   const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
   for (int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ch < nchunks;
        ch += (int64_t)gridDim.x * blockDim.x) {
@@ -151,15 +157,59 @@ __global__ __launch_bounds__(256) void k_sc_union(CsrView g, const uint32_t* __r
         uend = g.off[u + 1];
       }
       const uint32_t v = g.nbr[p];
-      if (v == (uint32_t)u || ids[v] != ids[u]) continue;
+      if (v == (uint32_t)u || (half && v < (uint32_t)u) || ids[v] != ids[u]) continue;
       if (method == 1) {
         const int64_t du = uend - g.off[u];
         const int64_t dv = g.off[(int64_t)v + 1] - g.off[v];
+        if (min(du, dv) > kScShortRow) {
+          const unsigned long long q = atomicAdd(nlong, 1ull);
+          if (longq) longq[q] = p;
+          continue;
+        }
+        if (!longq) continue;  // counting pass
         const uint32_t common = CommonNeighbours(g.nbr, g.off[u], uend, g.off[v], g.off[(int64_t)v + 1]);
         if (!(4 * (int64_t)common > du + dv)) continue;
       }
       UfUnite(uf, (uint32_t)u, v);
     }
+  }
+}
+
+// One wave per queued TRIANGLE edge: the lanes search the entries of the
+// shorter row in the longer one independently.
+__global__ __launch_bounds__(256) void k_sc_tri_long(CsrView g, const int64_t* __restrict__ longq,
+                                                     int64_t nq, uint32_t* __restrict__ uf) {
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int lane = LaneId();
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < nq; q += waves) {
+    const int64_t p = longq[q];
+    const int64_t u = VertexOfEdge(g.off, 0, g.n, p);
+    const uint32_t v = g.nbr[p];
+    int64_t a0 = g.off[u], a1 = g.off[u + 1], b0 = g.off[v], b1 = g.off[(int64_t)v + 1];
+    const int64_t du = a1 - a0, dv = b1 - b0;
+    if (du > dv) {
+      int64_t t = a0;
+      a0 = b0;
+      b0 = t;
+      t = a1;
+      a1 = b1;
+      b1 = t;
+    }
+    uint32_t common = 0;
+    for (int64_t i = a0 + lane; i < a1; i += kWave) {
+      const uint32_t x = g.nbr[i];
+      int64_t lo = b0, hi = b1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (g.nbr[mid] < x)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      common += (lo < b1 && g.nbr[lo] == x) ? 1u : 0u;
+    }
+    for (int s = 32; s > 0; s >>= 1) common += __shfl_xor(common, s);
+    if (lane == 0 && 4 * (int64_t)common > du + dv) UfUnite(uf, (uint32_t)u, v);
   }
 }
 
@@ -218,16 +268,46 @@ __global__ __launch_bounds__(256) void k_sc_outdeg(CsrView g, const uint32_t* __
   }
 }
 
-// SLOW_CONNECTIVITY (PCCI.cc:897-1013), one thread per cluster, in the
-// reference's member order.  The reference rebuilds the groups for every
-// member; here each group value t keeps its size, its edges leaving the
-// cluster (the sum over its members) and its smallest member, updated when a
-// member joins.  Member i (still alone, at least size - 1 edges leaving the
-// cluster) counts its edges into each group, then takes the last group t !=
-// its own, in ascending t, with leaving(t) >= |t| (size - |t|) and
-// weight - fma(offset, |t|, res |t|) >= 0 and maximal.  Scratch arrays are
-// indexed base + local index (base = the cluster's start in the sorted
-// member list), so clusters never share entries.
+// SLOW_CONNECTIVITY (PCCI.cc:897-1013).  Scratch arrays are indexed by the
+// position q in the (cluster, vertex)-sorted member list, so clusters never
+// share entries; k_sc_slow_init sets them up for every member at once:
+// group value t of a member = its local index, size 1, its edges leaving the
+// cluster, and cand = the member has at least size - 1 of them (the
+// reference's test, PCCI.cc:929-935, which depends on nothing that moves).
+__global__ __launch_bounds__(256) void k_sc_slow_init(int64_t n, const uint64_t* __restrict__ members,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint32_t* __restrict__ cnt,
+                                                      const uint32_t* __restrict__ outdeg,
+                                                      uint32_t* __restrict__ tmp,
+                                                      uint32_t* __restrict__ gsize,
+                                                      uint64_t* __restrict__ gout,
+                                                      uint32_t* __restrict__ gfirst,
+                                                      uint8_t* __restrict__ single,
+                                                      uint8_t* __restrict__ cand,
+                                                      uint32_t* __restrict__ wcnt) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = members[q];
+    const uint32_t c = (uint32_t)(k >> 32), v = (uint32_t)k;
+    const uint32_t j = (uint32_t)(q - (int64_t)start[c]);
+    const uint32_t size = cnt[c];
+    tmp[q] = j;
+    gsize[q] = 1;
+    gout[q] = outdeg[v];
+    gfirst[q] = j;
+    single[q] = 0;
+    cand[q] = size > 1 && outdeg[v] >= size - 1 ? 1 : 0;
+    wcnt[q] = 0;
+  }
+}
+
+// One thread per cluster, in the reference's member order.  The reference
+// rebuilds the groups for every member; here each group value t keeps its
+// size, its edges leaving the cluster (the sum over its members) and its
+// smallest member, updated when a member joins.  Member i (a candidate still
+// alone) counts its edges into each group, then takes the last group t != its
+// own, in ascending t, with leaving(t) >= |t| (size - |t|) and
+// weight - fma(offset, |t|, res |t|) >= 0 and maximal.
 __global__ __launch_bounds__(64) void k_sc_slow(int64_t n, CsrView g, const uint32_t* __restrict__ ids,
                                                 const uint32_t* __restrict__ cnt,
                                                 const uint32_t* __restrict__ start,
@@ -239,26 +319,16 @@ __global__ __launch_bounds__(64) void k_sc_slow(int64_t n, CsrView g, const uint
                                                 uint64_t* __restrict__ gout,
                                                 uint32_t* __restrict__ gfirst,
                                                 uint8_t* __restrict__ single,
-                                                uint32_t* __restrict__ wcnt,
-                                                uint32_t* __restrict__ lab,
-                                                uint32_t* __restrict__ span) {
+                                                const uint8_t* __restrict__ cand,
+                                                uint32_t* __restrict__ wcnt) {
   for (int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ci < n;
        ci += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t size = cnt[ci];
-    if (size == 0) continue;
+    if (size <= 1) continue;
     const int64_t base = start[ci];
-    for (uint32_t j = 0; j < size; ++j) {
-      tmp[base + j] = j;
-      gsize[base + j] = 1;
-      gout[base + j] = outdeg[(uint32_t)members[base + j]];
-      gfirst[base + j] = j;
-      single[base + j] = 0;
-      wcnt[base + j] = 0;
-    }
-    for (uint32_t i = 0; size > 1 && i < size; ++i) {
-      if (single[base + i]) continue;
+    for (uint32_t i = 0; i < size; ++i) {
+      if (!cand[base + i] || single[base + i]) continue;
       const uint32_t v = (uint32_t)members[base + i];
-      if (outdeg[v] < size - 1) continue;
       for (int64_t p = g.off[v]; p < g.off[(int64_t)v + 1]; ++p) {
         const uint32_t u = g.nbr[p];
         if (ids[u] == (uint32_t)ci) ++wcnt[base + tmp[base + pos[u]]];
@@ -293,13 +363,20 @@ __global__ __launch_bounds__(64) void k_sc_slow(int64_t n, CsrView g, const uint
       gout[base + bt] += outdeg[v];
       gfirst[base + bt] = f < i ? f : i;
     }
-    uint32_t mx = 0;
-    for (uint32_t j = 0; j < size; ++j) {
-      const uint32_t t = tmp[base + j];
-      lab[(uint32_t)members[base + j]] = t;
-      mx = t > mx ? t : mx;
-    }
-    span[ci] = mx + 1;
+  }
+}
+
+// Labels = group values; span = 1 + the largest (PCCI.cc:1006-1012).
+__global__ __launch_bounds__(256) void k_sc_slow_labels(int64_t n, const uint64_t* __restrict__ members,
+                                                        const uint32_t* __restrict__ tmp,
+                                                        uint32_t* __restrict__ lab,
+                                                        uint32_t* __restrict__ span) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = members[q];
+    const uint32_t t = tmp[q];
+    lab[(uint32_t)k] = t;
+    atomicMax(&span[k >> 32], t + 1);
   }
 }
 
@@ -325,7 +402,7 @@ bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
 
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
                       DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, double offset,
-                      double res, hipStream_t s, bool check_rows) {
+                      double res, hipStream_t s, bool check_rows, bool half) {
   const int64_t n = g.n;
   if (n <= 0) return 0;
   const int grid = ScGrid(n, 256, 8192);
@@ -354,11 +431,16 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
     DevBuf<uint32_t> outdeg(al, n), tmp(al, n), gsize(al, n), gfirst(al, n), wcnt(al, n);
     DevBuf<uint64_t> gout(al, n);
-    DevBuf<uint8_t> single(al, n);
+    DevBuf<uint8_t> single(al, n), cand(al, n);
     hipLaunchKernelGGL(k_sc_outdeg, dim3(grid), dim3(256), 0, s, g, ids, outdeg.get());
+    hipLaunchKernelGGL(k_sc_slow_init, dim3(grid), dim3(256), 0, s, n, keys.get(), start.get(),
+                       cnt.get(), outdeg.get(), tmp.get(), gsize.get(), gout.get(), gfirst.get(),
+                       single.get(), cand.get(), wcnt.get());
     hipLaunchKernelGGL(k_sc_slow, dim3(ScGrid(n, 64, 1 << 16)), dim3(64), 0, s, n, g, ids,
                        cnt.get(), start.get(), keys.get(), pos.get(), outdeg.get(), offset, res,
-                       tmp.get(), gsize.get(), gout.get(), gfirst.get(), single.get(), wcnt.get(),
+                       tmp.get(), gsize.get(), gout.get(), gfirst.get(), single.get(), cand.get(),
+                       wcnt.get());
+    hipLaunchKernelGGL(k_sc_slow_labels, dim3(grid), dim3(256), 0, s, n, keys.get(), tmp.get(),
                        lab.get(), span.get());
     PCC_HIP_CHECK(hipGetLastError());
     ExclusiveScanU32toU32(al, span.get(), toff.get(), n, nullptr, s);
@@ -367,9 +449,33 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
   keys.Release();
   if (method != 3 && g.m > 0) {
     const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
-    hipLaunchKernelGGL(k_sc_union, dim3(ScGrid(nchunks, 256, 8192)), dim3(256), 0, s, g, ids,
-                       method, uf.get());
-    PCC_HIP_CHECK(hipGetLastError());
+    const int ugrid = ScGrid(nchunks, 256, 8192);
+    const int h = half ? 1 : 0;
+    if (method == 1) {
+      // Count the long-row edges, then queue them while the short ones unite.
+      DevBuf<unsigned long long> nlong(al, 1);
+      PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(k_sc_union, dim3(ugrid), dim3(256), 0, s, g, ids, method, h, uf.get(),
+                         nullptr, nlong.get());
+      PCC_HIP_CHECK(hipGetLastError());
+      unsigned long long nq = 0;
+      PCC_HIP_CHECK(hipMemcpyAsync(&nq, nlong.get(), sizeof(nq), hipMemcpyDeviceToHost, s));
+      PCC_HIP_CHECK(hipStreamSynchronize(s));
+      DevBuf<int64_t> longq(al, nq > 0 ? (int64_t)nq : 1);
+      PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(k_sc_union, dim3(ugrid), dim3(256), 0, s, g, ids, method, h, uf.get(),
+                         longq.get(), nlong.get());
+      PCC_HIP_CHECK(hipGetLastError());
+      if (nq > 0) {
+        hipLaunchKernelGGL(k_sc_tri_long, dim3(ScGrid((int64_t)nq, 256 / kWave, 1 << 16)), dim3(256),
+                           0, s, g, longq.get(), (int64_t)nq, uf.get());
+        PCC_HIP_CHECK(hipGetLastError());
+      }
+    } else {
+      hipLaunchKernelGGL(k_sc_union, dim3(ugrid), dim3(256), 0, s, g, ids, method, h, uf.get(),
+                         nullptr, nullptr);
+      PCC_HIP_CHECK(hipGetLastError());
+    }
   }
   if (method == 1) {
     span.Reset(al, n);

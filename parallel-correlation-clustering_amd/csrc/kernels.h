@@ -483,11 +483,12 @@ void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, c
 // vertex of every vertex) and parent (n2 entries: the cluster every next-level
 // vertex starts in) and returns n2 = 1 + max sub, or -1 when method 1 meets an
 // adjacency row that is not strictly ascending.  Synchronizes the stream.
-// check_rows false skips the row-order check (the caller made it already).
+// check_rows false skips the row-order check (the caller made it already);
+// half: the graph is known to be symmetric, so each edge is visited once.
 bool RowsAscending(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int method,
                       DevBuf<uint32_t>* sub, DevBuf<uint32_t>* parent, double offset,
-                      double res, hipStream_t s, bool check_rows = true);
+                      double res, hipStream_t s, bool check_rows = true, bool half = false);
 
 // FlattenClustering (parallel/parallel-graph-utils.cc:233-243).
 void LaunchFlatten(int64_t n, const uint32_t* ids, const uint32_t* compressed, uint32_t* out,
