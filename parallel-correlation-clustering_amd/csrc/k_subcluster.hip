@@ -256,15 +256,29 @@ __global__ __launch_bounds__(256) void k_sc_parent(int64_t n2, const uint8_t* __
     parent[j] = ex[j] + mark[j] - 1u;
 }
 
-// Edges of every vertex that leave its cluster.
+// Edges of every vertex that leave its cluster (outdeg zeroed by the
+// caller), kScItems consecutive CSR edges per thread, one atomic per row run.
 __global__ __launch_bounds__(256) void k_sc_outdeg(CsrView g, const uint32_t* __restrict__ ids,
                                                    uint32_t* __restrict__ outdeg) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t c = ids[v];
-    uint32_t k = 0;
-    for (int64_t p = g.off[v]; p < g.off[v + 1]; ++p) k += ids[g.nbr[p]] != c ? 1u : 0u;
-    outdeg[v] = k;
+  const int64_t nchunks = (g.m + kScItems - 1) / kScItems;
+  for (int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ch < nchunks;
+       ch += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p0 = ch * kScItems;
+    const int64_t p1 = min(p0 + (int64_t)kScItems, g.m);
+    int64_t u = VertexOfEdge(g.off, 0, g.n, p0);
+    int64_t uend = g.off[u + 1];
+    uint32_t cu = ids[u], k = 0;
+    for (int64_t p = p0; p < p1; ++p) {
+      while (p >= uend) {
+        if (k) atomicAdd(&outdeg[u], k);
+        k = 0;
+        ++u;
+        uend = g.off[u + 1];
+        cu = ids[u];
+      }
+      k += ids[g.nbr[p]] != cu ? 1u : 0u;
+    }
+    if (k) atomicAdd(&outdeg[u], k);
   }
 }
 
@@ -282,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sc_slow_init(int64_t n, const uint64_t*
                                                       uint32_t* __restrict__ gsize,
                                                       uint64_t* __restrict__ gout,
                                                       uint32_t* __restrict__ gfirst,
-                                                      uint8_t* __restrict__ single,
+                                                      uint32_t* __restrict__ single,
                                                       uint8_t* __restrict__ cand,
                                                       uint32_t* __restrict__ wcnt) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
@@ -301,67 +315,93 @@ __global__ __launch_bounds__(256) void k_sc_slow_init(int64_t n, const uint64_t*
   }
 }
 
-// One thread per cluster, in the reference's member order.  The reference
+// One wave per cluster, in the reference's member order.  The reference
 // rebuilds the groups for every member; here each group value t keeps its
 // size, its edges leaving the cluster (the sum over its members) and its
-// smallest member, updated when a member joins.  Member i (a candidate still
-// alone) counts its edges into each group, then takes the last group t != its
-// own, in ascending t, with leaving(t) >= |t| (size - |t|) and
-// weight - fma(offset, |t|, res |t|) >= 0 and maximal.
-__global__ __launch_bounds__(64) void k_sc_slow(int64_t n, CsrView g, const uint32_t* __restrict__ ids,
-                                                const uint32_t* __restrict__ cnt,
-                                                const uint32_t* __restrict__ start,
-                                                const uint64_t* __restrict__ members,
-                                                const uint32_t* __restrict__ pos,
-                                                const uint32_t* __restrict__ outdeg, double offset,
-                                                double res, uint32_t* __restrict__ tmp,
-                                                uint32_t* __restrict__ gsize,
-                                                uint64_t* __restrict__ gout,
-                                                uint32_t* __restrict__ gfirst,
-                                                uint8_t* __restrict__ single,
-                                                const uint8_t* __restrict__ cand,
-                                                uint32_t* __restrict__ wcnt) {
-  for (int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ci < n;
-       ci += (int64_t)gridDim.x * blockDim.x) {
+// smallest member, updated by lane 0 when a member joins.  Member i (a
+// candidate still alone) counts its edges into each group (lanes over its
+// row, atomics into wcnt), then takes the last group t != its own, in
+// ascending t, with leaving(t) >= |t| (size - |t|) and
+// weight - fma(offset, |t|, res |t|) >= 0 and maximal: each lane keeps the
+// largest surplus of its t's (the later t on ties) and the wave takes the
+// largest surplus, then the largest t.  The state lanes share is read with
+// agent-scope loads (no stale L1 lines) after a fence.
+template <typename T>
+__device__ __forceinline__ T ScLoad(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_sc_slow(int64_t n, CsrView g, const uint32_t* __restrict__ ids,
+                                                 const uint32_t* __restrict__ cnt,
+                                                 const uint32_t* __restrict__ start,
+                                                 const uint64_t* __restrict__ members,
+                                                 const uint32_t* __restrict__ pos,
+                                                 const uint32_t* __restrict__ outdeg, double offset,
+                                                 double res, uint32_t* tmp, uint32_t* gsize,
+                                                 uint64_t* gout, uint32_t* gfirst,
+                                                 uint32_t* single,
+                                                 const uint8_t* __restrict__ cand,
+                                                 uint32_t* wcnt) {
+  const int lane = LaneId();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t ci = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; ci < n; ci += waves) {
     const uint32_t size = cnt[ci];
     if (size <= 1) continue;
     const int64_t base = start[ci];
-    for (uint32_t i = 0; i < size; ++i) {
-      if (!cand[base + i] || single[base + i]) continue;
-      const uint32_t v = (uint32_t)members[base + i];
-      for (int64_t p = g.off[v]; p < g.off[(int64_t)v + 1]; ++p) {
-        const uint32_t u = g.nbr[p];
-        if (ids[u] == (uint32_t)ci) ++wcnt[base + tmp[base + pos[u]]];
-      }
-      double best = 0.0;
-      uint32_t bt = kNoneId;
-      for (uint32_t t = 0; t < size; ++t) {
-        const uint32_t gs = gsize[base + t];
-        if (gs == 0 || t == i) continue;
-        if (gout[base + t] < (uint64_t)gs * (uint64_t)(size - gs)) continue;
-        const double w = (double)wcnt[base + t];
-        const double cs = (double)gs;
-        const double off = __fma_rn(offset, cs, __dmul_rn(res, cs));
-        if (w < off) continue;
-        const double surplus = __dsub_rn(w, off);
-        if (surplus >= best) {
-          best = surplus;
-          bt = t;
+    for (uint32_t i0 = 0; i0 < size; i0 += kWave) {
+      uint64_t cm = __ballot(i0 + lane < size && cand[base + i0 + lane] != 0);
+      while (cm) {
+        const uint32_t i = i0 + (uint32_t)(__ffsll((unsigned long long)cm) - 1);
+        cm &= cm - 1;
+        if (ScLoad(&single[base + i])) continue;
+        const uint32_t v = (uint32_t)members[base + i];
+        const int64_t r0 = g.off[v], r1 = g.off[(int64_t)v + 1];
+        for (int64_t p = r0 + lane; p < r1; p += kWave) {
+          const uint32_t u = g.nbr[p];
+          if (ids[u] == (uint32_t)ci) atomicAdd(&wcnt[base + ScLoad(&tmp[base + pos[u]])], 1u);
         }
+        __threadfence();
+        double best = -1.0;
+        uint32_t bt = kNoneId;
+        for (uint32_t t = lane; t < size; t += kWave) {
+          const uint32_t gs = ScLoad(&gsize[base + t]);
+          if (gs == 0 || t == i) continue;
+          if (ScLoad(&gout[base + t]) < (uint64_t)gs * (uint64_t)(size - gs)) continue;
+          const double w = (double)ScLoad(&wcnt[base + t]);
+          const double cs = (double)gs;
+          const double off = __fma_rn(offset, cs, __dmul_rn(res, cs));
+          if (w < off) continue;
+          const double surplus = __dsub_rn(w, off);
+          if (surplus >= best) {
+            best = surplus;
+            bt = t;
+          }
+        }
+        for (int sh = 32; sh > 0; sh >>= 1) {
+          const double ob = __shfl_xor(best, sh);
+          const uint32_t ot = __shfl_xor(bt, sh);
+          if (ot != kNoneId && (bt == kNoneId || ob > best || (ob == best && ot > bt))) {
+            best = ob;
+            bt = ot;
+          }
+        }
+        __threadfence();
+        for (int64_t p = r0 + lane; p < r1; p += kWave) {
+          const uint32_t u = g.nbr[p];
+          if (ids[u] == (uint32_t)ci) atomicExch(&wcnt[base + ScLoad(&tmp[base + pos[u]])], 0u);
+        }
+        if (bt != kNoneId && lane == 0) {
+          const uint32_t f = ScLoad(&gfirst[base + bt]);
+          atomicExch(&single[base + i], 1u);
+          atomicExch(&single[base + f], 1u);
+          atomicExch(&tmp[base + i], bt);
+          atomicExch(&gsize[base + i], 0u);
+          atomicAdd(&gsize[base + bt], 1u);
+          atomicAdd((unsigned long long*)&gout[base + bt], (unsigned long long)outdeg[v]);
+          atomicExch(&gfirst[base + bt], f < i ? f : i);
+        }
+        __threadfence();
       }
-      for (int64_t p = g.off[v]; p < g.off[(int64_t)v + 1]; ++p) {
-        const uint32_t u = g.nbr[p];
-        if (ids[u] == (uint32_t)ci) wcnt[base + tmp[base + pos[u]]] = 0;
-      }
-      if (bt == kNoneId) continue;
-      const uint32_t f = gfirst[base + bt];
-      single[base + i] = 1;
-      single[base + f] = 1;
-      tmp[base + i] = bt;
-      gsize[base + i] = 0;
-      gsize[base + bt] += 1;
-      gout[base + bt] += outdeg[v];
-      gfirst[base + bt] = f < i ? f : i;
     }
   }
 }
@@ -431,12 +471,16 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
     DevBuf<uint32_t> outdeg(al, n), tmp(al, n), gsize(al, n), gfirst(al, n), wcnt(al, n);
     DevBuf<uint64_t> gout(al, n);
-    DevBuf<uint8_t> single(al, n), cand(al, n);
-    hipLaunchKernelGGL(k_sc_outdeg, dim3(grid), dim3(256), 0, s, g, ids, outdeg.get());
+    DevBuf<uint32_t> single(al, n);
+    DevBuf<uint8_t> cand(al, n);
+    PCC_HIP_CHECK(hipMemsetAsync(outdeg.get(), 0, 4 * (size_t)n, s));
+    if (g.m > 0)
+      hipLaunchKernelGGL(k_sc_outdeg, dim3(ScGrid((g.m + kScItems - 1) / kScItems, 256, 8192)),
+                         dim3(256), 0, s, g, ids, outdeg.get());
     hipLaunchKernelGGL(k_sc_slow_init, dim3(grid), dim3(256), 0, s, n, keys.get(), start.get(),
                        cnt.get(), outdeg.get(), tmp.get(), gsize.get(), gout.get(), gfirst.get(),
                        single.get(), cand.get(), wcnt.get());
-    hipLaunchKernelGGL(k_sc_slow, dim3(ScGrid(n, 64, 1 << 16)), dim3(64), 0, s, n, g, ids,
+    hipLaunchKernelGGL(k_sc_slow, dim3(ScGrid(n, 256 / kWave, 1 << 16)), dim3(256), 0, s, n, g, ids,
                        cnt.get(), start.get(), keys.get(), pos.get(), outdeg.get(), offset, res,
                        tmp.get(), gsize.get(), gout.get(), gfirst.get(), single.get(), cand.get(),
                        wcnt.get());
