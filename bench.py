@@ -91,8 +91,9 @@ def main() -> int:
                     help="rmat22: run pcc_autotune on the workload first and report its choice")
     ap.add_argument("--hash-waves", type=int, default=None,
                     help="hash-bin workgroups per launch, in chip-fulls (0: one per vertex)")
-    ap.add_argument("--subclustering-method", type=int, default=1,
-                    help="subclustering: 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY")
+    ap.add_argument("--subclustering-method", type=int, default=None,
+                    help="subclustering / cluster-moves: 1 TRIANGLE, 2 CONNECTIVITY, "
+                         "3 SLOW_CONNECTIVITY (defaults 1 / 0)")
     ap.add_argument("--pull-fraction", type=int, default=None,
                     help="frontier pull threshold in 1/1024 of m + n (1024: push only)")
     args = ap.parse_args()
@@ -253,7 +254,7 @@ def bench_subclustering(args):
     import pcc_amd
 
     names = {1: "TRIANGLE", 2: "CONNECTIVITY", 3: "SLOW_CONNECTIVITY"}
-    method = args.subclustering_method
+    method = 1 if args.subclustering_method is None else args.subclustering_method
     csr = pcc_amd.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
     cfg = pcc_amd.make_config(resolution=0.05, move_method=pcc_amd.NBHR_MOVE, refine=1,
                               subclustering_method=method)
@@ -319,8 +320,12 @@ def bench_cluster_moves(args):
     import pcc_amd
 
     csr = pcc_amd.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
+    sub = args.subclustering_method or 0
+    names = {0: "", 1: ", TRIANGLE subclustering", 2: ", CONNECTIVITY subclustering",
+             3: ", SLOW_CONNECTIVITY subclustering"}
     cfg = pcc_amd.make_config(resolution=0.05, move_method=pcc_amd.NBHR_MOVE,
-                              clustering_moves_method=pcc_amd.DEFAULT_CLUSTER_MOVES)
+                              clustering_moves_method=pcc_amd.DEFAULT_CLUSTER_MOVES,
+                              subclustering_method=sub)
     g = pcc_amd.DeviceGraph.from_csr(csr)
     g.set_option(pcc_amd.OPT_TIMING, 1)
     for _ in range(args.warmup):
@@ -353,7 +358,8 @@ def bench_cluster_moves(args):
                "sample": "round 0 (vertex and whole-cluster phases) from singletons, same graph; "
                          "the whole-cluster phase of the oracle is single-threaded"}
     line = {
-        "metric": "whole-cluster-move Cluster time (DEFAULT_CLUSTER_MOVES, sync NBHR_MOVE)",
+        "metric": "whole-cluster-move Cluster time (DEFAULT_CLUSTER_MOVES, sync NBHR_MOVE"
+                  f"{names.get(sub, sub)})",
         "value": round(st.cluster_seconds * 1e3, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -361,7 +367,8 @@ def bench_cluster_moves(args):
         "data": "synthetic (seeded RMAT generated on the host, resident in HBM before timing)",
         "config": {"workload": f"RMAT scale {args.scale} edge factor {args.edge_factor:g}, "
                                "U[-1,1] weights, seed 2; sync NBHR_MOVE resolution 0.05, "
-                               "DEFAULT_CLUSTER_MOVES (one level, up to 20 rounds)",
+                               "DEFAULT_CLUSTER_MOVES (one level, up to 20 rounds)"
+                               f"{names.get(sub, sub)}",
                    "n": csr.n, "m_directed": csr.m, "parallelism": "single GPU"},
         "rounds": st.rounds, "cluster_moves": st.cluster_moves,
         "cluster_phase_ms": round(st.cluster_move_seconds * 1e3, 3),

@@ -324,8 +324,12 @@ __global__ __launch_bounds__(256) void k_sc_slow_init(int64_t n, const uint64_t*
 // ascending t, with leaving(t) >= |t| (size - |t|) and
 // weight - fma(offset, |t|, res |t|) >= 0 and maximal: each lane keeps the
 // largest surplus of its t's (the later t on ties) and the wave takes the
-// largest surplus, then the largest t.  The state lanes share is read with
-// agent-scope loads (no stale L1 lines) after a fence.
+// largest surplus, then the largest t.  Only candidates' groups can be
+// valid: a singleton group {t} is valid iff t has at least size - 1 leaving
+// edges (the candidate test), and a larger group grew from a valid singleton
+// by joins, so both loops run over the cluster's candidates, compacted into
+// clist in ascending order.  The state lanes share is read with agent-scope
+// loads (no stale L1 lines) after a fence.
 template <typename T>
 __device__ __forceinline__ T ScLoad(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -341,18 +345,26 @@ __global__ __launch_bounds__(256) void k_sc_slow(int64_t n, CsrView g, const uin
                                                  uint64_t* gout, uint32_t* gfirst,
                                                  uint32_t* single,
                                                  const uint8_t* __restrict__ cand,
-                                                 uint32_t* wcnt) {
+                                                 uint32_t* wcnt, uint32_t* clist) {
   const int lane = LaneId();
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
   for (int64_t ci = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; ci < n; ci += waves) {
     const uint32_t size = cnt[ci];
     if (size <= 1) continue;
     const int64_t base = start[ci];
+    uint32_t nc = 0;
     for (uint32_t i0 = 0; i0 < size; i0 += kWave) {
-      uint64_t cm = __ballot(i0 + lane < size && cand[base + i0 + lane] != 0);
-      while (cm) {
-        const uint32_t i = i0 + (uint32_t)(__ffsll((unsigned long long)cm) - 1);
-        cm &= cm - 1;
+      const bool f = i0 + lane < size && cand[base + i0 + lane] != 0;
+      const uint64_t m = __ballot(f);
+      if (f) clist[base + nc + (uint32_t)__popcll(m & below)] = i0 + lane;
+      nc += (uint32_t)__popcll(m);
+    }
+    if (nc < 2) continue;  // a lone candidate has only its own group
+    __threadfence();
+    for (uint32_t ck = 0; ck < nc; ++ck) {
+      {
+        const uint32_t i = ScLoad(&clist[base + ck]);
         if (ScLoad(&single[base + i])) continue;
         const uint32_t v = (uint32_t)members[base + i];
         const int64_t r0 = g.off[v], r1 = g.off[(int64_t)v + 1];
@@ -363,7 +375,8 @@ __global__ __launch_bounds__(256) void k_sc_slow(int64_t n, CsrView g, const uin
         __threadfence();
         double best = -1.0;
         uint32_t bt = kNoneId;
-        for (uint32_t t = lane; t < size; t += kWave) {
+        for (uint32_t kk = lane; kk < nc; kk += kWave) {
+          const uint32_t t = ScLoad(&clist[base + kk]);
           const uint32_t gs = ScLoad(&gsize[base + t]);
           if (gs == 0 || t == i) continue;
           if (ScLoad(&gout[base + t]) < (uint64_t)gs * (uint64_t)(size - gs)) continue;
@@ -471,7 +484,7 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     PCC_HIP_CHECK(hipMemsetAsync(span.get(), 0, 4 * (size_t)n, s));
     DevBuf<uint32_t> outdeg(al, n), tmp(al, n), gsize(al, n), gfirst(al, n), wcnt(al, n);
     DevBuf<uint64_t> gout(al, n);
-    DevBuf<uint32_t> single(al, n);
+    DevBuf<uint32_t> single(al, n), clist(al, n);
     DevBuf<uint8_t> cand(al, n);
     PCC_HIP_CHECK(hipMemsetAsync(outdeg.get(), 0, 4 * (size_t)n, s));
     if (g.m > 0)
@@ -483,7 +496,7 @@ int64_t SubclusterIds(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     hipLaunchKernelGGL(k_sc_slow, dim3(ScGrid(n, 256 / kWave, 1 << 16)), dim3(256), 0, s, n, g, ids,
                        cnt.get(), start.get(), keys.get(), pos.get(), outdeg.get(), offset, res,
                        tmp.get(), gsize.get(), gout.get(), gfirst.get(), single.get(), cand.get(),
-                       wcnt.get());
+                       wcnt.get(), clist.get());
     hipLaunchKernelGGL(k_sc_slow_labels, dim3(grid), dim3(256), 0, s, n, keys.get(), tmp.get(),
                        lab.get(), span.get());
     PCC_HIP_CHECK(hipGetLastError());
