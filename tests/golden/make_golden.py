@@ -40,6 +40,19 @@ CASES = {
     "rmat10_cluster_moves": (dict(kind="rmat", scale=10, ef=16, lo=-1.0, hi=1.0, seed=4),
                              "correlation_clusterer_config { resolution: 0.05 move_method: NBHR_MOVE "
                              "clustering_moves_method: DEFAULT_CLUSTER_MOVES num_iterations: 12 }", 0),
+    # RMAT, signed FP weights, TRIANGLE subclusters as the next levels'
+    # vertices, sync NBHR_MOVE, res 0.05, refine.
+    "rmat10_triangle_subclusters": (dict(kind="rmat", scale=10, ef=16, lo=-1.0, hi=1.0, seed=5),
+                                    "correlation_clusterer_config { resolution: 0.05 "
+                                    "move_method: NBHR_MOVE refine: true "
+                                    "subclustering_method: TRIANGLE_SUBCLUSTERING }", 0),
+    # SBM, SLOW_CONNECTIVITY subcluster moves inside DEFAULT_CLUSTER_MOVES
+    # rounds, sync, res 0.02.
+    "sbm1000_slow_subcluster_moves": (dict(kind="sbm", n=1000, blocks=10, p_in=0.2, p_out=0.01,
+                                           seed=6),
+                                      "correlation_clusterer_config { resolution: 0.02 "
+                                      "clustering_moves_method: DEFAULT_CLUSTER_MOVES "
+                                      "subclustering_method: SLOW_CONNECTIVITY_SUBCLUSTERING }", 0),
 }
 
 
