@@ -31,6 +31,18 @@
 //     sparse_additive_map atomics in the reference (PCCI.cc:518-536), whose
 //     order is nondeterministic there.  The oracle uses adjacency order, one
 //     of the reference's possible interleavings.
+//
+// Subclustering (PCC.cc:175-249, 436-474; PCCI.cc:784-1120):
+//   * TRIANGLE labels a component by its smallest member's position; the
+//     reference's workefficient_cc::CC labels come from a randomized
+//     decomposition, so only the partition is pinned, not the ids.
+//   * Between LOUVAIN levels the oracle composes the levels through the
+//     subcluster ids (the next level's vertices).  The reference composes
+//     cluster ids there (PCC.cc:550, 594-595) and so indexes the next level by
+//     the wrong vertex; its results on that step cannot pin this restatement,
+//     which is parity unpinned against the reference and pinned only by the
+//     composition test (tests/test_oracle_subclustering.py).
+//   * Async subcluster moves are not restated.
 
 #include "pcc_oracle.h"
 
