@@ -57,6 +57,12 @@ __device__ __forceinline__ int MagExpFor(double m) {
   (void)frexp(m * (1.0 + 0x1p-20), &e);  // m' < 2^e
   return e;
 }
+// Threshold of a vertex none of whose values is fine (GranExp(x) >= the
+// threshold its bound B gives, for every value x): all its per-cluster totals
+// are exact in every summation order, so the hashed kernels keep no per-slot
+// term records for it.  ThresholdFor(0) gives the same value.
+constexpr int kNoFineThr = -32768;
+
 __device__ __forceinline__ int32_t PackScale(int thr, int mexp) {
   return (int32_t)(((uint32_t)(thr & 0xFFFF) << 16) | (uint32_t)(mexp & 0xFFFF));
 }

@@ -325,6 +325,20 @@ __global__ __launch_bounds__(kSubThreads) void k_sub_scatter(const uint32_t* __r
   }
 }
 
+// The value EfficientBestMove adds for an edge of weight wf: (double)w -
+// offset on the low-degree path, the float w -= offset on the hub path
+// (PCCI.cc:466, 529).
+__device__ __forceinline__ double EdgeValue(float wf, bool hub, double offset) {
+  if (hub) {
+    float wh = wf;
+    wh -= offset;
+    return (double)wh;
+  }
+  double x = (double)wf;
+  x -= offset;
+  return x;
+}
+
 // gscale[v] = PackScale(ThresholdFor(B), MagExpFor(M)) for deg > 64, with B
 // and M the sum and the maximum over v's edges of |w| + |offset|; one wave
 // per such vertex.
@@ -370,7 +384,15 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
       }
       const double s = WaveSum((s0 + s1) + (s2 + s3));
       for (int sh = 32; sh > 0; sh >>= 1) mx = fmax(mx, __shfl_xor(mx, sh));
-      if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(ThresholdFor(s), MagExpFor(mx));
+      // A second pass over the (cached) row: with no fine value the vertex
+      // gets kNoFineThr.
+      int thr = ThresholdFor(s);
+      const bool hub = vd > kHubDegree;
+      bool fine = false;
+      for (int64_t q = LaneId(); q < vd; q += kWave)
+        fine = fine || GranExp(EdgeValue(w[vs + q], hub, offset)) < thr;
+      if (__ballot(fine) == 0) thr = kNoFineThr;
+      if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(thr, MagExpFor(mx));
     }
   }
 }
@@ -761,19 +783,6 @@ __device__ __forceinline__ int HashFind(KeyPtr keys, uint32_t mask, uint32_t c) 
   return -1;
 }
 
-// The value EfficientBestMove adds for an edge of weight wf: (double)w -
-// offset on the low-degree path, the float w -= offset on the hub path
-// (PCCI.cc:466, 529).
-__device__ __forceinline__ double EdgeValue(float wf, bool hub, double offset) {
-  if (hub) {
-    float wh = wf;
-    wh -= offset;
-    return (double)wh;
-  }
-  double x = (double)wf;
-  x -= offset;
-  return x;
-}
 
 // Block size of the pbbs reduce of curr_together_seq (hub path, PCCI.cc:545):
 // max(1024, 4 * ceil(sqrt(deg))).
@@ -894,6 +903,7 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
 
 // Counts a term of a slot (capped) and records how fine it is.
 __device__ __forceinline__ void RecordTerm(uint32_t* cnt, int slot, double x, int thr) {
+  if (thr == kNoFineThr) return;  // every total of the vertex is exact
   if ((cnt[slot] & kCountMask) < kCountCap) atomicAdd(&cnt[slot], 1u);
   const uint32_t fb = FineBits(GranExp(x), thr);
   if (fb) atomicOr(&cnt[slot], fb);
@@ -1217,7 +1227,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
       const uint32_t cv = cur.cv;
       const int own = HashFind(keys, mask, cv);
       double T = own >= 0 ? vals[own] : 0.0;
-      if (!kAsync) {
+      if (!kAsync && thr != kNoFineThr) {
         bool inexact = false;
         for (uint32_t k = threadIdx.x; k < n_occ; k += THREADS)
           inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
@@ -1486,7 +1496,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const int own = PartOf(cv, parts) == part ? HashFind(keys, mask, cv) : -1;
   double T = own >= 0 ? vals[own] : 0.0;
   const uint32_t n_occ = min(nocc, kList);
-  if (!kAsync) {
+  if (!kAsync && thr != kNoFineThr) {
     bool inexact = false;
     for (uint32_t k = threadIdx.x; k < n_occ; k += kHubThreads)
       inexact = inexact || SlotInexact(cnt[occ[k]], thr, mexp);
