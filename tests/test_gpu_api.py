@@ -136,6 +136,18 @@ def test_cli_clone(pcc, orc, tmp_path):
     bad = subprocess.run([CLI, "--clusterer_name=Nope", "--clusterer_config=", f"--input_graph={path}"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode != 0 and "UNIMPLEMENTED" in bad.stderr
+    # Subclustering through the text config (enum names as the proto spells them).
+    for method in ("TRIANGLE_SUBCLUSTERING", "CONNECTIVITY_SUBCLUSTERING",
+                   "SLOW_CONNECTIVITY_SUBCLUSTERING"):
+        text = ("correlation_clusterer_config { resolution: 0.01 move_method: NBHR_MOVE refine: true "
+                f"subclustering_method: {method} }}")
+        p = subprocess.run([CLI, "--clusterer_name=ParallelCorrelationClusterer",
+                            f"--clusterer_config={text}", f"--input_graph={path}", "--float_weighted",
+                            f"--output_clustering={out}"], capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        got = [[int(x) for x in l.split("\t") if x] for l in open(out).read().splitlines()]
+        r = orc.cluster(orc.G.from_csr(csr), orc.config_from(pcc.parse_config(text)))
+        assert got == pcc.output_indices_by_id(r.ids), method
 
 
 def test_autotune_keeps_results(pcc):
