@@ -103,3 +103,47 @@ def test_subclustering_rejects_batches(pcc):
         pcc.DeviceBatch.from_csrs(graphs).cluster_batch(
             pcc.make_config(resolution=0.05, subclustering_method=2))
     assert e.value.code == pcc.UNIMPLEMENTED
+
+
+def _csr_from_edges(pcc, n, edges, w=1.0):
+    adj = [[] for _ in range(n)]
+    for u, v in edges:
+        adj[u].append(v)
+        if u != v:
+            adj[v].append(u)
+    off = np.zeros(n + 1, np.int64)
+    nbrs = []
+    for u in range(n):
+        row = sorted(adj[u])
+        off[u + 1] = off[u] + len(row)
+        nbrs += row
+    nb = np.array(nbrs, np.uint32)
+    return pcc.Csr(n, off, nb, np.full(nb.shape[0], w, np.float32))
+
+
+EDGE_CASES = {
+    "one_vertex": (1, []),
+    "isolated": (6, [(1, 2)]),
+    "self_loops_only": (4, [(0, 0), (1, 1), (3, 3)]),
+    "two_triangles_bridge": (7, [(0, 1), (1, 2), (0, 2), (3, 4), (4, 5), (3, 5), (2, 3), (6, 6)]),
+    "star": (9, [(0, k) for k in range(1, 9)]),
+    "clique8": (8, [(a, b) for a in range(8) for b in range(a + 1, 8)]),
+}
+
+
+@pytest.mark.parametrize("method", [1, 2, 3])
+@pytest.mark.parametrize("name", sorted(EDGE_CASES))
+@pytest.mark.parametrize("moves", [0, 1])
+def test_subclustering_edge_cases(pcc, orc, name, method, moves):
+    n, edges = EDGE_CASES[name]
+    print(f"subclustering edge case {name} method={method} moves={moves}", flush=True)
+    csr = _csr_from_edges(pcc, n, edges)
+    # Without refinement at 0.1: with it, a run whose last moving level is
+    # level 0 returns singletons (PCC.cc:526-530, 589), as every
+    # DEFAULT_CLUSTER_MOVES run would.
+    for res, refine in ((0.1, 0), (0.6, 1)):
+        cfg = pcc.make_config(resolution=res, subclustering_method=method,
+                              clustering_moves_method=moves, refine=refine)
+        ids, st = pcc.DeviceGraph.from_csr(csr).cluster(cfg)
+        r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg))
+        assert np.array_equal(ids, r.ids), (name, method, moves, res)
