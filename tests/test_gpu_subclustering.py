@@ -147,3 +147,36 @@ def test_subclustering_edge_cases(pcc, orc, name, method, moves):
         ids, st = pcc.DeviceGraph.from_csr(csr).cluster(cfg)
         r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg))
         assert np.array_equal(ids, r.ids), (name, method, moves, res)
+
+
+# Async LOUVAIN rounds with subclustering between levels: the statistical
+# contract of test_gpu_async.py (the oracle's async runs over eight
+# processing orders).
+ASYNC_STRICT = [
+    (dict(kind="sbm", n=4000, blocks=40, seed=32),
+     dict(resolution=0.01, move_method=2, refine=0, subclustering_method=2), False),
+    (dict(kind="rmat", scale=15, ef=16, w_lo=-1.0, w_hi=1.0, seed=35),
+     dict(resolution=0.05, move_method=2, refine=1, subclustering_method=3), False),
+]
+ASYNC_RANGE = [
+    (dict(kind="rmat", scale=14, ef=16, seed=31),
+     dict(resolution=1.0, move_method=0, refine=1, permute=1, subclustering_method=2), True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ASYNC_STRICT)))
+def test_async_subclustering_within_half_percent(pcc, orc, case):
+    from test_gpu_async import _run
+
+    spec, kw, modularity = ASYNC_STRICT[case]
+    gpu, ref = _run(pcc, orc, spec, kw, modularity)
+    assert gpu >= ref.mean() - 0.005 * abs(ref.mean())
+
+
+@pytest.mark.parametrize("case", range(len(ASYNC_RANGE)))
+def test_async_subclustering_not_below_order_range(pcc, orc, case):
+    from test_gpu_async import _run
+
+    spec, kw, modularity = ASYNC_RANGE[case]
+    gpu, ref = _run(pcc, orc, spec, kw, modularity)
+    assert gpu >= ref.min() - 0.005 * abs(ref.min())
