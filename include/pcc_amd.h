@@ -236,6 +236,8 @@ typedef struct pcc_server_stats {
 } pcc_server_stats;
 
 void pcc_server_options_default(pcc_server_options* opt);
+/* A config with subclustering_method != 0 runs every request as a batch of
+ * its own graph (max_batch_graphs is taken as 1). */
 int pcc_server_create(const pcc_config* cfg, int objective, const pcc_server_options* opt,
                       pcc_server** out);
 /* Copies the graph (local ids, n >= 1) and queues it; returns its request id. */

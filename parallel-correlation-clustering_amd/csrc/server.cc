@@ -218,6 +218,8 @@ int pcc_server_create(const pcc_config* cfg, int objective, const pcc_server_opt
     pcc_server_options_default(&s->opt);
   if (s->opt.workers < 1) s->opt.workers = 1;
   if (s->opt.max_batch_graphs < 1) s->opt.max_batch_graphs = 1;
+  // Subclustering runs on one graph at a time: each request is its own batch.
+  if (cfg->subclustering_method != 0) s->opt.max_batch_graphs = 1;
   if (s->opt.max_batch_edges < 1) s->opt.max_batch_edges = (int64_t)1 << 31;
   if (hipGetDevice(&s->device) != hipSuccess)
     return pcc::SetError(PCC_INTERNAL, "no HIP device");

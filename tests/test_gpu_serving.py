@@ -53,6 +53,22 @@ def test_server_results_match_single_runs(pcc):
         assert obj == pytest.approx(sst.objective, rel=1e-12, abs=1e-9)
 
 
+def test_server_with_subclustering_runs_requests_alone(pcc):
+    cfg = pcc.make_config(move_method=2, refine=1, subclustering_method=2)
+    reqs = _requests(pcc, 6, 71)
+    srv = pcc.Server(cfg, workers=2, max_batch_graphs=8)
+    ids = [srv.submit(g, res) for g, res in reqs]
+    results = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st.completed == len(reqs) and st.failed == 0
+    assert st.mean_batch_graphs == 1.0
+    for (g, res), (got, obj, lat) in zip(reqs, results):
+        single, _ = pcc.DeviceGraph.from_csr(g).cluster(
+            pcc.make_config(resolution=res, move_method=2, refine=1, subclustering_method=2))
+        assert np.array_equal(got, single)
+
+
 def test_server_rejects_request_over_budget(pcc):
     cfg = pcc.make_config(move_method=0)
     srv = pcc.Server(cfg, workers=1, hbm_budget_bytes=4e6)
