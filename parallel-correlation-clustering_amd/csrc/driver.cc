@@ -57,8 +57,97 @@ DevBuf<double> NodeWeights(pcc_graph* h, int objective, const double* host_nw) {
 // graph (may be null).
 int ClusterImpl(pcc_graph* g, const pcc_config* cfg_in, int objective, const double* host_nw,
                 const double* resolutions, const uint32_t* host_init, uint32_t* out_ids,
+                double* objectives, double* modularities, pcc_stats* stats);
+
+// Subclustering numbers subclusters and lays out levels per graph, so a batch
+// with it is clustered graph by graph: each graph on a handle of its own with
+// the batch's options, through the single-graph path.  Statistics are summed
+// (levels: the maximum; objective and modularity: the first graph's, as for
+// batches).
+int ClusterEachGraph(pcc_graph* g, const pcc_config* cfg_in, int objective, const double* host_nw,
+                     const double* resolutions, const uint32_t* host_init, uint32_t* out_ids,
+                     double* objectives, double* modularities, pcc_stats* stats) {
+  const int G = g->num_graphs();
+  const std::vector<int64_t>& base = g->gbase;
+  const int64_t n = g->csr.n, m = g->csr.m;
+  std::vector<int64_t> off((size_t)n + 1);
+  std::vector<uint32_t> nbr((size_t)std::max<int64_t>(m, 1));
+  std::vector<float> w((size_t)std::max<int64_t>(m, 1));
+  PCC_HIP_CHECK(hipMemcpyAsync(off.data(), g->csr.off.get(), 8 * (size_t)(n + 1),
+                               hipMemcpyDeviceToHost, g->stream));
+  if (m > 0) {
+    PCC_HIP_CHECK(hipMemcpyAsync(nbr.data(), g->csr.nbr.get(), 4 * (size_t)m,
+                                 hipMemcpyDeviceToHost, g->stream));
+    PCC_HIP_CHECK(hipMemcpyAsync(w.data(), g->csr.w.get(), 4 * (size_t)m, hipMemcpyDeviceToHost,
+                                 g->stream));
+  }
+  PCC_HIP_CHECK(hipStreamSynchronize(g->stream));
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  for (int o = 0; o < G; ++o) {
+    const int64_t v0 = base[(size_t)o], v1 = base[(size_t)o + 1];
+    const int64_t e0 = off[(size_t)v0], e1 = off[(size_t)v1];
+    std::vector<int64_t> lo((size_t)(v1 - v0) + 1);
+    for (int64_t i = v0; i <= v1; ++i) lo[(size_t)(i - v0)] = off[(size_t)i] - e0;
+    std::vector<uint32_t> ln((size_t)std::max<int64_t>(e1 - e0, 1));
+    for (int64_t k = e0; k < e1; ++k) ln[(size_t)(k - e0)] = nbr[(size_t)k] - (uint32_t)v0;
+    pcc_graph* h = nullptr;
+    PCC_RETURN_IF_ERROR(NewHandle(&h));
+    std::unique_ptr<pcc_graph> hold(h);
+    h->timing = g->timing;
+    h->force_ordered = g->force_ordered;
+    h->async_subrounds = g->async_subrounds;
+    h->async_singleton_rule = g->async_singleton_rule;
+    h->cluster_subrounds = g->cluster_subrounds;
+    h->hub_part_edges = g->hub_part_edges;
+    h->pull_fraction = g->pull_fraction;
+    h->th = g->th;
+    PCC_RETURN_IF_ERROR(UploadCsr(h, v1 - v0, e1 - e0, lo.data(), ln.data(), w.data() + e0));
+    pcc_config c = *cfg_in;
+    if (resolutions) c.resolution = resolutions[o];
+    pcc_stats so;
+    double ob = 0.0, mo = 0.0;
+    PCC_RETURN_IF_ERROR(ClusterImpl(h, &c, objective, host_nw ? host_nw + v0 : nullptr, nullptr,
+                                    host_init ? host_init + v0 : nullptr, out_ids + v0, &ob, &mo,
+                                    &so));
+    if (objectives) objectives[o] = ob;
+    if (modularities) modularities[o] = mo;
+    if (!stats) continue;
+    if (o == 0) {
+      stats->objective = so.objective;
+      stats->modularity = so.modularity;
+    }
+    stats->levels = std::max(stats->levels, so.levels);
+    stats->rounds += so.rounds;
+    stats->edges_scanned += so.edges_scanned;
+    stats->frontier_vertices += so.frontier_vertices;
+    stats->num_clusters += so.num_clusters;
+    stats->fallback_vertices += so.fallback_vertices;
+    stats->cluster_seconds += so.cluster_seconds;
+    stats->bestmove_seconds += so.bestmove_seconds;
+    stats->bestmove_bytes += so.bestmove_bytes;
+    stats->contract_seconds += so.contract_seconds;
+    stats->apply_seconds += so.apply_seconds;
+    stats->bestmove_launches += so.bestmove_launches;
+    stats->ordered_seconds += so.ordered_seconds;
+    for (int b = 0; b < 8; ++b) {
+      stats->bin_seconds[b] += so.bin_seconds[b];
+      stats->bin_edges[b] += so.bin_edges[b];
+      stats->bin_vertices[b] += so.bin_vertices[b];
+      stats->bin_launches[b] += so.bin_launches[b];
+    }
+    stats->cluster_move_seconds += so.cluster_move_seconds;
+    stats->cluster_moves += so.cluster_moves;
+  }
+  return PCC_OK;
+}
+
+int ClusterImpl(pcc_graph* g, const pcc_config* cfg_in, int objective, const double* host_nw,
+                const double* resolutions, const uint32_t* host_init, uint32_t* out_ids,
                 double* objectives, double* modularities, pcc_stats* stats) {
   PCC_RETURN_IF_ERROR(CheckGraph(g));
+  if (cfg_in && cfg_in->subclustering_method != 0 && g->num_graphs() > 1)
+    return ClusterEachGraph(g, cfg_in, objective, host_nw, resolutions, host_init, out_ids,
+                            objectives, modularities, stats);
   if (!cfg_in || (!out_ids && g->csr.n > 0)) return SetError(PCC_INVALID_ARGUMENT, "null argument");
   if (objective != PCC_OBJECTIVE_CORRELATION && objective != PCC_OBJECTIVE_MODULARITY)
     return SetError(PCC_INVALID_ARGUMENT, "unknown objective");

@@ -97,12 +97,19 @@ def test_async_subcluster_moves_unimplemented(pcc):
     assert e.value.code == pcc.UNIMPLEMENTED
 
 
-def test_subclustering_rejects_batches(pcc):
-    graphs = [pcc.gen_sbm(200, 4, 0.3, 0.02, seed=s) for s in (1, 2)]
-    with pytest.raises(pcc.StatusError) as e:
-        pcc.DeviceBatch.from_csrs(graphs).cluster_batch(
-            pcc.make_config(resolution=0.05, subclustering_method=2))
-    assert e.value.code == pcc.UNIMPLEMENTED
+def test_subclustering_batches_run_graph_by_graph(pcc):
+    graphs = [pcc.gen_sbm(200 + 100 * s, 4, 0.3, 0.02, seed=s) for s in (1, 2, 3)]
+    graphs.append(pcc.gen_rmat(9, 8, w_lo=-1.0, w_hi=1.0, seed=4))
+    res = np.array([0.05, 0.02, 0.1, 0.05])
+    for method in (1, 2, 3):
+        cfg = pcc.make_config(subclustering_method=method, move_method=2, refine=1)
+        ids, obj, _, st = pcc.DeviceBatch.from_csrs(graphs).cluster_batch(cfg, res)
+        for k, csr in enumerate(graphs):
+            single, sst = pcc.DeviceGraph.from_csr(csr).cluster(
+                pcc.make_config(subclustering_method=method, move_method=2, refine=1,
+                                resolution=float(res[k])))
+            assert np.array_equal(ids[k], single), (method, k)
+            assert obj[k] == pytest.approx(sst.objective, rel=1e-12, abs=1e-9)
 
 
 def _csr_from_edges(pcc, n, edges, w=1.0):
