@@ -62,7 +62,7 @@ typedef struct pcc_config {
   int64_t louvain_num_iterations;        /* 8.1 LouvainConfig.num_iterations */
   int64_t louvain_num_inner_iterations;  /* 8.2 LouvainConfig.num_inner_iterations */
   int32_t clustering_moves_method;       /* 9, default PCC_LOUVAIN */
-  int32_t subclustering_method;          /* 10: 0 NONE, 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY (one graph) */
+  int32_t subclustering_method;          /* 10: 0 NONE, 1 TRIANGLE, 2 CONNECTIVITY, 3 SLOW_CONNECTIVITY (batches: graph by graph) */
   int32_t preclustering_method;          /* 11, only NONE_PRECLUSTERING (0) */
   int32_t move_method;                   /* 16, default PCC_NBHR_CLUSTER_MOVE */
   uint8_t refine;                        /* 13 */
