@@ -364,34 +364,40 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
       big &= big - 1;
       const int64_t vs = __shfl(st, l);
       const int64_t vd = __shfl(deg, l);
+      // Besides B and M, the smallest granularity of the values the kernels
+      // add (EdgeValue): with none below the threshold the vertex gets
+      // kNoFineThr.
+      const bool hub = vd > kHubDegree;
       double s0 = 0, s1 = 0, s2 = 0, s3 = 0, mx = 0;
+      int gmin = 1 << 20;
       int64_t p = LaneId();
       for (; p + 3 * kWave < vd; p += 4 * kWave) {
-        const double a0 = fabs((double)w[vs + p]) + ao;
-        const double a1 = fabs((double)w[vs + p + kWave]) + ao;
-        const double a2 = fabs((double)w[vs + p + 2 * kWave]) + ao;
-        const double a3 = fabs((double)w[vs + p + 3 * kWave]) + ao;
+        const float w0 = w[vs + p], w1 = w[vs + p + kWave];
+        const float w2 = w[vs + p + 2 * kWave], w3 = w[vs + p + 3 * kWave];
+        const double a0 = fabs((double)w0) + ao;
+        const double a1 = fabs((double)w1) + ao;
+        const double a2 = fabs((double)w2) + ao;
+        const double a3 = fabs((double)w3) + ao;
         s0 += a0;
         s1 += a1;
         s2 += a2;
         s3 += a3;
         mx = fmax(mx, fmax(fmax(a0, a1), fmax(a2, a3)));
+        gmin = min(gmin, min(min(GranExp(EdgeValue(w0, hub, offset)), GranExp(EdgeValue(w1, hub, offset))),
+                             min(GranExp(EdgeValue(w2, hub, offset)), GranExp(EdgeValue(w3, hub, offset)))));
       }
       for (; p < vd; p += kWave) {
-        const double a = fabs((double)w[vs + p]) + ao;
+        const float wq = w[vs + p];
+        const double a = fabs((double)wq) + ao;
         s0 += a;
         mx = fmax(mx, a);
+        gmin = min(gmin, GranExp(EdgeValue(wq, hub, offset)));
       }
       const double s = WaveSum((s0 + s1) + (s2 + s3));
       for (int sh = 32; sh > 0; sh >>= 1) mx = fmax(mx, __shfl_xor(mx, sh));
-      // A second pass over the (cached) row: with no fine value the vertex
-      // gets kNoFineThr.
+      gmin = WaveMinInt(gmin);
       int thr = ThresholdFor(s);
-      const bool hub = vd > kHubDegree;
-      bool fine = false;
-      for (int64_t q = LaneId(); q < vd; q += kWave)
-        fine = fine || GranExp(EdgeValue(w[vs + q], hub, offset)) < thr;
-      if (__ballot(fine) == 0) thr = kNoFineThr;
+      if (!(gmin < thr)) thr = kNoFineThr;
       if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(thr, MagExpFor(mx));
     }
   }
