@@ -411,22 +411,28 @@ __global__ __launch_bounds__(256) void k_gran_long(const int64_t* __restrict__ o
                                                    const uint32_t* __restrict__ longq,
                                                    const unsigned int* __restrict__ nlong,
                                                    double* __restrict__ acc_s,
-                                                   unsigned long long* __restrict__ acc_m) {
+                                                   unsigned long long* __restrict__ acc_m,
+                                                   int* __restrict__ acc_g) {
   __shared__ double sm[256 / kWave];
   const double ao = fabs(offset);
   ForLongChunks(longq, *nlong, off, [&](uint32_t q, uint32_t, int64_t e0, int64_t e1) {
     double s = 0, mx = 0;
+    int gmin = 1 << 20;
     for (int64_t p = e0 + threadIdx.x; p < e1; p += blockDim.x) {
-      const double a = fabs((double)w[p]) + ao;
+      const float wp = w[p];
+      const double a = fabs((double)wp) + ao;
       s += a;
       mx = fmax(mx, a);
+      gmin = min(gmin, GranExp(EdgeValue(wp, true, offset)));  // rows > kLongRow are hubs
     }
     s = BlockReduce(s, sm, [](double x, double y) { return x + y; });
     mx = BlockReduce(mx, sm, [](double x, double y) { return fmax(x, y); });
+    const double gm = BlockReduce((double)gmin, sm, [](double x, double y) { return fmin(x, y); });
     if (threadIdx.x == 0) {
       atomicAdd(&acc_s[q], s);
       // Non-negative doubles order like their bit patterns.
       atomicMax(&acc_m[q], (unsigned long long)__double_as_longlong(mx));
+      atomicMin(&acc_g[q], (int)gm);
     }
   });
 }
@@ -435,10 +441,13 @@ __global__ __launch_bounds__(256) void k_gran_long_final(const uint32_t* __restr
                                                          const unsigned int* __restrict__ nlong,
                                                          const double* __restrict__ acc_s,
                                                          const unsigned long long* __restrict__ acc_m,
+                                                         const int* __restrict__ acc_g,
                                                          int32_t* __restrict__ gscale) {
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < *nlong; q += gridDim.x * blockDim.x)
-    gscale[longq[q]] = PackScale(ThresholdFor(acc_s[q]),
-                                 MagExpFor(__longlong_as_double((long long)acc_m[q])));
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < *nlong; q += gridDim.x * blockDim.x) {
+    int thr = ThresholdFor(acc_s[q]);
+    if (!(acc_g[q] < thr)) thr = kNoFineThr;  // no fine value in the row
+    gscale[longq[q]] = PackScale(thr, MagExpFor(__longlong_as_double((long long)acc_m[q])));
+  }
 }
 
 // Bin-7 prefix sums of a list, one workgroup.
@@ -1802,6 +1811,8 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
   DevBuf<unsigned int> nlong(al, 1);
   DevBuf<double> acc_s(al, maxlong);
   DevBuf<unsigned long long> acc_m(al, maxlong);
+  DevBuf<int> acc_g(al, maxlong);
+  PCC_HIP_CHECK(hipMemsetAsync(acc_g.get(), 0x7F, sizeof(int) * (size_t)maxlong, s));  // > 2^20
   PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned int), s));
   PCC_HIP_CHECK(hipMemsetAsync(acc_s.get(), 0, sizeof(double) * (size_t)maxlong, s));
   PCC_HIP_CHECK(hipMemsetAsync(acc_m.get(), 0, sizeof(unsigned long long) * (size_t)maxlong, s));
@@ -1810,9 +1821,9 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
                      gscale, longq.get(), nlong.get());
   if (g.m > kLongRow) {
     hipLaunchKernelGGL(k_gran_long, dim3(1024), dim3(256), 0, s, g.off, g.w, offset, longq.get(),
-                       nlong.get(), acc_s.get(), acc_m.get());
+                       nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
     hipLaunchKernelGGL(k_gran_long_final, dim3(16), dim3(256), 0, s, longq.get(), nlong.get(),
-                       acc_s.get(), acc_m.get(), gscale);
+                       acc_s.get(), acc_m.get(), acc_g.get(), gscale);
   }
   PCC_HIP_CHECK(hipGetLastError());
 }

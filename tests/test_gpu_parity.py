@@ -408,3 +408,49 @@ def test_objective_long_rows(pcc, orc, weights):
     for gamma in (0.5, 1.0):
         assert dg.modularity(gamma, ids) == pytest.approx(orc.modularity(g, ids, gamma), rel=1e-12,
                                                          abs=1e-12)
+
+
+def _hub_graph(pcc, fine_weight):
+    """An SBM with three vertices joined to 20,000 others each (rows longer
+    than the 16384 entries of the per-wave bound pass); unit weights, except
+    one hub edge of weight fine_weight (both directions) when given."""
+    base = pcc.gen_sbm(60000, 600, 0.08, 0.0002, seed=51)
+    rng = np.random.default_rng(52)
+    n = base.n
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(base.offsets))
+    us, vs, ws = [src], [base.nbrs.astype(np.int64)], [base.weights]
+    for h in (7, 20011, 41017):
+        nb = rng.choice(n, size=20000, replace=False)
+        nb = nb[nb != h]
+        us += [np.full(nb.shape[0], h), nb]
+        vs += [nb, np.full(nb.shape[0], h)]
+        w = np.ones(nb.shape[0], np.float32)
+        if fine_weight is not None and h == 20011:
+            w[5] = np.float32(fine_weight)
+        ws += [w, w]
+    u = np.concatenate(us)
+    v = np.concatenate(vs)
+    w = np.concatenate(ws)
+    key = u * n + v
+    order = np.argsort(key, kind="stable")
+    key, u, v, w = key[order], u[order], v[order], w[order]
+    keep = np.concatenate([[True], key[1:] != key[:-1]])  # drop duplicate pairs
+    u, v, w = u[keep], v[keep], w[keep]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, u + 1, 1)
+    return pcc.Csr(n, np.cumsum(off), v.astype(np.uint32), w.astype(np.float32))
+
+
+@pytest.mark.parametrize("fine", [None, 2.0 ** -40])
+def test_long_rows_coarse_and_fine(pcc, orc, fine):
+    """Rows longer than 16384 entries take the chunked bound pass; with unit
+    weights they are tagged coarse (no per-slot records), with one 2^-40
+    weight the hub's slots are tracked again.  Both bit-exact."""
+    csr = _hub_graph(pcc, fine)
+    assert int(np.diff(csr.offsets).max()) > 16384
+    print(f"long rows fine={fine} n={csr.n} m={csr.m}", flush=True)
+    cfg = pcc.make_config(resolution=0.02, move_method=2, refine=1)
+    ids, st = _dev(pcc, csr).cluster(cfg)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg))
+    assert np.array_equal(ids, r.ids)
+    assert st.objective == pytest.approx(r.objective, rel=1e-9, abs=1e-6)
