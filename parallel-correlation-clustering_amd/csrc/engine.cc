@@ -424,8 +424,10 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     const int64_t bcap = HubBucketCap(h_->hub_part_edges);
     c->hb_bkt.Reset(al_, items * bcap);
     c->hb_bcur.Reset(al_, items);
+    c->hb_pfine.Reset(al_, items);
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_flag.get(), 0, sizeof(uint32_t) * (size_t)c7, s_));
     PCC_HIP_CHECK(hipMemsetAsync(c->hb_bcur.get(), 0, sizeof(uint32_t) * (size_t)items, s_));
+    PCC_HIP_CHECK(hipMemsetAsync(c->hb_pfine.get(), 0, sizeof(uint32_t) * (size_t)items, s_));
     c->hb.part_edges = h_->hub_part_edges;
     c->hb.part_off = c->hb_part.get();
     c->hb.ent_off = c->hb_ent.get();
@@ -437,6 +439,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.px2 = c->hb_px2.get();
     c->hb.bkt = c->hb_bkt.get();
     c->hb.bcur = c->hb_bcur.get();
+    c->hb.pfine = c->hb_pfine.get();
     c->hb.bcap = bcap;
     c->hb.item_of = c->hb_item_of.get();
     if (cfg_.move_method != PCC_ALL_MOVE) {

@@ -1363,6 +1363,9 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
   const int64_t e0 = part * per, e1 = min(deg, (part + 1) * per);
   const bool local = parts <= kGatherMaxBuckets;
   uint32_t* out = hb.cbuf + hb.ent_off[i];
+  // Sync rounds flag the partitions that receive a fine value (bin-7
+  // vertices take the hub path's float values, EdgeValue(w, true, offset)).
+  const int thr = kAsync ? kNoFineThr : ScaleThr(mp.gscale[v]);
   constexpr int U = kGatherU;
   for (int64_t q0 = e0; q0 < e1; q0 += (int64_t)kGatherThreads * U) {
     uint32_t c[U];
@@ -1407,6 +1410,12 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
     for (int k = 0; k < U; ++k)
       if (c[k] != kNoneId && pos[k] < (uint32_t)hb.bcap)
         hb.bkt[(p0 + b[k]) * hb.bcap + pos[k]] = make_uint2(c[k], __float_as_uint(w[k]));
+    if (thr != kNoFineThr) {
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (c[k] != kNoneId && GranExp(EdgeValue(w[k], true, mp.offset)) < thr)
+          atomicOr(&hb.pfine[p0 + b[k]], 1u);
+    }
   }
 }
 
@@ -1487,7 +1496,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const SlotList sl{occ, &nocc, kList};
   const bool hub = deg > kHubDegree;
   const int32_t scale = kAsync ? 0 : mp.gscale[v];
-    const int thr = ScaleThr(scale), mexp = ScaleMexp(scale);
+  // A partition that received no fine value keeps no per-slot records.
+  const int thr = (kAsync || hb.pfine[t] != 0) ? ScaleThr(scale) : kNoFineThr;
+  const int mexp = ScaleMexp(scale);
   const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
   const uint32_t nb = hb.bcur[t];
   // The partition's bucket when it did not overflow; otherwise the whole
@@ -1501,8 +1512,11 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
                                                     cnt, mask, parts, part,
                                                     reinterpret_cast<uint32_t*>(&stg), sl);
   const bool ovf = __syncthreads_or(overflow);
-  // Every thread has read bcur[t] before the barrier.
-  if (threadIdx.x == 0) hb.bcur[t] = 0;
+  // Every thread has read bcur[t] and pfine[t] before the barrier.
+  if (threadIdx.x == 0) {
+    hb.bcur[t] = 0;
+    hb.pfine[t] = 0;
+  }
   if (ovf) {
     if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
     return;

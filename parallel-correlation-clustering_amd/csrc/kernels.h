@@ -236,6 +236,10 @@ struct HubScratch {
   uint2* bkt = nullptr;
   uint32_t* bcur = nullptr;
   int64_t bcap = 0;
+  // pfine[t] != 0: partition item t received a fine value (sync rounds; see
+  // kNoFineThr).  Without one its totals are exact in every order.  The
+  // partition pass resets it with bcur[t].
+  uint32_t* pfine = nullptr;
 };
 
 // Bucket capacity per partition item: a partition's share of a vertex's
