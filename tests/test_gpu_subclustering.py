@@ -187,3 +187,21 @@ def test_async_subclustering_not_below_order_range(pcc, orc, case):
     spec, kw, modularity = ASYNC_RANGE[case]
     gpu, ref = _run(pcc, orc, spec, kw, modularity)
     assert gpu >= ref.min() - 0.005 * abs(ref.min())
+
+
+@pytest.mark.parametrize("method", [1, 2, 3])
+@pytest.mark.parametrize("moves", [0, 1])
+def test_refine_from_clustering_with_subclustering(pcc, orc, method, moves):
+    """RefineClusters from a given clustering (pcc_refine_clusters) with node
+    weights: the first level starts from the clustering, whose clusters are
+    split into subclusters before compression (LOUVAIN) or inside the rounds
+    (DEFAULT_CLUSTER_MOVES)."""
+    csr = GRAPHS["sbm_dense"](pcc)
+    rng = np.random.default_rng(60 + method)
+    init = (np.arange(csr.n) // 90).astype(np.uint32)  # blocks of 90 across the planted ones
+    nw = rng.integers(1, 4, size=csr.n).astype(np.float64)
+    cfg = pcc.make_config(resolution=0.05, move_method=2, subclustering_method=method,
+                          clustering_moves_method=moves)
+    out, st = pcc.DeviceGraph.from_csr(csr).refine(cfg, init, node_weights=nw)
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(cfg), False, nw, init)
+    assert np.array_equal(out, r.ids)
