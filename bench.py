@@ -508,6 +508,8 @@ def bench_batched(args):
         plain_s += time.perf_counter() - t1
         same += int(np.array_equal(ids, ids_b[i]) and sst.objective == obj_b[i])
         g.close()
+        if (i + 1) % 64 == 0 or i + 1 == plain_n:
+            print(f"# plain runs: {i + 1}/{plain_n}, {plain_s:.1f} s", file=sys.stderr, flush=True)
     plain_gps = plain_n / plain_s
     batch_gps = len(graphs) / batch_s
     line = {
