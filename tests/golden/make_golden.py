@@ -46,6 +46,19 @@ CASES = {
                                     "correlation_clusterer_config { resolution: 0.05 "
                                     "move_method: NBHR_MOVE refine: true "
                                     "subclustering_method: TRIANGLE_SUBCLUSTERING }", 0),
+    # SBM, CONNECTIVITY subclusters between levels, modularity, refine.
+    "sbm1500_connectivity_subclusters": (dict(kind="sbm", n=1500, blocks=15, p_in=0.1, p_out=0.005,
+                                              seed=7),
+                                         "correlation_clusterer_config { resolution: 1.0 "
+                                         "move_method: NBHR_MOVE refine: true "
+                                         "subclustering_method: CONNECTIVITY_SUBCLUSTERING }", 1),
+    # RMAT, signed FP weights, TRIANGLE subcluster moves inside
+    # DEFAULT_CLUSTER_MOVES rounds, NBHR_MOVE, 10 rounds.
+    "rmat10_triangle_subcluster_moves": (dict(kind="rmat", scale=10, ef=16, lo=-1.0, hi=1.0, seed=8),
+                                         "correlation_clusterer_config { resolution: 0.05 "
+                                         "move_method: NBHR_MOVE num_iterations: 10 "
+                                         "clustering_moves_method: DEFAULT_CLUSTER_MOVES "
+                                         "subclustering_method: TRIANGLE_SUBCLUSTERING }", 0),
     # SBM, SLOW_CONNECTIVITY subcluster moves inside DEFAULT_CLUSTER_MOVES
     # rounds, sync, res 0.02.
     "sbm1000_slow_subcluster_moves": (dict(kind="sbm", n=1000, blocks=10, p_in=0.2, p_out=0.01,
