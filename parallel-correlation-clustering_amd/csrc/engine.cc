@@ -667,11 +667,21 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
                         cfg_.edge_weight_offset, cfg_.resolution, s_, false,
                         h_->symmetric == 1);
     parent.Release();
-    cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, sub.get(), ws.flags.get(),
-                                    ws.new_flag.get(), ws.rc.get(), s_);
+    const int64_t sub_moves = RunGroupMoves(al_, g, st, c.mp, sub.get(), ws.flags.get(),
+                                            ws.new_flag.get(), ws.rc.get(), s_);
+    cluster_moves_ += sub_moves;
     sub.Release();
-    cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, pre.get(), ws.flags.get(),
-                                    ws.new_flag.get(), ws.rc.get(), s_);
+    if (sub_moves > 0) {
+      cluster_moves_ += RunGroupMoves(al_, g, st, c.mp, pre.get(), ws.flags.get(),
+                                      ws.new_flag.get(), ws.rc.get(), s_);
+    } else {
+      // Nothing moved: the earlier clusters are the current ones, whose set
+      // moves the whole-cluster kernels compute (class-1-together and class-2
+      // terms are empty, so both give the same values).
+      cluster_moves_ += RunClusterMoves(al_, g, st, c.mp, G, counts, c.layout, ws.list.get(),
+                                        false, 1, 0, ws.flags.get(), ws.new_flag.get(),
+                                        ws.rc.get(), s_);
+    }
     StopTimer(tm, 12);
   } else if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES) {
     // Whole-cluster moves after the vertex moves (PCC.cc:167-297); the
