@@ -162,8 +162,12 @@ int pcc_cluster(pcc_graph* g, const pcc_config* cfg, int objective,
  * (parallel-correlation-clusterer.cc:399-412, 631-639;
  * parallel-modularity-clusterer.cc:68-96).  inout_cluster_ids[n] holds the
  * initial clustering as per-vertex cluster indices in [0, n) (cluster i of the
- * reference's Clustering vector); it is overwritten with the result.
- * node_weights may be NULL (1.0, or degrees for modularity). */
+ * reference's Clustering vector, its members taken in ascending order); it is
+ * overwritten with the result.  node_weights may be NULL (1.0, or degrees for
+ * modularity).  Any finite weights are accepted: when they are not all
+ * integers (or their magnitudes sum to 2^53 or more) the cluster weights are
+ * updated in the reference's order instead of by atomics, so sync results stay
+ * exact; DEFAULT_CLUSTER_MOVES then returns PCC_INVALID_ARGUMENT. */
 int pcc_refine_clusters(pcc_graph* g, const pcc_config* cfg, int objective,
                         const double* node_weights, uint32_t* inout_cluster_ids,
                         pcc_stats* stats);
@@ -245,7 +249,9 @@ int pcc_server_submit(pcc_server* s, int64_t n, int64_t m, const int64_t* offset
                       const uint32_t* nbrs, const float* weights, double resolution,
                       uint64_t* out_request_id);
 /* Blocks until the request is done; fills its n cluster ids (local), objective
- * and submit-to-completion latency.  Returns the request's status. */
+ * and submit-to-completion latency.  Returns the request's status.  A request
+ * is claimed by its first wait: another wait on the same id (concurrent or
+ * later) returns PCC_NOT_FOUND. */
 int pcc_server_wait(pcc_server* s, uint64_t request_id, uint32_t* out_cluster_ids,
                     double* out_objective, double* out_latency_seconds);
 int pcc_server_get_stats(pcc_server* s, pcc_server_stats* out);
@@ -271,7 +277,8 @@ int pcc_best_moves(pcc_graph* g, const pcc_config* cfg, const uint32_t* cluster_
 /* One BestMovesForVertexSubset round (parallel-correlation-clusterer.cc:115-338)
  * in place: state updated as MoveNodesToCluster / AsyncMove leave it, frontier
  * (n bytes) replaced by the next frontier, whose size is returned in
- * out_frontier_size. */
+ * out_frontier_size.  Config checks and node-weight handling as in
+ * pcc_refine_clusters. */
 int pcc_round(pcc_graph* g, const pcc_config* cfg, uint32_t* cluster_ids,
               uint32_t* cluster_sizes, const double* node_weights,
               double* cluster_weights, uint8_t* frontier, int64_t* out_frontier_size);

@@ -598,6 +598,7 @@ int pcc_best_moves(pcc_graph* g, const pcc_config* cfg, const uint32_t* cluster_
         !out_moves || !out_moved)
       return SetError(PCC_INVALID_ARGUMENT, "null argument");
     PCC_RETURN_IF_ERROR(CheckIds(g, cluster_ids));
+    PCC_RETURN_IF_ERROR(CheckMovesConfig(g, *cfg));
     const int64_t n = g->csr.n;
     if (n == 0) return PCC_OK;
     hipStream_t s = g->stream;
@@ -698,6 +699,7 @@ int pcc_round(pcc_graph* g, const pcc_config* cfg, uint32_t* cluster_ids, uint32
     if (!cfg || !cluster_ids || !cluster_sizes || !node_weights || !cluster_weights || !frontier)
       return SetError(PCC_INVALID_ARGUMENT, "null argument");
     PCC_RETURN_IF_ERROR(CheckIds(g, cluster_ids));
+    PCC_RETURN_IF_ERROR(CheckMovesConfig(g, *cfg));
     const int64_t n = g->csr.n;
     if (n == 0) {
       if (out_frontier_size) *out_frontier_size = 0;
@@ -705,7 +707,13 @@ int pcc_round(pcc_graph* g, const pcc_config* cfg, uint32_t* cluster_ids, uint32
     }
     hipStream_t s = g->stream;
     DevState st(g, cluster_ids, cluster_sizes, node_weights, cluster_weights);
+    const bool exact = NodeWeightsExact(&g->alloc, st.nw.get(), n, s);
+    if (!exact && cfg->clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES)
+      return SetError(PCC_INVALID_ARGUMENT,
+                      "DEFAULT_CLUSTER_MOVES needs integer node weights whose magnitudes sum "
+                      "below 2^53");
     Engine e(g, *cfg, nullptr);
+    e.SetExactWeights(exact);
     Engine::LevelCtx c;
     Layout lay;
     e.PrepareLevel(g->csr.View(), st.View(), lay, &c);
@@ -745,7 +753,19 @@ int pcc_batch_from_csr(int64_t num_graphs, const int64_t* vertex_counts, const i
       base[(size_t)gi + 1] = base[(size_t)gi] + vertex_counts[gi];
     }
     const int64_t n = base.back();
+    if (n >= (int64_t)0xFFFFFFF0ll)
+      return SetError(PCC_INVALID_ARGUMENT, "vertex count must lie in [0, 2^32 - 16)");
     const int64_t m = offsets[n];
+    if (offsets[0] != 0 || m < 0)
+      return SetError(PCC_INVALID_ARGUMENT, "offsets must start at 0 and end at m >= 0");
+    if (m > 0 && !nbrs) return SetError(PCC_INVALID_ARGUMENT, "null CSR array");
+    // The graph boundaries must be non-decreasing within [0, m] before the
+    // rows are read (UploadCsr checks every offset afterwards).
+    for (int64_t gi = 0; gi < num_graphs; ++gi) {
+      const int64_t a = offsets[base[(size_t)gi]], b = offsets[base[(size_t)gi + 1]];
+      if (a < 0 || b < a || b > m)
+        return SetError(PCC_INVALID_ARGUMENT, "offsets must be non-decreasing within [0, m]");
+    }
     // Local neighbour ids -> ids of the disjoint union.
     std::vector<uint32_t> gl((size_t)std::max<int64_t>(m, 1));
     for (int64_t gi = 0; gi < num_graphs; ++gi) {

@@ -11,6 +11,8 @@
 
 #include "engine.h"
 
+#include "device_common.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -277,6 +279,70 @@ int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint
   h->symmetric = -1;
   h->finished = true;
   return PCC_OK;
+}
+
+int CheckMovesConfig(pcc_graph* h, const pcc_config& cfg) {
+  if (cfg.clustering_moves_method != PCC_LOUVAIN &&
+      cfg.clustering_moves_method != PCC_DEFAULT_CLUSTER_MOVES)
+    return SetError(PCC_UNIMPLEMENTED,
+                    "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
+  if (cfg.move_method < PCC_NBHR_CLUSTER_MOVE || cfg.move_method > PCC_NBHR_MOVE)
+    return SetError(PCC_INVALID_ARGUMENT, "unknown move_method");
+  const int subcl = cfg.subclustering_method;
+  if (subcl == 0) return PCC_OK;
+  if (subcl < 1 || subcl > 3) return SetError(PCC_INVALID_ARGUMENT, "unknown subclustering_method");
+  // The async in-round form charges a set spanning several clusters to its
+  // first member's cluster (PCCI.cc:413-418) and marks moved clusters by
+  // subcluster index (PCC.cc:217-219), past the end of the array.
+  if (cfg.clustering_moves_method != PCC_LOUVAIN && cfg.async)
+    return SetError(PCC_UNIMPLEMENTED,
+                    "async subcluster moves inside DEFAULT_CLUSTER_MOVES rounds are not on this path");
+  if (h->num_graphs() > 1)
+    return SetError(PCC_UNIMPLEMENTED, "subclustering of batched graphs is not on this path");
+  // Triangle scores merge adjacency lists (GBBS's intersect assumes them
+  // ascending); contracted levels are built ascending.
+  if (subcl == 1 && !RowsAscending(&h->alloc, h->csr.View(), h->stream))
+    return SetError(PCC_INVALID_ARGUMENT,
+                    "TRIANGLE_SUBCLUSTERING needs strictly ascending adjacency lists");
+  return PCC_OK;
+}
+
+namespace {
+// flag = 1 if a weight is not a finite integer; mag += sum |nw| (an
+// order-dependent double sum, compared against 2^53 with a margin).
+__global__ void k_weights_exact(const double* __restrict__ nw, int64_t n,
+                                unsigned long long* __restrict__ flag, double* __restrict__ mag) {
+  double local = 0.0;
+  bool bad = false;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double x = nw[v];
+    bad = bad || !(x == floor(x)) || isinf(x);
+    local += fabs(x);
+  }
+  if (bad) atomicOr(flag, 1ull);
+  for (int s = 32; s > 0; s >>= 1) local += __shfl_xor(local, s);
+  if ((threadIdx.x & (kWave - 1)) == 0 && local != 0.0) atomicAdd(mag, local);
+}
+}  // namespace
+
+bool NodeWeightsExact(DeviceAllocator* al, const double* nw, int64_t n, hipStream_t s) {
+  if (n <= 0) return true;
+  struct Acc {
+    unsigned long long flag;
+    double mag;
+  };
+  DevBuf<Acc> acc(al, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(acc.get(), 0, sizeof(Acc), s));
+  Acc* a = acc.get();
+  hipLaunchKernelGGL(k_weights_exact, dim3(Grid(n, 256, 2048)), dim3(256), 0, s, nw, n, &a->flag,
+                     &a->mag);
+  PCC_HIP_CHECK(hipGetLastError());
+  Acc h;
+  PCC_HIP_CHECK(hipMemcpyAsync(&h, a, sizeof(h), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  // Relative error of the double sum is far below 2^-20 for any n < 2^32.
+  return h.flag == 0 && h.mag * (1.0 + 0x1p-20) < 0x1p53;
 }
 
 void LaunchIota(uint32_t* p, int64_t n, hipStream_t s) {
@@ -651,13 +717,18 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   Timer ta = StartTimer();
   if (!async) {
     // MoveNodesToCluster (PCCI.cc:256-369).
-    LaunchApply(counts, c.layout, ws.list.get(), st, c.mp.new_id, ws.moves.get(), ws.flags.get(),
-                ws.new_flag.get(), ws.rc.get(), s_);
+    if (exact_nw_)
+      LaunchApply(counts, c.layout, ws.list.get(), st, c.mp.new_id, ws.moves.get(), ws.flags.get(),
+                  ws.new_flag.get(), ws.rc.get(), s_);
+    else
+      LaunchApplyOrdered(al_, counts, c.layout, ws.list.get(), st, (uint32_t)n, c.mp.new_id,
+                         ws.moves.get(), ws.flags.get(), ws.new_flag.get(), ws.rc.get(), s_);
     RunNewClusterAssign(al_, st, (uint32_t)n, c.mp.vgraph, c.mp.gbase, ws.new_flag.get(),
                         ws.flags.get(), ws.rc.get(), s_);
   }
   StopTimer(ta, 9);
-  if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES && cfg_.subclustering_method != 0) {
+  if (cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES && cfg_.subclustering_method != 0 &&
+      !async) {
     // Subcluster moves, then moves of the clusters as they were before them
     // (PCC.cc:175-297, sync): both are set moves of vertex groups.
     Timer tm = StartTimer();
@@ -808,23 +879,15 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
                     "Correlation clustering moves must be DEFAULT_CLUSTER_MOVES or LOUVAIN.");
   }
   // CompressSubclusters (PCC.cc:564-567) runs between LOUVAIN levels and the
-  // subcluster moves inside sync DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249);
-  // their async form is not on this path.
+  // subcluster moves inside sync DEFAULT_CLUSTER_MOVES rounds (PCC.cc:184-249).
+  PCC_RETURN_IF_ERROR(CheckMovesConfig(h_, cfg_));
   const int subcl = cfg_.subclustering_method;
-  if (subcl != 0) {
-    if (subcl < 1 || subcl > 3)
-      return SetError(PCC_INVALID_ARGUMENT, "unknown subclustering_method");
-    if (cfg_.clustering_moves_method != PCC_LOUVAIN && cfg_.async)
-      return SetError(PCC_UNIMPLEMENTED,
-                      "async subcluster moves inside DEFAULT_CLUSTER_MOVES rounds are not on this path");
-    if (h_->num_graphs() > 1)
-      return SetError(PCC_UNIMPLEMENTED, "subclustering of batched graphs is not on this path");
-    // Triangle scores merge adjacency lists (GBBS's intersect assumes them
-    // ascending); contracted levels are built ascending.
-    if (subcl == 1 && !RowsAscending(al_, g0, s_))
-      return SetError(PCC_INVALID_ARGUMENT,
-                      "TRIANGLE_SUBCLUSTERING needs strictly ascending adjacency lists");
-  }
+  exact_nw_ = NodeWeightsExact(al_, nw0.get(), n, s_);
+  // The set moves accumulate group and cluster weights with atomics.
+  if (!exact_nw_ && cfg_.clustering_moves_method == PCC_DEFAULT_CLUSTER_MOVES)
+    return SetError(PCC_INVALID_ARGUMENT,
+                    "DEFAULT_CLUSTER_MOVES needs integer node weights whose magnitudes sum "
+                    "below 2^53");
 
   hipEvent_t e0, e1;
   PCC_HIP_CHECK(hipEventCreate(&e0));
@@ -856,10 +919,12 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
                                    hipMemcpyDeviceToDevice, s_));
       L.lay.gres = Upload(al_, res0, s_);
     }
-    if (init)
+    if (init) {
       LaunchSetClustering(n, init, L.State(), s_);
-    else
+      if (!exact_nw_) OrderedClusterWeights(al_, n, L.State(), s_);
+    } else {
       LaunchSingletons(n, L.State(), s_);
+    }
   }
   const uint32_t* vgraph0 = batch ? h_->vgraph.get() : nullptr;
   DevBuf<uint32_t> cluster_ids(al_, n > 0 ? n : 1);
@@ -919,6 +984,7 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       next->sizes.Reset(al_, n2);
       next->K.Reset(al_, n2);
       LaunchSetClustering(n2, parent.get(), next->State(), s_);
+      if (!exact_nw_) OrderedClusterWeights(al_, n2, next->State(), s_);
       StopTimer(tc, 10);
       if (!cfg_.refine) {
         LaunchFlatten(n, cluster_ids.get(), L.map.get(), cluster_ids.get(), s_);
@@ -1035,6 +1101,7 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
         ResetClusteringBatch(al_, Li.g.n, Ln.g.n, Ln.ids.get(), nullptr, nullptr, nullptr,
                              nullptr, Li.State(), s_);
       }
+      if (!exact_nw_) OrderedClusterWeights(al_, Li.g.n, Li.State(), s_);
       IterateBestMoves(Li, &sel, num_inner);
     }
     for (int o = 0; o < G0; ++o) use_ids0[(size_t)o] = iter_o[(size_t)o] > 0 ? 1 : 0;

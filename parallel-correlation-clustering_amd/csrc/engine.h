@@ -178,6 +178,10 @@ class Engine {
                     int64_t counts[kBins], int64_t te[kBins]);
   void BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],
                    int64_t te[kBins]);
+  // Node weights on which FP64 atomics are exact (see NodeWeightsExact);
+  // otherwise the sync apply and the cluster-weight rebuilds take the ordered
+  // passes.  Run sets it from the level-0 weights.
+  void SetExactWeights(bool exact) { exact_nw_ = exact; }
 
  private:
   void ResetCounters(RoundCounters* rc, bool all);
@@ -200,14 +204,28 @@ class Engine {
   int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
   int64_t cluster_moves_ = 0;
   double bestmove_bytes_ = 0;
+  bool exact_nw_ = true;
   // With refinement every level's graph is visited twice; its best-move
   // context (bins, thresholds, hub lists and scratch) depends only on the
   // graph and its node weights, so the second visit reuses it.
   std::vector<std::pair<const Level*, std::unique_ptr<LevelCtx>>> ctx_cache_;
 };
 
-// Resolution used for the objective: for modularity runs res' = gamma / m_g
-// (ComputeModularityConfig, parallel/parallel-graph-utils.cc:141-153).
+// The config checks every entry point that runs best moves shares
+// (RefineClusters_subroutine, parallel-correlation-clusterer.cc:487-500, and
+// the proto's enums, config.proto:115-160): clustering_moves_method LOUVAIN or
+// DEFAULT_CLUSTER_MOVES (UNIMPLEMENTED otherwise, as the reference), a known
+// move_method and subclustering_method (INVALID_ARGUMENT), no async subcluster
+// moves inside DEFAULT_CLUSTER_MOVES rounds and no subclustering of batches
+// (UNIMPLEMENTED), strictly ascending rows for TRIANGLE (INVALID_ARGUMENT).
+int CheckMovesConfig(pcc_graph* h, const pcc_config& cfg);
+
+// Node weights on which FP64 atomics are exact: every weight an integer and
+// the sum of their magnitudes below 2^53, so every partial sum of cluster
+// weights is exactly representable and its order cannot change a bit.
+// Reads back one flag (synchronizes the stream).
+bool NodeWeightsExact(DeviceAllocator* al, const double* nw, int64_t n, hipStream_t s);
+
 void LaunchIota(uint32_t* p, int64_t n, hipStream_t s);
 
 }  // namespace pcc

@@ -455,6 +455,171 @@ void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uin
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------- ordered weight updates --
+// Node weights that are not integers (or whose sums reach 2^53) make FP64
+// atomics order-dependent.  These passes update the cluster weights in the
+// reference's order instead: movers are sorted stably by cluster (keys
+// cluster << vb | vertex keep ascending vertex order within a cluster), and
+// one thread folds each cluster's run left to right.
+
+namespace {
+
+int BitsFor(uint64_t x) {  // bits to hold every value in [0, x]
+  int b = 1;
+  while (b < 64 && (x >> b) != 0) ++b;
+  return b;
+}
+
+__global__ __launch_bounds__(256) void k_mark_movers(Bins bins, const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ ids,
+                                                     const uint32_t* __restrict__ moves,
+                                                     uint8_t* __restrict__ flag) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int b = 0; b < kBins; ++b)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bins.count[b];
+         i += stride) {
+      const uint32_t v = list[bins.base[b] + i];
+      const uint32_t t = moves[v];
+      flag[v] = (t != kNoneId && t != ids[v]) ? 1 : 0;
+    }
+}
+
+// Movers in ascending vertex order, keyed by old and by new cluster.
+__global__ __launch_bounds__(256) void k_emit_movers(int64_t n, const uint8_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ rank,
+                                                     StateView st, const uint32_t* __restrict__ moves,
+                                                     int vb, uint64_t* __restrict__ kold,
+                                                     double* __restrict__ wold,
+                                                     uint64_t* __restrict__ knew,
+                                                     double* __restrict__ wnew) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (!flag[v]) continue;
+    const uint32_t i = rank[v];
+    kold[i] = ((uint64_t)st.ids[v] << vb) | (uint64_t)v;
+    knew[i] = ((uint64_t)moves[v] << vb) | (uint64_t)v;
+    wold[i] = st.nw[v];
+    wnew[i] = st.nw[v];
+  }
+}
+
+// Every vertex keyed by its cluster (OrderedClusterWeights).
+__global__ __launch_bounds__(256) void k_emit_members(int64_t n, StateView st, int vb,
+                                                      uint64_t* __restrict__ keys,
+                                                      double* __restrict__ w) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    keys[v] = ((uint64_t)st.ids[v] << vb) | (uint64_t)v;
+    w[v] = st.nw[v];
+    st.K[v] = 0.0;
+  }
+}
+
+// mode 0: a run of old cluster c loses its members (sizes -= count,
+// K[c] -= w one after another, PCCI.cc:283-292); mode 1: a run of new cluster
+// c != new_id gains them (sizes += count, ids = c, K[c] += w, PCCI.cc:309-320),
+// a run of new_id flags its members for the empty-id assignment; mode 2: K[c]
+// = 0 + w + w + ... over all members (SetClustering, PCCI.cc:68-75).
+__global__ __launch_bounds__(256) void k_fold_runs(int64_t cnt, const uint64_t* __restrict__ keys,
+                                                   const double* __restrict__ w, int vb, int mode,
+                                                   StateView st, uint32_t new_id,
+                                                   uint8_t* __restrict__ modified,
+                                                   uint8_t* __restrict__ new_flag,
+                                                   RoundCounters* __restrict__ rc) {
+  const uint64_t vmask = vb >= 64 ? ~0ull : ((1ull << vb) - 1ull);
+  unsigned long long news = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = (uint32_t)(keys[i] >> vb);
+    if (mode == 1 && c == new_id) {
+      new_flag[keys[i] & vmask] = 1;
+      ++news;
+      continue;
+    }
+    if (i > 0 && (uint32_t)(keys[i - 1] >> vb) == c) continue;  // not the run's head
+    int64_t j = i;
+    if (mode == 0) {
+      double k = st.K[c];
+      for (; j < cnt && (uint32_t)(keys[j] >> vb) == c; ++j) k -= w[j];
+      st.K[c] = k;
+      st.sizes[c] -= (uint32_t)(j - i);
+      modified[c] = 1;
+    } else if (mode == 1) {
+      double k = st.K[c];
+      for (; j < cnt && (uint32_t)(keys[j] >> vb) == c; ++j) {
+        st.ids[keys[j] & vmask] = c;
+        k += w[j];
+      }
+      st.K[c] = k;
+      st.sizes[c] += (uint32_t)(j - i);
+      modified[c] = 1;
+    } else {
+      double k = 0.0;
+      for (; j < cnt && (uint32_t)(keys[j] >> vb) == c; ++j) k += w[j];
+      st.K[c] = k;
+    }
+  }
+  if (mode == 1) {
+    for (int s = 32; s > 0; s >>= 1) news += __shfl_xor(news, s);
+    if (LaneId() == 0 && news) atomicAdd(&rc->new_count, news);
+  }
+}
+
+}  // namespace
+
+void LaunchApplyOrdered(DeviceAllocator* al, const int64_t counts[kBins], const BinLayout& layout,
+                        const uint32_t* list, const StateView& st, uint32_t n, uint32_t new_id,
+                        const uint32_t* moves, uint8_t* modified, uint8_t* new_flag,
+                        RoundCounters* rc, hipStream_t s) {
+  int64_t total;
+  const Bins b = MakeBins(counts, layout, &total);
+  if (total == 0 || n == 0) return;
+  DevBuf<uint8_t> flag(al, n);
+  DevBuf<uint32_t> rank(al, n), dcount(al, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(flag.get(), 0, n, s));
+  hipLaunchKernelGGL(k_mark_movers, dim3(Grid(total, 256, 4096)), dim3(256), 0, s, b, list, st.ids,
+                     moves, flag.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  ExclusiveScanU8toU32(al, flag.get(), rank.get(), n, dcount.get(), s);
+  uint32_t movers = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&movers, dcount.get(), 4, hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  if (movers == 0) return;
+  const int vb = BitsFor((uint64_t)n - 1);
+  const int cb = BitsFor((uint64_t)new_id > (uint64_t)n ? (uint64_t)new_id : (uint64_t)n);
+  DevBuf<uint64_t> kold(al, movers), knew(al, movers);
+  DevBuf<double> wold(al, movers), wnew(al, movers);
+  hipLaunchKernelGGL(k_emit_movers, dim3(Grid(n, 256, 4096)), dim3(256), 0, s, (int64_t)n,
+                     flag.get(), rank.get(), st, moves, vb, kold.get(), wold.get(), knew.get(),
+                     wnew.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  RadixSortPairsF64(al, kold.get(), wold.get(), movers, vb, vb + cb, s);
+  RadixSortPairsF64(al, knew.get(), wnew.get(), movers, vb, vb + cb, s);
+  const int grid = Grid(movers, 256, 4096);
+  // Every loss is applied before any gain, as the reference's two passes.
+  hipLaunchKernelGGL(k_fold_runs, dim3(grid), dim3(256), 0, s, (int64_t)movers, kold.get(),
+                     wold.get(), vb, 0, st, new_id, modified, new_flag, rc);
+  hipLaunchKernelGGL(k_fold_runs, dim3(grid), dim3(256), 0, s, (int64_t)movers, knew.get(),
+                     wnew.get(), vb, 1, st, new_id, modified, new_flag, rc);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void OrderedClusterWeights(DeviceAllocator* al, int64_t n, const StateView& st, hipStream_t s) {
+  if (n <= 0) return;
+  const int vb = BitsFor((uint64_t)n - 1);
+  const int cb = BitsFor((uint64_t)n);
+  DevBuf<uint64_t> keys(al, n);
+  DevBuf<double> w(al, n);
+  hipLaunchKernelGGL(k_emit_members, dim3(Grid(n, 256, 4096)), dim3(256), 0, s, n, st, vb,
+                     keys.get(), w.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  RadixSortPairsF64(al, keys.get(), w.get(), n, vb, vb + cb, s);
+  hipLaunchKernelGGL(k_fold_runs, dim3(Grid(n, 256, 4096)), dim3(256), 0, s, n, keys.get(),
+                     w.get(), vb, 2, st, 0u, (uint8_t*)nullptr, (uint8_t*)nullptr,
+                     (RoundCounters*)nullptr);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 __global__ __launch_bounds__(256) void k_graph_flags(int64_t n, const uint8_t* __restrict__ next,
                                                      const uint32_t* __restrict__ vgraph,
                                                      uint8_t* __restrict__ flags) {

@@ -322,6 +322,20 @@ void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uin
                  const StateView& st, uint32_t new_id, const uint32_t* moves, uint8_t* modified,
                  uint8_t* new_flag, RoundCounters* rc, hipStream_t s);
 
+// The same apply with the cluster-weight arithmetic in the reference's order
+// (losses of each old cluster, then gains of each new one, members in
+// ascending vertex order), for node weights on which FP64 atomics are not
+// exact.  Synchronizes the stream once (the mover count).
+void LaunchApplyOrdered(DeviceAllocator* al, const int64_t counts[kBins], const BinLayout& layout,
+                        const uint32_t* list, const StateView& st, uint32_t n, uint32_t new_id,
+                        const uint32_t* moves, uint8_t* modified, uint8_t* new_flag,
+                        RoundCounters* rc, hipStream_t s);
+
+// K[c] = left fold of nw over the members of c in ascending vertex order,
+// from 0 (SetClustering, parallel-correlation-clusterer-internal.cc:60-77);
+// sizes are left as they are.
+void OrderedClusterWeights(DeviceAllocator* al, int64_t n, const StateView& st, hipStream_t s);
+
 // Empty-id reuse for vertices forming new clusters, per graph of a batch
 // (vgraph / gbase null for one graph); gated on rc->new_count.
 void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,

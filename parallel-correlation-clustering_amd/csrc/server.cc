@@ -279,9 +279,11 @@ int pcc_server_wait(pcc_server* s, uint64_t request_id, uint32_t* out_cluster_id
     std::unique_lock<std::mutex> lock(s->mu);
     auto it = s->requests.find(request_id);
     if (it == s->requests.end()) return pcc::SetError(PCC_NOT_FOUND, "unknown request id");
+    // The first waiter claims the request: later waits on the same id get
+    // NOT_FOUND instead of sharing (and erasing) the entry.
     r = it->second;
-    s->cv_done.wait(lock, [&] { return r->done; });
     s->requests.erase(it);
+    s->cv_done.wait(lock, [&] { return r->done; });
   }
   if (out_latency_seconds) *out_latency_seconds = r->latency;
   if (r->status != PCC_OK) return pcc::SetError(r->status, r->error);
