@@ -15,7 +15,9 @@ from typing import Optional, Tuple
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "libpcc_oracle.so")
+# PCC_ORACLE_LIB loads another build of the same source (a build with a
+# planted mistake, to check that the oracle's tests catch it).
+LIB_PATH = os.environ.get("PCC_ORACLE_LIB") or os.path.join(_HERE, "build", "libpcc_oracle.so")
 NO_MOVE = 0xFFFFFFFF
 
 
@@ -85,7 +87,10 @@ def lib() -> ctypes.CDLL:
         P = ctypes.c_void_p
         l.oracle_set_threads.argtypes = [ctypes.c_int]
         l.oracle_get_threads.restype = ctypes.c_int
+        l.oracle_phase_seconds.argtypes = [P, ctypes.c_int]
         l.oracle_best_move.argtypes = [P, P, P, ctypes.c_uint32, P, P]
+        l.oracle_best_move_candidates.argtypes = [P, P, P, ctypes.c_uint32, P, P, ctypes.c_int64, P]
+        l.oracle_best_move_candidates.restype = ctypes.c_int64
         l.oracle_best_move_set.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         l.oracle_sync_decisions.argtypes = [P, P, P, P, P, P, P, P]
         l.oracle_apply_moves.argtypes = [ctypes.c_int64, P, P, P]
@@ -121,6 +126,16 @@ def _p(a):
 
 def set_threads(t: int) -> None:
     lib().oracle_set_threads(int(t))
+
+
+PHASES = ("best_move", "apply", "frontier", "contraction", "refine_relabel")
+
+
+def phase_seconds(reset: bool = True) -> dict:
+    """Wall seconds per phase of the oracle's rounds and pipeline runs."""
+    out = np.zeros(5)
+    lib().oracle_phase_seconds(_p(out), int(reset))
+    return dict(zip(PHASES, out.tolist()))
 
 
 class G:
@@ -190,6 +205,20 @@ def best_move(g: G, s: S, cfg: OConfig, v: int) -> Tuple[int, float]:
     lib().oracle_best_move(ctypes.byref(g.view), ctypes.byref(s.view), ctypes.byref(cfg), v,
                            ctypes.byref(t), ctypes.byref(d))
     return int(t.value), float(d.value)
+
+
+def best_move_candidates(g: G, s: S, cfg: OConfig, v: int):
+    """Every candidate EfficientBestMove evaluates for v: ({cluster: delta},
+    change), change being the "new cluster" value."""
+    cap = int(g.offsets[v + 1] - g.offsets[v]) + 1
+    cl = np.zeros(cap, np.uint32)
+    dl = np.zeros(cap, np.float64)
+    ch = ctypes.c_double()
+    cnt = lib().oracle_best_move_candidates(ctypes.byref(g.view), ctypes.byref(s.view),
+                                            ctypes.byref(cfg), v, _p(cl), _p(dl), cap,
+                                            ctypes.byref(ch))
+    assert cnt <= cap
+    return {int(c): float(d) for c, d in zip(cl[:cnt], dl[:cnt])}, float(ch.value)
 
 
 def best_move_set(g: G, s: S, cfg: OConfig, members) -> Tuple[int, float]:

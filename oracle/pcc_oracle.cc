@@ -56,6 +56,7 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#include <parallel/algorithm>
 #endif
 
 namespace {
@@ -65,6 +66,35 @@ constexpr int64_t kHubDegree = 1000;  // PCCI.cc:456
 
 int g_threads = 1;
 int64_t g_edges_scanned = 0;  // sum over rounds of sum_{v in frontier} deg(v)
+
+// Wall seconds per phase of the whole-pipeline runs: 0 best-move map, 1 sync
+// apply, 2 frontier expansion, 3 contraction, 4 refinement relabel.
+double g_phase[5] = {0, 0, 0, 0, 0};
+double Now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+struct PhaseTimer {
+  int k;
+  double t0;
+  explicit PhaseTimer(int kk) : k(kk), t0(Now()) {}
+  ~PhaseTimer() { g_phase[k] += Now() - t0; }
+};
+
+// Stable sort on g_threads threads (libstdc++'s parallel multiway mergesort).
+// A stable sort has one possible output, so the thread count never changes a
+// result.
+template <class It, class Cmp>
+void StableSort(It b, It e, Cmp cmp) {
+#ifdef _OPENMP
+  if (g_threads > 1 && e - b > 65536) {
+    __gnu_parallel::stable_sort(b, e, cmp,
+                                __gnu_parallel::multiway_mergesort_tag((unsigned)g_threads));
+    return;
+  }
+#endif
+  std::stable_sort(b, e, cmp);
+}
 
 // pbbslib reduce: block size and left folds (see the header comment).
 template <class T, class F>
@@ -133,8 +163,17 @@ thread_local ClusterMap t_map;
 // ClusteringHelper::EfficientBestMove, PCCI.cc:448-583.  flat: every degree
 // takes the adjacency-order path, as SeqClusteringHelper::EfficientBestMove
 // (correlation-clusterer-internal.cc:116-174) does.
+// cands (may be null) receives every candidate (cluster, delta) the
+// reference evaluates, in evaluation order, and change the "new cluster"
+// value.
+struct Candidates {
+  std::vector<std::pair<uint32_t, double>> list;
+  double change = 0.0;
+};
+
 void BestMove(const oracle_graph& g, const oracle_state& s, double res,
-              double offset, uint32_t v, uint32_t* out_t, double* out_d, bool flat = false) {
+              double offset, uint32_t v, uint32_t* out_t, double* out_d, bool flat = false,
+              Candidates* cands = nullptr) {
   const int64_t b = g.offsets[v], e = g.offsets[(int64_t)v + 1];
   const int64_t deg = e - b;
   const uint32_t n = (uint32_t)g.n;
@@ -164,10 +203,12 @@ void BestMove(const oracle_graph& g, const oracle_state& s, double res,
     bool has = false;
     uint32_t best = 0;
     double best_delta = change;
+    if (cands) cands->change = change;
     for (uint32_t slot : after.used) {
       const uint32_t c = after.keys[slot];
       max_edges = moving_nodes_weight * s.cluster_weights[c];
       const double overall = change + NetWeight(after.vals[slot], max_edges, res);
+      if (cands) cands->list.emplace_back(c, overall);
       if (overall > best_delta || (overall == best_delta && has && c < best)) {
         best = c;
         best_delta = overall;
@@ -209,6 +250,10 @@ void BestMove(const oracle_graph& g, const oracle_state& s, double res,
     const double me = moving_nodes_weight * s.cluster_weights[c];
     data = NetWeight(data, me, res);  // data -= res * max_edges
     seq.emplace_back(c, change + data);
+  }
+  if (cands) {
+    cands->change = change;
+    cands->list = seq;
   }
   auto f_max = [](const std::pair<uint32_t, double>& a,
                   const std::pair<uint32_t, double>& bb) {
@@ -480,9 +525,12 @@ uint64_t SplitMix64(uint64_t z) {
 int64_t ExpandFrontier(const oracle_graph& g, const std::vector<uint8_t>& seed,
                        uint8_t* frontier) {
   std::memset(frontier, 0, (size_t)g.n);
+  // Every write stores 1, so the set does not depend on the thread count.
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
   for (int64_t v = 0; v < g.n; ++v) {
     if (!seed[(size_t)v]) continue;
-    for (int64_t p = g.offsets[v]; p < g.offsets[v + 1]; ++p) frontier[g.nbrs[p]] = 1;
+    for (int64_t p = g.offsets[v]; p < g.offsets[v + 1]; ++p)
+      __atomic_store_n(&frontier[g.nbrs[p]], (uint8_t)1, __ATOMIC_RELAXED);
   }
   int64_t cnt = 0;
   for (int64_t v = 0; v < g.n; ++v) cnt += frontier[v];
@@ -555,6 +603,7 @@ struct OwnedCsr {
 // restatement lists them by ascending id.
 void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
               OwnedCsr* out, bool seq = false) {
+  PhaseTimer timer(3);
   uint32_t max_id = 0;
   for (int64_t i = 0; i < g.n; ++i) max_id = std::max(max_id, ids[i]);
   const int64_t n2 = 1 + (int64_t)max_id;  // PCCI.cc:713-717
@@ -562,16 +611,28 @@ void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
     uint32_t cu, cv;
     float w;
   };
-  std::vector<E> edges;
-  edges.reserve((size_t)g.m);
+  // PGU.cc:93-103: keep (u, v) iff v <= u or the clusters differ, in CSR
+  // order (counted per vertex, then written at each vertex's prefix).
+  auto keep = [&](int64_t u, int64_t p) {
+    const uint32_t v = g.nbrs[p];
+    return (int64_t)v <= u || ids[v] != ids[u];
+  };
+  std::vector<int64_t> kept_at((size_t)g.n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
   for (int64_t u = 0; u < g.n; ++u) {
-    for (int64_t p = g.offsets[u]; p < g.offsets[u + 1]; ++p) {
-      const uint32_t v = g.nbrs[p];
-      // PGU.cc:93-103: keep iff v <= u or the clusters differ.
-      if ((int64_t)v <= u || ids[v] != ids[u]) edges.push_back(E{ids[u], ids[v], g.weights[p]});
-    }
+    int64_t c = 0;
+    for (int64_t p = g.offsets[u]; p < g.offsets[u + 1]; ++p) c += keep(u, p) ? 1 : 0;
+    kept_at[(size_t)u + 1] = c;
   }
-  std::stable_sort(edges.begin(), edges.end(), [](const E& a, const E& b) {
+  for (int64_t u = 0; u < g.n; ++u) kept_at[(size_t)u + 1] += kept_at[(size_t)u];
+  std::vector<E> edges((size_t)kept_at[(size_t)g.n]);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
+  for (int64_t u = 0; u < g.n; ++u) {
+    int64_t q = kept_at[(size_t)u];
+    for (int64_t p = g.offsets[u]; p < g.offsets[u + 1]; ++p)
+      if (keep(u, p)) edges[(size_t)q++] = E{ids[u], ids[g.nbrs[p]], g.weights[p]};
+  }
+  StableSort(edges.begin(), edges.end(), [](const E& a, const E& b) {
     return a.cu != b.cu ? a.cu < b.cu : a.cv < b.cv;
   });
   out->n = n2;
@@ -607,8 +668,7 @@ void Compress(const oracle_graph& g, const uint32_t* ids, const double* nw,
   // pbbs reduce per group.
   std::vector<uint32_t> order((size_t)g.n);
   std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return ids[a] < ids[b]; });
+  StableSort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ids[a] < ids[b]; });
   out->node_weights.assign((size_t)n2, 0.0);
   std::vector<double> grp;
   for (size_t i = 0; i < order.size();) {
@@ -854,6 +914,28 @@ void oracle_set_threads(int threads) {
 }
 int oracle_get_threads(void) { return g_threads; }
 
+int64_t oracle_best_move_candidates(const oracle_graph* g, const oracle_state* s,
+                                    const oracle_config* cfg, uint32_t v, uint32_t* clusters,
+                                    double* deltas, int64_t cap, double* out_change) {
+  Candidates c;
+  uint32_t t;
+  double d;
+  BestMove(*g, *s, cfg->resolution, cfg->edge_weight_offset, v, &t, &d, false, &c);
+  for (int64_t i = 0; i < (int64_t)c.list.size() && i < cap; ++i) {
+    clusters[i] = c.list[(size_t)i].first;
+    deltas[i] = c.list[(size_t)i].second;
+  }
+  if (out_change) *out_change = c.change;
+  return (int64_t)c.list.size();
+}
+
+void oracle_phase_seconds(double out[5], int reset) {
+  for (int i = 0; i < 5; ++i) {
+    out[i] = g_phase[i];
+    if (reset) g_phase[i] = 0;
+  }
+}
+
 void oracle_best_move(const oracle_graph* g, const oracle_state* s,
                       const oracle_config* cfg, uint32_t v, uint32_t* out_target,
                       double* out_delta) {
@@ -896,7 +978,7 @@ void oracle_apply_moves(int64_t n, oracle_state* s, const uint32_t* moves,
     if (moves[v] != kNone && moves[v] != s->cluster_ids[v]) moving.push_back((uint32_t)v);
   if (moving.empty()) return;
   std::vector<uint32_t> by_old(moving);
-  std::stable_sort(by_old.begin(), by_old.end(), [&](uint32_t a, uint32_t b) {
+  StableSort(by_old.begin(), by_old.end(), [&](uint32_t a, uint32_t b) {
     return s->cluster_ids[a] < s->cluster_ids[b];
   });
   for (size_t i = 0; i < by_old.size();) {  // PCCI.cc:294-303
@@ -909,8 +991,8 @@ void oracle_apply_moves(int64_t n, oracle_state* s, const uint32_t* moves,
     i = j;
   }
   std::vector<uint32_t> by_new(moving);
-  std::stable_sort(by_new.begin(), by_new.end(),
-                   [&](uint32_t a, uint32_t b) { return moves[a] < moves[b]; });
+  StableSort(by_new.begin(), by_new.end(),
+             [&](uint32_t a, uint32_t b) { return moves[a] < moves[b]; });
   size_t new_start = by_new.size();
   for (size_t i = 0; i < by_new.size();) {  // PCCI.cc:325-337
     size_t j = i;
@@ -972,8 +1054,12 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
     }
   } else {
     std::vector<uint32_t> moves((size_t)n);
-    oracle_sync_decisions(g, s, cfg, frontier, moves.data(), moved_vertex.data(), nullptr,
-                          nullptr);
+    {
+      PhaseTimer t(0);
+      oracle_sync_decisions(g, s, cfg, frontier, moves.data(), moved_vertex.data(), nullptr,
+                            nullptr);
+    }
+    PhaseTimer t(1);
     oracle_apply_moves(n, s, moves.data(), moved_clusters.data());  // PCC.cc:163-165
   }
   if (cfg->clustering_moves_method == 0) {  // DEFAULT_CLUSTER_MOVES, PCC.cc:167-297
@@ -987,6 +1073,7 @@ int64_t oracle_round(const oracle_graph* g, oracle_state* s,
     std::memset(frontier, 1, (size_t)n);
     return n;
   }
+  PhaseTimer t(2);
   std::vector<uint8_t> seed((size_t)n);
   for (int64_t v = 0; v < n; ++v)  // PCC.cc:312-324
     seed[(size_t)v] = cfg->move_method == 0 ? moved_clusters[s->cluster_ids[v]] : moved_vertex[v];
@@ -1258,7 +1345,10 @@ int oracle_cluster(const oracle_graph* g, const oracle_config* cfg_in, int modul
       oracle_flatten(ni, subclust ? L.map.data() : L.ids.data(), levels[(size_t)i + 1].ids.data(),
                      flat.data());
       oracle_state st = L.State();
-      ResetFromIds(ni, st, flat);
+      {
+        PhaseTimer t(4);
+        ResetFromIds(ni, st, flat);
+      }
       oracle_iterate_best_moves(&cg, &st, &cfg, num_inner, cfg.permute_seed + 7919ull * (uint64_t)(i + 1),
                                 &rounds);
     }

@@ -67,11 +67,24 @@ typedef struct oracle_state {
 void oracle_set_threads(int threads);
 int oracle_get_threads(void);
 
+/* Wall seconds spent by oracle_round / oracle_cluster since the last reset,
+ * per phase: 0 best-move map, 1 sync apply, 2 frontier expansion,
+ * 3 contraction, 4 refinement relabel.  reset != 0 zeroes them after copying. */
+void oracle_phase_seconds(double out[5], int reset);
+
 /* EfficientBestMove, PCCI.cc:448-583.  Returns (target, delta); target == n
  * means "form a new cluster". */
 void oracle_best_move(const oracle_graph* g, const oracle_state* s,
                       const oracle_config* cfg, uint32_t v,
                       uint32_t* out_target, double* out_delta);
+
+/* Every candidate EfficientBestMove evaluates for v (PCCI.cc:483-504 for
+ * degree <= 1000, best_move_seq of PCCI.cc:551-563 above): clusters[i] /
+ * deltas[i] for i < min(count, cap) in evaluation order, and *out_change the
+ * "new cluster" value (change_in_objective).  Returns the candidate count. */
+int64_t oracle_best_move_candidates(const oracle_graph* g, const oracle_state* s,
+                                    const oracle_config* cfg, uint32_t v, uint32_t* clusters,
+                                    double* deltas, int64_t cap, double* out_change);
 
 /* BestMove(graph, moving_nodes), PCCI.cc:644-707 with BestMoveFromStats,
  * UTIL.h:70-113: the best cluster for the whole set members[0..count) (in
