@@ -5,14 +5,15 @@ the state (parallel-correlation-clusterer-internal.cc:371-403), so its result
 depends on thread timing and processing order (PAPER §4, footnote 3).  Even
 the sequential oracle moves by 0.2-0.7% between processing orders on small
 RMAT graphs.  The contract is therefore statistical, against the oracle's
-async runs over eight seeded processing orders (permute seeds 1..8):
+async runs over eight seeded processing orders (permute seeds 1..8), and
+two-sided:
 
-* on graphs whose order-to-order spread is small (std <= 0.25%), the GPU
-  objective is at most 0.5% below the oracle's mean;
-* on the noisier small graphs, it is no lower than the oracle's worst order
-  minus 0.5%.
-A result above the sequential runs is accepted (it is not a loss of quality);
-the objective the GPU reports is checked separately against the oracle's own
+* on graphs whose order-to-order spread is small (std <= 0.25% of the mean),
+  the GPU objective lies within 0.5% of the oracle's mean;
+* on the noisier small graphs a single reference run is itself only within
+  about 3 standard deviations of its mean, so the band is
+  |gpu - mean| <= 0.5% |mean| + 3 std.
+The objective the GPU reports is checked separately against the oracle's own
 evaluation of the GPU's cluster ids, so an inflated value cannot pass.
 """
 
@@ -86,17 +87,26 @@ def _run(pcc, orc, spec, kw, modularity):
     return gpu, ref
 
 
+def async_band(ref):
+    """Half-width of the accepted band around the oracle's mean: 0.5% of the
+    mean, plus 3 standard deviations of the order-to-order spread when that
+    spread exceeds 0.25% of the mean."""
+    mean, std = ref.mean(), ref.std()
+    extra = 3 * std if std > 0.0025 * abs(mean) else 0.0
+    return 0.005 * abs(mean) + extra
+
+
 @pytest.mark.parametrize("case", range(len(STRICT_CASES)))
 def test_async_within_half_percent_of_oracle_mean(pcc, orc, case):
     spec, kw, modularity = STRICT_CASES[case]
     gpu, ref = _run(pcc, orc, spec, kw, modularity)
     mean = ref.mean()
-    assert gpu >= mean - 0.005 * abs(mean)
+    assert ref.std() <= 0.0025 * abs(mean)  # the case is a strict one
+    assert abs(gpu - mean) <= 0.005 * abs(mean)
 
 
 @pytest.mark.parametrize("case", range(len(RANGE_CASES)))
-def test_async_not_below_oracle_order_range(pcc, orc, case):
+def test_async_within_band_of_noisy_oracle_orders(pcc, orc, case):
     spec, kw, modularity = RANGE_CASES[case]
     gpu, ref = _run(pcc, orc, spec, kw, modularity)
-    lo = ref.min()
-    assert gpu >= lo - 0.005 * abs(lo)
+    assert abs(gpu - ref.mean()) <= async_band(ref)

@@ -173,7 +173,11 @@ def test_async_default_cluster_moves(pcc, orc, case):
     else:
         assert got == pytest.approx(orc.objective(og, ids, np.ones(csr.n), kw["resolution"]),
                                     rel=1e-9)
-    lo = min(vals)
-    print(f"  gpu {got:.6f} oracle mean {np.mean(vals):.6f} range [{lo:.6f}, {max(vals):.6f}]",
-          flush=True)
-    assert got >= lo - 0.005 * abs(np.mean(vals))
+    ref = np.array(vals)
+    print(f"  gpu {got:.6f} oracle mean {ref.mean():.6f} std {ref.std() / abs(ref.mean()) * 100:.3f}% "
+          f"range [{ref.min():.6f}, {ref.max():.6f}]", flush=True)
+    # Two-sided band around the oracle's mean: 0.5%, plus 3 standard
+    # deviations of its order-to-order spread when that is above 0.25%.
+    std = ref.std()
+    band = 0.005 * abs(ref.mean()) + (3 * std if std > 0.0025 * abs(ref.mean()) else 0.0)
+    assert abs(got - ref.mean()) <= band

@@ -158,3 +158,61 @@ def test_output_indices_by_id(pcc):
     assert pcc.clustering_to_ids([[3], [1, 4, 5], [0, 2]], 6).tolist() == [2, 1, 2, 0, 1, 1]
     with pytest.raises(pcc.StatusError):
         pcc.clustering_to_ids([[0, 1]], 3)
+
+
+def _batch_rc(pcc, counts, offsets, nbrs):
+    import ctypes
+    counts = np.ascontiguousarray(counts, np.int64)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    nb = None if nbrs is None else np.ascontiguousarray(nbrs, np.uint32)
+    h = ctypes.c_void_p()
+    w = None
+    rc = pcc.lib().pcc_batch_from_csr(len(counts), pcc._ptr(counts), pcc._ptr(offsets),
+                                      pcc._ptr(nb), pcc._ptr(w), ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("counts,offsets,nbrs,why", [
+    ([1, 1, 1], [0, 3, 2, 4], [0, 0, 0, 0], "decreasing graph boundary"),
+    ([2, 2], [0, 1, 9, 3, 4], [1, 0, 1, 0], "graph boundary past m"),
+    ([2, 2], [1, 1, 2, 3, 4], [1, 0, 1, 0], "offsets not starting at 0"),
+    ([2, 2], [0, 1, 2, 3, 4], None, "null neighbours with edges"),
+])
+def test_batch_rejects_malformed_offsets_before_reading_rows(pcc, counts, offsets, nbrs, why):
+    # The checks run before any row is read and before any device work.
+    rc, h = _batch_rc(pcc, counts, offsets, nbrs)
+    assert rc == pcc.INVALID_ARGUMENT, why
+    assert not h.value
+
+
+def test_fullsize_sbm_fixture_reproduces(pcc, orc):
+    """tests/golden/fullsize_sbm10000_allmove.json: the generator's CSR and
+    the oracle's ids hash to the recorded digests."""
+    import hashlib
+    import json
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fullsize_sbm10000_allmove.json")))
+    s = fx["spec"]
+    csr = pcc.gen_sbm(s["n"], s["blocks"], s["p_in"], s["p_out"], seed=s["seed"])
+    h = hashlib.sha256()
+    for a in (csr.offsets.astype("<i8"), csr.nbrs.astype("<u4"), csr.weights.view("<u4")):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == fx["csr_sha256"]
+    r = orc.cluster(orc.G.from_csr(csr), orc.config_from(pcc.parse_config(fx["config"])))
+    assert hashlib.sha256(r.ids.astype("<u4").tobytes()).hexdigest() == fx["ids_sha256"]
+    assert float.fromhex(fx["objective_hex"]) == r.objective
+
+
+def test_fullsize_rmat22_fixture_graph_reproduces(pcc):
+    """The RMAT scale-22 graph of tests/golden/fullsize_rmat22_nbhr_refine.json
+    is what the seeded generator makes (the oracle's run on it takes minutes and
+    is recorded, not repeated here)."""
+    import hashlib
+    import json
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fullsize_rmat22_nbhr_refine.json")))
+    s = fx["spec"]
+    csr = pcc.gen_rmat(s["scale"], s["ef"], 0.57, 0.19, 0.19, s["lo"], s["hi"], seed=s["seed"])
+    assert (csr.n, csr.m) == (fx["n"], fx["m"])
+    h = hashlib.sha256()
+    for a in (csr.offsets.astype("<i8"), csr.nbrs.astype("<u4"), csr.weights.view("<u4")):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == fx["csr_sha256"]
