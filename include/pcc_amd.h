@@ -178,6 +178,23 @@ int pcc_compute_objective(pcc_graph* g, const pcc_config* cfg,
                           const uint32_t* cluster_ids, const double* node_weights,
                           double* out_objective);
 
+/* No counterpart in the reference: pcc_compute_objective for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels in the same
+ * order give the same value bit for bit, but no device memory is allocated
+ * and nothing synchronizes: cluster_ids_dev (n entries) and node_weights_dev
+ * (n entries, NULL: unit weights) are device arrays, *out_objective_dev a
+ * device double written on `stream` (a hipStream_t; NULL: the handle's
+ * stream).  The work memory is one fixed array per device (about 50 MiB,
+ * kept until the process ends), so the graph may have at most 2^21 vertices
+ * and 2^30 directed edges (PCC_INVALID_ARGUMENT beyond).  Cluster ids
+ * outside [0, n) make the objective NaN.  Calls that overlap on one device
+ * share that array and race: serialize them (one stream, or events). */
+int pcc_compute_objective_noalloc(pcc_graph* g, const pcc_config* cfg,
+                                  const uint32_t* cluster_ids_dev,
+                                  const double* node_weights_dev,
+                                  double* out_objective_dev, void* stream);
+
 /* ComputeModularity (parallel-modularity-clusterer.cc:36-66) on device. */
 int pcc_compute_modularity(pcc_graph* g, double resolution,
                            const uint32_t* cluster_ids, double* out_modularity);

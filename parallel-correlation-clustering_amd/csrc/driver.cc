@@ -562,6 +562,28 @@ int pcc_compute_objective(pcc_graph* g, const pcc_config* cfg, const uint32_t* c
   });
 }
 
+int pcc_compute_objective_noalloc(pcc_graph* g, const pcc_config* cfg,
+                                  const uint32_t* cluster_ids_dev, const double* node_weights_dev,
+                                  double* out_objective_dev, void* stream) {
+  ClearError();
+  return Guard("pcc_compute_objective_noalloc", [&]() -> int {
+    PCC_RETURN_IF_ERROR(CheckGraph(g));
+    PCC_RETURN_IF_ERROR(CheckSingle(g));
+    const int64_t n = g->csr.n, m = g->csr.m;
+    if (!cfg || !out_objective_dev || (n > 0 && !cluster_ids_dev))
+      return SetError(PCC_INVALID_ARGUMENT, "null argument");
+    if (n > kNoAllocMaxVertices || m > kNoAllocMaxEdges)
+      return SetError(PCC_INVALID_ARGUMENT,
+                      "pcc_compute_objective_noalloc takes at most 2^21 vertices and 2^30 "
+                      "directed edges (its work memory is a fixed per-device array)");
+    void* work = NoAllocScratch(g->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->stream;
+    ComputeObjectiveNoAlloc(work, g->csr.View(), cluster_ids_dev, node_weights_dev,
+                            cfg->resolution, cfg->edge_weight_offset, out_objective_dev, s);
+    return PCC_OK;
+  });
+}
+
 int pcc_compute_modularity(pcc_graph* g, double resolution, const uint32_t* cluster_ids,
                            double* out_modularity) {
   ClearError();

@@ -14,7 +14,9 @@
 //   * node weights of a cluster are summed in ascending vertex order the same
 //     way; n' = 1 + max cluster id, so unused ids become isolated vertices.
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "device_common.h"
@@ -571,14 +573,23 @@ __device__ __forceinline__ void WaveKeyedAdd(bool act, uint32_t key, double x, d
   }
 }
 
+// Node weight of v; nw null means unit weights.
+__device__ __forceinline__ double NwAt(const double* __restrict__ nw, int64_t v) {
+  return nw ? nw[v] : 1.0;
+}
+
+// gate (may be null): when *gate != 0 the kernel does nothing (the cluster
+// ids were found out of range by the allocation-free entry point).
 __global__ __launch_bounds__(256) void k_cluster_weights(int64_t n, const uint32_t* __restrict__ ids,
                                                          const double* __restrict__ nw,
-                                                         double* __restrict__ K) {
+                                                         double* __restrict__ K,
+                                                         const unsigned int* __restrict__ gate) {
+  if (gate && *gate) return;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = base + threadIdx.x;
     const bool in = v < n;
-    WaveKeyedAdd(in, in ? ids[v] : 0u, in ? nw[v] : 0.0, K);
+    WaveKeyedAdd(in, in ? ids[v] : 0u, in ? NwAt(nw, v) : 0.0, K);
   }
 }
 
@@ -630,7 +641,9 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
                                                          double offset, double* __restrict__ e,
                                                          double* __restrict__ r,
                                                          uint32_t* __restrict__ longq,
-                                                         unsigned int* __restrict__ nlong) {
+                                                         unsigned int* __restrict__ nlong,
+                                                         const unsigned int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int lane = LaneId();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -642,7 +655,8 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
       cv = ids[v];
       b = g.off[v];
       deg = g.off[v + 1] - b;
-      r[v] = nw[v] * (K[cv] - nw[v]);
+      const double k = NwAt(nw, v);
+      r[v] = k * (K[cv] - k);
       if (deg > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;  // k_objective_long
       if (deg <= 16) {
         double tot = 0.0;
@@ -909,11 +923,11 @@ __global__ __launch_bounds__(256) void k_graph_sums(const double* __restrict__ a
 
 __global__ void k_objective_final(int G, const double* __restrict__ esum,
                                   const double* __restrict__ rsum, const double* __restrict__ res,
-                                  double* __restrict__ out) {
+                                  double* __restrict__ out, const unsigned int* __restrict__ gate) {
   for (int gi = threadIdx.x; gi < G; gi += blockDim.x) {
     double objective = esum[gi];
     objective -= res[gi] * rsum[gi] / 2;
-    out[gi] = objective;
+    out[gi] = (gate && *gate) ? __longlong_as_double(0x7FF8000000000000ll) : objective;
   }
 }
 
@@ -1217,9 +1231,10 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     PCC_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)gmin.get(), 1 << 20, (size_t)maxlong, s));
     PCC_HIP_CHECK(hipMemsetAsync(nz.get(), 0, sizeof(unsigned long long) * (size_t)maxlong, s));
     const int grid = Grid(n, 256, 8192);
-    hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, K.get());
+    hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, K.get(),
+                       (const unsigned int*)nullptr);
     hipLaunchKernelGGL(k_objective_terms, dim3(grid), dim3(256), 0, s, g, ids, nw, K.get(), offset,
-                       e.get(), r.get(), longq.get(), nlong.get());
+                       e.get(), r.get(), longq.get(), nlong.get(), (const unsigned int*)nullptr);
     if (g.m > kLongRow) {
       hipLaunchKernelGGL(k_objective_long, dim3(1024), dim3(256), 0, s, g, ids, offset, longq.get(),
                          nlong.get(), acc.get(), gmin.get(), nz.get());
@@ -1230,7 +1245,7 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
   }
   GraphSums(al, e.get(), r.get(), G, gbase, se.get(), sr.get(), s);
   hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(256), 0, s, G, se.get(), sr.get(), res,
-                     out_dev);
+                     out_dev, (const unsigned int*)nullptr);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -1263,6 +1278,187 @@ void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* id
   GraphSums(al, intra.get(), t.get(), G, gbase, si.get(), st.get(), s);
   hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, g, G, gbase, si.get(), st.get(),
                      out_dev);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------- allocation-free objective --
+// ComputeObjective for one graph with every work array carved out of a fixed
+// per-device array instead of allocations: callers that cannot allocate
+// (code under HIP graph capture, serving loops, harnesses that forbid
+// hipMalloc) get the same kernels, in the same order, as the allocating
+// path, so the same value bit for bit.  Work memory per call: the cluster
+// weights and the two per-vertex term arrays (24 B per vertex), the chunk
+// partial sums of the fixed-order reduction, and the long-row accumulators
+// (32 B per row of more than kLongRow edges).
+
+namespace {
+
+constexpr int64_t kSumChunk = 4096;  // GraphSums' chunk of vertices
+constexpr int64_t kNaMaxChunks = (kNoAllocMaxVertices + kSumChunk - 1) / kSumChunk;
+constexpr int64_t kNaMaxLong = kNoAllocMaxEdges / kLongRow + 1;
+
+constexpr size_t Align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Byte offsets of the work arrays inside the scratch array.
+constexpr size_t kNaK = 0;
+constexpr size_t kNaE = kNaK + Align256(8 * (size_t)kNoAllocMaxVertices);
+constexpr size_t kNaR = kNaE + Align256(8 * (size_t)kNoAllocMaxVertices);
+constexpr size_t kNaPa = kNaR + Align256(8 * (size_t)kNoAllocMaxVertices);
+constexpr size_t kNaPb = kNaPa + Align256(8 * (size_t)kNaMaxChunks);
+constexpr size_t kNaCs = kNaPb + Align256(8 * (size_t)kNaMaxChunks);
+constexpr size_t kNaSmall = kNaCs + Align256(8 * (size_t)(kNaMaxChunks + 1));
+// small: coff[2] (int64), se, sr, res (double), nlong, bad (uint32)
+constexpr size_t kNaLongq = kNaSmall + 256;
+constexpr size_t kNaAcc = kNaLongq + Align256(4 * (size_t)kNaMaxLong);
+constexpr size_t kNaGmin = kNaAcc + Align256(16 * (size_t)kNaMaxLong);
+constexpr size_t kNaNz = kNaGmin + Align256(4 * (size_t)kNaMaxLong);
+constexpr size_t kNaBytes = kNaNz + Align256(8 * (size_t)kNaMaxLong);
+
+}  // namespace
+
+}  // namespace pcc
+
+// The per-device work memory of pcc_compute_objective_noalloc (about 50 MiB;
+// every device the library runs on keeps its copy until the process ends).
+// Host code must not pass this name to kernels: it stands for the symbol, not
+// for a device address.  NoAllocScratch() returns the address.
+static __device__ __align__(256) unsigned char scratch[pcc::kNaBytes];
+
+namespace pcc {
+
+namespace {
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+struct NaWork {
+  double* K;
+  double* e;
+  double* r;
+  double* pa;
+  double* pb;
+  int64_t* cs;
+  int64_t* coff;
+  double* se;
+  double* sr;
+  double* res;
+  unsigned int* nlong;
+  unsigned int* bad;
+  uint32_t* longq;
+  double* acc;
+  int* gmin;
+  unsigned long long* nz;
+};
+
+NaWork Carve(void* base) {
+  unsigned char* b = static_cast<unsigned char*>(base);
+  NaWork w;
+  w.K = reinterpret_cast<double*>(b + kNaK);
+  w.e = reinterpret_cast<double*>(b + kNaE);
+  w.r = reinterpret_cast<double*>(b + kNaR);
+  w.pa = reinterpret_cast<double*>(b + kNaPa);
+  w.pb = reinterpret_cast<double*>(b + kNaPb);
+  w.cs = reinterpret_cast<int64_t*>(b + kNaCs);
+  w.coff = reinterpret_cast<int64_t*>(b + kNaSmall);
+  w.se = reinterpret_cast<double*>(b + kNaSmall + 16);
+  w.sr = reinterpret_cast<double*>(b + kNaSmall + 24);
+  w.res = reinterpret_cast<double*>(b + kNaSmall + 32);
+  w.nlong = reinterpret_cast<unsigned int*>(b + kNaSmall + 40);
+  w.bad = reinterpret_cast<unsigned int*>(b + kNaSmall + 44);
+  w.longq = reinterpret_cast<uint32_t*>(b + kNaLongq);
+  w.acc = reinterpret_cast<double*>(b + kNaAcc);
+  w.gmin = reinterpret_cast<int*>(b + kNaGmin);
+  w.nz = reinterpret_cast<unsigned long long*>(b + kNaNz);
+  return w;
+}
+
+// Clears what the kernels accumulate into (as the allocating path's memsets
+// do, but as a kernel: memset nodes are not safe in captured graphs) and
+// writes the chunk starts of the single graph's fixed-order sum.
+__global__ __launch_bounds__(256) void k_na_prepare(int64_t n, int64_t nlongmax, int64_t nch,
+                                                    double res, NaWork w) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = tid; v < n; v += stride) {
+    w.K[v] = 0.0;
+    w.e[v] = 0.0;
+    w.r[v] = 0.0;
+  }
+  for (int64_t q = tid; q < nlongmax; q += stride) {
+    w.acc[2 * q] = 0.0;
+    w.acc[2 * q + 1] = 0.0;
+    w.gmin[q] = 1 << 20;
+    w.nz[q] = 0;
+  }
+  for (int64_t j = tid; j <= nch; j += stride) w.cs[j] = j < nch ? j * kSumChunk : n;
+  if (tid == 0) {
+    w.coff[0] = 0;
+    w.coff[1] = nch;
+    w.se[0] = 0.0;
+    w.sr[0] = 0.0;
+    w.res[0] = res;
+    *w.nlong = 0;
+    *w.bad = 0;
+  }
+}
+
+// bad = 1 if a cluster id is outside [0, n) (the kernels that index by
+// cluster id then do nothing and the objective is NaN).
+__global__ __launch_bounds__(256) void k_na_check_ids(int64_t n, const uint32_t* __restrict__ ids,
+                                                      unsigned int* __restrict__ bad) {
+  bool out = false;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    out = out || (int64_t)ids[v] >= n;
+  if (__ballot(out) && LaneId() == 0) atomicOr(bad, 1u);
+}
+
+}  // namespace
+
+void* NoAllocScratch(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    throw HipError{hipErrorInvalidDevice, "device index beyond the scratch table", __FILE__,
+                   __LINE__};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // With `device` current: each device holds its own copy of the array.
+    void* p = nullptr;
+    PCC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+void ComputeObjectiveNoAlloc(void* scratch_dev, const CsrView& g, const uint32_t* ids,
+                             const double* nw, double res, double offset, double* out_dev,
+                             hipStream_t s) {
+  const int64_t n = g.n;
+  const NaWork w = Carve(scratch_dev);
+  const int64_t nch = (n + kSumChunk - 1) / kSumChunk;
+  const int64_t nlongmax = g.m / kLongRow + 1;
+  const int64_t prep = std::max<int64_t>(std::max<int64_t>(n, nlongmax), nch + 1);
+  hipLaunchKernelGGL(k_na_prepare, dim3(Grid(prep, 256, 8192)), dim3(256), 0, s, n, nlongmax, nch,
+                     res, w);
+  if (n > 0) {
+    const int grid = Grid(n, 256, 8192);
+    hipLaunchKernelGGL(k_na_check_ids, dim3(grid), dim3(256), 0, s, n, ids, w.bad);
+    hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, w.K,
+                       (const unsigned int*)w.bad);
+    hipLaunchKernelGGL(k_objective_terms, dim3(grid), dim3(256), 0, s, g, ids, nw, w.K, offset,
+                       w.e, w.r, w.longq, w.nlong, (const unsigned int*)w.bad);
+    if (g.m > kLongRow) {
+      hipLaunchKernelGGL(k_objective_long, dim3(1024), dim3(256), 0, s, g, ids, offset, w.longq,
+                         w.nlong, w.acc, w.gmin, w.nz);
+      hipLaunchKernelGGL(k_objective_long_final, dim3(64), dim3(256), 0, s, g, ids, offset,
+                         w.longq, w.nlong, w.acc, w.gmin, w.nz, w.e);
+    }
+    hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)nch), dim3(256), 0, s, w.cs, w.e, w.r, w.pa,
+                       w.pb);
+  }
+  hipLaunchKernelGGL(k_graph_sums, dim3(1), dim3(256), 0, s, w.pa, w.pb, w.coff, w.se, w.sr);
+  hipLaunchKernelGGL(k_objective_final, dim3(1), dim3(256), 0, s, 1, w.se, w.sr, w.res, out_dev,
+                     (const unsigned int*)w.bad);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

@@ -553,6 +553,19 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
                       const double* nw, int G, const int64_t* gbase, const double* res,
                       double offset, double* out_dev, hipStream_t s);
 
+// The same objective for one graph with no device allocation: every work
+// array lives in a fixed per-device array (NoAllocScratch(device), with that
+// device current), so graphs are limited to kNoAllocMaxVertices vertices and
+// kNoAllocMaxEdges directed edges.  nw null means unit weights.  Cluster ids
+// outside [0, n) give NaN.  Asynchronous on s; calls that overlap on one
+// device share the array and race.
+constexpr int64_t kNoAllocMaxVertices = (int64_t)1 << 21;
+constexpr int64_t kNoAllocMaxEdges = (int64_t)1 << 30;
+void* NoAllocScratch(int device);
+void ComputeObjectiveNoAlloc(void* scratch_dev, const CsrView& g, const uint32_t* ids,
+                             const double* nw, double res, double offset, double* out_dev,
+                             hipStream_t s);
+
 // ComputeModularity (parallel-modularity-clusterer.cc:36-66) of each graph
 // with resolution gamma[g].
 void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, int G,

@@ -187,6 +187,7 @@ _SIGS = {
     "pcc_cluster": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(Stats)]),
     "pcc_refine_clusters": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_int, _P, _P, ctypes.POINTER(Stats)]),
     "pcc_compute_objective": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, ctypes.POINTER(_D)]),
+    "pcc_compute_objective_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, _P, _P]),
     "pcc_compute_modularity": (ctypes.c_int, [_P, _D, _P, ctypes.POINTER(_D)]),
     "pcc_best_moves": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pcc_round": (ctypes.c_int, [_P, ctypes.POINTER(Config), _P, _P, _P, _P, _P, ctypes.POINTER(_I64)]),
@@ -419,6 +420,13 @@ class DeviceGraph:
         out = ctypes.c_double()
         _check(lib().pcc_compute_objective(self._h, ctypes.byref(cfg), _ptr(ids), _ptr(nw), ctypes.byref(out)))
         return out.value
+
+    def objective_noalloc(self, cfg: Config, ids_dev: int, nw_dev: Optional[int], out_dev: int,
+                          stream: Optional[int] = None) -> None:
+        """pcc_compute_objective_noalloc: device pointers (integers) in and out;
+        asynchronous on `stream` (a hipStream_t, None: the handle's stream)."""
+        _check(lib().pcc_compute_objective_noalloc(self._h, ctypes.byref(cfg), ids_dev, nw_dev,
+                                                   out_dev, stream))
 
     def modularity(self, resolution: float, ids: np.ndarray) -> float:
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
