@@ -10,8 +10,15 @@ all levels and the refinement on device-resident data.
 Prints one JSON line.  `value` is best-move GTEPS: edges scanned by the
 best-move kernels over all rounds of the timed steps divided by those kernels'
 time (HIP events on the library's stream).  `roofline` describes the dominant
-best-move kernel (the degree bin with the most time); `cpu_baseline` times the
-oracle's first best-move round of level 0 on the same graph on the host cores.
+best-move kernel (the degree bin with the most time).  The same line carries
+the metric's second half, `batched`: graphs/s of one batched launch sequence
+over a size-stratified sample of the 4,096-graph batched workload against
+plain per-graph runs of the same graphs, with the count of identical results;
+`parity_full_size`: the headline run's ids against the digest of the oracle's
+ids on the same graph (tests/golden/fullsize_rmat22_nbhr_refine.json); and
+`cpu_baseline`: the oracle's whole Cluster span on a bounded sample (the same
+generator and configuration at RMAT scale 20) on the host cores, beside the
+GPU's span on that sample.
 """
 
 import argparse
@@ -68,6 +75,12 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads for the CPU baseline (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="rmat22: skip the batched sample of the headline line")
+    ap.add_argument("--batched-sample", type=int, default=512,
+                    help="rmat22: graphs of the size-stratified batched sample (of 4,096)")
+    ap.add_argument("--cpu-scale", type=int, default=20,
+                    help="rmat22: RMAT scale of the CPU baseline's sample")
     ap.add_argument("--no-gather-ceiling", action="store_true",
                     help="skip the random-gather ceiling (a child process using torch)")
     ap.add_argument("--workload", default="rmat22",
@@ -168,15 +181,23 @@ def main() -> int:
     dom_gbs = dom_bytes / bin_s[dom] / 1e9 if bin_s[dom] > 0 else 0.0
     all_gbs = sum(s.bestmove_bytes for s in stats) / bm_s / 1e9 if bm_s > 0 else 0.0
 
+    parity = full_size_parity(ids, stats[-1], args)
+    g.close()
     # torch and this library bring separate HIP runtimes, so the ceiling is
     # measured in a child process.
     gceil = None if args.no_gather_ceiling else gather_ceiling_child(args.scale)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(csr, cfg, args.cpu_threads)
+        cpu = cpu_baseline(csr, cfg, args)
+    n_v, m_e = csr.n, csr.m
+    del csr
+    batched = None if args.no_batched else batched_sample(args)
 
     line = {
-        "metric": "best-move GTEPS (sync NBHR_MOVE Louvain rounds, all levels + refinement)",
+        # The headline metric has two halves: `value` is best-move GTEPS over
+        # the sync NBHR_MOVE rounds of all levels and the refinement, the
+        # `batched` object the batched graphs/s against plain runs.
+        "metric": "best-move GTEPS + batched graphs/s vs plain GPU path at reference-equal objective",
         "value": round(gteps, 4),
         "unit": "GTEPS",
         "n_gpus": 1,
@@ -192,8 +213,8 @@ def main() -> int:
             "workload": f"RMAT scale {args.scale} edge factor {args.edge_factor:g} "
                         "(a .57 b .19 c .19), U[-1,1] weights, seed 2; "
                         "ParallelCorrelationClusterer sync NBHR_MOVE resolution 0.05 refine",
-            "n": csr.n,
-            "m_directed": csr.m,
+            "n": n_v,
+            "m_directed": m_e,
             "parallelism": "single GPU",
         },
         "end_to_end_ms": round(cl_s / max(args.steps, 1) * 1e3, 3),
@@ -233,15 +254,40 @@ def main() -> int:
             "peak": round(gceil / 1e9, 2),
             "unit": "G gathers/s",
             "frac": round(bin_e[dom] / bin_s[dom] / gceil, 4) if bin_s[dom] > 0 else 0.0,
-            "peak_source": f"torch.index_select of 45M random indices into {csr.n} u32 entries "
+            "peak_source": f"torch.index_select of 45M random indices into {n_v} u32 entries "
                            "(bench.py --workload gather)",
         },
         "cpu_baseline": cpu,
+        "parity_full_size": parity,
+        "batched": batched,
     }
     if tune:
         line["autotune"] = tune
     print(json.dumps(line), flush=True)
     return 0
+
+
+def full_size_parity(ids, st, args):
+    """The headline run's ids against the oracle's (sha256 digests from
+    tests/golden/fullsize_rmat22_nbhr_refine.json, written by
+    tests/golden/make_fullsize.py)."""
+    import hashlib
+
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", "fullsize_rmat22_nbhr_refine.json")
+    if args.scale != 22 or args.edge_factor != 16.0 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        fx = json.load(f)
+    digest = hashlib.sha256(np.ascontiguousarray(ids, "<u4").tobytes()).hexdigest()
+    return {
+        "reference": "oracle (CPU restatement), whole Cluster run on the same graph",
+        "ids_equal": digest == fx["ids_sha256"],
+        "num_clusters_equal": int(st.num_clusters) == fx["num_clusters"],
+        "objective_equal": float(st.objective) == float.fromhex(fx["objective_hex"]),
+        "rounds_equal": int(st.rounds) == fx["rounds"],
+    }
 
 
 def bench_subclustering(args):
@@ -395,14 +441,19 @@ def measured_traffic(args, dom):
     return grp["traffic_bytes_per_round"] if grp else None
 
 
-def c4_graphs(pcc_amd, count, seed=4):
+def c4_graphs(pcc_amd, count, seed=4, select=None):
     """Batched workload: SBM graphs, n ~ logU[200, 2e5], mean degree ~ U[8, 64],
     power-law (exponent 2) community sizes, p_out / p_in = 0.01, weights
     U[0, 1] rounded to float32, resolution ~ U[0.001, 0.1].  The parameters
-    are drawn in order from one seeded stream; the graphs are generated by a
-    pool of host threads (the generator releases the interpreter lock)."""
-    from concurrent.futures import ThreadPoolExecutor
+    of all `count` graphs are drawn in order from one seeded stream; the
+    graphs listed in `select` (all by default) are generated, in that order,
+    by a pool of host threads (the generator releases the interpreter lock)."""
+    params, res = c4_params(count, seed)
+    sel = list(range(count)) if select is None else list(select)
+    return c4_generate(pcc_amd, [params[i] for i in sel]), res[sel]
 
+
+def c4_params(count, seed=4):
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -425,6 +476,13 @@ def c4_graphs(pcc_amd, count, seed=4):
         p_in = min(1.0, d / max(denom, 1e-9))
         params.append((sizes, p_in, seed * 100003 + i))
         res.append(rng.uniform(0.001, 0.1))
+    return params, np.array(res)
+
+
+def c4_generate(pcc_amd, params):
+    from concurrent.futures import ThreadPoolExecutor
+
+    count = len(params)
     graphs = [None] * count
     step = max(1, count // 10)
     workers = min(16, os.cpu_count() or 1)
@@ -438,7 +496,70 @@ def c4_graphs(pcc_amd, count, seed=4):
         for done, _ in enumerate(ex.map(make, range(count)), 1):
             if count >= 512 and done % step == 0:
                 print(f"# generated {done}/{count} graphs", file=sys.stderr, flush=True)
-    return graphs, np.array(res)
+    return graphs
+
+
+# The degree-bin thresholds and hub partition size pcc_autotune chose for the
+# batched workload (on its first device batch): wave bound 512, block bound
+# 4096, 1024-edge hub partitions.  They only pick kernel shapes.
+C4_TUNED = {"wave_max_deg": 512, "block_max_deg": 4096, "hub_part_edges": 1024}
+
+
+def batched_sample(args, steps=3):
+    """The metric's second half on a size-stratified sample of the batched
+    workload: the 4,096 graphs ranked by vertex count, every k-th of them
+    (k = 4096 / sample), clustered in one batch (the saved autotune choice),
+    against plain per-graph pcc_cluster runs of the same graphs; results must
+    be identical."""
+    import numpy as np
+    import pcc_amd
+
+    total = 4096
+    k = max(1, total // max(1, args.batched_sample))
+    params, res_all = c4_params(total)
+    order = np.argsort([int(p[0].sum()) for p in params], kind="stable")
+    sel = [int(i) for i in order[k // 2::k]]
+    t0 = time.time()
+    graphs = c4_generate(pcc_amd, [params[i] for i in sel])
+    res = res_all[sel]
+    nv, ne = sum(g.n for g in graphs), sum(g.m for g in graphs)
+    print(f"# batched sample: {len(graphs)} of {total} graphs (every {k}th by size), {nv} vertices, "
+          f"{ne} edges, generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    cfg = pcc_amd.make_config(move_method=pcc_amd.NBHR_MOVE, refine=1)
+    b = pcc_amd.DeviceBatch.from_csrs(graphs)
+    set_tuned(b, C4_TUNED, pcc_amd)
+    b.cluster_batch(cfg, res)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ids_b, obj_b, _, st = b.cluster_batch(cfg, res)
+    batch_s = (time.perf_counter() - t0) / steps
+    b.close()
+    print(f"# batched sample: {batch_s * 1e3:.1f} ms per batch call", file=sys.stderr, flush=True)
+    plain_s, same = 0.0, 0
+    for j, gr in enumerate(graphs):
+        g = pcc_amd.DeviceGraph.from_csr(gr)
+        set_tuned(g, C4_TUNED, pcc_amd)
+        c1 = pcc_amd.make_config(resolution=float(res[j]), move_method=pcc_amd.NBHR_MOVE, refine=1)
+        t1 = time.perf_counter()
+        ids, sst = g.cluster(c1)
+        plain_s += time.perf_counter() - t1
+        same += int(np.array_equal(ids, ids_b[j]) and sst.objective == obj_b[j])
+        g.close()
+    print(f"# batched sample: plain runs {plain_s:.1f} s", file=sys.stderr, flush=True)
+    bgps, pgps = len(graphs) / batch_s, len(graphs) / plain_s
+    return {
+        "metric": "batched graphs/s vs plain per-graph GPU runs (identical ids and objective)",
+        "value": round(bgps, 3), "unit": "graphs/s",
+        "plain_graphs_per_s": round(pgps, 3), "speedup_vs_plain": round(bgps / pgps, 2),
+        "identical_results": f"{same}/{len(graphs)}",
+        "batch_ms": round(batch_s * 1e3, 3), "plain_s": round(plain_s, 3),
+        "sample": f"{len(graphs)} of the 4,096 seeded SBM graphs (n logU[200,2e5], mean degree "
+                  f"U[8,64], power-law blocks, p_out/p_in 0.01, U[0,1] weights, res U[.001,.1]), "
+                  f"every {k}th by vertex count: {nv} vertices, {ne} directed edges; sync NBHR_MOVE "
+                  f"refine; plain runs time each pcc_cluster call on a resident graph",
+        "rounds_per_batch_call": st.rounds, "edges_scanned_per_batch_call": st.edges_scanned,
+        "tuning": C4_TUNED,
+    }
 
 
 def bench_batched(args):
@@ -698,27 +819,45 @@ def bench_gather(args):
     return 0
 
 
-def cpu_baseline(csr, cfg, threads):
-    """Oracle best-move round 0 of level 0 (all vertices, singleton state)."""
+def cpu_baseline(csr, cfg, args):
+    """The oracle (the CPU restatement; the reference cannot be built here)
+    on the host cores: its whole Cluster span (all levels and the refinement,
+    the reference's "Cluster Time:" span) on a bounded sample, the same RMAT
+    generator and configuration at scale --cpu-scale, beside the GPU's span on
+    that sample; plus its best-move round 0 on the full graph and one round of
+    the sequential clusterer (SEQ-CC)."""
     import numpy as np
+    import pcc_amd
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    t = threads if threads > 0 else min(16, os.cpu_count() or 1)
+    t = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
+    ocfg = oracle.config_from(cfg)
+    small = pcc_amd.gen_rmat(args.cpu_scale, args.edge_factor, 0.57, 0.19, 0.19, -1.0, 1.0, seed=2)
+    gs = pcc_amd.DeviceGraph.from_csr(small)
+    gs.cluster(cfg)
+    gpu_ids, gst = gs.cluster(cfg)
+    gs.close()
     oracle.set_threads(t)
+    oracle.phase_seconds(reset=True)
+    r = oracle.cluster(oracle.G.from_csr(small), ocfg)
+    phases = {k: round(v, 3) for k, v in oracle.phase_seconds(reset=True).items()}
+    same = bool(np.array_equal(r.ids, gpu_ids))
+    print(f"# cpu baseline: scale {args.cpu_scale} Cluster span {r.seconds:.1f} s on {t} threads "
+          f"(GPU {gst.cluster_seconds * 1e3:.1f} ms), ids equal {same}", file=sys.stderr, flush=True)
+    # Round 0 of level 0 on the full graph (all vertices, singleton state).
     g = oracle.G.from_csr(csr)
     s = oracle.S.singletons(csr.n)
-    ocfg = oracle.config_from(cfg)
     reps = 0
     t0 = time.perf_counter()
     while True:
         oracle.sync_decisions(g, s, ocfg)
         reps += 1
         el = time.perf_counter() - t0
-        if el > 10.0 or reps >= 20:
+        if el > 4.0 or reps >= 10:
             break
-    gteps = reps * csr.m / el / 1e9
+    round0 = reps * csr.m / el / 1e9
     # The paper's sequential baseline (SEQ-CC): one sequential round over all
     # vertices from singletons, single-threaded.
     oracle.set_threads(1)
@@ -729,16 +868,31 @@ def cpu_baseline(csr, cfg, threads):
     oracle.round_(g, s, scfg, np.ones(csr.n, np.uint8))
     seq_s = time.perf_counter() - t1
     return {
-        "value": round(gteps, 4),
+        "value": round(r.edges_scanned / r.seconds / 1e9, 4),
         "unit": "GTEPS",
         "cores": t,
         "kind": "port",
-        "sample": f"{reps} x best-move round 0 of level 0 (all {csr.n} vertices, {csr.m} edges) "
-                  f"of the same graph, {el:.1f} s",
+        "sample": f"whole Cluster span (all {r.levels} levels, {r.rounds} rounds, refinement) on "
+                  f"RMAT scale {args.cpu_scale} edge factor {args.edge_factor:g} with the same "
+                  f"generator and configuration (n={small.n}, m={small.m}): {r.edges_scanned} edges "
+                  f"scanned in {r.seconds:.2f} s",
+        "end_to_end_s": round(r.seconds, 3),
+        "phase_seconds": phases,
+        "gpu_same_sample": {
+            "end_to_end_ms": round(gst.cluster_seconds * 1e3, 3),
+            "end_to_end_gteps": round(gst.edges_scanned / gst.cluster_seconds / 1e9, 4),
+            "ids_equal_to_cpu": same,
+            "speedup_end_to_end": round(r.seconds / gst.cluster_seconds, 1),
+        },
+        "round0": {
+            "value": round(round0, 4), "unit": "GTEPS", "cores": t, "kind": "port",
+            "sample": f"{reps} x best-move round 0 of level 0 (all {csr.n} vertices, {csr.m} edges) "
+                      f"of the headline graph, {el:.1f} s",
+        },
         "sequential": {
             "value": round(csr.m / seq_s / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
             "sample": f"round 0 of level 0 of the sequential clusterer (SEQ-CC, vertices moving "
-                      f"one after another) on the same graph, {seq_s:.1f} s",
+                      f"one after another) on the headline graph, {seq_s:.1f} s",
         },
     }
 

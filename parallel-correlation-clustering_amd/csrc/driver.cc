@@ -43,10 +43,7 @@ DevBuf<double> NodeWeights(pcc_graph* h, int objective, const double* host_nw) {
   } else if (objective == PCC_OBJECTIVE_MODULARITY) {
     LaunchDegrees(h->csr.View(), nw.get(), h->stream);  // PGU.cc:141-153
   } else {
-    std::vector<double> ones((size_t)(n > 0 ? n : 1), 1.0);
-    PCC_HIP_CHECK(hipMemcpyAsync(nw.get(), ones.data(), sizeof(double) * ones.size(),
-                                 hipMemcpyHostToDevice, h->stream));
-    PCC_HIP_CHECK(hipStreamSynchronize(h->stream));
+    LaunchFillF64(nw.get(), n > 0 ? n : 1, 1.0, h->stream);
   }
   return nw;
 }
@@ -159,13 +156,11 @@ int ClusterImpl(pcc_graph* g, const pcc_config* cfg_in, int objective, const dou
   std::vector<double> gamma((size_t)G, cfg.resolution);
   if (resolutions) gamma.assign(resolutions, resolutions + G);
   std::vector<double> res = gamma;
-  std::vector<int64_t> off_at(base.size());
   if (objective == PCC_OBJECTIVE_MODULARITY) {
     // ComputeModularityConfig + PMC.cc:76-79: k_v = deg(v), res' = gamma / sum deg.
-    PCC_HIP_CHECK(hipMemcpy(&off_at[0], g->csr.off.get(), 8, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < base.size(); ++i)
-      PCC_HIP_CHECK(hipMemcpy(&off_at[i], g->csr.off.get() + base[i], 8, hipMemcpyDeviceToHost));
-    for (int o = 0; o < G; ++o) res[(size_t)o] = gamma[(size_t)o] / (double)(off_at[(size_t)o + 1] - off_at[(size_t)o]);
+    const std::vector<int64_t>& off_at = g->goff;
+    for (int o = 0; o < G; ++o)
+      res[(size_t)o] = gamma[(size_t)o] / (double)(off_at[(size_t)o + 1] - off_at[(size_t)o]);
     cfg.edge_weight_offset = 0;
   }
   cfg.resolution = res[0];
@@ -188,22 +183,16 @@ int ClusterImpl(pcc_graph* g, const pcc_config* cfg_in, int objective, const dou
   DevBuf<double> nw_keep(&g->alloc, n > 0 ? n : 1);
   PCC_HIP_CHECK(hipMemcpyAsync(nw_keep.get(), nw.get(), sizeof(double) * (size_t)(n > 0 ? n : 1),
                                hipMemcpyDeviceToDevice, g->stream));
+  DevBuf<uint32_t> ids;  // final ids as vertex ids of the whole batch
   {
     Engine e(g, cfg, stats);
     PCC_RETURN_IF_ERROR(e.Run(std::move(nw), G > 1 ? res.data() : nullptr,
-                              host_init ? init.get() : nullptr, out_ids, nullptr));
+                              host_init ? init.get() : nullptr, out_ids, nullptr, &ids));
   }
   if (stats || objectives || modularities) {
     // Final objective on the input graph (PCC.cc:619-624) and modularity
     // (PMC.cc:91-93), per graph; outside the timed span like the reference.
-    std::vector<uint32_t> gl((size_t)std::max<int64_t>(n, 1));
-    for (int o = 0; o < G; ++o)
-      for (int64_t v = base[(size_t)o]; v < base[(size_t)o + 1]; ++v)
-        gl[(size_t)v] = (uint32_t)(base[(size_t)o] + out_ids[v]);
-    DevBuf<uint32_t> ids(&g->alloc, n > 0 ? n : 1);
-    if (n > 0)
-      PCC_HIP_CHECK(hipMemcpyAsync(ids.get(), gl.data(), sizeof(uint32_t) * (size_t)n,
-                                   hipMemcpyHostToDevice, g->stream));
+    if (n == 0) ids.Reset(&g->alloc, 1);
     DevBuf<int64_t> dbase(&g->alloc, (int64_t)base.size());
     DevBuf<double> dres(&g->alloc, G), dgam(&g->alloc, G);
     PCC_HIP_CHECK(hipMemcpyAsync(dbase.get(), base.data(), 8 * base.size(), hipMemcpyHostToDevice, g->stream));
@@ -225,13 +214,9 @@ int ClusterImpl(pcc_graph* g, const pcc_config* cfg_in, int objective, const dou
     if (stats) {
       stats->objective = hobj[0];
       stats->modularity = hmod[0];
-      int64_t k = 0;
-      for (int o = 0; o < G; ++o) {
-        std::vector<uint32_t> sorted(out_ids + base[(size_t)o], out_ids + base[(size_t)o + 1]);
-        std::sort(sorted.begin(), sorted.end());
-        k += (int64_t)(std::unique(sorted.begin(), sorted.end()) - sorted.begin());
-      }
-      stats->num_clusters = k;
+      // Graphs own disjoint id ranges, so the distinct ids of the batch are
+      // the sum of every graph's cluster count.
+      stats->num_clusters = CountDistinctIds(&g->alloc, n, ids.get(), g->stream);
     }
   }
   return PCC_OK;

@@ -214,17 +214,22 @@ __global__ void k_sel_frontier(int64_t n, const uint32_t* __restrict__ vgraph,
     fr[v] = sel[vgraph ? vgraph[v] : 0u];
 }
 
-// Final per-vertex ids, local to each graph.
+// Final per-vertex ids, local to each graph (out) and as vertex ids of the
+// whole batch (global: local + the graph's first vertex).
 __global__ void k_output_ids(int64_t n, const uint32_t* __restrict__ vgraph0,
+                             const int64_t* __restrict__ base0,
                              const uint8_t* __restrict__ use_ids0,
                              const uint32_t* __restrict__ ids0,
                              const uint32_t* __restrict__ chain,
-                             const int64_t* __restrict__ sub, uint32_t* __restrict__ out) {
+                             const int64_t* __restrict__ sub, uint32_t* __restrict__ out,
+                             uint32_t* __restrict__ global) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t o = vgraph0 ? vgraph0[v] : 0u;
     const int64_t x = use_ids0[o] ? (int64_t)ids0[v] : (int64_t)chain[v];
-    out[v] = (uint32_t)(x - sub[o]);
+    const uint32_t local = (uint32_t)(x - sub[o]);
+    out[v] = local;
+    global[v] = local + (uint32_t)(vgraph0 ? base0[o] : 0);
   }
 }
 
@@ -276,6 +281,8 @@ int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint
                     "CSR has out-of-range neighbour ids or decreasing offsets");
   }
   if (h->gbase.empty()) h->gbase = {0, n};
+  h->goff.resize(h->gbase.size());
+  for (size_t i = 0; i < h->gbase.size(); ++i) h->goff[i] = off[h->gbase[i]];
   h->symmetric = -1;
   h->finished = true;
   return PCC_OK;
@@ -325,6 +332,54 @@ __global__ void k_weights_exact(const double* __restrict__ nw, int64_t n,
   if ((threadIdx.x & (kWave - 1)) == 0 && local != 0.0) atomicAdd(mag, local);
 }
 }  // namespace
+
+namespace {
+__global__ void k_mark_present(int64_t n, const uint32_t* __restrict__ ids,
+                               uint8_t* __restrict__ present) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    present[ids[v]] = 1;
+}
+
+__global__ void k_count_bytes(int64_t n, const uint8_t* __restrict__ flag,
+                              unsigned long long* __restrict__ total) {
+  unsigned long long c = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    c += flag[v];
+  for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(total, c);
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double x) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    p[v] = x;
+}
+}  // namespace
+
+int64_t CountDistinctIds(DeviceAllocator* al, int64_t n, const uint32_t* ids, hipStream_t s) {
+  if (n <= 0) return 0;
+  DevBuf<uint8_t> present(al, n);
+  DevBuf<unsigned long long> total(al, 1);
+  PCC_HIP_CHECK(hipMemsetAsync(present.get(), 0, (size_t)n, s));
+  PCC_HIP_CHECK(hipMemsetAsync(total.get(), 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_mark_present, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, n, ids,
+                     present.get());
+  hipLaunchKernelGGL(k_count_bytes, dim3(Grid(n, 256, 2048)), dim3(256), 0, s, n, present.get(),
+                     total.get());
+  PCC_HIP_CHECK(hipGetLastError());
+  unsigned long long h = 0;
+  PCC_HIP_CHECK(hipMemcpyAsync(&h, total.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  return (int64_t)h;
+}
+
+void LaunchFillF64(double* p, int64_t n, double x, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_f64, dim3(Grid(n, 256, 8192)), dim3(256), 0, s, p, n, x);
+  PCC_HIP_CHECK(hipGetLastError());
+}
 
 bool NodeWeightsExact(DeviceAllocator* al, const double* nw, int64_t n, hipStream_t s) {
   if (n <= 0) return true;
@@ -863,7 +918,7 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
 }
 
 int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, uint32_t* host_out,
-                std::vector<int64_t>* out_iter) {
+                std::vector<int64_t>* out_iter, DevBuf<uint32_t>* dev_global) {
   const CsrView g0 = h_->csr.View();
   const int64_t n = g0.n;
   const int G0 = h_->num_graphs();
@@ -1119,9 +1174,12 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
   if (n > 0) {
     DevBuf<uint8_t> duse = Upload(al_, use_ids0, s_);
     DevBuf<int64_t> dsub = Upload(al_, sub, s_);
-    DevBuf<uint32_t> out(al_, n);
+    DevBuf<uint32_t> out(al_, n), global;
+    DevBuf<uint32_t>& gl = dev_global ? *dev_global : global;
+    gl.Reset(al_, n);
     hipLaunchKernelGGL(k_output_ids, dim3(Grid(n, 256, 8192)), dim3(256), 0, s_, n, vgraph0,
-                       duse.get(), levels[0]->ids.get(), cluster_ids.get(), dsub.get(), out.get());
+                       batch ? levels[0]->lay.dbase.get() : (const int64_t*)nullptr, duse.get(),
+                       levels[0]->ids.get(), cluster_ids.get(), dsub.get(), out.get(), gl.get());
     PCC_HIP_CHECK(hipGetLastError());
     PCC_HIP_CHECK(hipMemcpyAsync(host_out, out.get(), sizeof(uint32_t) * (size_t)n,
                                  hipMemcpyDeviceToHost, s_));

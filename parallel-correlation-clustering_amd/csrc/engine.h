@@ -34,6 +34,7 @@ struct pcc_graph {
   // Batch layout: graph g owns vertices [gbase[g], gbase[g + 1]); one graph
   // has gbase = {0, n} and no vgraph.
   std::vector<int64_t> gbase;
+  std::vector<int64_t> goff;  // CSR offset at every gbase entry (edges before graph g)
   pcc::DevBuf<uint32_t> vgraph;
   // Options.
   bool timing = false;
@@ -109,9 +110,10 @@ class Engine {
   // gres: per-graph resolutions (null: cfg.resolution for all); init: device
   // initial cluster ids (global vertex ids of the level) or null.
   // host_out receives each vertex's final cluster id local to its graph;
-  // out_iter (may be null) the final level index of each graph.
+  // out_iter (may be null) the final level index of each graph; dev_global
+  // (may be null) the final ids on device as vertex ids of the whole batch.
   int Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, uint32_t* host_out,
-          std::vector<int64_t>* out_iter);
+          std::vector<int64_t>* out_iter, DevBuf<uint32_t>* dev_global = nullptr);
 
   struct Workspace {
     DevBuf<uint32_t> list, list2, fb_list, moves;
@@ -227,6 +229,9 @@ int CheckMovesConfig(pcc_graph* h, const pcc_config& cfg);
 bool NodeWeightsExact(DeviceAllocator* al, const double* nw, int64_t n, hipStream_t s);
 
 void LaunchIota(uint32_t* p, int64_t n, hipStream_t s);
+void LaunchFillF64(double* p, int64_t n, double x, hipStream_t s);
+// Number of distinct values among ids[0..n), each below n.  Synchronizes.
+int64_t CountDistinctIds(DeviceAllocator* al, int64_t n, const uint32_t* ids, hipStream_t s);
 
 }  // namespace pcc
 
