@@ -243,6 +243,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(args, dom),
+            "l2_hit_rate": measured_group(args, dom, "l2_rmat22.json", "l2_hit_rate"),
             "algorithmic_bytes_per_launch": round(dom_bytes / max(bin_l[dom], 1)),
         },
         # The dominant kernels wait on random lookups rather than on bytes:
@@ -427,18 +428,24 @@ def bench_cluster_moves(args):
     return 0
 
 
-def measured_traffic(args, dom):
-    """HBM bytes per launch of the dominant kernel group, from the calibrated
-    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/ for the default
-    workload and tuning (null otherwise)."""
+def measured_group(args, dom, name, key):
+    """A per-group figure of the dominant kernel group recorded in profiles/
+    (rocprofv3 counter runs of the default workload and tuning; null for
+    other workloads)."""
     default = (args.scale == 22 and args.edge_factor == 16.0 and not args.wave_max
                and not args.block_max and not args.hub_part_edges)
-    path = os.path.join(ROOT, "profiles", "traffic_rmat22.json")
+    path = os.path.join(ROOT, "profiles", name)
     if not default or not os.path.exists(path):
         return None
     with open(path) as f:
         grp = json.load(f).get("groups", {}).get(str(dom))
-    return grp["traffic_bytes_per_round"] if grp else None
+    return grp.get(key) if grp else None
+
+
+def measured_traffic(args, dom):
+    """HBM bytes per launch of the dominant kernel group, from the calibrated
+    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_rmat22.json)."""
+    return measured_group(args, dom, "traffic_rmat22.json", "traffic_bytes_per_round")
 
 
 def c4_graphs(pcc_amd, count, seed=4, select=None):
