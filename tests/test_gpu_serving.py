@@ -81,3 +81,36 @@ def test_server_rejects_request_over_budget(pcc):
         srv.wait(b)
     assert e.value.code == pcc.RESOURCE_EXHAUSTED
     srv.close()
+
+
+def test_server_mixed_sizes_up_to_200k_vertices(pcc):
+    """Requests from the serving workload's size range (1,000 up to 200,000
+    vertices, SBM and RMAT, mean degree 16, resolution U[0.001, 0.1]),
+    submitted in a Poisson stream while batches of other sizes run: every
+    result equals the request's own pcc_cluster run."""
+    rng = np.random.default_rng(72)
+    reqs = []
+    for i in range(16):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        if i % 2:
+            g = pcc.gen_rmat(max(10, int(round(np.log2(n)))), 8.0, w_lo=0.0, w_hi=1.0, seed=7200 + i)
+        else:
+            g = pcc.gen_sbm(n, max(1, n // 50), min(1.0, 16 / 50.0 * 0.9), 16 * 0.1 / n, 0.0, 1.0,
+                            seed=7200 + i)
+        reqs.append((g, float(rng.uniform(0.001, 0.1))))
+    print(f"serving mixed sizes, max n={max(g.n for g, _ in reqs)}", flush=True)
+    cfg = pcc.make_config(move_method=2, refine=1)
+    srv = pcc.Server(cfg, workers=2, max_batch_graphs=16)
+    ids = []
+    for g, res in reqs:
+        ids.append(srv.submit(g, res))
+        time.sleep(float(rng.exponential(0.01)))
+    results = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st.completed == len(reqs) and st.failed == 0
+    for (g, res), (got, obj, _) in zip(reqs, results):
+        single, sst = pcc.DeviceGraph.from_csr(g).cluster(
+            pcc.make_config(resolution=res, move_method=2, refine=1))
+        assert np.array_equal(got, single)
+        assert obj == pytest.approx(sst.objective, rel=1e-12, abs=1e-9)
