@@ -325,6 +325,12 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
                                        8192) */
 #define PCC_OPT_CLUSTER_SUBROUNDS 9 /* async whole-cluster moves: sub-rounds per round
                                        (default 16, range [1, 1024]) */
+#define PCC_OPT_DEVICE_ROUNDS 11   /* 1: sync LOUVAIN rounds with their counts on device,
+                                      one round captured as a HIP graph and replayed when
+                                      PCC_OPT_TIMING is off; 2: the same rounds launched one
+                                      by one; 0 (default, measured fastest): the host reads
+                                      every round's frontier counts back.  All three give
+                                      the same results. */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* Autotuner of the degree-bin thresholds (PCC_OPT_WAVE_MAX_DEG,

@@ -97,6 +97,7 @@ int ClusterEachGraph(pcc_graph* g, const pcc_config* cfg_in, int objective, cons
     h->cluster_subrounds = g->cluster_subrounds;
     h->hub_part_edges = g->hub_part_edges;
     h->pull_fraction = g->pull_fraction;
+    h->device_rounds = g->device_rounds;
     h->th = g->th;
     PCC_RETURN_IF_ERROR(UploadCsr(h, v1 - v0, e1 - e0, lo.data(), ln.data(), w.data() + e0));
     pcc_config c = *cfg_in;
@@ -486,6 +487,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
       if (value < 1 || value > 1024)
         return SetError(PCC_INVALID_ARGUMENT, "cluster sub-rounds must lie in [1, 1024]");
       g->cluster_subrounds = (int)value;
+      return PCC_OK;
+    case PCC_OPT_DEVICE_ROUNDS:
+      if (value < 0 || value > 2)
+        return SetError(PCC_INVALID_ARGUMENT, "device rounds must be 0, 1 or 2");
+      g->device_rounds = (int)value;
       return PCC_OK;
     case PCC_OPT_PULL_FRACTION:
       if (value < 0 || value > 1024)

@@ -419,7 +419,7 @@ Engine::~Engine() {
   }
 }
 
-Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) {
+Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) : na(al, n) {
   const int64_t nn = n > 0 ? n : 1;
   list.Reset(al, nn);
   list2.Reset(al, nn);
@@ -433,6 +433,8 @@ Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) {
   next.Reset(al, nn);
   new_flag.Reset(al, nn);
   rc.Reset(al, 1);
+  dc.Reset(al, 1);
+  dstats.Reset(al, 1);
   // new_flag is kept all-zero between rounds by the new-cluster assignment.
   PCC_HIP_CHECK(hipMemsetAsync(new_flag.get(), 0, (size_t)nn, s));
   PCC_HIP_CHECK(hipMemsetAsync(rc.get(), 0, sizeof(RoundCounters), s));
@@ -856,6 +858,98 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   return f;
 }
 
+bool Engine::DeviceRounds() const {
+  return h_->device_rounds != 0 && !cfg_.async && cfg_.clustering_moves_method == PCC_LOUVAIN &&
+         cfg_.subclustering_method == 0 && exact_nw_;
+}
+
+void Engine::RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, uint8_t* gmoved) {
+  const CsrView& g = c.g;
+  const StateView& st = c.st;
+  const int64_t n = g.n;
+  const bool all_move = cfg_.move_method == PCC_ALL_MOVE;
+  const bool nbhr = cfg_.move_method == PCC_NBHR_MOVE;
+  DeviceCounts* dc = ws.dc.get();
+  const double pull_thr = ((double)g.m + (double)n) * h_->pull_fraction / 1024.0;
+  // The counts binned by the previous round become this round's; the ALL_MOVE
+  // frontier never changes, so its counts stay.
+  LaunchRoundBegin(ws.rc.get(), dc, ws.dstats.get(), all_move, nbhr, pull_thr, s_);
+  LaunchRoundReset(n, ws.flags.get(), all_move ? nullptr : ws.next.get(),
+                   (c.pull && nbhr) ? ws.moved.get() : nullptr, ws.rc.get(), 2,
+                   c.pull ? c.vol.get() : nullptr, s_, &dc->pull_possible);
+  MoveOut out;
+  out.moves = ws.moves.get();
+  out.moved = ws.moved.get();
+  out.moved_clusters = ws.flags.get();
+  Timer all = StartTimer();
+  for (int b0 = 0; b0 < kHubBin;) {
+    const int grp = StatGroupOf(b0);
+    int b1 = b0;
+    bool any = false;
+    for (; b1 < kHubBin && StatGroupOf(b1) == grp; ++b1) any = any || c.caps[b1] > 0;
+    if (any) {
+      Timer tb = StartTimer();
+      for (int b = b0; b < b1; ++b)
+        if (c.caps[b] > 0)
+          LaunchBestMoveBin(b, c.caps[b], false, h_->th, g, st, c.mp,
+                            ws.list.get() + c.layout.base[b], out, ws.fb_list.get(), ws.rc.get(), 0,
+                            s_, dc);
+      StopTimer(tb, grp);
+    }
+    b0 = b1;
+  }
+  const int64_t c7 = c.caps[kBins - 1];
+  if (c7 > 0) {
+    const uint32_t* list7 = ws.list.get() + c.layout.base[kBins - 1];
+    Timer tb = StartTimer();
+    LaunchHubScan(g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_,
+                  &dc->cnt[kBins - 1]);
+    LaunchBestMoveHub(c7, c.caps_te[kBins - 1], false, g, st, c.mp, list7, c.hb, out,
+                      ws.fb_list.get(), ws.rc.get(), s_, dc);
+    StopTimer(tb, StatGroupOf(kHubBin));
+  }
+  int64_t hashed = 0;
+  for (int b = kWaveBin0; b < kBins; ++b) hashed += c.caps[b];
+  if (hashed > 0) {
+    Timer tf = StartTimer();
+    LaunchBestMoveOrdered(g, st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s_);
+    StopTimer(tf, 8);
+  }
+  StopTimer(all, 11);
+  Timer ta = StartTimer();
+  LaunchApply(c.caps, c.layout, ws.list.get(), st, c.mp.new_id, ws.moves.get(), ws.flags.get(),
+              ws.new_flag.get(), ws.rc.get(), s_, dc);
+  RunNewClusterAssign(al_, st, (uint32_t)n, c.mp.vgraph, c.mp.gbase, ws.new_flag.get(),
+                      ws.flags.get(), ws.rc.get(), s_, &ws.na);
+  if (all_move) {  // PCC.cc:299-307: frontier unchanged
+    StopTimer(ta, 9);
+    return;
+  }
+  // Next frontier = neighbours of the seeds (PCC.cc:309-331).
+  const bool by_cluster = cfg_.move_method == PCC_NBHR_CLUSTER_MOVE;
+  const uint32_t* seed_ids = by_cluster ? st.ids : nullptr;
+  ExpandDir dir;
+  if (c.pull) {
+    LaunchSeedVolume(g, seed_ids, ws.flags.get(), ws.moved.get(), c.vol.get(), s_,
+                     &dc->pull_possible);
+    dir.vol = c.vol.get();
+    dir.thr = (unsigned long long)pull_thr;
+  }
+  if (by_cluster) {
+    LaunchExpandClusters(g, st.ids, ws.flags.get(), h_->th.block_max, c.Hubs(), h_->hub_part_edges,
+                         dir, ws.next.get(), s_);
+  } else {
+    LaunchExpandMoved(c.caps, c.caps_te, c.layout, ws.list.get(), c.hb.part_off, c.hb.item_of,
+                      c.hb.part_edges, g, ws.moved.get(), dir, ws.next.get(), s_, dc);
+  }
+  if (c.pull)
+    LaunchPullFrontier(g, seed_ids, ws.flags.get(), ws.moved.get(), h_->th.block_max, c.Hubs(),
+                       h_->hub_part_edges, dir, ws.next.get(), s_);
+  LaunchGraphFlags(n, ws.next.get(), vgraph, gmoved, s_);
+  LaunchBinFrontier(g, ws.next.get(), h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
+  StopTimer(ta, 9);
+}
+
 std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<uint8_t>* sel,
                                               int64_t inner) {
   const int64_t n = L.g.n;
@@ -887,15 +981,69 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
                          L.lay.vgraph.get(), dsel.get(), fr0.get());
   }
   int64_t counts[kBins], te[kBins];
-  BinFrontier(c, ws, all ? nullptr : fr0.get(), counts, te);
-  bool local_moved = true;
   int64_t rounds = 0;
-  for (int64_t it = 0; it < inner && local_moved; ++it) {
-    const int64_t fsize = RoundStep(c, ws, L.lay.vgraph.get(), G, gmoved.get(), counts, te);
-    local_moved = fsize != 0;
-    ++rounds;
-  }
-  {
+  if (DeviceRounds() && inner > 0) {
+    // Every round enqueued back to back; the frontier, its bins and the loop
+    // statistics stay on device.  Rounds after the frontier empties run as
+    // no-ops (every list is empty), as the reference's loop would have
+    // stopped there (PCC.cc:356-363).
+    ResetCounters(ws.rc.get(), true);
+    LaunchBinFrontier(c.g, all ? nullptr : fr0.get(), h_->th, c.layout, ws.list.get(), ws.rc.get(),
+                      s_);
+    LaunchRoundStatsInit(ws.dstats.get(), s_);
+    if (h_->timing || inner == 1 || h_->device_rounds == 2) {
+      for (int64_t it = 0; it < inner; ++it) RoundStepDevice(c, ws, L.lay.vgraph.get(), gmoved.get());
+    } else {
+      // One round captured as a graph (no memset nodes: every clear is a
+      // kernel) and replayed for each round.
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      PCC_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+      try {
+        RoundStepDevice(c, ws, L.lay.vgraph.get(), gmoved.get());
+      } catch (...) {
+        (void)hipStreamEndCapture(s_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      PCC_HIP_CHECK(hipStreamEndCapture(s_, &graph));
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        PCC_HIP_CHECK(ie);
+      }
+      hipError_t le = hipSuccess;
+      for (int64_t it = 0; it < inner && le == hipSuccess; ++it) le = hipGraphLaunch(exec, s_);
+      const hipError_t se = hipStreamSynchronize(s_);
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      PCC_HIP_CHECK(le);
+      PCC_HIP_CHECK(se);
+    }
+    DeviceRoundStats hs;
+    RoundCounters hrc;
+    PCC_HIP_CHECK(hipMemcpyAsync(&hs, ws.dstats.get(), sizeof(hs), hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipMemcpyAsync(&hrc, ws.rc.get(), sizeof(hrc), hipMemcpyDeviceToHost, s_));
+    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    CollectTimers();
+    rounds = (int64_t)hs.rounds;
+    rounds_ += rounds;
+    edges_scanned_ += (int64_t)hs.edges;
+    frontier_vertices_ += (int64_t)hs.verts;
+    bestmove_bytes_ += 12.0 * (double)hs.edges + 40.0 * (double)hs.verts;
+    for (int k = 0; k < kStatGroups; ++k) {
+      bin_edges_[k] += (int64_t)hs.bin_edges[k];
+      bin_vertices_[k] += (int64_t)hs.bin_verts[k];
+    }
+    fallback_ += (int64_t)(hrc.fb_total + hrc.fb_count + hrc.repair_count);
+  } else {
+    BinFrontier(c, ws, all ? nullptr : fr0.get(), counts, te);
+    bool local_moved = true;
+    for (int64_t it = 0; it < inner && local_moved; ++it) {
+      const int64_t fsize = RoundStep(c, ws, L.lay.vgraph.get(), G, gmoved.get(), counts, te);
+      local_moved = fsize != 0;
+      ++rounds;
+    }
     // Ordered-path and repaired vertices of every round of this call.
     RoundCounters hrc;
     int64_t c2[kBins], t2[kBins];

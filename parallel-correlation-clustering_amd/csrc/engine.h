@@ -42,6 +42,10 @@ struct pcc_graph {
   int async_subrounds = 0;  // 0: automatic (see Engine::RoundStep)
   int async_singleton_rule = 0;
   int cluster_subrounds = (int)pcc::kClusterSubRounds;  // async whole-cluster moves
+  // Sync LOUVAIN rounds driven from device (PCC_OPT_DEVICE_ROUNDS): 0 host
+  // read-back per round (measured fastest, the default), 1 graph replay,
+  // 2 device counts with plain launches.
+  int device_rounds = 0;
   int64_t hub_part_edges = pcc::kHubPartEdges;
   // Pull expansion when the seed volume exceeds pull_fraction / 1024 of the
   // graph's (edges + vertices); 1024 disables it.
@@ -121,6 +125,11 @@ class Engine {
     DevBuf<unsigned long long> sub_te;  // bin-7 degree sum per sub-round, 1024
     DevBuf<uint8_t> moved, flags, next, new_flag;
     DevBuf<RoundCounters> rc;
+    // Device-driven rounds: the current round's list counts, the statistics
+    // of the call's rounds, and the new-cluster assignment's temporaries.
+    DevBuf<DeviceCounts> dc;
+    DevBuf<DeviceRoundStats> dstats;
+    NewAssignWork na;
     Workspace(DeviceAllocator* al, int64_t n, hipStream_t s);
   };
 
@@ -178,6 +187,13 @@ class Engine {
   // frontier and receive the next one.  Returns the next frontier size.
   int64_t RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, int G, uint8_t* gmoved,
                     int64_t counts[kBins], int64_t te[kBins]);
+  // The same round with every count on device (DeviceCounts): launches sized
+  // by the level's bin capacities, no host read-back, no allocation, so the
+  // rounds of a call run back to back and one round can be captured as a HIP
+  // graph and replayed.  Sync LOUVAIN rounds without subclustering and with
+  // exact node weights (DeviceRounds()).
+  void RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, uint8_t* gmoved);
+  bool DeviceRounds() const;
   void BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, int64_t counts[kBins],
                    int64_t te[kBins]);
   // Node weights on which FP64 atomics are exact (see NodeWeightsExact);

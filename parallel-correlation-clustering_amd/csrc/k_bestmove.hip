@@ -457,7 +457,9 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            int64_t count, int64_t part_edges,
                                                            int64_t* __restrict__ part_off,
                                                            int64_t* __restrict__ ent_off,
-                                                           uint32_t* __restrict__ item_of) {
+                                                           uint32_t* __restrict__ item_of,
+                                                           const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
   int64_t carry_p = 0, carry_e = 0;
   for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
@@ -620,7 +622,9 @@ __device__ __forceinline__ int64_t PermuteIndex(int64_t i, int64_t count, uint64
 template <int G, bool kAsync>
 __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveParams mp,
                                                   const uint32_t* __restrict__ list,
-                                                  int64_t count, MoveOut out, uint64_t salt) {
+                                                  int64_t count, MoveOut out, uint64_t salt,
+                                                  const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   constexpr int kGroups = 256 / G;
   const int lane = threadIdx.x & (G - 1);
   const int wlane = LaneId();
@@ -1087,7 +1091,9 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
                                                      MoveOut out,
                                                      uint32_t* __restrict__ fb_list,
                                                      RoundCounters* __restrict__ rc,
-                                                     uint64_t salt) {
+                                                     uint64_t salt,
+                                                     const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   constexpr int NW = THREADS / kWave;
   constexpr int E = TABLE / (2 * THREADS);
   static_assert(E >= 1 && E * THREADS * 2 == TABLE, "table holds two slots per edge slot");
@@ -1347,7 +1353,9 @@ template <bool kAsync>
 __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateView st,
                                                                MoveParams mp,
                                                                const uint32_t* __restrict__ list,
-                                                               int64_t count, HubScratch hb) {
+                                                               int64_t count, HubScratch hb,
+                                                               const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   __shared__ uint32_t lh[kGatherMaxBuckets];
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
@@ -1465,7 +1473,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
                                                              MoveParams mp,
                                                              const uint32_t* __restrict__ list,
                                                              int64_t count, HubScratch hb,
-                                                             RoundCounters* __restrict__ rc) {
+                                                             RoundCounters* __restrict__ rc,
+                                                             const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   __shared__ uint32_t keys[TABLE];
   __shared__ double vals[TABLE];
   __shared__ uint32_t cnt[TABLE];
@@ -1598,7 +1608,9 @@ __global__ __launch_bounds__(256) void k_bm_hub_final(CsrView g, StateView st, M
                                                       const uint32_t* __restrict__ list,
                                                       int64_t count, HubScratch hb, MoveOut out,
                                                       uint32_t* __restrict__ fb_list,
-                                                      RoundCounters* __restrict__ rc) {
+                                                      RoundCounters* __restrict__ rc,
+                                                      const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
   const int lane = LaneId();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
@@ -1843,9 +1855,10 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 }
 
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
-                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s) {
+                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
+                   const unsigned long long* dcount) {
   hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
-                     part_edges, part_off, ent_off, item_of);
+                     part_edges, part_off, ent_off, item_of, dcount);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -1858,12 +1871,18 @@ void LaunchCollectHubs(const CsrView& g, const BinThresholds& th, uint32_t* list
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+// A device-driven round sizes the grid by the bin's capacity, so it is
+// capped (the kernel strides): a late round's short list then does not pay
+// for a grid of the level's whole bin.
+constexpr int kDeviceSmallGrid = 4096;
+
 template <int G, bool A>
 static void LaunchSmall(int64_t count, const CsrView& g, const StateView& st, const MoveParams& mp,
-                        const uint32_t* list, const MoveOut& out, uint64_t salt, hipStream_t s) {
-  const int grid = GridFor(count, 256 / G, 1 << 20);
+                        const uint32_t* list, const MoveOut& out, uint64_t salt, hipStream_t s,
+                        const unsigned long long* dcount) {
+  const int grid = GridFor(count, 256 / G, dcount ? kDeviceSmallGrid : 1 << 20);
   hipLaunchKernelGGL((k_bm_small<G, A>), dim3(grid), dim3(256), 0, s, g, st, mp, list, count, out,
-                     salt);
+                     salt, dcount);
 }
 
 // One hash bin: the kernel whose table fits the bin's degree range (dlo, dhi]
@@ -1873,7 +1892,8 @@ template <int THREADS, int TABLE, bool A>
 static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_waves,
                        const CsrView& g, const StateView& st, const MoveParams& mp,
                        const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
+                       RoundCounters* rc, uint64_t salt, hipStream_t s,
+                       const unsigned long long* dcount) {
   if (dhi <= dlo || count <= 0) return;
   int max_grid = 8192;
   if (hash_waves > 0) {
@@ -1886,54 +1906,56 @@ static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_wav
   }
   const int grid = GridFor(count, 1, max_grid);
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
-                     list, count, dlo, dhi, out, fb_list, rc, salt);
+                     list, count, dlo, dhi, out, fb_list, rc, salt, dcount);
 }
 
 template <bool A>
 static void LaunchHashBin(int bin, int64_t count, const BinThresholds& th, const CsrView& g,
                           const StateView& st, const MoveParams& mp, const uint32_t* list,
                           const MoveOut& out, uint32_t* fb_list, RoundCounters* rc,
-                          uint64_t salt, hipStream_t s) {
+                          uint64_t salt, hipStream_t s, const unsigned long long* dcount) {
   const HashBounds h = HashBoundsOf(th.wave_max, th.block_max);
   const int k = bin - kWaveBin0;
   const int64_t lo = k == 0 ? 64 : h.b[k - 1], hi = h.b[k];
   const int64_t hw = th.hash_waves;
   switch (bin) {
-    case 5: LaunchHash<64, 256, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
-    case 6: LaunchHash<64, 512, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
-    case 7: LaunchHash<128, 1024, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
-    case 8: LaunchHash<256, 2048, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
-    case 9: LaunchHash<512, 4096, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s); break;
+    case 5: LaunchHash<64, 256, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
+    case 6: LaunchHash<64, 512, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
+    case 7: LaunchHash<128, 1024, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
+    case 8: LaunchHash<256, 2048, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
+    case 9: LaunchHash<512, 4096, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
     default:
-      LaunchHash<1024, 8192, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s);
+      LaunchHash<1024, 8192, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount);
   }
 }
 
 void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& th,
                        const CsrView& g, const StateView& st, const MoveParams& mp,
                        const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, uint64_t salt, hipStream_t s) {
+                       RoundCounters* rc, uint64_t salt, hipStream_t s,
+                       const DeviceCounts* dc) {
   if (count <= 0) return;
+  const unsigned long long* dcount = dc ? &dc->cnt[bin] : nullptr;
   switch (bin) {
     case 0:
-      async ? LaunchSmall<4, true>(count, g, st, mp, list, out, salt, s)
-            : LaunchSmall<4, false>(count, g, st, mp, list, out, salt, s);
+      async ? LaunchSmall<4, true>(count, g, st, mp, list, out, salt, s, dcount)
+            : LaunchSmall<4, false>(count, g, st, mp, list, out, salt, s, dcount);
       break;
     case 1:
-      async ? LaunchSmall<8, true>(count, g, st, mp, list, out, salt, s)
-            : LaunchSmall<8, false>(count, g, st, mp, list, out, salt, s);
+      async ? LaunchSmall<8, true>(count, g, st, mp, list, out, salt, s, dcount)
+            : LaunchSmall<8, false>(count, g, st, mp, list, out, salt, s, dcount);
       break;
     case 2:
-      async ? LaunchSmall<16, true>(count, g, st, mp, list, out, salt, s)
-            : LaunchSmall<16, false>(count, g, st, mp, list, out, salt, s);
+      async ? LaunchSmall<16, true>(count, g, st, mp, list, out, salt, s, dcount)
+            : LaunchSmall<16, false>(count, g, st, mp, list, out, salt, s, dcount);
       break;
     case 3:
-      async ? LaunchSmall<32, true>(count, g, st, mp, list, out, salt, s)
-            : LaunchSmall<32, false>(count, g, st, mp, list, out, salt, s);
+      async ? LaunchSmall<32, true>(count, g, st, mp, list, out, salt, s, dcount)
+            : LaunchSmall<32, false>(count, g, st, mp, list, out, salt, s, dcount);
       break;
     case 4:
-      async ? LaunchSmall<64, true>(count, g, st, mp, list, out, salt, s)
-            : LaunchSmall<64, false>(count, g, st, mp, list, out, salt, s);
+      async ? LaunchSmall<64, true>(count, g, st, mp, list, out, salt, s, dcount)
+            : LaunchSmall<64, false>(count, g, st, mp, list, out, salt, s, dcount);
       break;
     case 5:
     case 6:
@@ -1941,8 +1963,8 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
     case 8:
     case 9:
     case 10:
-      async ? LaunchHashBin<true>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s)
-            : LaunchHashBin<false>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s);
+      async ? LaunchHashBin<true>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s, dcount)
+            : LaunchHashBin<false>(bin, count, th, g, st, mp, list, out, fb_list, rc, salt, s, dcount);
       break;
     default:
       throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: the hub bin goes through LaunchBestMoveHub",
@@ -1956,21 +1978,23 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
 template <bool A>
 static void LaunchHubPart(int64_t items, const CsrView& g, const StateView& st,
                           const MoveParams& mp, const uint32_t* list, int64_t count,
-                          const HubScratch& hb, RoundCounters* rc, hipStream_t s) {
+                          const HubScratch& hb, RoundCounters* rc, hipStream_t s,
+                          const unsigned long long* dcount) {
   const dim3 grid((unsigned)items), block(kHubThreads);
   if (hb.part_edges <= 1024)
-    hipLaunchKernelGGL((k_bm_hub_part<2048, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+    hipLaunchKernelGGL((k_bm_hub_part<2048, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc, dcount);
   else if (hb.part_edges <= 2048)
-    hipLaunchKernelGGL((k_bm_hub_part<4096, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+    hipLaunchKernelGGL((k_bm_hub_part<4096, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc, dcount);
   else
-    hipLaunchKernelGGL((k_bm_hub_part<8192, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc);
+    hipLaunchKernelGGL((k_bm_hub_part<8192, A>), grid, block, 0, s, g, st, mp, list, count, hb, rc, dcount);
 }
 
 void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
                        const StateView& st, const MoveParams& mp, const uint32_t* list,
                        const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, hipStream_t s) {
+                       RoundCounters* rc, hipStream_t s, const DeviceCounts* dc) {
   if (count <= 0) return;
+  const unsigned long long* dcount = dc ? &dc->cnt[kHubBin] : nullptr;
   // sum_i ceil(deg_i / E) < te / E + count bounds the partition items; the
   // grid is a whole number of XCD chunks (XcdChunkItem).
   int64_t items = count + te / hb.part_edges + 1;
@@ -1981,16 +2005,16 @@ void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
   const int fin = GridFor(count, 256 / kWave, 1 << 16);
   if (async) {
     hipLaunchKernelGGL(k_hub_gather<true>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
-                       count, hb);
-    LaunchHubPart<true>(items, g, st, mp, list, count, hb, rc, s);
+                       count, hb, dcount);
+    LaunchHubPart<true>(items, g, st, mp, list, count, hb, rc, s, dcount);
     hipLaunchKernelGGL(k_bm_hub_final<true>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count, hb,
-                       out, fb_list, rc);
+                       out, fb_list, rc, dcount);
   } else {
     hipLaunchKernelGGL(k_hub_gather<false>, dim3((unsigned)items), dim3(256), 0, s, g, st, mp, list,
-                       count, hb);
-    LaunchHubPart<false>(items, g, st, mp, list, count, hb, rc, s);
+                       count, hb, dcount);
+    LaunchHubPart<false>(items, g, st, mp, list, count, hb, rc, s, dcount);
     hipLaunchKernelGGL(k_bm_hub_final<false>, dim3(fin), dim3(256), 0, s, g, st, mp, list, count,
-                       hb, out, fb_list, rc);
+                       hb, out, fb_list, rc, dcount);
   }
   PCC_HIP_CHECK(hipGetLastError());
 }

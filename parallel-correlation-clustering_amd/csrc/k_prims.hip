@@ -84,9 +84,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const Tin* __restri
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
 
+// work (may be null): ScanWorkElems(n) elements for the tile sums of every
+// level of the recursion, so the scan allocates nothing (and can run inside a
+// captured graph); null draws them from the allocator.
 template <typename Tin, typename Tout>
 void ScanImpl(DeviceAllocator* al, const Tin* in, Tout* out, int64_t n, Tout* total, hipStream_t s,
-              const unsigned long long* gate) {
+              const unsigned long long* gate, Tout* work = nullptr) {
   if (n <= 0) {
     if (total) PCC_HIP_CHECK(hipMemsetAsync(total, 0, sizeof(Tout), s));
     return;
@@ -98,13 +101,22 @@ void ScanImpl(DeviceAllocator* al, const Tin* in, Tout* out, int64_t n, Tout* to
     PCC_HIP_CHECK(hipGetLastError());
     return;
   }
-  DevBuf<Tout> sums(al, tiles), offs(al, tiles);
+  DevBuf<Tout> sums_b, offs_b;
+  Tout* sums = work;
+  Tout* offs = work ? work + tiles : nullptr;
+  if (!work) {
+    sums_b.Reset(al, tiles);
+    offs_b.Reset(al, tiles);
+    sums = sums_b.get();
+    offs = offs_b.get();
+  }
   hipLaunchKernelGGL((k_scan_tile_sums<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
-                     s, in, n, sums.get(), gate);
+                     s, in, n, sums, gate);
   PCC_HIP_CHECK(hipGetLastError());
-  ScanImpl<Tout, Tout>(al, sums.get(), offs.get(), tiles, (Tout*)nullptr, s, gate);
+  ScanImpl<Tout, Tout>(al, sums, offs, tiles, (Tout*)nullptr, s, gate,
+                       work ? work + 2 * tiles : nullptr);
   hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanThreads), 0, s,
-                     in, out, n, (const Tout*)offs.get(), total, gate);
+                     in, out, n, (const Tout*)offs, total, gate);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -310,9 +322,17 @@ void RadixSortImpl(DeviceAllocator* al, uint64_t* keys, V* vals, int64_t n, int 
 
 }  // namespace
 
+int64_t ScanWorkElems(int64_t n) {
+  int64_t e = 0;
+  for (int64_t t = (n + kScanTile - 1) / kScanTile; t > 1; t = (t + kScanTile - 1) / kScanTile)
+    e += 2 * t;
+  return e;
+}
+
 void ExclusiveScanU8toU32(DeviceAllocator* al, const uint8_t* in, uint32_t* out, int64_t n,
-                          uint32_t* total, hipStream_t s, const unsigned long long* gate) {
-  ScanImpl<uint8_t, uint32_t>(al, in, out, n, total, s, gate);
+                          uint32_t* total, hipStream_t s, const unsigned long long* gate,
+                          uint32_t* work) {
+  ScanImpl<uint8_t, uint32_t>(al, in, out, n, total, s, gate, work);
 }
 void ExclusiveScanU32toI64(DeviceAllocator* al, const uint32_t* in, int64_t* out, int64_t n,
                            int64_t* total, hipStream_t s) {

@@ -10,6 +10,9 @@
 // weights, degrees, and their aggregates on contracted levels) every partial
 // sum is an exact integer, so the atomic order cannot change a result.
 
+#include <algorithm>
+#include <memory>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -22,12 +25,25 @@ struct Bins {
   int64_t count[kBins];
 };
 
+// Device-driven rounds: the launch sized bins by their capacities; the bins
+// a launch covers (count > 0) take their current counts from dcnt.
+__device__ __forceinline__ int64_t TakeCounts(Bins& bins, const unsigned long long* dcnt) {
+  int64_t total = 0;
+  for (int b = 0; b < kBins; ++b) {
+    if (dcnt && bins.count[b] > 0) bins.count[b] = (int64_t)dcnt[b];
+    total += bins.count[b];
+  }
+  return total;
+}
+
 __global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __restrict__ list,
                                                StateView st, uint32_t new_id,
                                                const uint32_t* __restrict__ moves,
                                                uint8_t* __restrict__ modified,
                                                uint8_t* __restrict__ new_flag,
-                                               RoundCounters* __restrict__ rc) {
+                                               RoundCounters* __restrict__ rc,
+                                               const unsigned long long* __restrict__ dcnt) {
+  TakeCounts(bins, dcnt);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long new_movers = 0;
   for (int b = 0; b < kBins; ++b) {
@@ -170,7 +186,9 @@ __global__ __launch_bounds__(256) void k_round_reset(int64_t n, uint8_t* __restr
                                                      uint8_t* __restrict__ b,
                                                      uint8_t* __restrict__ c,
                                                      RoundCounters* __restrict__ rc, int counters,
-                                                     unsigned long long* __restrict__ acc) {
+                                                     unsigned long long* __restrict__ acc,
+                                                     const unsigned int* __restrict__ c_gate) {
+  if (c_gate && *c_gate == 0) c = nullptr;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t n16 = n / 16;
@@ -225,8 +243,10 @@ __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint8_t* __restrict__ moved,
-                                                      ExpandDir dir, uint8_t* __restrict__ next) {
+                                                      ExpandDir dir, uint8_t* __restrict__ next,
+                                                      const unsigned long long* __restrict__ dcnt) {
   if (dir.Pull()) return;
+  if (dcnt) total = TakeCounts(bins, dcnt);
   for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
     int64_t r = t;
     int b = 0;
@@ -244,8 +264,10 @@ __global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
                                                       const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint8_t* __restrict__ moved,
-                                                      ExpandDir dir, uint8_t* __restrict__ next) {
+                                                      ExpandDir dir, uint8_t* __restrict__ next,
+                                                      const unsigned long long* __restrict__ dcnt) {
   if (dir.Pull()) return;
+  if (dcnt) total = TakeCounts(bins, dcnt);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t = base + threadIdx.x;
@@ -299,8 +321,10 @@ __global__ __launch_bounds__(256) void k_expand_hub(const uint32_t* __restrict__
                                                     const uint8_t* __restrict__ moved,
                                                     const uint32_t* __restrict__ ids,
                                                     const uint8_t* __restrict__ modified,
-                                                    ExpandDir dir, uint8_t* __restrict__ next) {
+                                                    ExpandDir dir, uint8_t* __restrict__ next,
+                                                    const unsigned long long* __restrict__ dcount) {
   if (dir.Pull()) return;
+  if (dcount) count = (int64_t)*dcount;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= part_off[count]) return;
   const int64_t i = item_of[t];
@@ -330,7 +354,9 @@ __global__ __launch_bounds__(256) void k_seed_volume(int64_t n, const int64_t* _
                                                      const uint32_t* __restrict__ ids,
                                                      const uint8_t* __restrict__ modified,
                                                      const uint8_t* __restrict__ moved,
-                                                     unsigned long long* __restrict__ vol) {
+                                                     unsigned long long* __restrict__ vol,
+                                                     const unsigned int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ unsigned long long sm[256 / kWave];
   unsigned long long x = 0;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
@@ -446,12 +472,12 @@ Bins MakeBins(const int64_t counts[kBins], const BinLayout& layout, int64_t* tot
 
 void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
                  const StateView& st, uint32_t new_id, const uint32_t* moves, uint8_t* modified,
-                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s) {
+                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s, const DeviceCounts* dc) {
   int64_t total;
   const Bins b = MakeBins(counts, layout, &total);
   if (total == 0) return;
   hipLaunchKernelGGL(k_apply, dim3(Grid(total, 256, 4096)), dim3(256), 0, s, b, list, st, new_id,
-                     moves, modified, new_flag, rc);
+                     moves, modified, new_flag, rc, dc ? dc->cnt : (const unsigned long long*)nullptr);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -629,10 +655,66 @@ __global__ __launch_bounds__(256) void k_graph_flags(int64_t n, const uint8_t* _
 }
 
 void LaunchRoundReset(int64_t n, uint8_t* a, uint8_t* b, uint8_t* c, RoundCounters* rc,
-                      int counters, unsigned long long* acc, hipStream_t s) {
+                      int counters, unsigned long long* acc, hipStream_t s,
+                      const unsigned int* c_gate) {
   const int64_t work = (a || b || c) ? (n + 15) / 16 : 1;
   hipLaunchKernelGGL(k_round_reset, dim3(Grid(work, 256, 2048)), dim3(256), 0, s, n, a, b, c, rc,
-                     counters, acc);
+                     counters, acc, c_gate);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+// One wave: the round's list counts move from rc (where the binning
+// accumulated them) to dc, and the round's statistics are added up.
+__global__ __launch_bounds__(64) void k_round_begin(RoundCounters* __restrict__ rc,
+                                                    DeviceCounts* __restrict__ dc,
+                                                    DeviceRoundStats* __restrict__ st, int keep,
+                                                    int nbhr_move, double pull_thr) {
+  const int b = threadIdx.x;
+  unsigned long long c = 0, t = 0;
+  if (b < kBins) {
+    c = rc->cnt[b];
+    t = rc->te[b];
+    dc->cnt[b] = c;
+    dc->te[b] = t;
+    if (!keep) {
+      rc->cnt[b] = 0;
+      rc->te[b] = 0;
+    }
+    const int grp = StatGroupOf(b);
+    if (c) atomicAdd(&st->bin_verts[grp], c);
+    if (t) atomicAdd(&st->bin_edges[grp], t);
+  }
+  unsigned long long F = c, TE = t;
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    F += __shfl_xor(F, sh);
+    TE += __shfl_xor(TE, sh);
+  }
+  if (b == 0) {
+    st->edges += TE;
+    st->verts += F;
+    if (st->first || F > 0) st->rounds += 1;
+    st->first = 0;
+    dc->pull_possible = (!nbhr_move || (double)(TE + F) > pull_thr) ? 1u : 0u;
+  }
+}
+
+__global__ void k_round_stats_init(DeviceRoundStats* __restrict__ st) {
+  if (threadIdx.x == 0) {
+    st->rounds = st->edges = st->verts = 0;
+    for (int k = 0; k < kStatGroups; ++k) st->bin_edges[k] = st->bin_verts[k] = 0;
+    st->first = 1;
+  }
+}
+
+void LaunchRoundStatsInit(DeviceRoundStats* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_round_stats_init, dim3(1), dim3(64), 0, s, stats);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchRoundBegin(RoundCounters* rc, DeviceCounts* dc, DeviceRoundStats* stats, bool keep,
+                      bool nbhr_move, double pull_thr, hipStream_t s) {
+  hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, s, rc, dc, stats, keep ? 1 : 0,
+                     nbhr_move ? 1 : 0, pull_thr);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -644,29 +726,55 @@ void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, ui
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+__global__ __launch_bounds__(256) void k_fill_u32_gated(uint32_t* __restrict__ p, int64_t n,
+                                                        uint32_t x,
+                                                        const unsigned long long* __restrict__ gate) {
+  if (*gate == 0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = x;
+}
+
+NewAssignWork::NewAssignWork(DeviceAllocator* al, int64_t n) {
+  const int64_t nn = n > 0 ? n : 1;
+  zrank.Reset(al, nn);
+  zero_list.Reset(al, nn);
+  nrank.Reset(al, nn);
+  if (n > (int64_t)kSmallAssignMax) {
+    z.Reset(al, nn);
+    scan.Reset(al, std::max<int64_t>(1, ScanWorkElems(n)));
+  }
+}
+
 void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
-                         uint8_t* modified, RoundCounters* rc, hipStream_t s) {
+                         uint8_t* modified, RoundCounters* rc, hipStream_t s,
+                         const NewAssignWork* work) {
   const unsigned long long* gate = &rc->new_count;
   if (n == 0) return;
-  DevBuf<uint32_t> zrank(al, n), zero_list(al, n), nrank(al, n);
+  std::unique_ptr<NewAssignWork> own;
+  if (!work) {
+    own.reset(new NewAssignWork(al, n));
+    work = own.get();
+  }
   if (n <= kSmallAssignMax) {
     hipLaunchKernelGGL(k_new_assign_small, dim3(1), dim3(kSmallAssignThreads), 0, s, n, st,
-                       vgraph, gbase, new_flag, zero_list.get(), zrank.get(), nrank.get(),
-                       modified, gate);
+                       vgraph, gbase, new_flag, work->zero_list.get(), work->zrank.get(),
+                       work->nrank.get(), modified, gate);
     PCC_HIP_CHECK(hipGetLastError());
     return;
   }
-  DevBuf<uint8_t> z(al, n);
   const int grid = Grid(n, 256, 4096);
-  PCC_HIP_CHECK(hipMemsetAsync(zero_list.get(), 0xFF, sizeof(uint32_t) * (size_t)n, s));
-  hipLaunchKernelGGL(k_zero_flags, dim3(grid), dim3(256), 0, s, n, st.sizes, z.get(), gate);
-  ExclusiveScanU8toU32(al, z.get(), zrank.get(), n, nullptr, s, gate);
-  hipLaunchKernelGGL(k_zero_compact, dim3(grid), dim3(256), 0, s, n, z.get(), zrank.get(),
-                     zero_list.get(), gate);
-  ExclusiveScanU8toU32(al, new_flag, nrank.get(), n, nullptr, s, gate);
+  hipLaunchKernelGGL(k_fill_u32_gated, dim3(grid), dim3(256), 0, s, work->zero_list.get(),
+                     (int64_t)n, 0xFFFFFFFFu, gate);
+  hipLaunchKernelGGL(k_zero_flags, dim3(grid), dim3(256), 0, s, n, st.sizes, work->z.get(), gate);
+  ExclusiveScanU8toU32(al, work->z.get(), work->zrank.get(), n, nullptr, s, gate,
+                       work->scan.get());
+  hipLaunchKernelGGL(k_zero_compact, dim3(grid), dim3(256), 0, s, n, work->z.get(),
+                     work->zrank.get(), work->zero_list.get(), gate);
+  ExclusiveScanU8toU32(al, new_flag, work->nrank.get(), n, nullptr, s, gate, work->scan.get());
   hipLaunchKernelGGL(k_new_assign, dim3(grid), dim3(256), 0, s, n, st, vgraph, gbase, new_flag,
-                     nrank.get(), zrank.get(), zero_list.get(), modified, gate);
+                     work->nrank.get(), work->zrank.get(), work->zero_list.get(), modified, gate);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
@@ -725,7 +833,8 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
                        const int64_t* hub_part_off, const uint32_t* hub_item_of,
                        int64_t part_edges, const CsrView& g, const uint8_t* moved,
-                       const ExpandDir& dir, uint8_t* next, hipStream_t s) {
+                       const ExpandDir& dir, uint8_t* next, hipStream_t s, const DeviceCounts* dc) {
+  const unsigned long long* dcnt = dc ? dc->cnt : nullptr;
   int64_t total;
   const Bins all = MakeBins(counts, layout, &total);
   Bins lo = all, mid = all;
@@ -738,13 +847,13 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
     nmid += mid.count[b];
   }
   if (nlo > 0) {
-    hipLaunchKernelGGL(k_expand_moved, dim3(Grid(nlo, 256, 8192)), dim3(256), 0, s, lo, nlo, list,
-                       g.off, g.nbr, moved, dir, next);
+    hipLaunchKernelGGL(k_expand_moved, dim3(Grid(nlo, 256, dc ? 4096 : 8192)), dim3(256), 0, s, lo,
+                       nlo, list, g.off, g.nbr, moved, dir, next, dcnt);
     PCC_HIP_CHECK(hipGetLastError());
   }
   if (nmid > 0) {
     hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid, nmid, list,
-                       g.off, g.nbr, moved, dir, next);
+                       g.off, g.nbr, moved, dir, next, dcnt);
     PCC_HIP_CHECK(hipGetLastError());
   }
   const int64_t c7 = counts[kBins - 1];
@@ -753,7 +862,8 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
     items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
     hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s,
                        list + layout.base[kBins - 1], c7, hub_part_off, hub_item_of, g.off, g.nbr,
-                       moved, (const uint32_t*)nullptr, (const uint8_t*)nullptr, dir, next);
+                       moved, (const uint32_t*)nullptr, (const uint8_t*)nullptr, dir, next,
+                       dc ? &dc->cnt[kBins - 1] : (const unsigned long long*)nullptr);
     PCC_HIP_CHECK(hipGetLastError());
   }
 }
@@ -770,16 +880,18 @@ void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* 
     items = (items + kXcdGrid - 1) / kXcdGrid * kXcdGrid;
     hipLaunchKernelGGL(k_expand_hub, dim3((unsigned)items), dim3(256), 0, s, hubs.list,
                        hubs.count, hubs.part_off, hubs.item_of, g.off, g.nbr,
-                       (const uint8_t*)nullptr, ids, modified, dir, next);
+                       (const uint8_t*)nullptr, ids, modified, dir, next,
+                       (const unsigned long long*)nullptr);
     PCC_HIP_CHECK(hipGetLastError());
   }
 }
 
 void LaunchSeedVolume(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
-                      const uint8_t* moved, unsigned long long* vol, hipStream_t s) {
+                      const uint8_t* moved, unsigned long long* vol, hipStream_t s,
+                      const unsigned int* gate) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_seed_volume, dim3(Grid(g.n, 256, 2048)), dim3(256), 0, s, g.n, g.off, ids,
-                     modified, moved, vol);
+                     modified, moved, vol, gate);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

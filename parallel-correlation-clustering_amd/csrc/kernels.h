@@ -133,7 +133,7 @@ constexpr int kWaveBin0 = 5;   // first one-wave hash bin
 constexpr int kBlockBin0 = 8;  // first one-workgroup hash bin
 constexpr int kHubBin = 11;
 constexpr int kStatGroups = 8;
-inline int StatGroupOf(int b) {
+__host__ __device__ inline int StatGroupOf(int b) {
   return b < kWaveBin0 ? b : (b < kBlockBin0 ? 5 : (b < kHubBin ? 6 : 7));
 }
 struct BinThresholds {
@@ -162,6 +162,27 @@ struct RoundCounters {
 
 struct BinLayout {
   int64_t base[kBins];
+};
+
+// Device-driven rounds: the counts of the current round's lists (the
+// previous round's binning, moved here when the round starts) live on device,
+// so a round's launches are sized by the level's capacities, every kernel
+// reads its list's count, and nothing is read back between rounds.
+struct DeviceCounts {
+  unsigned long long cnt[kBins];
+  unsigned long long te[kBins];
+  // 1 when the frontier's volume can exceed the pull threshold (a pull
+  // expansion may then be chosen; NBHR_MOVE must then clear every mover flag).
+  unsigned int pull_possible;
+};
+
+// Statistics of device-driven rounds, accumulated on device and read once per
+// IterateBestMoves call.  A round counts when its frontier is non-empty, or
+// when it is the call's first (IterateBestMoves, PCC.cc:346-363).
+struct DeviceRoundStats {
+  unsigned long long rounds, edges, verts;
+  unsigned long long bin_edges[kStatGroups], bin_verts[kStatGroups];
+  unsigned int first;
 };
 
 // Parameters of the best-move kernels.  Cluster ids are global vertex ids of
@@ -291,7 +312,8 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 // partitions of part_edges edges, and the list item of every partition
 // (item_of[part_off[i] + p] = i).
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
-                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s);
+                   int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
+                   const unsigned long long* dcount = nullptr);
 
 // Every vertex of bin 7 (deg > th.block_max) appended to list; *count zeroed
 // by the caller.
@@ -300,17 +322,20 @@ void LaunchCollectHubs(const CsrView& g, const BinThresholds& th, uint32_t* list
 
 // One best-move kernel per bin 0..6.  count = number of listed vertices in
 // the bin.
+// dc (may be null): device-driven round; count is then the bin's capacity
+// on the level (the launch bound) and the kernel reads dc->cnt[bin].
 void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& th,
                        const CsrView& g, const StateView& st, const MoveParams& mp,
                        const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, uint64_t permute_salt, hipStream_t s);
+                       RoundCounters* rc, uint64_t permute_salt, hipStream_t s,
+                       const DeviceCounts* dc = nullptr);
 
 // Bin 7: partition pass and selection pass over a list whose HubScratch
 // prefix sums are current; te = sum of the listed degrees.
 void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
                        const StateView& st, const MoveParams& mp, const uint32_t* list,
                        const HubScratch& hb, const MoveOut& out, uint32_t* fb_list,
-                       RoundCounters* rc, hipStream_t s);
+                       RoundCounters* rc, hipStream_t s, const DeviceCounts* dc = nullptr);
 
 // Ordered (sequential-order-exact) best moves for the fallback list.
 void LaunchBestMoveOrdered(const CsrView& g, const StateView& st, const MoveParams& mp,
@@ -320,7 +345,8 @@ void LaunchBestMoveOrdered(const CsrView& g, const StateView& st, const MovePara
 // Sync apply, MoveNodesToCluster (parallel-correlation-clusterer-internal.cc:256-369).
 void LaunchApply(const int64_t counts[kBins], const BinLayout& layout, const uint32_t* list,
                  const StateView& st, uint32_t new_id, const uint32_t* moves, uint8_t* modified,
-                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s);
+                 uint8_t* new_flag, RoundCounters* rc, hipStream_t s,
+                 const DeviceCounts* dc = nullptr);
 
 // The same apply with the cluster-weight arithmetic in the reference's order
 // (losses of each old cluster, then gains of each new one, members in
@@ -337,10 +363,18 @@ void LaunchApplyOrdered(DeviceAllocator* al, const int64_t counts[kBins], const 
 void OrderedClusterWeights(DeviceAllocator* al, int64_t n, const StateView& st, hipStream_t s);
 
 // Empty-id reuse for vertices forming new clusters, per graph of a batch
-// (vgraph / gbase null for one graph); gated on rc->new_count.
+// (vgraph / gbase null for one graph); gated on rc->new_count.  work (may be
+// null: allocated per call) holds its n-sized temporaries, so a caller that
+// keeps one per level runs the assignment with no allocation.
+struct NewAssignWork {
+  DevBuf<uint32_t> zrank, zero_list, nrank, scan;
+  DevBuf<uint8_t> z;
+  NewAssignWork(DeviceAllocator* al, int64_t n);
+};
 void RunNewClusterAssign(DeviceAllocator* al, const StateView& st, uint32_t n,
                          const uint32_t* vgraph, const int64_t* gbase, uint8_t* new_flag,
-                         uint8_t* modified, RoundCounters* rc, hipStream_t s);
+                         uint8_t* modified, RoundCounters* rc, hipStream_t s,
+                         const NewAssignWork* work = nullptr);
 
 // Sync set moves of arbitrary vertex groups (grp[v] < n; one graph): every
 // group with members gets BestMove(graph, members)
@@ -381,8 +415,20 @@ void PrepareFreeIds(DeviceAllocator* al, const StateView& st, uint32_t n, int G,
 // bit 1: fb_count / new_count / repair_count, after adding fb_count and
 // repair_count to fb_total) and *acc (may be null).  Byte
 // arrays must start 16-byte aligned (device allocations are).
+// c_gate (may be null): c is cleared only when *c_gate != 0.
 void LaunchRoundReset(int64_t n, uint8_t* a, uint8_t* b, uint8_t* c, RoundCounters* rc,
-                      int counters, unsigned long long* acc, hipStream_t s);
+                      int counters, unsigned long long* acc, hipStream_t s,
+                      const unsigned int* c_gate = nullptr);
+
+// Start of a device-driven round: dc <- the binned counts of rc (then zeroed
+// there unless keep, the ALL_MOVE frontier that never changes), the round's
+// statistics into stats, and dc->pull_possible from the frontier's volume
+// against pull_thr (always 1 unless nbhr_move: the NBHR_MOVE seeds are a
+// subset of the frontier).
+void LaunchRoundBegin(RoundCounters* rc, DeviceCounts* dc, DeviceRoundStats* stats, bool keep,
+                      bool nbhr_move, double pull_thr, hipStream_t s);
+// Zero statistics with the next round marked as a call's first.
+void LaunchRoundStatsInit(DeviceRoundStats* stats, hipStream_t s);
 
 // flags[graph of v] = 1 for every v with next[v] != 0.
 void LaunchGraphFlags(int64_t n, const uint8_t* next, const uint32_t* vgraph, uint8_t* flags,
@@ -415,7 +461,8 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
                        const BinLayout& layout, const uint32_t* list,
                        const int64_t* hub_part_off, const uint32_t* hub_item_of,
                        int64_t part_edges, const CsrView& g, const uint8_t* moved,
-                       const ExpandDir& dir, uint8_t* next, hipStream_t s);
+                       const ExpandDir& dir, uint8_t* next, hipStream_t s,
+                       const DeviceCounts* dc = nullptr);
 // Neighbours of every vertex whose cluster is modified.  Vertices with
 // deg > block_max are taken from hubs instead of the vertex sweep.
 void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
@@ -424,7 +471,8 @@ void LaunchExpandClusters(const CsrView& g, const uint32_t* ids, const uint8_t* 
 // vol += sum of (deg + 1) over the seeds: modified[ids[v]] if ids is non-null,
 // else moved[v] (which must then hold 0 for every vertex that did not move).
 void LaunchSeedVolume(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
-                      const uint8_t* moved, unsigned long long* vol, hipStream_t s);
+                      const uint8_t* moved, unsigned long long* vol, hipStream_t s,
+                      const unsigned int* gate = nullptr);
 // Pull side of the expansion (no-op unless dir selects it); hubs must list
 // every vertex with deg > block_max.
 void LaunchPullFrontier(const CsrView& g, const uint32_t* ids, const uint8_t* modified,
@@ -439,9 +487,12 @@ bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 
 // Exclusive scan; total (device pointer, may be null) receives the sum.  If
 // gate is non-null and *gate == 0 the kernels return immediately.
+// work (may be null): ScanWorkElems(n) preallocated elements, so the scan
+// allocates nothing.
+int64_t ScanWorkElems(int64_t n);
 void ExclusiveScanU8toU32(DeviceAllocator* al, const uint8_t* in, uint32_t* out, int64_t n,
                           uint32_t* total, hipStream_t s,
-                          const unsigned long long* gate = nullptr);
+                          const unsigned long long* gate = nullptr, uint32_t* work = nullptr);
 void ExclusiveScanU32toI64(DeviceAllocator* al, const uint32_t* in, int64_t* out, int64_t n,
                            int64_t* total, hipStream_t s);
 void ExclusiveScanU32toU32(DeviceAllocator* al, const uint32_t* in, uint32_t* out, int64_t n,

@@ -52,6 +52,7 @@ OPT_HUB_PART_EDGES = 7
 OPT_PULL_FRACTION = 8
 OPT_CLUSTER_SUBROUNDS = 9
 OPT_HASH_WAVES = 10
+OPT_DEVICE_ROUNDS = 11
 
 
 class StatusError(RuntimeError):
