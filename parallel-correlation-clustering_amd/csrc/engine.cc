@@ -535,7 +535,6 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   if (c7 > 0) {
     c->hb_part.Reset(al_, c7 + 1);
     c->hb_ent.Reset(al_, c7 + 1);
-    c->hb_cbuf.Reset(al_, t7);
     c->hb_flag.Reset(al_, c7);
     c->hb_tval.Reset(al_, c7);
     // Partition items of any sub-list: sum ceil(deg / E) < t7 / E + c7.
@@ -544,6 +543,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb_pc1.Reset(al_, items);
     c->hb_px2.Reset(al_, items);
     c->hb_item_of.Reset(al_, items);
+    c->hb_hpart.Reset(al_, items);
     const int64_t bcap = HubBucketCap(h_->hub_part_edges);
     c->hb_bkt.Reset(al_, items * bcap);
     c->hb_bcur.Reset(al_, items);
@@ -554,7 +554,6 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.part_edges = h_->hub_part_edges;
     c->hb.part_off = c->hb_part.get();
     c->hb.ent_off = c->hb_ent.get();
-    c->hb.cbuf = c->hb_cbuf.get();
     c->hb.hflag = c->hb_flag.get();
     c->hb.tval = c->hb_tval.get();
     c->hb.px1 = c->hb_px1.get();
@@ -565,6 +564,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.pfine = c->hb_pfine.get();
     c->hb.bcap = bcap;
     c->hb.item_of = c->hb_item_of.get();
+    c->hb.hpart = c->hb_hpart.get();
     if (cfg_.move_method != PCC_ALL_MOVE) {
       c->hubs.Reset(al_, c7);
       c->hubs_part.Reset(al_, c7 + 1);
@@ -640,7 +640,8 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
     const uint32_t* list7 = lists[kBins - 1];
     Timer tb = StartTimer();
     if (scan_hubs)
-      LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
+      LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_,
+                    nullptr, c.hb.hpart, c.gscale.get());
     LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_);
     StopTimer(tb, StatGroupOf(kHubBin));
@@ -760,7 +761,8 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     PCC_HIP_CHECK(hipStreamSynchronize(s_));  // hoff / hcnt stay valid until here
     if (counts[kBins - 1] > 0)
       LaunchHubScan(g, ws.list.get() + c.layout.base[kBins - 1], counts[kBins - 1],
-                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
+                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_, nullptr,
+                    c.hb.hpart, c.gscale.get());
   }
   int64_t hashed_count = 0;
   for (int b = kWaveBin0; b < kBins; ++b) hashed_count += counts[b];
@@ -903,7 +905,7 @@ void Engine::RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph,
     const uint32_t* list7 = ws.list.get() + c.layout.base[kBins - 1];
     Timer tb = StartTimer();
     LaunchHubScan(g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_,
-                  &dc->cnt[kBins - 1]);
+                  &dc->cnt[kBins - 1], c.hb.hpart, c.gscale.get());
     LaunchBestMoveHub(c7, c.caps_te[kBins - 1], false, g, st, c.mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_, dc);
     StopTimer(tb, StatGroupOf(kHubBin));

@@ -74,8 +74,15 @@ __device__ __forceinline__ int BinOf(int64_t deg, int64_t wave_max, int64_t bloc
   if (deg <= 64) return 4;
   if (deg > block_max) return kHubBin;
   const HashBounds h = HashBoundsOf(wave_max, block_max);
+  // The first bound >= deg names the bin (unrolled: a variable index into h
+  // would put it in scratch memory).
   int b = kWaveBin0;
-  while (deg > h.b[b - kWaveBin0]) ++b;
+  bool above = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    above = above && deg > h.b[j];
+    b += above ? 1 : 0;
+  }
   return b;
 }
 
@@ -458,17 +465,27 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            int64_t* __restrict__ part_off,
                                                            int64_t* __restrict__ ent_off,
                                                            uint32_t* __restrict__ item_of,
-                                                           const unsigned long long* __restrict__ dcount) {
+                                                           const unsigned long long* __restrict__ dcount,
+                                                           HubPart* __restrict__ rec,
+                                                           const int32_t* __restrict__ gscale) {
   if (dcount) count = (int64_t)*dcount;
   __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
+  // The chunk's items, for the partition writes.
+  __shared__ uint32_t s_xp[kScanThreads], s_v[kScanThreads];
+  __shared__ int32_t s_scale[kScanThreads];
+  __shared__ int64_t s_start[kScanThreads], s_deg[kScanThreads];
   int64_t carry_p = 0, carry_e = 0;
   for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
     const int64_t i = b0 + threadIdx.x;
-    int64_t parts = 0, deg = 0;
+    int64_t parts = 0, deg = 0, start = 0;
+    uint32_t v = 0;
+    int32_t scale = 0;
     if (i < count) {
-      const uint32_t v = list[i];
-      deg = off[v + 1] - off[v];
+      v = list[i];
+      start = off[v];
+      deg = off[v + 1] - start;
       parts = deg > 0 ? (deg + part_edges - 1) / part_edges : 1;
+      if (rec && gscale) scale = gscale[v];
     }
     int64_t tp = 0, tdeg = 0;
     const int64_t xp = BlockExclusiveScan(parts, sp, &tp);
@@ -476,8 +493,37 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
     if (i < count) {
       part_off[i] = carry_p + xp;
       ent_off[i] = carry_e + xe;
-      for (int64_t q = 0; q < parts; ++q) item_of[carry_p + xp + q] = (uint32_t)i;
     }
+    // Partition q of the chunk belongs to the last item whose first
+    // partition is <= q (every item has at least one); the workgroup writes
+    // the chunk's partitions together.
+    s_xp[threadIdx.x] = i < count ? (uint32_t)xp : ~0u;
+    s_v[threadIdx.x] = v;
+    s_scale[threadIdx.x] = scale;
+    s_start[threadIdx.x] = start;
+    s_deg[threadIdx.x] = deg;
+    __syncthreads();
+    for (int64_t q = threadIdx.x; q < tp; q += kScanThreads) {
+      int lo = 0, hi = kScanThreads - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_xp[mid] <= (uint32_t)q) lo = mid; else hi = mid - 1;
+      }
+      item_of[carry_p + q] = (uint32_t)(b0 + lo);
+      if (rec) {
+        HubPart h;
+        h.start = s_start[lo];
+        h.deg = s_deg[lo];
+        h.v = s_v[lo];
+        h.item = (uint32_t)(b0 + lo);
+        h.part = (uint32_t)(q - s_xp[lo]);
+        h.parts = (uint32_t)((lo + 1 < kScanThreads && s_xp[lo + 1] != ~0u ? (int64_t)s_xp[lo + 1] : tp) -
+                             s_xp[lo]);
+        h.scale = s_scale[lo];
+        rec[carry_p + q] = h;
+      }
+    }
+    __syncthreads();
     carry_p += tp;
     carry_e += tdeg;
   }
@@ -750,17 +796,15 @@ template <typename KeyPtr>
 __device__ __forceinline__ int HashSlot(KeyPtr keys, uint32_t mask, uint32_t c, bool* fresh) {
   uint32_t h = HashId(c) & mask;
   *fresh = false;
+  // Compare-and-swap first: one LDS round trip per probe whether the slot is
+  // free, holds c, or holds another key.
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (k == c) return (int)h;
-    if (k == kNoneId) {
-      const uint32_t old = atomicCAS(&keys[h], kNoneId, c);
-      if (old == kNoneId) {
-        *fresh = true;
-        return (int)h;
-      }
-      if (old == c) return (int)h;
+    const uint32_t old = atomicCAS(&keys[h], kNoneId, c);
+    if (old == kNoneId) {
+      *fresh = true;
+      return (int)h;
     }
+    if (old == c) return (int)h;
     h = (h + 1) & mask;
   }
   return -1;
@@ -810,6 +854,11 @@ __device__ __forceinline__ int64_t PbbsBlock(int64_t deg) {
   return bs < 1024 ? 1024 : bs;
 }
 
+// Chunks of THREADS edges whose gathers the order repair issues together
+// (one for the 1024-thread kernel, whose 128 VGPRs per thread are taken).
+template <int THREADS>
+constexpr int kRepairChunks = THREADS >= 1024 ? 1 : 8;
+
 // LDS staging of the order repair: one chunk of THREADS edges.
 template <int THREADS>
 struct RepairStage {
@@ -819,9 +868,6 @@ struct RepairStage {
   uint32_t wsum[THREADS / kWave];
   double T;
 };
-// The partition kernel reuses the first 4 * THREADS words as its compaction
-// queue (AccumulateBuffered).
-static_assert(sizeof(RepairStage<512>) >= 4 * 512 * sizeof(uint32_t), "queue fits");
 
 // Recomputes, in adjacency order, the sums of the slots whose summation order
 // matters (SlotInexact), after Accumulate filled the table.  Edges of those
@@ -851,60 +897,79 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
   double t_blk = 0.0, t_tot = 0.0;
   int64_t cur_blk = -1;
   bool have_tot = false;
-  for (int64_t q0 = 0; q0 < deg; q0 += THREADS) {
-    const int64_t q = q0 + threadIdx.x;
-    int slot = -1;
-    double x = 0.0;
-    if (q < deg) {
-      const uint32_t c = src.Cluster(src.Neighbour(q));
-      if (c != kNoneId) {
-        if (parts == 1 || PartOf(c, parts) == part) {
-          const int s = HashFind(keys, mask, c);
-          if (s >= 0 && SlotInexact(cnt[s], thr, mexp)) {
-            slot = s;
-            x = EdgeValue(g.w[start + q], hub, mp.offset);
-          }
-        }
+  // U chunks of THREADS edges per trip: their neighbour and cluster gathers
+  // are issued together (a hub row is long and its repair would otherwise
+  // pay two dependent global loads per chunk), then the chunks are folded
+  // one after another in adjacency order.
+  constexpr int U = kRepairChunks<THREADS>;
+  for (int64_t q0 = 0; q0 < deg; q0 += (int64_t)THREADS * U) {
+    uint32_t c[U];
+    int slot[U];
+    double x[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * THREADS + threadIdx.x;
+      c[k] = q < deg ? src.Neighbour(q) : kNoneId;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) c[k] = c[k] != kNoneId ? src.Cluster(c[k]) : kNoneId;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      slot[k] = -1;
+      if (c[k] != kNoneId && (parts == 1 || PartOf(c[k], parts) == part)) {
+        const int s = HashFind(keys, mask, c[k]);
+        if (s >= 0 && SlotInexact(cnt[s], thr, mexp)) slot[k] = s;
       }
+      any = any || slot[k] >= 0;
     }
-    if (!__syncthreads_or(slot >= 0)) continue;
-    const uint64_t m = __ballot(slot >= 0);
-    if (lane == 0) stg->wsum[w] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int i = 0; i < THREADS / kWave; ++i) {
-      const uint32_t t = stg->wsum[i];
-      if (i < w) before += t;
-      total += t;
+    if (!__syncthreads_or(any)) continue;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * THREADS + threadIdx.x;
+      x[k] = slot[k] >= 0 ? EdgeValue(g.w[start + q], hub, mp.offset) : 0.0;
     }
-    if (slot >= 0) {
-      const uint32_t k = before + (uint32_t)__popcll(m & LaneMaskLt());
-      stg->slot[k] = (uint32_t)slot;
-      stg->pos[k] = (uint32_t)q;
-      stg->x[k] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (uint32_t k = 0; k < total; ++k) {
-        const uint32_t sl = stg->slot[k];
-        const double xx = stg->x[k];
-        vals[sl] += xx;
-        if (own_ordered && (int)sl == own) {
-          const int64_t blk = (int64_t)stg->pos[k] / bs;
-          if (blk != cur_blk) {
-            if (cur_blk >= 0) {
-              t_tot = have_tot ? t_tot + t_blk : t_blk;
-              have_tot = true;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (!__syncthreads_or(slot[k] >= 0)) continue;
+      const uint64_t m = __ballot(slot[k] >= 0);
+      if (lane == 0) stg->wsum[w] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+      for (int i = 0; i < THREADS / kWave; ++i) {
+        const uint32_t t = stg->wsum[i];
+        if (i < w) before += t;
+        total += t;
+      }
+      if (slot[k] >= 0) {
+        const uint32_t j = before + (uint32_t)__popcll(m & LaneMaskLt());
+        stg->slot[j] = (uint32_t)slot[k];
+        stg->pos[j] = (uint32_t)(q0 + (int64_t)k * THREADS + threadIdx.x);
+        stg->x[j] = x[k];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (uint32_t j = 0; j < total; ++j) {
+          const uint32_t sl = stg->slot[j];
+          const double xx = stg->x[j];
+          vals[sl] += xx;
+          if (own_ordered && (int)sl == own) {
+            const int64_t blk = (int64_t)stg->pos[j] / bs;
+            if (blk != cur_blk) {
+              if (cur_blk >= 0) {
+                t_tot = have_tot ? t_tot + t_blk : t_blk;
+                have_tot = true;
+              }
+              cur_blk = blk;
+              t_blk = xx;
+            } else {
+              t_blk += xx;
             }
-            cur_blk = blk;
-            t_blk = xx;
-          } else {
-            t_blk += xx;
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (threadIdx.x == 0) {
     double T = 0.0;
@@ -928,9 +993,9 @@ __device__ __forceinline__ void RecordTerm(uint32_t* cnt, int slot, double x, in
   if (fb) atomicOr(&cnt[slot], fb);
 }
 
-// Where the hashed kernels take the cluster of v's q-th neighbour from:
-// the adjacency and the cluster ids (one gather per edge), or a buffer the
-// hub gather pass filled with those clusters (kNoneId for self-loops).
+// Where the hashed kernels take the cluster of v's q-th neighbour from: the
+// adjacency and the cluster ids, one gather per edge (kNoneId for
+// self-loops).
 struct AdjacencyClusters {
   const uint32_t* nbr;  // v's adjacency
   const uint32_t* ids;
@@ -940,60 +1005,23 @@ struct AdjacencyClusters {
     return u != v ? ids[u] : kNoneId;
   }
 };
-struct BufferedClusters {
-  const uint32_t* c;  // v's gathered clusters
-  __device__ __forceinline__ uint32_t Neighbour(int64_t q) const { return c[q]; }
-  __device__ __forceinline__ uint32_t Cluster(uint32_t x) const { return x; }
-};
 
 // Accumulate over one partition's bucket of (cluster, weight bits) pairs:
-// every pair belongs to the partition, so no filtering is needed.
-template <int THREADS, bool kAsync>
-__device__ __forceinline__ bool AccumulateBucket(const MoveParams& mp, const uint2* __restrict__ bk,
-                                                 uint32_t nb, bool hub, int thr, uint32_t* keys,
-                                                 double* vals, uint32_t* cnt, uint32_t mask,
-                                                 const SlotList& sl) {
+// every pair belongs to the partition, so no filtering is needed.  The first
+// P * THREADS pairs arrive in pre (loaded before the table was cleared); the
+// rest are loaded Q * THREADS at a time, all of a trip's loads issued before
+// its inserts.
+template <int THREADS, bool kAsync, int P>
+__device__ __forceinline__ bool AccumulateBucket(const MoveParams& mp, const uint2 (&pre)[P],
+                                                 const uint2* __restrict__ bk, uint32_t nb,
+                                                 bool hub, int thr, uint32_t* keys, double* vals,
+                                                 uint32_t* cnt, uint32_t mask, const SlotList& sl) {
   bool overflow = false;
-  for (uint32_t k0 = 0; k0 < nb; k0 += THREADS) {
-    const uint32_t k = k0 + threadIdx.x;
-    const bool act = k < nb;
-    const uint2 e = act ? bk[k] : make_uint2(kNoneId, 0u);
+  auto insert = [&](bool act, uint2 e) {
     const double x = act ? EdgeValue(__uint_as_float(e.y), hub, mp.offset) : 0.0;
     bool fresh = false;
     const int slot = act ? HashSlot(keys, mask, e.x, &fresh) : -1;
-    const bool listed = sl.AddWave(fresh, slot);
-    if (!act) continue;
-    if (slot < 0 || !listed) {
-      overflow = true;
-      continue;
-    }
-    atomicAdd(&vals[slot], x);
-    if (!kAsync) RecordTerm(cnt, slot, x, thr);
-  }
-  return overflow;
-}
-
-// Accumulate over gathered clusters cb[0, deg) (partition `part` of
-// `parts`).  Only about 1/parts of the entries belong to the partition, so
-// each wave first compacts the positions of its matching entries into a
-// per-wave LDS queue (wq, 4 * kWave entries per wave) and then inserts them
-// with all lanes busy, instead of running the insert code masked for every
-// entry.  The buffer is streamed with 16-byte loads, two per thread in
-// flight.
-template <int THREADS, bool kAsync>
-__device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveParams& mp,
-                                                   const uint32_t* __restrict__ cb, int64_t start,
-                                                   int64_t deg, bool hub, int thr, uint32_t* keys,
-                                                   double* vals, uint32_t* cnt, uint32_t mask,
-                                                   uint32_t parts, uint32_t part, uint32_t* wq,
-                                                   const SlotList& sl) {
-  bool overflow = false;
-  // Called by every lane of the wave; act: this lane has an entry.
-  auto insert = [&](bool act, uint32_t c, float wf) {
-    const double x = act ? EdgeValue(wf, hub, mp.offset) : 0.0;
-    bool fresh = false;
-    const int slot = act ? HashSlot(keys, mask, c, &fresh) : -1;
-    const bool listed = sl.AddWave(fresh, slot);
+    const bool listed = sl.AddWave(fresh, slot);  // every lane of the wave
     if (!act) return;
     if (slot < 0 || !listed) {
       overflow = true;
@@ -1002,65 +1030,22 @@ __device__ __forceinline__ bool AccumulateBuffered(const CsrView& g, const MoveP
     atomicAdd(&vals[slot], x);
     if (!kAsync) RecordTerm(cnt, slot, x, thr);
   };
-  auto mine = [&](uint32_t c) {
-    return c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
-  };
-  const int lane = LaneId();
-  uint32_t* q = wq + (threadIdx.x / kWave) * (4 * kWave);
-  const float* wv = g.w + start;
-  // Compacts the matches among entries q0 .. q0 + 3 (m: bit e = entry q0 + e)
-  // of every lane, then inserts them.
-  auto drain = [&](uint32_t m, uint32_t q0) {
-    const uint32_t n = (uint32_t)__popc(m);
-    uint32_t inc = n;
-    for (int sh = 1; sh < kWave; sh <<= 1) {
-      const uint32_t y = __shfl_up(inc, sh);
-      if (lane >= sh) inc += y;
+#pragma unroll
+  for (int r = 0; r < P; ++r) insert(r * THREADS + threadIdx.x < nb, pre[r]);
+  constexpr int Q = 4;
+  for (uint32_t k0 = P * THREADS; k0 < nb; k0 += Q * THREADS) {
+    uint2 e[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+      const uint32_t k = k0 + r * THREADS + threadIdx.x;
+      e[r] = k < nb ? bk[k] : make_uint2(kNoneId, 0u);
     }
-    const uint32_t tot = __shfl(inc, kWave - 1);
-    if (tot == 0) return;
-    uint32_t at = inc - n;
-    for (uint32_t e = 0; e < 4; ++e)
-      if (m & (1u << e)) q[at++] = q0 + e;
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t k0 = 0; k0 < tot; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      const bool act = k < tot;
-      const uint32_t qq = act ? q[k] : 0u;
-      insert(act, act ? cb[qq] : 0u, act ? wv[qq] : 0.0f);
-    }
-    __builtin_amdgcn_wave_barrier();
-  };
-  const int64_t lead = (int64_t)((4 - (((uintptr_t)cb >> 2) & 3)) & 3);
-  const int64_t head = lead < deg ? lead : deg;
-  const int64_t nvec = (deg - head) >> 2;
-  const uint4* vp = reinterpret_cast<const uint4*>(cb + head);
-  const uint4 none = make_uint4(kNoneId, kNoneId, kNoneId, kNoneId);
-  // Head and tail entries (fewer than 4 each) go through the first drain.
-  {
-    uint32_t m = 0, q0 = 0;
-    if ((int64_t)threadIdx.x < head) {
-      q0 = threadIdx.x;
-      m = mine(cb[q0]) ? 1u : 0u;
-    } else if ((int64_t)threadIdx.x < head + (deg - head - 4 * nvec)) {
-      q0 = (uint32_t)(head + 4 * nvec + (threadIdx.x - head));
-      m = mine(cb[q0]) ? 1u : 0u;
-    }
-    drain(m, q0);
-  }
-  for (int64_t b0 = 0; b0 < nvec; b0 += 2 * THREADS) {
-    const int64_t j0 = b0 + threadIdx.x, j1 = j0 + THREADS;
-    const uint4 x0 = j0 < nvec ? vp[j0] : none;
-    const uint4 x1 = j1 < nvec ? vp[j1] : none;
-    uint32_t m0 = (mine(x0.x) ? 1u : 0u) | (mine(x0.y) ? 2u : 0u) | (mine(x0.z) ? 4u : 0u) |
-                  (mine(x0.w) ? 8u : 0u);
-    uint32_t m1 = (mine(x1.x) ? 1u : 0u) | (mine(x1.y) ? 2u : 0u) | (mine(x1.z) ? 4u : 0u) |
-                  (mine(x1.w) ? 8u : 0u);
-    drain(m0, (uint32_t)(head + 4 * j0));
-    drain(m1, (uint32_t)(head + 4 * j1));
+#pragma unroll
+    for (int r = 0; r < Q; ++r) insert(k0 + r * THREADS + threadIdx.x < nb, e[r]);
   }
   return overflow;
 }
+
 
 // Per-vertex values the hashed kernels read besides the adjacency.
 struct HashVertex {
@@ -1339,12 +1324,50 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
     SyncEpilogue(pv, pt, pd, mp.new_id, st, out);
 }
 
+// Accumulate over v's whole adjacency (partition `part` of `parts`), the
+// cluster of every neighbour gathered again: the fallback of a partition
+// whose bucket overflowed (a heavy cluster), rare enough not to need the
+// compaction queue of the buffered form.
+template <int THREADS, bool kAsync>
+__device__ __forceinline__ bool AccumulateGathered(const CsrView& g, const StateView& st,
+                                                   const MoveParams& mp, uint32_t v, int64_t start,
+                                                   int64_t deg, bool hub, int thr, uint32_t* keys,
+                                                   double* vals, uint32_t* cnt, uint32_t mask,
+                                                   uint32_t parts, uint32_t part,
+                                                   const SlotList& sl) {
+  bool overflow = false;
+  for (int64_t q0 = 0; q0 < deg; q0 += THREADS) {
+    const int64_t q = q0 + threadIdx.x;
+    uint32_t c = kNoneId;
+    float wf = 0.0f;
+    if (q < deg) {
+      const uint32_t u = g.nbr[start + q];
+      if (u != v) c = st.ids[u];
+      wf = g.w[start + q];
+    }
+    const bool act = c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
+    const double x = act ? EdgeValue(wf, hub, mp.offset) : 0.0;
+    bool fresh = false;
+    const int slot = act ? HashSlot(keys, mask, c, &fresh) : -1;
+    const bool listed = sl.AddWave(fresh, slot);
+    if (!act) continue;
+    if (slot < 0 || !listed) {
+      overflow = true;
+      continue;
+    }
+    atomicAdd(&vals[slot], x);
+    if (!kAsync) RecordTerm(cnt, slot, x, thr);
+  }
+  return overflow;
+}
+
 // Bin 7, gather pass: the cluster of every neighbour of the listed vertices,
-// cbuf[ent_off[i] + q] (kNoneId for self-loops), one workgroup per partition
-// of edges, so the partition pass reads clusters without gathers.  The same
-// (cluster, weight) pairs are also appended to the bucket of the partition
-// their cluster hashes to (HubScratch::bkt): counted per bucket in LDS, then
-// one global reservation per bucket and workgroup.
+// one workgroup per partition of edges, appended with its weight to the
+// bucket of the partition the cluster hashes to (HubScratch::bkt): counted
+// per bucket in LDS, then one global reservation per bucket and workgroup.
+// The partition pass then reads only its bucket; the rare passes that need
+// a vertex's whole adjacency in order (an overflowing bucket, the in-order
+// repair of inexact sums) gather the clusters again.
 constexpr int kGatherThreads = 256;
 constexpr int kGatherU = 8;
 constexpr int kGatherMaxBuckets = 4096;
@@ -1357,23 +1380,22 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
                                                                const unsigned long long* __restrict__ dcount) {
   if (dcount) count = (int64_t)*dcount;
   __shared__ uint32_t lh[kGatherMaxBuckets];
+  if (!kAsync && mp.force_ordered) return;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const int64_t i = hb.item_of[t];
-  const uint32_t v = list[i];
+  const HubPart hp = hb.hpart[t];
+  const uint32_t v = hp.v;
   if (kAsync && !InSubRound(mp, v)) return;
-  if (!kAsync && mp.force_ordered) return;
-  const int64_t p0 = hb.part_off[i];
-  const int64_t parts = hb.part_off[i + 1] - p0;
-  const int64_t part = t - p0;
-  const int64_t start = g.off[v], deg = g.off[v + 1] - start;
+  const int64_t part = hp.part;
+  const int64_t p0 = t - part;
+  const int64_t parts = hp.parts;
+  const int64_t start = hp.start, deg = hp.deg;
   const int64_t per = (deg + parts - 1) / parts;
   const int64_t e0 = part * per, e1 = min(deg, (part + 1) * per);
   const bool local = parts <= kGatherMaxBuckets;
-  uint32_t* out = hb.cbuf + hb.ent_off[i];
   // Sync rounds flag the partitions that receive a fine value (bin-7
   // vertices take the hub path's float values, EdgeValue(w, true, offset)).
-  const int thr = kAsync ? kNoFineThr : ScaleThr(mp.gscale[v]);
+  const int thr = kAsync ? kNoFineThr : ScaleThr(hp.scale);
   constexpr int U = kGatherU;
   for (int64_t q0 = e0; q0 < e1; q0 += (int64_t)kGatherThreads * U) {
     uint32_t c[U];
@@ -1388,11 +1410,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
 #pragma unroll
     for (int k = 0; k < U; ++k) c[k] = c[k] != v ? st.ids[c[k]] : kNoneId;
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t q = q0 + threadIdx.x + (int64_t)k * kGatherThreads;
-      if (q < e1) out[q] = c[k];
-      b[k] = c[k] != kNoneId ? PartOf(c[k], (uint32_t)parts) : 0u;
-    }
+    for (int k = 0; k < U; ++k) b[k] = c[k] != kNoneId ? PartOf(c[k], (uint32_t)parts) : 0u;
     if (local) {
       __syncthreads();
       for (int64_t j = threadIdx.x; j < parts; j += kGatherThreads) lh[j] = 0;
@@ -1483,44 +1501,59 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   constexpr uint32_t kList = TABLE * 3 / 4;
   __shared__ uint16_t occ[kList];
   __shared__ uint32_t nocc;
+  if (!kAsync && mp.force_ordered) return;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const int64_t i = hb.item_of[t];
-  const uint32_t v = list[i];
-  if (kAsync && !InSubRound(mp, v)) return;
-  if (!kAsync && mp.force_ordered) return;
-  const uint32_t parts = (uint32_t)(hb.part_off[i + 1] - hb.part_off[i]);
-  const uint32_t part = (uint32_t)(t - hb.part_off[i]);
-  const int64_t start = g.off[v];
-  const int64_t deg = g.off[v + 1] - start;
-  const int64_t per = (deg + parts - 1) / parts;
-  const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * per + 2))));
-  const uint32_t mask = cap - 1;
-  for (uint32_t s = threadIdx.x; s < cap; s += kHubThreads) {
+  // One round of loads: the partition's record, its bucket's fill and fine
+  // flag, and its first P * kHubThreads pairs, speculatively (slots past the
+  // fill, within the bucket's capacity, hold stale pairs and are masked).
+  const HubPart hp = hb.hpart[t];
+  const uint32_t nb = hb.bcur[t];
+  const uint32_t fine = hb.pfine[t];
+  const uint2* bk = hb.bkt + t * hb.bcap;
+  constexpr int P = 3;
+  uint2 pre[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+    const int64_t k = (int64_t)r * kHubThreads + threadIdx.x;
+    pre[r] = k < hb.bcap ? bk[k] : make_uint2(kNoneId, 0u);
+  }
+  // The whole table is cleared (its probe range needs the record).
+  for (uint32_t s = threadIdx.x; s < (uint32_t)TABLE; s += kHubThreads) {
     keys[s] = kNoneId;
     vals[s] = 0.0;
     cnt[s] = 0;
   }
   if (threadIdx.x == 0) nocc = 0;
+  const uint32_t v = hp.v;
+  if (kAsync && !InSubRound(mp, v)) return;
+  // Read after the table is built; issued now.
+  const uint32_t cv = st.ids[v];
+  const double kv = st.nw[v];
   __syncthreads();
+  const int64_t i = hp.item;
+  const uint32_t parts = hp.parts;
+  const uint32_t part = hp.part;
+  const int64_t start = hp.start;
+  const int64_t deg = hp.deg;
+  const int64_t per = (deg + parts - 1) / parts;
+  const uint32_t cap = min((uint32_t)TABLE, max(64u, NextPow2((uint64_t)(2 * per + 2))));
+  const uint32_t mask = cap - 1;
   const SlotList sl{occ, &nocc, kList};
   const bool hub = deg > kHubDegree;
-  const int32_t scale = kAsync ? 0 : mp.gscale[v];
+  const int32_t scale = kAsync ? 0 : hp.scale;
   // A partition that received no fine value keeps no per-slot records.
-  const int thr = (kAsync || hb.pfine[t] != 0) ? ScaleThr(scale) : kNoFineThr;
+  const int thr = (kAsync || fine != 0) ? ScaleThr(scale) : kNoFineThr;
   const int mexp = ScaleMexp(scale);
-  const BufferedClusters src{hb.cbuf + hb.ent_off[i]};
-  const uint32_t nb = hb.bcur[t];
+  const AdjacencyClusters src{g.nbr + start, st.ids, v};
   // The partition's bucket when it did not overflow; otherwise the whole
-  // cluster buffer, filtered (the compaction queue shares the repair stage's
-  // LDS, never live together).
+  // adjacency, gathered and filtered.
   const bool overflow =
       nb <= (uint32_t)hb.bcap
-          ? AccumulateBucket<kHubThreads, kAsync>(mp, hb.bkt + t * hb.bcap, nb, hub, thr, keys,
-                                                  vals, cnt, mask, sl)
-          : AccumulateBuffered<kHubThreads, kAsync>(g, mp, src.c, start, deg, hub, thr, keys, vals,
-                                                    cnt, mask, parts, part,
-                                                    reinterpret_cast<uint32_t*>(&stg), sl);
+          ? AccumulateBucket<kHubThreads, kAsync, P>(mp, pre, bk, nb, hub, thr, keys, vals, cnt,
+                                                     mask, sl)
+          : AccumulateGathered<kHubThreads, kAsync>(g, st, mp, v, start, deg, hub, thr, keys, vals,
+                                                    cnt, mask, parts, part, sl);
   const bool ovf = __syncthreads_or(overflow);
   // Every thread has read bcur[t] and pfine[t] before the barrier.
   if (threadIdx.x == 0) {
@@ -1531,7 +1564,6 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     if (threadIdx.x == 0) atomicOr(&hb.hflag[i], 2u);
     return;
   }
-  const uint32_t cv = st.ids[v];
   const int own = PartOf(cv, parts) == part ? HashFind(keys, mask, cv) : -1;
   double T = own >= 0 ? vals[own] : 0.0;
   const uint32_t n_occ = min(nocc, kList);
@@ -1554,7 +1586,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   // term, which only the partition holding c(v) knows, afterwards.
   const double res = ResOf(mp, v);
   double mw = 0;
-  mw += st.nw[v];
+  mw += kv;
   // A thread's (at most KS) candidate clusters have their weights loaded
   // together, not one dependent load per loop trip.
   constexpr int KS = (int)((kList + kHubThreads - 1) / kHubThreads);
@@ -1856,9 +1888,9 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
                    int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
-                   const unsigned long long* dcount) {
+                   const unsigned long long* dcount, HubPart* rec, const int32_t* gscale) {
   hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
-                     part_edges, part_off, ent_off, item_of, dcount);
+                     part_edges, part_off, ent_off, item_of, dcount, rec, gscale);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

@@ -26,13 +26,17 @@ struct Bins {
 };
 
 // Device-driven rounds: the launch sized bins by their capacities; the bins
-// a launch covers (count > 0) take their current counts from dcnt.
-__device__ __forceinline__ int64_t TakeCounts(Bins& bins, const unsigned long long* dcnt) {
+// a launch covers (count > 0) take their current counts from dcnt.  The
+// kernel argument itself is never written (a written copy indexed by a
+// variable would live in scratch memory).
+__device__ __forceinline__ int64_t BinCount(const Bins& bins, const unsigned long long* dcnt,
+                                            int b) {
+  return (dcnt && bins.count[b] > 0) ? (int64_t)dcnt[b] : bins.count[b];
+}
+__device__ __forceinline__ int64_t BinTotal(const Bins& bins, const unsigned long long* dcnt) {
   int64_t total = 0;
-  for (int b = 0; b < kBins; ++b) {
-    if (dcnt && bins.count[b] > 0) bins.count[b] = (int64_t)dcnt[b];
-    total += bins.count[b];
-  }
+#pragma unroll
+  for (int b = 0; b < kBins; ++b) total += BinCount(bins, dcnt, b);
   return total;
 }
 
@@ -43,12 +47,11 @@ __global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __rest
                                                uint8_t* __restrict__ new_flag,
                                                RoundCounters* __restrict__ rc,
                                                const unsigned long long* __restrict__ dcnt) {
-  TakeCounts(bins, dcnt);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long new_movers = 0;
   for (int b = 0; b < kBins; ++b) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bins.count[b];
-         i += stride) {
+    const int64_t cb = BinCount(bins, dcnt, b);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cb; i += stride) {
       const uint32_t v = list[bins.base[b] + i];
       const uint32_t t = moves[v];
       if (t == kNoneId) continue;
@@ -237,7 +240,18 @@ __device__ __forceinline__ void MarkNeighbours(bool sel, int64_t st, int64_t deg
   }
 }
 
-// Moved vertices of bins 5 and 6: one workgroup per vertex.
+__device__ __forceinline__ int64_t WaveUniform(int64_t x) {
+  const uint64_t u = (uint64_t)x;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Moved vertices of bins 5 and 6: one workgroup per vertex.  The vertex
+// index is uniform over the workgroup, so the bin search runs on the scalar
+// unit; with device counts (kDev) the loaded counts are made wave-uniform
+// explicitly so that it stays there.
+template <bool kDev>
 __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       const uint32_t* __restrict__ list,
                                                       const int64_t* __restrict__ off,
@@ -246,11 +260,15 @@ __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       ExpandDir dir, uint8_t* __restrict__ next,
                                                       const unsigned long long* __restrict__ dcnt) {
   if (dir.Pull()) return;
-  if (dcnt) total = TakeCounts(bins, dcnt);
+  if (kDev) total = WaveUniform(BinTotal(bins, dcnt));
   for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
     int64_t r = t;
     int b = 0;
-    while (r >= bins.count[b]) r -= bins.count[b++];
+    if (kDev) {
+      while (r >= WaveUniform(BinCount(bins, dcnt, b))) r -= WaveUniform(BinCount(bins, dcnt, b++));
+    } else {
+      while (r >= bins.count[b]) r -= bins.count[b++];
+    }
     const uint32_t v = list[bins.base[b] + r];
     if (!moved[v]) continue;
     const int64_t st = off[v], en = off[v + 1];
@@ -267,7 +285,7 @@ __global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
                                                       ExpandDir dir, uint8_t* __restrict__ next,
                                                       const unsigned long long* __restrict__ dcnt) {
   if (dir.Pull()) return;
-  if (dcnt) total = TakeCounts(bins, dcnt);
+  if (dcnt) total = BinTotal(bins, dcnt);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t = base + threadIdx.x;
@@ -276,7 +294,7 @@ __global__ __launch_bounds__(256) void k_expand_moved(Bins bins, int64_t total,
     if (t < total) {
       int64_t r = t;
       int b = 0;
-      while (r >= bins.count[b]) r -= bins.count[b++];
+      while (r >= BinCount(bins, dcnt, b)) r -= BinCount(bins, dcnt, b++);
       const uint32_t v = list[bins.base[b] + r];
       if (moved[v]) {
         sel = true;
@@ -852,8 +870,12 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
     PCC_HIP_CHECK(hipGetLastError());
   }
   if (nmid > 0) {
-    hipLaunchKernelGGL(k_expand_block, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid, nmid, list,
-                       g.off, g.nbr, moved, dir, next, dcnt);
+    if (dcnt)
+      hipLaunchKernelGGL(k_expand_block<true>, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid,
+                         nmid, list, g.off, g.nbr, moved, dir, next, dcnt);
+    else
+      hipLaunchKernelGGL(k_expand_block<false>, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid,
+                         nmid, list, g.off, g.nbr, moved, dir, next, dcnt);
     PCC_HIP_CHECK(hipGetLastError());
   }
   const int64_t c7 = counts[kBins - 1];

@@ -233,15 +233,28 @@ constexpr int64_t kHubPartEdges = 2048;
 
 // Bin-7 scratch.  part_off / ent_off: exclusive prefix sums over the bin's
 // list of the partition counts ceil(deg / kHubPartEdges) and of the degrees;
-// ekeys / evals[ent_off[i] ...]: the (cluster, sum) entries of list item i;
-// ecur / hflag (bit 0: own-cluster sum found in tval, bit 1: inexact or table
-// overflow) are zero between launches.
+// hflag (bit 0: own-cluster sum found in tval, bit 1: inexact or table
+// overflow, bit 2: a candidate exists) is zero between launches.
+// One partition item of a bin-7 list, written by the partition scan so that
+// the passes over partitions read their vertex, its adjacency range and its
+// order-test scale with one load instead of a chain of dependent ones.
+struct alignas(16) HubPart {
+  int64_t start = 0;   // offset of v's adjacency
+  int64_t deg = 0;     // degree of v
+  uint32_t v = 0;      // the vertex
+  uint32_t item = 0;   // its list item
+  uint32_t part = 0;   // this partition's index among v's
+  uint32_t parts = 0;  // v's partition count
+  int32_t scale = 0;   // gscale[v] (0 without order-test scales)
+  uint32_t pad[3] = {0, 0, 0};
+};
+
 struct HubScratch {
   int64_t part_edges = kHubPartEdges;
   int64_t* part_off = nullptr;
   uint32_t* item_of = nullptr;  // list item of each partition
+  HubPart* hpart = nullptr;     // record of each partition
   int64_t* ent_off = nullptr;
-  uint32_t* cbuf = nullptr;  // neighbour clusters of list item i at ent_off[i]
   uint32_t* hflag = nullptr;
   double* tval = nullptr;
   // Per partition item t, over its candidate clusters c with
@@ -252,8 +265,8 @@ struct HubScratch {
   double* px2 = nullptr;
   // The same clusters bucketed by partition: item t's (cluster, weight bits)
   // pairs at bkt[t * bcap ...], bcur[t] of them; bcur[t] > bcap means the
-  // bucket overflowed and the partition filters cbuf instead.  The partition
-  // pass resets bcur[t] to 0.
+  // bucket overflowed and the partition gathers and filters the whole
+  // adjacency instead.  The partition pass resets bcur[t] to 0.
   uint2* bkt = nullptr;
   uint32_t* bcur = nullptr;
   int64_t bcap = 0;
@@ -311,9 +324,11 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 // Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
 // partitions of part_edges edges, and the list item of every partition
 // (item_of[part_off[i] + p] = i).
+// rec (optional): the record of every partition, with gscale[v] (optional).
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
                    int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
-                   const unsigned long long* dcount = nullptr);
+                   const unsigned long long* dcount = nullptr, HubPart* rec = nullptr,
+                   const int32_t* gscale = nullptr);
 
 // Every vertex of bin 7 (deg > th.block_max) appended to list; *count zeroed
 // by the caller.
