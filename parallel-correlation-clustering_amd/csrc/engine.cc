@@ -640,8 +640,7 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
     const uint32_t* list7 = lists[kBins - 1];
     Timer tb = StartTimer();
     if (scan_hubs)
-      LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_,
-                    nullptr, c.hb.hpart, c.gscale.get());
+      LaunchHubScan(c.g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
     LaunchBestMoveHub(c7, te[kBins - 1], async, c.g, c.st, mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_);
     StopTimer(tb, StatGroupOf(kHubBin));
@@ -761,8 +760,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     PCC_HIP_CHECK(hipStreamSynchronize(s_));  // hoff / hcnt stay valid until here
     if (counts[kBins - 1] > 0)
       LaunchHubScan(g, ws.list.get() + c.layout.base[kBins - 1], counts[kBins - 1],
-                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_, nullptr,
-                    c.hb.hpart, c.gscale.get());
+                    c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_);
   }
   int64_t hashed_count = 0;
   for (int b = kWaveBin0; b < kBins; ++b) hashed_count += counts[b];
@@ -905,7 +903,7 @@ void Engine::RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph,
     const uint32_t* list7 = ws.list.get() + c.layout.base[kBins - 1];
     Timer tb = StartTimer();
     LaunchHubScan(g, list7, c7, c.hb.part_edges, c.hb.part_off, c.hb.ent_off, c.hb.item_of, s_,
-                  &dc->cnt[kBins - 1], c.hb.hpart, c.gscale.get());
+                  &dc->cnt[kBins - 1]);
     LaunchBestMoveHub(c7, c.caps_te[kBins - 1], false, g, st, c.mp, list7, c.hb, out,
                       ws.fb_list.get(), ws.rc.get(), s_, dc);
     StopTimer(tb, StatGroupOf(kHubBin));

@@ -465,27 +465,18 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            int64_t* __restrict__ part_off,
                                                            int64_t* __restrict__ ent_off,
                                                            uint32_t* __restrict__ item_of,
-                                                           const unsigned long long* __restrict__ dcount,
-                                                           HubPart* __restrict__ rec,
-                                                           const int32_t* __restrict__ gscale) {
+                                                           const unsigned long long* __restrict__ dcount) {
   if (dcount) count = (int64_t)*dcount;
   __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
-  // The chunk's items, for the partition writes.
-  __shared__ uint32_t s_xp[kScanThreads], s_v[kScanThreads];
-  __shared__ int32_t s_scale[kScanThreads];
-  __shared__ int64_t s_start[kScanThreads], s_deg[kScanThreads];
+  __shared__ uint32_t s_xp[kScanThreads];  // the chunk's first partition per item
   int64_t carry_p = 0, carry_e = 0;
   for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
     const int64_t i = b0 + threadIdx.x;
-    int64_t parts = 0, deg = 0, start = 0;
-    uint32_t v = 0;
-    int32_t scale = 0;
+    int64_t parts = 0, deg = 0;
     if (i < count) {
-      v = list[i];
-      start = off[v];
-      deg = off[v + 1] - start;
+      const uint32_t v = list[i];
+      deg = off[v + 1] - off[v];
       parts = deg > 0 ? (deg + part_edges - 1) / part_edges : 1;
-      if (rec && gscale) scale = gscale[v];
     }
     int64_t tp = 0, tdeg = 0;
     const int64_t xp = BlockExclusiveScan(parts, sp, &tp);
@@ -496,12 +487,8 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
     }
     // Partition q of the chunk belongs to the last item whose first
     // partition is <= q (every item has at least one); the workgroup writes
-    // the chunk's partitions together.
+    // the chunk's owners together.
     s_xp[threadIdx.x] = i < count ? (uint32_t)xp : ~0u;
-    s_v[threadIdx.x] = v;
-    s_scale[threadIdx.x] = scale;
-    s_start[threadIdx.x] = start;
-    s_deg[threadIdx.x] = deg;
     __syncthreads();
     for (int64_t q = threadIdx.x; q < tp; q += kScanThreads) {
       int lo = 0, hi = kScanThreads - 1;
@@ -510,18 +497,6 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
         if (s_xp[mid] <= (uint32_t)q) lo = mid; else hi = mid - 1;
       }
       item_of[carry_p + q] = (uint32_t)(b0 + lo);
-      if (rec) {
-        HubPart h;
-        h.start = s_start[lo];
-        h.deg = s_deg[lo];
-        h.v = s_v[lo];
-        h.item = (uint32_t)(b0 + lo);
-        h.part = (uint32_t)(q - s_xp[lo]);
-        h.parts = (uint32_t)((lo + 1 < kScanThreads && s_xp[lo + 1] != ~0u ? (int64_t)s_xp[lo + 1] : tp) -
-                             s_xp[lo]);
-        h.scale = s_scale[lo];
-        rec[carry_p + q] = h;
-      }
     }
     __syncthreads();
     carry_p += tp;
@@ -1383,19 +1358,36 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
   if (!kAsync && mp.force_ordered) return;
   const int64_t t = XcdChunkItem(blockIdx.x);
   if (t >= hb.part_off[count]) return;
-  const HubPart hp = hb.hpart[t];
-  const uint32_t v = hp.v;
+  const int64_t i = hb.item_of[t];
+  const uint32_t v = list[i];
+  const int64_t p0 = hb.part_off[i];
+  const int64_t parts = hb.part_off[i + 1] - p0;
+  const int64_t part = t - p0;
+  const int64_t start = g.off[v], deg = g.off[v + 1] - start;
+  const int32_t scale = kAsync ? 0 : mp.gscale[v];
+  // Every item's record, also of vertices outside the sub-round: the
+  // partition pass tests the sub-round on the record's vertex.
+  if (threadIdx.x == 0) {
+    HubPart h;
+    h.start = start;
+    h.deg = deg;
+    h.v = v;
+    h.item = (uint32_t)i;
+    h.part = (uint32_t)part;
+    h.parts = (uint32_t)parts;
+    h.scale = scale;
+    h.cv = st.ids[v];
+    h.kv = st.nw[v];
+    h.res = ResOf(mp, v);
+    hb.hpart[t] = h;
+  }
   if (kAsync && !InSubRound(mp, v)) return;
-  const int64_t part = hp.part;
-  const int64_t p0 = t - part;
-  const int64_t parts = hp.parts;
-  const int64_t start = hp.start, deg = hp.deg;
   const int64_t per = (deg + parts - 1) / parts;
   const int64_t e0 = part * per, e1 = min(deg, (part + 1) * per);
   const bool local = parts <= kGatherMaxBuckets;
   // Sync rounds flag the partitions that receive a fine value (bin-7
   // vertices take the hub path's float values, EdgeValue(w, true, offset)).
-  const int thr = kAsync ? kNoFineThr : ScaleThr(hp.scale);
+  const int thr = kAsync ? kNoFineThr : ScaleThr(scale);
   constexpr int U = kGatherU;
   for (int64_t q0 = e0; q0 < e1; q0 += (int64_t)kGatherThreads * U) {
     uint32_t c[U];
@@ -1511,7 +1503,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const uint32_t nb = hb.bcur[t];
   const uint32_t fine = hb.pfine[t];
   const uint2* bk = hb.bkt + t * hb.bcap;
-  constexpr int P = 3;
+  constexpr int P = 4;
   uint2 pre[P];
 #pragma unroll
   for (int r = 0; r < P; ++r) {
@@ -1527,9 +1519,8 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   if (threadIdx.x == 0) nocc = 0;
   const uint32_t v = hp.v;
   if (kAsync && !InSubRound(mp, v)) return;
-  // Read after the table is built; issued now.
-  const uint32_t cv = st.ids[v];
-  const double kv = st.nw[v];
+  const uint32_t cv = hp.cv;
+  const double kv = hp.kv;
   __syncthreads();
   const int64_t i = hp.item;
   const uint32_t parts = hp.parts;
@@ -1584,7 +1575,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   // The partition's candidates, reduced to the top two values of
   // x_c = NetWeight(S_c, k_v K_c, res); the decision adds the own-cluster
   // term, which only the partition holding c(v) knows, afterwards.
-  const double res = ResOf(mp, v);
+  const double res = hp.res;
   double mw = 0;
   mw += kv;
   // A thread's (at most KS) candidate clusters have their weights loaded
@@ -1888,9 +1879,9 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
                    int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
-                   const unsigned long long* dcount, HubPart* rec, const int32_t* gscale) {
+                   const unsigned long long* dcount) {
   hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
-                     part_edges, part_off, ent_off, item_of, dcount, rec, gscale);
+                     part_edges, part_off, ent_off, item_of, dcount);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

@@ -235,9 +235,10 @@ constexpr int64_t kHubPartEdges = 2048;
 // list of the partition counts ceil(deg / kHubPartEdges) and of the degrees;
 // hflag (bit 0: own-cluster sum found in tval, bit 1: inexact or table
 // overflow, bit 2: a candidate exists) is zero between launches.
-// One partition item of a bin-7 list, written by the partition scan so that
-// the passes over partitions read their vertex, its adjacency range and its
-// order-test scale with one load instead of a chain of dependent ones.
+// One partition item of a bin-7 list, written by the gather pass so that the
+// partition pass reads its vertex, its adjacency range and the vertex state
+// it needs with one load instead of a chain of dependent ones (no vertex
+// moves between the two passes).
 struct alignas(16) HubPart {
   int64_t start = 0;   // offset of v's adjacency
   int64_t deg = 0;     // degree of v
@@ -246,8 +247,12 @@ struct alignas(16) HubPart {
   uint32_t part = 0;   // this partition's index among v's
   uint32_t parts = 0;  // v's partition count
   int32_t scale = 0;   // gscale[v] (0 without order-test scales)
-  uint32_t pad[3] = {0, 0, 0};
+  uint32_t cv = 0;     // ids[v]
+  double kv = 0.0;     // nw[v]
+  double res = 0.0;    // resolution of v's graph
+  uint32_t pad[2] = {0, 0};
 };
+
 
 struct HubScratch {
   int64_t part_edges = kHubPartEdges;
@@ -324,11 +329,9 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 // Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
 // partitions of part_edges edges, and the list item of every partition
 // (item_of[part_off[i] + p] = i).
-// rec (optional): the record of every partition, with gscale[v] (optional).
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
                    int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
-                   const unsigned long long* dcount = nullptr, HubPart* rec = nullptr,
-                   const int32_t* gscale = nullptr);
+                   const unsigned long long* dcount = nullptr);
 
 // Every vertex of bin 7 (deg > th.block_max) appended to list; *count zeroed
 // by the caller.
