@@ -331,6 +331,9 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
                                       by one; 0 (default, measured fastest): the host reads
                                       every round's frontier counts back.  All three give
                                       the same results. */
+#define PCC_OPT_CONTRACT_METHOD 12 /* contraction: 0 (default) cluster-grouped sorts on chip;
+                                       1 one LSD radix sort of every kept edge.  Both give
+                                       the same contracted graph. */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* Autotuner of the degree-bin thresholds (PCC_OPT_WAVE_MAX_DEG,

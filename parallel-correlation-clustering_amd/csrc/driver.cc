@@ -98,6 +98,7 @@ int ClusterEachGraph(pcc_graph* g, const pcc_config* cfg_in, int objective, cons
     h->hub_part_edges = g->hub_part_edges;
     h->pull_fraction = g->pull_fraction;
     h->device_rounds = g->device_rounds;
+    h->contract_method = g->contract_method;
     h->th = g->th;
     PCC_RETURN_IF_ERROR(UploadCsr(h, v1 - v0, e1 - e0, lo.data(), ln.data(), w.data() + e0));
     pcc_config c = *cfg_in;
@@ -493,6 +494,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
         return SetError(PCC_INVALID_ARGUMENT, "device rounds must be 0, 1 or 2");
       g->device_rounds = (int)value;
       return PCC_OK;
+    case PCC_OPT_CONTRACT_METHOD:
+      if (value < 0 || value > 1)
+        return SetError(PCC_INVALID_ARGUMENT, "contraction method must be 0 or 1");
+      g->contract_method = (int)value;
+      return PCC_OK;
     case PCC_OPT_PULL_FRACTION:
       if (value < 0 || value > 1024)
         return SetError(PCC_INVALID_ARGUMENT, "pull fraction must lie in [0, 1024]");
@@ -678,7 +684,7 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
     DevCsr c;
     DevBuf<double> nw2;
     CompressGraph(&g->alloc, g->csr.View(), ids.get(), nw.get(), nullptr, nullptr, nullptr,
-                  (int64_t)max_id + 1, &c, &nw2, s);
+                  (int64_t)max_id + 1, &c, &nw2, s, g->contract_method);
     // Hand the result to a new handle (host round trip keeps ownership simple).
     std::vector<int64_t> off((size_t)c.n + 1);
     std::vector<uint32_t> nb((size_t)std::max<int64_t>(c.m, 1));

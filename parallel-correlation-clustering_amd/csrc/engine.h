@@ -47,6 +47,7 @@ struct pcc_graph {
   // 2 device counts with plain launches.
   int device_rounds = 0;
   int64_t hub_part_edges = pcc::kHubPartEdges;
+  int contract_method = 0;  // PCC_OPT_CONTRACT_METHOD
   // Pull expansion when the seed volume exceeds pull_fraction / 1024 of the
   // graph's (edges + vertices); 1024 disables it.
   int pull_fraction = 512;

@@ -1027,9 +1027,27 @@ void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, u
   PCC_HIP_CHECK(hipGetLastError());
 }
 
+void SegmentSumsF32(DeviceAllocator* al, const float* vals, const int64_t* start, int64_t nseg,
+                    int64_t nvals, float* out, hipStream_t s) {
+  SegmentSums<float>(al, vals, start, nseg, nvals, out, s);
+}
+void SegmentSumsF64(DeviceAllocator* al, const double* vals, const int64_t* start, int64_t nseg,
+                    int64_t nvals, double* out, hipStream_t s) {
+  SegmentSums<double>(al, vals, start, nseg, nvals, out, s);
+}
+
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
                    const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
-                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s) {
+                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s, int method) {
+  if (method == 0 && CompressGraphGrouped(al, g, ids, nw, vgraph, gbase, newbase, n2, out, out_nw, s))
+    return;
+  CompressGraphSorted(al, g, ids, nw, vgraph, gbase, newbase, n2, out, out_nw, s);
+}
+
+void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
+                         const double* nw, const uint32_t* vgraph, const int64_t* gbase,
+                         const int64_t* newbase, int64_t n2, DevCsr* out,
+                         DevBuf<double>* out_nw, hipStream_t s) {
   const int64_t n = g.n;
   const GraphMap gm{vgraph, gbase, newbase};
   const int vbits = BitsFor((uint64_t)(n2 > 1 ? n2 - 1 : 1));

@@ -369,5 +369,9 @@ void RadixSortPairsF64(DeviceAllocator* al, uint64_t* keys, double* vals, int64_
                        int begin_bit, int end_bit, hipStream_t s) {
   RadixSortImpl<double>(al, keys, vals, n, begin_bit, end_bit, s);
 }
+void RadixSortPairsU32(DeviceAllocator* al, uint64_t* keys, uint32_t* vals, int64_t n,
+                       int begin_bit, int end_bit, hipStream_t s) {
+  RadixSortImpl<uint32_t>(al, keys, vals, n, begin_bit, end_bit, s);
+}
 
 }  // namespace pcc

@@ -524,11 +524,20 @@ void RadixSortPairsF32(DeviceAllocator* al, uint64_t* keys, float* vals, int64_t
                        int end_bit, hipStream_t s);
 void RadixSortPairsF64(DeviceAllocator* al, uint64_t* keys, double* vals, int64_t n,
                        int begin_bit, int end_bit, hipStream_t s);
+void RadixSortPairsU32(DeviceAllocator* al, uint64_t* keys, uint32_t* vals, int64_t n,
+                       int begin_bit, int end_bit, hipStream_t s);
 
 // Segment boundaries of sorted keys: returns the number of runs of equal keys
 // and fills start[0..nseg] (start[nseg] = n).  Synchronizes the stream.
 int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<int64_t>* start,
                  hipStream_t s);
+
+// Sums of the runs [start[i], start[i + 1]) of vals, i < nseg, in pbbslib's
+// reduce order (left fold up to 1024 terms, blocked beyond), into out[i].
+void SegmentSumsF32(DeviceAllocator* al, const float* vals, const int64_t* start, int64_t nseg,
+                    int64_t nvals, float* out, hipStream_t s);
+void SegmentSumsF64(DeviceAllocator* al, const double* vals, const int64_t* start, int64_t nseg,
+                    int64_t nvals, double* out, hipStream_t s);
 
 // ------------------------------------------------------------ contraction --
 
@@ -559,9 +568,22 @@ void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, u
 // vertex newbase[g] + (c - gbase[g]) of the contracted level, whose size is n2.
 // Vertices of graphs with newbase < 0 are dropped.  vgraph / gbase / newbase
 // null: one graph, identity numbering (n2 = 1 + max id).
+// method 0: cluster-grouped (k_group.hip); 1: one LSD radix sort of every
+// kept edge by (c(u), c(v)) (k_contract.hip).  Both give the same graph.
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
                    const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
-                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s);
+                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s,
+                   int method = 0);
+void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
+                         const double* nw, const uint32_t* vgraph, const int64_t* gbase,
+                         const int64_t* newbase, int64_t n2, DevCsr* out,
+                         DevBuf<double>* out_nw, hipStream_t s);
+// Returns false (touching nothing) for inputs it does not take: n or n2 of
+// 2^32 - 1 or more, or keys of its radix path wider than 64 bits.
+bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
+                          const double* nw, const uint32_t* vgraph, const int64_t* gbase,
+                          const int64_t* newbase, int64_t n2, DevCsr* out,
+                          DevBuf<double>* out_nw, hipStream_t s);
 
 // The subcluster numbering of CompressSubclusters
 // (parallel-correlation-clusterer.cc:436-474) of one graph's clustering ids:

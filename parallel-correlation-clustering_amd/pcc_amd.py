@@ -53,6 +53,7 @@ OPT_PULL_FRACTION = 8
 OPT_CLUSTER_SUBROUNDS = 9
 OPT_HASH_WAVES = 10
 OPT_DEVICE_ROUNDS = 11
+OPT_CONTRACT_METHOD = 12
 
 
 class StatusError(RuntimeError):
