@@ -334,6 +334,9 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
 #define PCC_OPT_CONTRACT_METHOD 12 /* contraction: 0 (default) cluster-grouped sorts on chip;
                                        1 one LSD radix sort of every kept edge.  Both give
                                        the same contracted graph. */
+#define PCC_OPT_CONTRACT_CHUNK 13  /* cluster-grouped contraction: most terms one radix sort of
+                                      its large clusters takes (default and maximum 2^31);
+                                      smaller values bound the sort's temporary memory */
 int pcc_set_option(pcc_graph* g, int key, int64_t value);
 
 /* Autotuner of the degree-bin thresholds (PCC_OPT_WAVE_MAX_DEG,

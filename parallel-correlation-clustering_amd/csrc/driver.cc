@@ -99,6 +99,7 @@ int ClusterEachGraph(pcc_graph* g, const pcc_config* cfg_in, int objective, cons
     h->pull_fraction = g->pull_fraction;
     h->device_rounds = g->device_rounds;
     h->contract_method = g->contract_method;
+    h->contract_chunk = g->contract_chunk;
     h->th = g->th;
     PCC_RETURN_IF_ERROR(UploadCsr(h, v1 - v0, e1 - e0, lo.data(), ln.data(), w.data() + e0));
     pcc_config c = *cfg_in;
@@ -499,6 +500,11 @@ int pcc_set_option(pcc_graph* g, int key, int64_t value) {
         return SetError(PCC_INVALID_ARGUMENT, "contraction method must be 0 or 1");
       g->contract_method = (int)value;
       return PCC_OK;
+    case PCC_OPT_CONTRACT_CHUNK:
+      if (value < 1 || value > kContractChunkTerms)
+        return SetError(PCC_INVALID_ARGUMENT, "contraction chunk must lie in [1, 2^31]");
+      g->contract_chunk = value;
+      return PCC_OK;
     case PCC_OPT_PULL_FRACTION:
       if (value < 0 || value > 1024)
         return SetError(PCC_INVALID_ARGUMENT, "pull fraction must lie in [0, 1024]");
@@ -684,7 +690,8 @@ int pcc_compress(pcc_graph* g, const uint32_t* cluster_ids, const double* node_w
     DevCsr c;
     DevBuf<double> nw2;
     CompressGraph(&g->alloc, g->csr.View(), ids.get(), nw.get(), nullptr, nullptr, nullptr,
-                  (int64_t)max_id + 1, &c, &nw2, s, g->contract_method);
+                  (int64_t)max_id + 1, &c, &nw2, s, g->contract_method,
+                  g->contract_chunk);
     // Hand the result to a new handle (host round trip keeps ownership simple).
     std::vector<int64_t> off((size_t)c.n + 1);
     std::vector<uint32_t> nb((size_t)std::max<int64_t>(c.m, 1));

@@ -1181,7 +1181,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       next->lay.base = {0, n2};
       next->lay.orig.push_back(0);
       CompressGraph(al_, L.g, L.map.get(), L.nw.get(), nullptr, nullptr, nullptr, n2, &next->own,
-                    &next->nw, s_, h_->contract_method);
+                    &next->nw, s_, h_->contract_method,
+                    h_->contract_chunk);
       next->g = next->own.View();
       next->ids.Reset(al_, n2);
       next->sizes.Reset(al_, n2);
@@ -1233,7 +1234,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     if (batch) {
       DevBuf<int64_t> dnb = Upload(al_, newbase, s_);
       CompressGraph(al_, L.g, L.ids.get(), L.nw.get(), L.lay.vgraph.get(), L.lay.dbase.get(),
-                    dnb.get(), n2, &next->own, &next->nw, s_, h_->contract_method);
+                    dnb.get(), n2, &next->own, &next->nw, s_, h_->contract_method,
+                    h_->contract_chunk);
       next->lay.dbase = Upload(al_, next->lay.base, s_);
       next->lay.vgraph.Reset(al_, n2 > 0 ? n2 : 1);
       if (n2 > 0)
@@ -1242,7 +1244,8 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
       next->lay.gres = Upload(al_, res_next, s_);
     } else {
       CompressGraph(al_, L.g, L.ids.get(), L.nw.get(), nullptr, nullptr, nullptr, n2, &next->own,
-                    &next->nw, s_, h_->contract_method);
+                    &next->nw, s_, h_->contract_method,
+                    h_->contract_chunk);
     }
     next->g = next->own.View();
     next->ids.Reset(al_, n2);

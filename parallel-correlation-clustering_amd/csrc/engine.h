@@ -48,6 +48,7 @@ struct pcc_graph {
   int device_rounds = 0;
   int64_t hub_part_edges = pcc::kHubPartEdges;
   int contract_method = 0;  // PCC_OPT_CONTRACT_METHOD
+  int64_t contract_chunk = pcc::kContractChunkTerms;  // PCC_OPT_CONTRACT_CHUNK
   // Pull expansion when the seed volume exceeds pull_fraction / 1024 of the
   // graph's (edges + vertices); 1024 disables it.
   int pull_fraction = 512;

@@ -1038,8 +1038,10 @@ void SegmentSumsF64(DeviceAllocator* al, const double* vals, const int64_t* star
 
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
                    const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
-                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s, int method) {
-  if (method == 0 && CompressGraphGrouped(al, g, ids, nw, vgraph, gbase, newbase, n2, out, out_nw, s))
+                   int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s, int method,
+                   int64_t chunk_terms) {
+  if (method == 0 &&
+      CompressGraphGrouped(al, g, ids, nw, vgraph, gbase, newbase, n2, out, out_nw, s, chunk_terms))
     return;
   CompressGraphSorted(al, g, ids, nw, vgraph, gbase, newbase, n2, out, out_nw, s);
 }

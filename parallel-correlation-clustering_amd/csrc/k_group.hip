@@ -692,29 +692,29 @@ __global__ __launch_bounds__(kWave* W) void k_gv_multi(CsrView g, const uint32_t
   }
 }
 
-// Radix path, 1: the terms of the listed (large) clusters in CSR order, keyed
-// (rank << (vbits + 1)) | c(v), dropped terms | 1 << vbits (last in their
-// cluster).  Term q of the concatenation lies in cluster rank r with
-// boff[r] <= q < boff[r + 1].
+// Radix path, 1: the terms of a chunk of nr listed (large) clusters in CSR
+// order, keyed (rank << (vbits + 1)) | c(v), dropped terms | 1 << vbits (last
+// in their cluster).  Term q of the chunk's concatenation lies in cluster rank
+// a with boff[a] - boff[0] <= q < boff[a + 1] - boff[0].
 __global__ __launch_bounds__(256) void k_gv_big_emit(CsrView g, const uint32_t* __restrict__ ids,
-                                                     GroupIndex gi, GvItems L, int64_t nbig,
+                                                     GroupIndex gi, GvItems L, int64_t nr,
                                                      const int64_t* __restrict__ boff, int vbits,
                                                      uint64_t* __restrict__ keys,
                                                      float* __restrict__ vals) {
-  const int64_t total = boff[nbig];
+  const int64_t b0 = boff[0], total = boff[nr] - b0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (int64_t)gridDim.x * blockDim.x) {
-    int64_t a = 0, b = nbig - 1;
+    int64_t a = 0, b = nr - 1;
     while (a < b) {
       const int64_t mid = (a + b + 1) >> 1;
-      if (boff[mid] <= q)
+      if (boff[mid] - b0 <= q)
         a = mid;
       else
         b = mid - 1;
     }
     const uint4 itm = L.item[a];
     const uint32_t c = itm.x;
-    const int64_t P = L.rp0[a] + (q - boff[a]);
+    const int64_t P = L.rp0[a] + (q + b0 - boff[a]);
     int64_t x = itm.y, y = (int64_t)itm.z - 1;  // the last member whose row starts at or before P
     while (x < y) {
       const int64_t mid = (x + y + 1) >> 1;
@@ -852,7 +852,7 @@ void LaunchWave(const CsrView& g, const uint32_t* ids, const GroupIndex& gi, con
 bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
                           const double* nw, const uint32_t* vgraph, const int64_t* gbase,
                           const int64_t* newbase, int64_t n2, DevCsr* out, DevBuf<double>* out_nw,
-                          hipStream_t s) {
+                          hipStream_t s, int64_t chunk_terms) {
   const int64_t n = g.n;
   const int vbits = GvBitsFor((uint64_t)(n2 > 1 ? n2 - 1 : 1));
   if (n <= 0 || n2 <= 0 || n >= (int64_t)0xFFFFFFFFll || n2 >= (int64_t)0xFFFFFFFFll)
@@ -934,6 +934,9 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   LaunchMulti<16>(g, ids, gi, cls(11), cnt(11), np, wp, len.get(), s);
   PCC_HIP_CHECK(hipGetLastError());
   if (nbig > 0) {
+    // Large clusters through the radix sort, in chunks of consecutive ranks
+    // of at most chunk_terms terms (one cluster per chunk when it alone is
+    // larger), which bounds the sort's temporaries and its key count.
     const GvItems B = cls(kGvBig);
     DevBuf<uint32_t> bcap(al, nbig);
     DevBuf<int64_t> boff(al, nbig + 1);
@@ -941,28 +944,37 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
                        bcap.get());
     PCC_HIP_CHECK(hipGetLastError());
     ExclusiveScanU32toI64(al, bcap.get(), boff.get(), nbig, boff.get() + nbig, s);
-    int64_t mb = 0;
-    PCC_HIP_CHECK(hipMemcpyAsync(&mb, boff.get() + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> hoff((size_t)nbig + 1);
+    PCC_HIP_CHECK(hipMemcpyAsync(hoff.data(), boff.get(), sizeof(int64_t) * (size_t)(nbig + 1),
+                                 hipMemcpyDeviceToHost, s));
     PCC_HIP_CHECK(hipStreamSynchronize(s));
-    DevBuf<uint64_t> keys(al, mb);
-    DevBuf<float> vals(al, mb);
-    hipLaunchKernelGGL(k_gv_big_emit, dim3(GvGrid(mb, 256, 16384)), dim3(256), 0, s, g, ids, gi,
-                       B, nbig, boff.get(), vbits, keys.get(), vals.get());
-    PCC_HIP_CHECK(hipGetLastError());
-    RadixSortPairsF32(al, keys.get(), vals.get(), mb, 0, rbits + vbits + 1, s);
-    DevBuf<int64_t> start;
-    const int64_t nseg = Segments(al, keys.get(), mb, &start, s);
-    DevBuf<float> sums(al, nseg);
-    SegmentSumsF32(al, vals.get(), start.get(), nseg, mb, sums.get(), s);
-    DevBuf<int64_t> first(al, nbig);
-    hipLaunchKernelGGL(k_gv_big_first, dim3(GvGrid(nseg, 256, 8192)), dim3(256), 0, s, nseg,
-                       keys.get(), start.get(), vbits, first.get());
-    hipLaunchKernelGGL(k_gv_big_rows, dim3(GvGrid(nseg, 256, 8192)), dim3(256), 0, s, nseg,
-                       keys.get(), start.get(), sums.get(), first.get(), B, vbits, np, wp,
-                       len.get());
-    PCC_HIP_CHECK(hipGetLastError());
+    const int64_t chunk = std::max<int64_t>(chunk_terms, 1);
+    for (int64_t r0 = 0; r0 < nbig;) {
+      int64_t r1 = r0 + 1;
+      while (r1 < nbig && hoff[(size_t)r1 + 1] - hoff[(size_t)r0] <= chunk) ++r1;
+      const int64_t nr = r1 - r0, mb = hoff[(size_t)r1] - hoff[(size_t)r0];
+      const int cbits = GvBitsFor((uint64_t)(nr > 1 ? nr - 1 : 1)) + vbits + 1;
+      const GvItems C{B.item + r0, B.rp0 + r0};
+      DevBuf<uint64_t> keys(al, mb);
+      DevBuf<float> vals(al, mb);
+      hipLaunchKernelGGL(k_gv_big_emit, dim3(GvGrid(mb, 256, 16384)), dim3(256), 0, s, g, ids, gi,
+                         C, nr, boff.get() + r0, vbits, keys.get(), vals.get());
+      PCC_HIP_CHECK(hipGetLastError());
+      RadixSortPairsF32(al, keys.get(), vals.get(), mb, 0, cbits, s);
+      DevBuf<int64_t> start;
+      const int64_t nseg = Segments(al, keys.get(), mb, &start, s);
+      DevBuf<float> sums(al, nseg);
+      SegmentSumsF32(al, vals.get(), start.get(), nseg, mb, sums.get(), s);
+      DevBuf<int64_t> first(al, nr);
+      hipLaunchKernelGGL(k_gv_big_first, dim3(GvGrid(nseg, 256, 8192)), dim3(256), 0, s, nseg,
+                         keys.get(), start.get(), vbits, first.get());
+      hipLaunchKernelGGL(k_gv_big_rows, dim3(GvGrid(nseg, 256, 8192)), dim3(256), 0, s, nseg,
+                         keys.get(), start.get(), sums.get(), first.get(), C, vbits, np, wp,
+                         len.get());
+      PCC_HIP_CHECK(hipGetLastError());
+      r0 = r1;
+    }
   }
-
   // 4. Compaction.
   out->n = n2;
   out->off.Reset(al, n2 + 1);

@@ -570,10 +570,12 @@ void LaunchGraphMaxIds(int64_t n, const uint32_t* ids, const uint32_t* vgraph, u
 // null: one graph, identity numbering (n2 = 1 + max id).
 // method 0: cluster-grouped (k_group.hip); 1: one LSD radix sort of every
 // kept edge by (c(u), c(v)) (k_contract.hip).  Both give the same graph.
+// chunk_terms bounds the terms one radix sort of the grouped path takes.
+constexpr int64_t kContractChunkTerms = (int64_t)1 << 31;
 void CompressGraph(DeviceAllocator* al, const CsrView& g, const uint32_t* ids, const double* nw,
                    const uint32_t* vgraph, const int64_t* gbase, const int64_t* newbase,
                    int64_t n2, DevCsr* out, DevBuf<double>* out_nw, hipStream_t s,
-                   int method = 0);
+                   int method = 0, int64_t chunk_terms = kContractChunkTerms);
 void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
                          const double* nw, const uint32_t* vgraph, const int64_t* gbase,
                          const int64_t* newbase, int64_t n2, DevCsr* out,
@@ -583,7 +585,8 @@ void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* 
 bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t* ids,
                           const double* nw, const uint32_t* vgraph, const int64_t* gbase,
                           const int64_t* newbase, int64_t n2, DevCsr* out,
-                          DevBuf<double>* out_nw, hipStream_t s);
+                          DevBuf<double>* out_nw, hipStream_t s,
+                          int64_t chunk_terms = kContractChunkTerms);
 
 // The subcluster numbering of CompressSubclusters
 // (parallel-correlation-clusterer.cc:436-474) of one graph's clustering ids:

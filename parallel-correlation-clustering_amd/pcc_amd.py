@@ -54,6 +54,7 @@ OPT_CLUSTER_SUBROUNDS = 9
 OPT_HASH_WAVES = 10
 OPT_DEVICE_ROUNDS = 11
 OPT_CONTRACT_METHOD = 12
+OPT_CONTRACT_CHUNK = 13
 
 
 class StatusError(RuntimeError):

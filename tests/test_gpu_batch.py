@@ -72,3 +72,47 @@ def test_batch_result_independent_of_mix(pcc):
     part, _, _, _ = pcc.DeviceBatch.from_csrs(graphs[3:9][::-1]).cluster_batch(cfg)
     for j, i in enumerate(range(8, 2, -1)):
         assert np.array_equal(part[j], full[i])
+
+
+def _power_law_graphs(pcc, seed, count):
+    """The batched workload's generator at test sizes: n ~ logU[200, 4000],
+    mean degree ~ U[8, 64], community sizes from a discrete power law with
+    exponent 2 on [8, n], p_out / p_in = 0.01, U[0, 1] weights."""
+    rng = np.random.default_rng(seed)
+    graphs, res = [], []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(4000))))
+        d = rng.uniform(8, 64)
+        sizes, left = [], n
+        while left > 0:
+            u = rng.random()
+            s_ = max(1, min(int(8 / (1 - u * (1 - 8 / max(n, 9)))), left))
+            sizes.append(s_)
+            left -= s_
+        sizes = np.array(sizes, np.int64)
+        denom = float((sizes * (sizes - 1)).sum() + 0.01 * (sizes * (n - sizes)).sum()) / n
+        p_in = min(1.0, d / max(denom, 1e-9))
+        graphs.append(pcc.gen_sbm_blocks(sizes, p_in, 0.01 * p_in, 0.0, 1.0, seed=seed * 997 + i))
+        res.append(rng.uniform(0.001, 0.1))
+    return graphs, np.array(res)
+
+
+def test_power_law_batch_with_tuned_bins(pcc, orc):
+    """Power-law community graphs of the batched workload, with the degree-bin
+    bounds and hub partition size its autotuning picked (512, 4096, 1024
+    edges): every graph equals its own run and the oracle."""
+    graphs, res = _power_law_graphs(pcc, 71, 16)
+    print(f"power-law batch: {len(graphs)} graphs, {sum(g.m for g in graphs)} edges", flush=True)
+    cfg = pcc.make_config(move_method=2, refine=1)
+    b = pcc.DeviceBatch.from_csrs(graphs)
+    for key, val in ((pcc.OPT_WAVE_MAX_DEG, 512), (pcc.OPT_BLOCK_MAX_DEG, 4096),
+                     (pcc.OPT_HUB_PART_EDGES, 1024)):
+        b.set_option(key, val)
+    ids, obj, _, st = b.cluster_batch(cfg, res)
+    for i, g in enumerate(graphs):
+        c1 = pcc.make_config(resolution=float(res[i]), move_method=2, refine=1)
+        single, sst = pcc.DeviceGraph.from_csr(g).cluster(c1)
+        assert np.array_equal(ids[i], single), f"graph {i} differs from its single run"
+        assert obj[i] == sst.objective
+        r = orc.cluster(orc.G.from_csr(g), orc.config_from(c1))
+        assert np.array_equal(ids[i], r.ids), f"graph {i} differs from the oracle"

@@ -115,3 +115,29 @@ def test_cluster_runs_equal_between_methods(pcc):
     (i0, s0), (i1, s1) = out
     assert np.array_equal(i0, i1)
     assert s0.objective == s1.objective and s0.rounds == s1.rounds
+
+
+def test_radix_path_in_chunks(pcc, orc):
+    """Clusters beyond 8192 terms go through the radix sort in chunks of at
+    most PCC_OPT_CONTRACT_CHUNK terms (a cluster larger than a chunk alone);
+    every chunk size gives the oracle's graph."""
+    csr = pcc.gen_rmat(15, 16, w_lo=-1.0, w_hi=1.0, seed=33)
+    rng = np.random.default_rng(12)
+    ids = rng.integers(0, 40, size=csr.n).astype(np.uint32)  # ~40 clusters of ~13K terms
+    nw = rng.integers(1, 4, size=csr.n).astype(np.float64)
+    print(f"contract chunks n={csr.n} m={csr.m}", flush=True)
+    og, onw = orc.compress(orc.G.from_csr(csr), ids, nw)
+    for chunk in (1, 9000, 40000, 1 << 31):
+        dg = pcc.DeviceGraph.from_csr(csr)
+        dg.set_option(pcc.OPT_CONTRACT_CHUNK, chunk)
+        cg, gnw = dg.compress(ids, nw)
+        _same(cg.to_csr(), og)
+        assert np.array_equal(gnw.view(np.uint64), onw.view(np.uint64)), f"chunk {chunk}"
+
+
+def test_contract_options_checked(pcc):
+    dg = pcc.DeviceGraph.from_csr(pcc.gen_sbm(100, 4, 0.2, 0.01, seed=3))
+    for key, bad in ((pcc.OPT_CONTRACT_METHOD, 2), (pcc.OPT_CONTRACT_METHOD, -1),
+                     (pcc.OPT_CONTRACT_CHUNK, 0), (pcc.OPT_CONTRACT_CHUNK, (1 << 31) + 1)):
+        with pytest.raises(pcc.StatusError):
+            dg.set_option(key, bad)
