@@ -101,6 +101,12 @@ StatusOr<InMemoryClusterer::Clustering> ParallelCorrelationClusterer::Cluster(
   return OutputIndicesById(ids);
 }
 
+std::string InMemoryClusterer::StringId(NodeId id) const {
+  if (node_id_map_ == nullptr) return std::to_string(id);
+  if ((size_t)id < node_id_map_->size()) return (*node_id_map_)[id];
+  return "missing-id-" + std::to_string(id);
+}
+
 Status ParallelCorrelationClusterer::RefineClusters(const ClustererConfig& config,
                                                     Clustering* clustering) const {
   return RefineClusters(config, clustering, {});
@@ -108,7 +114,9 @@ Status ParallelCorrelationClusterer::RefineClusters(const ClustererConfig& confi
 
 Status ParallelCorrelationClusterer::RefineClusters(const ClustererConfig& config,
                                                     Clustering* clustering,
-                                                    const std::vector<double>& node_weights) const {
+                                                    const std::vector<double>& node_weights,
+                                                    double resolution) const {
+  (void)resolution;
   if (!graph_.handle()) return Status(PCC_INVALID_ARGUMENT, "graph not imported");
   if (!clustering) return Status(PCC_INVALID_ARGUMENT, "null clustering");
   const int64_t n = graph_.num_nodes();
@@ -119,13 +127,18 @@ Status ParallelCorrelationClusterer::RefineClusters(const ClustererConfig& confi
   std::vector<uint32_t> ids((size_t)std::max<int64_t>(n, 1), PCC_NO_MOVE);
   for (size_t i = 0; i < clustering->size(); ++i)
     for (uint32_t v : (*clustering)[i]) {
-      if ((int64_t)v >= n || ids[v] != PCC_NO_MOVE)
-        return Status(PCC_INVALID_ARGUMENT, "clustering must cover every node exactly once");
+      if ((int64_t)v >= n)
+        return Status(PCC_INVALID_ARGUMENT,
+                      "clustering names node " + StringId(v) + ", which is not in the graph");
+      if (ids[v] != PCC_NO_MOVE)
+        return Status(PCC_INVALID_ARGUMENT,
+                      "clustering holds node " + StringId(v) + " more than once");
       ids[v] = (uint32_t)i;
     }
   for (int64_t v = 0; v < n; ++v)
     if (ids[(size_t)v] == PCC_NO_MOVE)
-      return Status(PCC_INVALID_ARGUMENT, "clustering must cover every node exactly once");
+      return Status(PCC_INVALID_ARGUMENT,
+                    "clustering misses node " + StringId((NodeId)v));
   const int rc = pcc_refine_clusters(graph_.handle(), &config.correlation_clusterer_config,
                                      objective(), node_weights.empty() ? nullptr : node_weights.data(),
                                      ids.data(), &stats_);

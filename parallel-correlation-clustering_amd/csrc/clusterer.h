@@ -95,6 +95,21 @@ class InMemoryClusterer {
     (void)clustering;
     return OkStatus();
   }
+
+  // String ids of the NodeIds, used in error messages
+  // (in-memory-clusterer.h:101-108).  Not owned: the vector must outlive
+  // every call on this object.
+  void set_node_id_map(const std::vector<std::string>* node_id_map) {
+    node_id_map_ = node_id_map;
+  }
+
+ protected:
+  // The map's entry for id, "missing-id-<id>" for an id outside the map, or
+  // the decimal id when no map was set (in-memory-clusterer.cc:27-35).
+  std::string StringId(NodeId id) const;
+
+ private:
+  const std::vector<std::string>* node_id_map_ = nullptr;
 };
 
 // GbbsGraph stand-in: imports adjacency lists and keeps the CSR in HBM.
@@ -128,8 +143,11 @@ class ParallelCorrelationClusterer : public InMemoryClusterer {
   StatusOr<Clustering> Cluster(const ClustererConfig& config) const override;
   Status RefineClusters(const ClustererConfig& config, Clustering* clustering) const override;
   // RefineClusters with explicit node weights (parallel-correlation-clusterer.h:58-91).
+  // resolution is accepted and not used: the reference passes it on as
+  // original_resolution, which it never reads
+  // (parallel-correlation-clusterer.cc:407-412, 631-639).
   Status RefineClusters(const ClustererConfig& config, Clustering* clustering,
-                        const std::vector<double>& node_weights) const;
+                        const std::vector<double>& node_weights, double resolution = 0) const;
   const pcc_stats& last_stats() const { return stats_; }
   DeviceGraph& device_graph() { return graph_; }
 

@@ -836,10 +836,13 @@ constexpr int kRepairChunks = THREADS >= 1024 ? 1 : 8;
 
 // LDS staging of the order repair: one chunk of THREADS edges.
 template <int THREADS>
+// Per staged edge: its table slot (< 2^16), its pbbs block of the hub path's
+// own-cluster order (position / PbbsBlock(deg) < 2^16 for any degree below
+// 2^32) and its weight as stored (the fold recomputes EdgeValue).
 struct RepairStage {
-  uint32_t slot[THREADS];
-  uint32_t pos[THREADS];
-  double x[THREADS];
+  uint16_t slot[THREADS];
+  uint16_t blk[THREADS];
+  float w[THREADS];
   uint32_t wsum[THREADS / kWave];
   double T;
 };
@@ -880,7 +883,7 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
   for (int64_t q0 = 0; q0 < deg; q0 += (int64_t)THREADS * U) {
     uint32_t c[U];
     int slot[U];
-    double x[U];
+    float x[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t q = q0 + (int64_t)k * THREADS + threadIdx.x;
@@ -902,7 +905,7 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t q = q0 + (int64_t)k * THREADS + threadIdx.x;
-      x[k] = slot[k] >= 0 ? EdgeValue(g.w[start + q], hub, mp.offset) : 0.0;
+      x[k] = slot[k] >= 0 ? g.w[start + q] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -918,18 +921,22 @@ __device__ __forceinline__ double RepairInexactSlots(const CsrView& g, const Mov
       }
       if (slot[k] >= 0) {
         const uint32_t j = before + (uint32_t)__popcll(m & LaneMaskLt());
-        stg->slot[j] = (uint32_t)slot[k];
-        stg->pos[j] = (uint32_t)(q0 + (int64_t)k * THREADS + threadIdx.x);
-        stg->x[j] = x[k];
+        stg->slot[j] = (uint16_t)slot[k];
+        // Only the own cluster's edges on the hub path need their block.
+        stg->blk[j] = (own_ordered && slot[k] == own)
+                          ? (uint16_t)((uint32_t)(q0 + (int64_t)k * THREADS + threadIdx.x) /
+                                       (uint32_t)bs)
+                          : (uint16_t)0;
+        stg->w[j] = x[k];
       }
       __syncthreads();
       if (threadIdx.x == 0) {
         for (uint32_t j = 0; j < total; ++j) {
           const uint32_t sl = stg->slot[j];
-          const double xx = stg->x[j];
+          const double xx = EdgeValue(stg->w[j], hub, mp.offset);
           vals[sl] += xx;
           if (own_ordered && (int)sl == own) {
-            const int64_t blk = (int64_t)stg->pos[j] / bs;
+            const int64_t blk = stg->blk[j];
             if (blk != cur_blk) {
               if (cur_blk >= 0) {
                 t_tot = have_tot ? t_tot + t_blk : t_blk;
@@ -1444,12 +1451,24 @@ struct Top2 {
   uint32_t c1 = kNoneId;
   double x2 = -INFINITY;
 };
+// An empty Top2 (c1 = kNoneId) has x1 = x2 = -inf, so the general rule covers
+// it: written with selects, no divergent branches.
 __device__ __forceinline__ Top2 MergeTop2(Top2 a, Top2 b) {
-  if (a.c1 == kNoneId) return Top2{b.x1, b.c1, b.x2 > a.x2 ? b.x2 : a.x2};
-  if (b.c1 == kNoneId) return Top2{a.x1, a.c1, a.x2 > b.x2 ? a.x2 : b.x2};
-  if (a.x1 > b.x1) return Top2{a.x1, a.c1, fmax(a.x2, b.x1)};
-  if (b.x1 > a.x1) return Top2{b.x1, b.c1, fmax(b.x2, a.x1)};
-  return Top2{a.x1, min(a.c1, b.c1), fmax(a.x2, b.x2)};
+  const bool ga = a.x1 > b.x1, gb = b.x1 > a.x1;
+  Top2 r;
+  r.x1 = ga ? a.x1 : b.x1;
+  r.c1 = ga ? a.c1 : (gb ? b.c1 : min(a.c1, b.c1));
+  r.x2 = ga ? fmax(a.x2, b.x1) : (gb ? fmax(b.x2, a.x1) : fmax(a.x2, b.x2));
+  return r;
+}
+// MergeTop2(t, Top2{x, c, -inf}).
+__device__ __forceinline__ Top2 AddTop2(Top2 t, double x, uint32_t c) {
+  const bool gt = x > t.x1, eq = x == t.x1;
+  Top2 r;
+  r.x1 = gt ? x : t.x1;
+  r.c1 = gt ? c : (eq ? min(t.c1, c) : t.c1);
+  r.x2 = gt ? t.x1 : (eq ? t.x2 : fmax(t.x2, x));
+  return r;
 }
 __device__ __forceinline__ Top2 WaveTop2(Top2 x) {
   for (int s = kWave >> 1; s > 0; s >>= 1) {
@@ -1503,7 +1522,7 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   const uint32_t nb = hb.bcur[t];
   const uint32_t fine = hb.pfine[t];
   const uint2* bk = hb.bkt + t * hb.bcap;
-  constexpr int P = 4;
+  constexpr int P = 2048 / kHubThreads;
   uint2 pre[P];
 #pragma unroll
   for (int r = 0; r < P; ++r) {
@@ -1588,6 +1607,8 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
     const uint32_t k = threadIdx.x + (uint32_t)e * kHubThreads;
     cc[e] = kNoneId;
     sv[e] = 0.0;
+    kc[e] = 0.0;
+    if ((uint32_t)e * kHubThreads >= n_occ) continue;  // the same for the whole workgroup
     if (k < n_occ) {
       const uint32_t s = occ[k];
       sv[e] = vals[s];
@@ -1597,8 +1618,10 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
   }
   Top2 top;
 #pragma unroll
-  for (int e = 0; e < KS; ++e)
-    if (cc[e] != kNoneId) top = MergeTop2(top, Top2{NetWeight(sv[e], mw * kc[e], res), cc[e], -INFINITY});
+  for (int e = 0; e < KS; ++e) {
+    if ((uint32_t)e * kHubThreads >= n_occ) continue;
+    if (cc[e] != kNoneId) top = AddTop2(top, NetWeight(sv[e], mw * kc[e], res), cc[e]);
+  }
   top = BlockTop2<kHubThreads>(top, reinterpret_cast<Top2*>(&stg));
   if (threadIdx.x == 0) {
     hb.px1[t] = top.x1;

@@ -216,3 +216,32 @@ def test_fullsize_rmat22_fixture_graph_reproduces(pcc):
     for a in (csr.offsets.astype("<i8"), csr.nbrs.astype("<u4"), csr.weights.view("<u4")):
         h.update(np.ascontiguousarray(a).tobytes())
     assert h.hexdigest() == fx["csr_sha256"]
+
+
+def test_facade_string_ids(tmp_path):
+    """InMemoryClusterer::set_node_id_map / StringId (in-memory-clusterer.h:101-121,
+    in-memory-clusterer.cc:27-35): the decimal id without a map, the map's entry
+    inside it, "missing-id-<id>" outside; compiled against the facade, no GPU call."""
+    src = tmp_path / "ids.cc"
+    src.write_text(r'''
+#include <cstdio>
+#include "clusterer.h"
+struct Probe : pcc::in_memory::ParallelCorrelationClusterer {
+  std::string Id(NodeId v) const { return StringId(v); }
+};
+int main() {
+  Probe p;
+  std::printf("%s\n", p.Id(7).c_str());
+  const std::vector<std::string> names = {"a", "b"};
+  p.set_node_id_map(&names);
+  std::printf("%s %s %s\n", p.Id(0).c_str(), p.Id(1).c_str(), p.Id(5).c_str());
+  return 0;
+}
+''')
+    exe = tmp_path / "ids"
+    pkg = os.path.join(ROOT, "parallel-correlation-clustering_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}",
+                    f"-I{os.path.join(pkg, 'csrc')}", str(src), "-o", str(exe), f"-L{pkg}",
+                    "-lpcc_amd", f"-Wl,-rpath,{pkg}"], check=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=60).stdout
+    assert out.split("\n")[:2] == ["7", "a b missing-id-5"]
