@@ -565,6 +565,10 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb.bcap = bcap;
     c->hb.item_of = c->hb_item_of.get();
     c->hb.hpart = c->hb_hpart.get();
+    if (cfg_.async) {
+      c->hb_cbuf.Reset(al_, t7 > 0 ? t7 : 1);
+      c->hb.cbuf = c->hb_cbuf.get();
+    }
     if (cfg_.move_method != PCC_ALL_MOVE) {
       c->hubs.Reset(al_, c7);
       c->hubs_part.Reset(al_, c7 + 1);

@@ -151,7 +151,7 @@ class Engine {
     // Bin-7 scratch, sized for every bin-7 vertex of the level.
     HubScratch hb;
     DevBuf<int64_t> hb_part, hb_ent;
-    DevBuf<uint32_t> hb_flag, hb_pc1, hb_item_of, hb_bcur, hb_pfine;
+    DevBuf<uint32_t> hb_flag, hb_pc1, hb_item_of, hb_bcur, hb_pfine, hb_cbuf;
     DevBuf<uint2> hb_bkt;
     DevBuf<HubPart> hb_hpart;
     DevBuf<double> hb_tval, hb_px1, hb_px2;

@@ -1316,29 +1316,41 @@ __device__ __forceinline__ bool AccumulateGathered(const CsrView& g, const State
                                                    int64_t deg, bool hub, int thr, uint32_t* keys,
                                                    double* vals, uint32_t* cnt, uint32_t mask,
                                                    uint32_t parts, uint32_t part,
-                                                   const SlotList& sl) {
+                                                   const SlotList& sl,
+                                                   const uint32_t* __restrict__ cb) {
   bool overflow = false;
-  for (int64_t q0 = 0; q0 < deg; q0 += THREADS) {
-    const int64_t q = q0 + threadIdx.x;
-    uint32_t c = kNoneId;
-    float wf = 0.0f;
-    if (q < deg) {
-      const uint32_t u = g.nbr[start + q];
-      if (u != v) c = st.ids[u];
-      wf = g.w[start + q];
+  // U chunks of THREADS edges per trip: all their neighbour loads, then all
+  // their cluster gathers, then the inserts, so a long row costs one pair of
+  // memory round trips per U chunks rather than per chunk.
+  constexpr int U = 8;
+  for (int64_t q0 = 0; q0 < deg; q0 += (int64_t)U * THREADS) {
+    uint32_t c[U];
+    float wf[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t q = q0 + (int64_t)k * THREADS + threadIdx.x;
+      c[k] = q < deg ? (cb ? cb[q] : g.nbr[start + q]) : kNoneId;
+      wf[k] = q < deg ? g.w[start + q] : 0.0f;
     }
-    const bool act = c != kNoneId && (parts == 1 || PartOf(c, parts) == part);
-    const double x = act ? EdgeValue(wf, hub, mp.offset) : 0.0;
-    bool fresh = false;
-    const int slot = act ? HashSlot(keys, mask, c, &fresh) : -1;
-    const bool listed = sl.AddWave(fresh, slot);
-    if (!act) continue;
-    if (slot < 0 || !listed) {
-      overflow = true;
-      continue;
+    if (!cb) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) c[k] = (c[k] != kNoneId && c[k] != v) ? st.ids[c[k]] : kNoneId;
     }
-    atomicAdd(&vals[slot], x);
-    if (!kAsync) RecordTerm(cnt, slot, x, thr);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool act = c[k] != kNoneId && (parts == 1 || PartOf(c[k], parts) == part);
+      const double x = act ? EdgeValue(wf[k], hub, mp.offset) : 0.0;
+      bool fresh = false;
+      const int slot = act ? HashSlot(keys, mask, c[k], &fresh) : -1;
+      const bool listed = sl.AddWave(fresh, slot);
+      if (!act) continue;
+      if (slot < 0 || !listed) {
+        overflow = true;
+        continue;
+      }
+      atomicAdd(&vals[slot], x);
+      if (!kAsync) RecordTerm(cnt, slot, x, thr);
+    }
   }
   return overflow;
 }
@@ -1408,6 +1420,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_hub_gather(CsrView g, StateV
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) c[k] = c[k] != v ? st.ids[c[k]] : kNoneId;
+    if (kAsync && hb.cbuf) {
+      uint32_t* cb = hb.cbuf + hb.ent_off[i];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t q = q0 + threadIdx.x + (int64_t)k * kGatherThreads;
+        if (q < e1) cb[q] = c[k];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < U; ++k) b[k] = c[k] != kNoneId ? PartOf(c[k], (uint32_t)parts) : 0u;
     if (local) {
@@ -1563,7 +1583,9 @@ __global__ __launch_bounds__(kHubThreads) void k_bm_hub_part(CsrView g, StateVie
           ? AccumulateBucket<kHubThreads, kAsync, P>(mp, pre, bk, nb, hub, thr, keys, vals, cnt,
                                                      mask, sl)
           : AccumulateGathered<kHubThreads, kAsync>(g, st, mp, v, start, deg, hub, thr, keys, vals,
-                                                    cnt, mask, parts, part, sl);
+                                                    cnt, mask, parts, part, sl,
+                                                    kAsync && hb.cbuf ? hb.cbuf + hb.ent_off[i]
+                                                                      : nullptr);
   const bool ovf = __syncthreads_or(overflow);
   // Every thread has read bcur[t] and pfine[t] before the barrier.
   if (threadIdx.x == 0) {

@@ -112,23 +112,27 @@ struct GvMap {
   }
 };
 
-// Sort keys of the vertices (new id, or n2 for vertices of graphs that leave
-// the batch, which sort last) and member counts per new id.
+// Sort keys of the vertices: new id, or n2 for vertices of graphs that leave
+// the batch, which sort last.
 __global__ __launch_bounds__(256) void k_gv_keys(int64_t n, const uint32_t* __restrict__ ids,
                                                  GvMap gm, int64_t n2, uint64_t* __restrict__ vkey,
-                                                 uint32_t* __restrict__ vval,
-                                                 uint32_t* __restrict__ cnt) {
+                                                 uint32_t* __restrict__ vval) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
     const int64_t shift = gm.Shift(v);
-    if (shift == INT64_MIN) {
-      vkey[v] = (uint64_t)n2;
-    } else {
-      const uint64_t nid = (uint64_t)((int64_t)ids[v] + shift);
-      vkey[v] = nid;
-      atomicAdd(&cnt[nid], 1u);
-    }
+    vkey[v] = shift == INT64_MIN ? (uint64_t)n2 : (uint64_t)((int64_t)ids[v] + shift);
     vval[v] = (uint32_t)v;
+  }
+}
+
+// Member counts per new id from the runs of the sorted keys (cnt zeroed).
+__global__ __launch_bounds__(256) void k_gv_counts(const uint64_t* __restrict__ vkey,
+                                                   const int64_t* __restrict__ start, int64_t nseg,
+                                                   int64_t n2, uint32_t* __restrict__ cnt) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = vkey[start[s]];
+    if ((int64_t)k < n2) cnt[k] = (uint32_t)(start[s + 1] - start[s]);
   }
 }
 
@@ -863,13 +867,20 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   // 1. Members of every new vertex, in ascending vertex order.
   DevBuf<uint64_t> vkey(al, n);
   DevBuf<uint32_t> perm(al, n), vstart(al, n2 + 1);
+  DevBuf<int64_t> nstart;  // runs of equal new id in the vertex order
+  int64_t ng = 0;
   {
     DevBuf<uint32_t> cnt(al, n2);
     PCC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(uint32_t) * (size_t)n2, s));
     hipLaunchKernelGGL(k_gv_keys, dim3(grid_n), dim3(256), 0, s, n, ids, gm, n2, vkey.get(),
-                       perm.get(), cnt.get());
+                       perm.get());
     PCC_HIP_CHECK(hipGetLastError());
     RadixSortPairsU32(al, vkey.get(), perm.get(), n, 0, GvBitsFor((uint64_t)n2), s);
+    ng = Segments(al, vkey.get(), n, &nstart, s);
+    if (ng > 0)
+      hipLaunchKernelGGL(k_gv_counts, dim3(GvGrid(ng, 256, 8192)), dim3(256), 0, s, vkey.get(),
+                         nstart.get(), ng, n2, cnt.get());
+    PCC_HIP_CHECK(hipGetLastError());
     ExclusiveScanU32toU32(al, cnt.get(), vstart.get(), n2, vstart.get() + n2, s);
   }
   DevBuf<int64_t> rowpos(al, n + 1), moff(al, n);
@@ -1008,8 +1019,6 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
 
   // 5. Node weights: members' weights in ascending vertex order, summed per
   // new vertex in pbbslib's order.
-  DevBuf<int64_t> nstart;
-  const int64_t ng = Segments(al, vkey.get(), n, &nstart, s);
   DevBuf<double> gsum(al, ng > 0 ? ng : 1);
   SegmentSumsF64(al, nws.get(), nstart.get(), ng, n, gsum.get(), s);
   out_nw->Reset(al, n2);

@@ -247,10 +247,11 @@ __device__ __forceinline__ int64_t WaveUniform(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Moved vertices of bins 5 and 6: one workgroup per vertex.  The vertex
-// index is uniform over the workgroup, so the bin search runs on the scalar
-// unit; with device counts (kDev) the loaded counts are made wave-uniform
-// explicitly so that it stays there.
+// Moved vertices of bins 5 and 6: one wave per vertex, four per workgroup,
+// 64 neighbours per step (two steps' loads in flight).  The vertex index is
+// uniform over the wave, so the bin search runs on the scalar unit; with
+// device counts (kDev) the loaded counts are made wave-uniform explicitly so
+// that it stays there.
 template <bool kDev>
 __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       const uint32_t* __restrict__ list,
@@ -261,7 +262,10 @@ __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
                                                       const unsigned long long* __restrict__ dcnt) {
   if (dir.Pull()) return;
   if (kDev) total = WaveUniform(BinTotal(bins, dcnt));
-  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+  const int lane = LaneId();
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t t = WaveUniform(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave); t < total;
+       t += nwaves) {
     int64_t r = t;
     int b = 0;
     if (kDev) {
@@ -272,7 +276,13 @@ __global__ __launch_bounds__(256) void k_expand_block(Bins bins, int64_t total,
     const uint32_t v = list[bins.base[b] + r];
     if (!moved[v]) continue;
     const int64_t st = off[v], en = off[v + 1];
-    for (int64_t p = st + threadIdx.x; p < en; p += blockDim.x) next[nbr[p]] = 1;
+    int64_t p = st + lane;
+    for (; p + kWave < en; p += 2 * kWave) {
+      const uint32_t a = nbr[p], c = nbr[p + kWave];
+      next[a] = 1;
+      next[c] = 1;
+    }
+    if (p < en) next[nbr[p]] = 1;
   }
 }
 
@@ -871,10 +881,10 @@ void LaunchExpandMoved(const int64_t counts[kBins], const int64_t te[kBins],
   }
   if (nmid > 0) {
     if (dcnt)
-      hipLaunchKernelGGL(k_expand_block<true>, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid,
+      hipLaunchKernelGGL(k_expand_block<true>, dim3(Grid(nmid, 4, 4096)), dim3(256), 0, s, mid,
                          nmid, list, g.off, g.nbr, moved, dir, next, dcnt);
     else
-      hipLaunchKernelGGL(k_expand_block<false>, dim3(Grid(nmid, 1, 4096)), dim3(256), 0, s, mid,
+      hipLaunchKernelGGL(k_expand_block<false>, dim3(Grid(nmid, 4, 4096)), dim3(256), 0, s, mid,
                          nmid, list, g.off, g.nbr, moved, dir, next, dcnt);
     PCC_HIP_CHECK(hipGetLastError());
   }

@@ -279,6 +279,12 @@ struct HubScratch {
   // kNoFineThr).  Without one its totals are exact in every order.  The
   // partition pass resets it with bcur[t].
   uint32_t* pfine = nullptr;
+  // Async rounds: the cluster of every neighbour of list item i in adjacency
+  // order at cbuf[ent_off[i] ...] (kNoneId for a self-loop), written by the
+  // gather pass, so that a partition whose bucket overflowed (a heavy
+  // cluster, frequent on unit weights) streams it instead of gathering the
+  // whole row again.  Null in sync rounds.
+  uint32_t* cbuf = nullptr;
 };
 
 // Bucket capacity per partition item: a partition's share of a vertex's
