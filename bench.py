@@ -673,12 +673,15 @@ def bench_modularity_async(args):
     t0 = time.time()
     g = pcc_amd.DeviceGraph.from_csr(csr)
     print(f"# graph n={csr.n} m={csr.m} upload={time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    g.set_option(pcc_amd.OPT_TIMING, 1)
     apply_tuning(g, args, pcc_amd)
     for i in range(args.warmup):
         _, st = g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
         print(f"# warmup {i}: {st.cluster_seconds*1e3:.1f} ms modularity={st.modularity:.5f}",
               file=sys.stderr, flush=True)
+    # The timed steps run as a caller would (no per-launch events, so bins
+    # 0-4 and small hashed sub-rounds take one launch each); one more step
+    # with per-bin launches and HIP events gives the best-move time and the
+    # per-bin breakdown.
     t0 = time.perf_counter()
     stats = []
     for i in range(args.steps):
@@ -687,11 +690,17 @@ def bench_modularity_async(args):
         print(f"# step {i}: {st.cluster_seconds*1e3:.1f} ms rounds={st.rounds} "
               f"modularity={st.modularity:.5f} clusters={st.num_clusters}", file=sys.stderr, flush=True)
     wall = (time.perf_counter() - t0) / max(args.steps, 1)
+    g.set_option(pcc_amd.OPT_TIMING, 1)
+    _, ist = g.cluster(cfg, pcc_amd.OBJECTIVE_MODULARITY)
+    print(f"# instrumented step: {ist.cluster_seconds*1e3:.1f} ms rounds={ist.rounds} "
+          f"best-move {ist.bestmove_seconds*1e3:.1f} ms modularity={ist.modularity:.5f}",
+          file=sys.stderr, flush=True)
     te = sum(s.edges_scanned for s in stats)
-    bm = sum(s.bestmove_seconds for s in stats)
     line = {
         "metric": "best-move GTEPS (ParallelModularityClusterer async NBHR_CLUSTER_MOVE refine permute)",
-        "value": round(te / bm / 1e9, 4) if bm else 0.0, "unit": "GTEPS", "n_gpus": 1,
+        "value": round(ist.edges_scanned / ist.bestmove_seconds / 1e9, 4) if ist.bestmove_seconds else 0.0,
+        "value_from": "one instrumented step (per-bin launches timed with HIP events) after the timed steps",
+        "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (move deltas)", "data": "synthetic (seeded RMAT, unit weights)",
@@ -701,11 +710,13 @@ def bench_modularity_async(args):
         "end_to_end_ms": round(sum(s.cluster_seconds for s in stats) / len(stats) * 1e3, 3),
         "rounds_per_step": stats[0].rounds, "levels": stats[0].levels,
         "edges_scanned_per_step": te // max(len(stats), 1),
-        "contract_ms_per_step": round(sum(s.contract_seconds for s in stats) / len(stats) * 1e3, 3),
-        "apply_frontier_ms_per_step": round(sum(s.apply_seconds for s in stats) / len(stats) * 1e3, 3),
-        "bins": [{"bin": BIN_NAMES[b], "seconds": round(sum(s.bin_seconds[b] for s in stats), 6),
-                  "edges": sum(s.bin_edges[b] for s in stats),
-                  "launches": sum(s.bin_launches[b] for s in stats)} for b in range(8)],
+        "instrumented_step": {"cluster_ms": round(ist.cluster_seconds * 1e3, 3), "rounds": ist.rounds,
+                              "bestmove_ms": round(ist.bestmove_seconds * 1e3, 3),
+                              "contract_ms": round(ist.contract_seconds * 1e3, 3),
+                              "apply_frontier_ms": round(ist.apply_seconds * 1e3, 3),
+                              "modularity": ist.modularity},
+        "bins": [{"bin": BIN_NAMES[b], "seconds": round(ist.bin_seconds[b], 6),
+                  "edges": ist.bin_edges[b], "launches": ist.bin_launches[b]} for b in range(8)],
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -419,7 +419,9 @@ Engine::~Engine() {
   }
 }
 
-Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) : na(al, n) {
+Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s, uint8_t* also_zero,
+                             int64_t also_bytes)
+    : na(al, n) {
   const int64_t nn = n > 0 ? n : 1;
   list.Reset(al, nn);
   list2.Reset(al, nn);
@@ -436,8 +438,18 @@ Engine::Workspace::Workspace(DeviceAllocator* al, int64_t n, hipStream_t s) : na
   dc.Reset(al, 1);
   dstats.Reset(al, 1);
   // new_flag is kept all-zero between rounds by the new-cluster assignment.
-  PCC_HIP_CHECK(hipMemsetAsync(new_flag.get(), 0, (size_t)nn, s));
-  PCC_HIP_CHECK(hipMemsetAsync(rc.get(), 0, sizeof(RoundCounters), s));
+  ZeroSpans z{};
+  z.p[0] = new_flag.get();
+  z.bytes[0] = nn;
+  z.p[1] = reinterpret_cast<uint8_t*>(rc.get());
+  z.bytes[1] = sizeof(RoundCounters);
+  z.n = 2;
+  if (also_zero && also_bytes > 0) {
+    z.p[2] = also_zero;
+    z.bytes[2] = also_bytes;
+    z.n = 3;
+  }
+  LaunchZeroSpans(z, s);
 }
 
 void Engine::ResetCounters(RoundCounters* rc, bool all) {
@@ -623,8 +635,13 @@ void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool 
                          const uint32_t* const lists[kBins], const int64_t counts[kBins],
                          const int64_t te[kBins], const MoveOut& out, uint64_t salt,
                          bool scan_hubs) {
-  // One timer per reported group (the three ranges of a hash group together).
-  for (int b0 = 0; b0 < kHubBin;) {
+  if (!h_->timing) {
+    LaunchBestMoveBins(counts, lists, async, h_->th, c.g, c.st, mp, out, ws.fb_list.get(),
+                       ws.rc.get(), salt, s_);
+  }
+  // With timing, one launch per bin and one timer per reported group (the
+  // three ranges of a hash group together).
+  for (int b0 = 0; b0 < kHubBin && h_->timing;) {
     const int grp = StatGroupOf(b0);
     int b1 = b0;
     bool any = false;
@@ -886,7 +903,13 @@ void Engine::RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph,
   out.moved = ws.moved.get();
   out.moved_clusters = ws.flags.get();
   Timer all = StartTimer();
-  for (int b0 = 0; b0 < kHubBin;) {
+  if (!h_->timing) {
+    const uint32_t* lists[kHubBin];
+    for (int b = 0; b < kHubBin; ++b) lists[b] = ws.list.get() + c.layout.base[b];
+    LaunchBestMoveBins(c.caps, lists, false, h_->th, g, st, c.mp, out, ws.fb_list.get(),
+                       ws.rc.get(), 0, s_, dc);
+  }
+  for (int b0 = 0; b0 < kHubBin && h_->timing;) {
     const int grp = StatGroupOf(b0);
     int b1 = b0;
     bool any = false;
@@ -969,9 +992,8 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
     if (cfg_.refine) ctx_cache_.emplace_back(&L, std::move(fresh));
   }
   LevelCtx& c = *cp;
-  Workspace ws(al_, n, s_);
   DevBuf<uint8_t> gmoved(al_, G);
-  PCC_HIP_CHECK(hipMemsetAsync(gmoved.get(), 0, (size_t)G, s_));
+  Workspace ws(al_, n, s_, gmoved.get(), G);
   // Round 0 frontier: every vertex of the selected graphs (PCC.cc:346-352).
   bool all = true;
   if (sel)
@@ -1041,7 +1063,17 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
     }
     fallback_ += (int64_t)(hrc.fb_total + hrc.fb_count + hrc.repair_count);
   } else {
-    BinFrontier(c, ws, all ? nullptr : fr0.get(), counts, te);
+    if (all) {
+      // Every vertex: the bins hold the level's capacities, known already
+      // (the round counters are zero from the workspace's set-up).
+      LaunchBinFrontier(c.g, nullptr, h_->th, c.layout, ws.list.get(), ws.rc.get(), s_);
+      for (int b = 0; b < kBins; ++b) {
+        counts[b] = c.caps[b];
+        te[b] = c.caps_te[b];
+      }
+    } else {
+      BinFrontier(c, ws, fr0.get(), counts, te);
+    }
     bool local_moved = true;
     for (int64_t it = 0; it < inner && local_moved; ++it) {
       const int64_t fsize = RoundStep(c, ws, L.lay.vgraph.get(), G, gmoved.get(), counts, te);

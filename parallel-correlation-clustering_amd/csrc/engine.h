@@ -132,7 +132,10 @@ class Engine {
     DevBuf<DeviceCounts> dc;
     DevBuf<DeviceRoundStats> dstats;
     NewAssignWork na;
-    Workspace(DeviceAllocator* al, int64_t n, hipStream_t s);
+    // Clears new_flag, rc and (optionally) also_bytes bytes at also_zero in
+    // one launch.
+    Workspace(DeviceAllocator* al, int64_t n, hipStream_t s, uint8_t* also_zero = nullptr,
+              int64_t also_bytes = 0);
   };
 
   // One level's best-move machinery, shared by Run and the one-phase C ABI.

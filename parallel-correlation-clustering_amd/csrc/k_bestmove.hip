@@ -410,6 +410,21 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
   }
 }
 
+// The long-row accumulators of k_gran_long: sums and maxima zero, granularity
+// minima above every exponent (> 2^20), no row queued.
+__global__ __launch_bounds__(256) void k_gran_clear(int64_t maxlong, unsigned int* __restrict__ nlong,
+                                                    double* __restrict__ acc_s,
+                                                    unsigned long long* __restrict__ acc_m,
+                                                    int* __restrict__ acc_g) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < maxlong;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    acc_s[i] = 0.0;
+    acc_m[i] = 0ull;
+    acc_g[i] = 0x7F7F7F7F;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nlong = 0u;
+}
+
 // Rows longer than kLongRow: chunk sums and maxima combined with atomics (the
 // bound B only has to hold, not to be reproducible; ThresholdFor leaves a
 // margin for the rounding of any summation order), then packed.
@@ -640,19 +655,21 @@ __device__ __forceinline__ int64_t PermuteIndex(int64_t i, int64_t count, uint64
 
 // ------------------------------------------------------------ small bins --
 
+// G lanes per listed vertex; workgroup `block` of `nblocks` working on the
+// list (a launch of its own, or one bin's range of the fused launch).
 template <int G, bool kAsync>
-__global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveParams mp,
-                                                  const uint32_t* __restrict__ list,
-                                                  int64_t count, MoveOut out, uint64_t salt,
-                                                  const unsigned long long* __restrict__ dcount) {
-  if (dcount) count = (int64_t)*dcount;
+__device__ __forceinline__ void SmallBinBody(const CsrView& g, const StateView& st,
+                                             const MoveParams& mp,
+                                             const uint32_t* __restrict__ list, int64_t count,
+                                             const MoveOut& out, uint64_t salt, int64_t block,
+                                             int64_t nblocks) {
   constexpr int kGroups = 256 / G;
   const int lane = threadIdx.x & (G - 1);
   const int wlane = LaneId();
   const int gbase_lane = wlane & ~(G - 1);
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase_lane);
-  for (int64_t gb = XcdItem(blockIdx.x, gridDim.x) * kGroups; gb < count;
-       gb += (int64_t)gridDim.x * kGroups) {
+  for (int64_t gb = XcdItem(block, nblocks) * kGroups; gb < count;
+       gb += nblocks * kGroups) {
     const int64_t gi = gb + threadIdx.x / G;
     const uint32_t v = gi < count ? list[PermuteIndex(gi, count, salt)] : 0u;
     const bool active = gi < count && (!kAsync || InSubRound(mp, v));
@@ -712,6 +729,44 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
       else
         SyncEpilogue(v, t, d, mp.new_id, st, out);
     }
+  }
+}
+
+template <int G, bool kAsync>
+__global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveParams mp,
+                                                  const uint32_t* __restrict__ list,
+                                                  int64_t count, MoveOut out, uint64_t salt,
+                                                  const unsigned long long* __restrict__ dcount) {
+  if (dcount) count = (int64_t)*dcount;
+  SmallBinBody<G, kAsync>(g, st, mp, list, count, out, salt, blockIdx.x, gridDim.x);
+}
+
+// Bins 0-4 in one launch: workgroups [end[b-1], end[b]) take bin b (ranges
+// of whole multiples of 8 workgroups, so a workgroup's index within its bin
+// keeps the XCD it has in the grid).  In sync rounds the bins read the same
+// state and write disjoint vertices, so running them together changes no
+// result; it saves four dependent launches per round on small graphs.
+struct SmallBins {
+  const uint32_t* list[kWaveBin0];
+  int64_t count[kWaveBin0];
+  const unsigned long long* dcount[kWaveBin0];
+  uint32_t end[kWaveBin0];
+};
+
+template <bool kAsync>
+__global__ __launch_bounds__(256) void k_bm_small_all(CsrView g, StateView st, MoveParams mp,
+                                                      SmallBins sb, MoveOut out, uint64_t salt) {
+  int b = 0;
+  while (b < kWaveBin0 - 1 && blockIdx.x >= sb.end[b]) ++b;
+  const uint32_t b0 = b ? sb.end[b - 1] : 0u;
+  const int64_t blk = (int64_t)blockIdx.x - b0, nblk = (int64_t)sb.end[b] - b0;
+  const int64_t count = sb.dcount[b] ? (int64_t)*sb.dcount[b] : sb.count[b];
+  switch (b) {
+    case 0: SmallBinBody<4, kAsync>(g, st, mp, sb.list[0], count, out, salt, blk, nblk); break;
+    case 1: SmallBinBody<8, kAsync>(g, st, mp, sb.list[1], count, out, salt, blk, nblk); break;
+    case 2: SmallBinBody<16, kAsync>(g, st, mp, sb.list[2], count, out, salt, blk, nblk); break;
+    case 3: SmallBinBody<32, kAsync>(g, st, mp, sb.list[3], count, out, salt, blk, nblk); break;
+    default: SmallBinBody<64, kAsync>(g, st, mp, sb.list[4], count, out, salt, blk, nblk); break;
   }
 }
 
@@ -1051,9 +1106,25 @@ struct HashVertex {
 // async mode it is one more interleaving of the concurrent moves.
 // At least 4 waves per SIMD (128 VGPRs), so the LDS budget, not registers,
 // sets how many workgroups share a CU.
+// The items of a hash launch: one bin's list, or (sync rounds with few
+// hashed vertices) the lists of several bins one after another, so that one
+// launch of the largest table covers them; a vertex's result does not depend
+// on which table size handles it.
+struct ItemLists {
+  const uint32_t* list[kHubBin - kWaveBin0];
+  int64_t end[kHubBin - kWaveBin0];  // cumulative item counts
+  int n;                             // lists in use
+  __device__ __forceinline__ uint32_t At(int64_t i) const {
+    if (n == 1) return list[0][i];
+    int b = 0;
+    while (b < n - 1 && i >= end[b]) ++b;
+    return list[b][i - (b ? end[b - 1] : 0)];
+  }
+};
+
 template <int THREADS, int TABLE, bool kAsync>
 __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
-                                                     const uint32_t* __restrict__ list,
+                                                     ItemLists il,
                                                      int64_t count, int64_t dlo, int64_t dhi,
                                                      MoveOut out,
                                                      uint32_t* __restrict__ fb_list,
@@ -1088,7 +1159,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
   // Vertices routed to the ordered kernel are queued on the way.
   auto next_item = [&](int64_t& i, HashVertex& h) -> bool {
     for (; i < count; i += gridDim.x) {
-      const uint32_t v = list[PermuteIndex(i, count, salt)];
+      const uint32_t v = il.At(PermuteIndex(i, count, salt));
       if (kAsync && !InSubRound(mp, v)) continue;
       const int64_t start = g.off[v];
       const int64_t deg = g.off[v + 1] - start;
@@ -1906,10 +1977,8 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
   DevBuf<double> acc_s(al, maxlong);
   DevBuf<unsigned long long> acc_m(al, maxlong);
   DevBuf<int> acc_g(al, maxlong);
-  PCC_HIP_CHECK(hipMemsetAsync(acc_g.get(), 0x7F, sizeof(int) * (size_t)maxlong, s));  // > 2^20
-  PCC_HIP_CHECK(hipMemsetAsync(nlong.get(), 0, sizeof(unsigned int), s));
-  PCC_HIP_CHECK(hipMemsetAsync(acc_s.get(), 0, sizeof(double) * (size_t)maxlong, s));
-  PCC_HIP_CHECK(hipMemsetAsync(acc_m.get(), 0, sizeof(unsigned long long) * (size_t)maxlong, s));
+  hipLaunchKernelGGL(k_gran_clear, dim3(GridFor(maxlong, 256, 1024)), dim3(256), 0, s, maxlong,
+                     nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
   const int grid = GridFor(g.n, 256, 8192);
   hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset,
                      gscale, longq.get(), nlong.get());
@@ -1959,7 +2028,7 @@ static void LaunchSmall(int64_t count, const CsrView& g, const StateView& st, co
 template <int THREADS, int TABLE, bool A>
 static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_waves,
                        const CsrView& g, const StateView& st, const MoveParams& mp,
-                       const uint32_t* list, const MoveOut& out, uint32_t* fb_list,
+                       const ItemLists& il, const MoveOut& out, uint32_t* fb_list,
                        RoundCounters* rc, uint64_t salt, hipStream_t s,
                        const unsigned long long* dcount) {
   if (dhi <= dlo || count <= 0) return;
@@ -1974,7 +2043,25 @@ static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_wav
   }
   const int grid = GridFor(count, 1, max_grid);
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
-                     list, count, dlo, dhi, out, fb_list, rc, salt, dcount);
+                     il, count, dlo, dhi, out, fb_list, rc, salt, dcount);
+}
+
+// The kernel of hash list k (0 = bin 5) over the items il, degrees (lo, hi].
+template <bool A>
+static void LaunchHashList(int k, int64_t count, int64_t lo, int64_t hi, int64_t hw,
+                           const CsrView& g, const StateView& st, const MoveParams& mp,
+                           const ItemLists& il, const MoveOut& out, uint32_t* fb_list,
+                           RoundCounters* rc, uint64_t salt, hipStream_t s,
+                           const unsigned long long* dcount) {
+  switch (k) {
+    case 0: LaunchHash<64, 256, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount); break;
+    case 1: LaunchHash<64, 512, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount); break;
+    case 2: LaunchHash<128, 1024, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount); break;
+    case 3: LaunchHash<256, 2048, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount); break;
+    case 4: LaunchHash<512, 4096, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount); break;
+    default:
+      LaunchHash<1024, 8192, A>(count, lo, hi, hw, g, st, mp, il, out, fb_list, rc, salt, s, dcount);
+  }
 }
 
 template <bool A>
@@ -1985,17 +2072,17 @@ static void LaunchHashBin(int bin, int64_t count, const BinThresholds& th, const
   const HashBounds h = HashBoundsOf(th.wave_max, th.block_max);
   const int k = bin - kWaveBin0;
   const int64_t lo = k == 0 ? 64 : h.b[k - 1], hi = h.b[k];
-  const int64_t hw = th.hash_waves;
-  switch (bin) {
-    case 5: LaunchHash<64, 256, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
-    case 6: LaunchHash<64, 512, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
-    case 7: LaunchHash<128, 1024, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
-    case 8: LaunchHash<256, 2048, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
-    case 9: LaunchHash<512, 4096, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount); break;
-    default:
-      LaunchHash<1024, 8192, A>(count, lo, hi, hw, g, st, mp, list, out, fb_list, rc, salt, s, dcount);
-  }
+  ItemLists il{};
+  il.list[0] = list;
+  il.end[0] = count;
+  il.n = 1;
+  LaunchHashList<A>(k, count, lo, hi, th.hash_waves, g, st, mp, il, out, fb_list, rc, salt, s,
+                    dcount);
 }
+
+// Sync rounds with at most this many hashed vertices take one launch of the
+// largest table they need instead of one launch per table size.
+constexpr int64_t kMergeHashItems = 1024;
 
 void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& th,
                        const CsrView& g, const StateView& st, const MoveParams& mp,
@@ -2038,6 +2125,67 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
       throw HipError{hipErrorInvalidValue, "LaunchBestMoveBin: the hub bin goes through LaunchBestMoveHub",
                      __FILE__, __LINE__};
   }
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBestMoveBins(const int64_t counts[kHubBin], const uint32_t* const lists[kHubBin],
+                        bool async, const BinThresholds& th, const CsrView& g,
+                        const StateView& st, const MoveParams& mp, const MoveOut& out,
+                        uint32_t* fb_list, RoundCounters* rc, uint64_t salt, hipStream_t s,
+                        const DeviceCounts* dc) {
+  if (async) {
+    // Async rounds keep one launch per bin, each seeing the moves of the
+    // ones before: running the bins of a sub-round together lowered the
+    // objective of the async modularity runs by about half a percent.
+    for (int b = 0; b < kHubBin; ++b)
+      if (counts[b] > 0)
+        LaunchBestMoveBin(b, counts[b], true, th, g, st, mp, lists[b], out, fb_list, rc, salt, s,
+                          dc);
+    return;
+  }
+  SmallBins sb{};
+  uint32_t blocks = 0;
+  for (int b = 0; b < kWaveBin0; ++b) {
+    sb.list[b] = lists[b];
+    sb.count[b] = counts[b];
+    sb.dcount[b] = dc ? &dc->cnt[b] : nullptr;
+    if (counts[b] > 0) {
+      const int per = 256 >> (2 + b);  // G = 4 << b lanes per vertex
+      const int64_t grid = GridFor(counts[b], per, dc ? kDeviceSmallGrid : 1 << 20);
+      blocks += (uint32_t)((grid + 7) / 8 * 8);
+    }
+    sb.end[b] = blocks;
+  }
+  if (blocks > 0) {
+    hipLaunchKernelGGL(k_bm_small_all<false>, dim3(blocks), dim3(256), 0, s, g, st, mp, sb, out,
+                       salt);
+    PCC_HIP_CHECK(hipGetLastError());
+  }
+  int64_t hashed = 0;
+  int top = -1;
+  for (int b = kWaveBin0; b < kHubBin; ++b)
+    if (counts[b] > 0) {
+      hashed += counts[b];
+      top = b;
+    }
+  if (top < 0) return;
+  if (dc || hashed > kMergeHashItems) {
+    for (int b = kWaveBin0; b < kHubBin; ++b)
+      if (counts[b] > 0)
+        LaunchBestMoveBin(b, counts[b], false, th, g, st, mp, lists[b], out, fb_list, rc, salt, s,
+                          dc);
+    return;
+  }
+  const HashBounds h = HashBoundsOf(th.wave_max, th.block_max);
+  ItemLists il{};
+  for (int b = kWaveBin0; b < kHubBin; ++b)
+    if (counts[b] > 0) {
+      il.list[il.n] = lists[b];
+      il.end[il.n] = (il.n ? il.end[il.n - 1] : 0) + counts[b];
+      ++il.n;
+    }
+  LaunchHashList<false>(top - kWaveBin0, hashed, 64, h.b[top - kWaveBin0], th.hash_waves, g, st,
+                        mp, il, out, fb_list, rc, salt, s, nullptr);
   PCC_HIP_CHECK(hipGetLastError());
 }
 

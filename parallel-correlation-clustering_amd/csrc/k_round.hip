@@ -763,6 +763,32 @@ __global__ __launch_bounds__(256) void k_fill_u32_gated(uint32_t* __restrict__ p
     p[i] = x;
 }
 
+// Word stores over each range's aligned middle, byte stores at its ends.
+__global__ __launch_bounds__(256) void k_zero_spans(ZeroSpans z) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < z.n; ++k) {
+    uint8_t* p = z.p[k];
+    const int64_t nb = z.bytes[k];
+    const int64_t head = std::min<int64_t>(nb, (int64_t)((4 - ((uintptr_t)p & 3)) & 3));
+    const int64_t words = (nb - head) / 4;
+    uint32_t* w = reinterpret_cast<uint32_t*>(p + head);
+    for (int64_t i = t0; i < words; i += step) w[i] = 0u;
+    const int64_t tail0 = head + 4 * words;
+    for (int64_t i = t0; i < head; i += step) p[i] = 0;
+    for (int64_t i = tail0 + t0; i < nb; i += step) p[i] = 0;
+  }
+}
+
+void LaunchZeroSpans(const ZeroSpans& z, hipStream_t s) {
+  int64_t most = 0;
+  for (int k = 0; k < z.n; ++k) most = std::max(most, z.bytes[k]);
+  if (most <= 0) return;
+  const int64_t blocks = std::min<int64_t>(4096, (most / 4 + 255) / 256 + 1);
+  hipLaunchKernelGGL(k_zero_spans, dim3((unsigned)blocks), dim3(256), 0, s, z);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 NewAssignWork::NewAssignWork(DeviceAllocator* al, int64_t n) {
   const int64_t nn = n > 0 ? n : 1;
   zrank.Reset(al, nn);

@@ -354,6 +354,16 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
                        RoundCounters* rc, uint64_t permute_salt, hipStream_t s,
                        const DeviceCounts* dc = nullptr);
 
+// Bins 0..10 of a round (count / list per bin, as LaunchBestMoveBin).  Sync
+// rounds run bins 0-4 as one launch and, with at most 1024 hashed vertices
+// and host-known counts, bins 5-10 as one launch of the largest table
+// needed; async rounds launch bin by bin.  Results equal the per-bin launches.
+void LaunchBestMoveBins(const int64_t counts[kHubBin], const uint32_t* const lists[kHubBin],
+                        bool async, const BinThresholds& th, const CsrView& g,
+                        const StateView& st, const MoveParams& mp, const MoveOut& out,
+                        uint32_t* fb_list, RoundCounters* rc, uint64_t permute_salt,
+                        hipStream_t s, const DeviceCounts* dc = nullptr);
+
 // Bin 7: partition pass and selection pass over a list whose HubScratch
 // prefix sums are current; te = sum of the listed degrees.
 void LaunchBestMoveHub(int64_t count, int64_t te, bool async, const CsrView& g,
@@ -390,6 +400,15 @@ void OrderedClusterWeights(DeviceAllocator* al, int64_t n, const StateView& st, 
 // (vgraph / gbase null for one graph); gated on rc->new_count.  work (may be
 // null: allocated per call) holds its n-sized temporaries, so a caller that
 // keeps one per level runs the assignment with no allocation.
+// Up to four byte ranges cleared by one launch (a kernel, so the clear can
+// sit in a captured graph).
+struct ZeroSpans {
+  uint8_t* p[4];
+  int64_t bytes[4];
+  int n;
+};
+void LaunchZeroSpans(const ZeroSpans& z, hipStream_t s);
+
 struct NewAssignWork {
   DevBuf<uint32_t> zrank, zero_list, nrank, scan;
   DevBuf<uint8_t> z;
