@@ -273,8 +273,7 @@ int UploadCsr(pcc_graph* h, int64_t n, int64_t m, const int64_t* off, const uint
   hipLaunchKernelGGL(k_validate, dim3(2048), dim3(256), 0, s, n, m, h->csr.off.get(),
                      h->csr.nbr.get(), bad.get());
   unsigned long long nbad = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&nbad, bad.get(), sizeof(nbad), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&nbad, bad.get(), sizeof(nbad), s);
   if (nbad) {
     h->csr = DevCsr();
     return SetError(PCC_INVALID_ARGUMENT,
@@ -370,8 +369,7 @@ int64_t CountDistinctIds(DeviceAllocator* al, int64_t n, const uint32_t* ids, hi
                      total.get());
   PCC_HIP_CHECK(hipGetLastError());
   unsigned long long h = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&h, total.get(), sizeof(h), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&h, total.get(), sizeof(h), s);
   return (int64_t)h;
 }
 
@@ -394,8 +392,7 @@ bool NodeWeightsExact(DeviceAllocator* al, const double* nw, int64_t n, hipStrea
                      &a->mag);
   PCC_HIP_CHECK(hipGetLastError());
   Acc h;
-  PCC_HIP_CHECK(hipMemcpyAsync(&h, a, sizeof(h), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&h, a, sizeof(h), s);
   // Relative error of the double sum is far below 2^-20 for any n < 2^32.
   return h.flag == 0 && h.mag * (1.0 + 0x1p-20) < 0x1p53;
 }
@@ -463,8 +460,7 @@ void Engine::ResetCounters(RoundCounters* rc, bool all) {
 void Engine::ReadCounters(RoundCounters* rc, int64_t counts[kBins], int64_t te[kBins],
                           RoundCounters* full) {
   RoundCounters hrc;
-  PCC_HIP_CHECK(hipMemcpyAsync(&hrc, rc, sizeof(hrc), hipMemcpyDeviceToHost, s_));
-  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  ReadBack(&hrc, rc, sizeof(hrc), s_);
   for (int b = 0; b < kBins; ++b) {
     counts[b] = (int64_t)hrc.cnt[b];
     te[b] = (int64_t)hrc.te[b];
@@ -513,8 +509,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   PCC_HIP_CHECK(hipMemsetAsync(caps.get(), 0, sizeof(unsigned long long) * kLevelStats, s_));
   LaunchBinCapacity(g, h_->th, st.nw, lay.batch ? lay.gres.get() : nullptr, lay.G, caps.get(), s_);
   unsigned long long hcaps[kLevelStats];
-  PCC_HIP_CHECK(hipMemcpyAsync(hcaps, caps.get(), sizeof(hcaps), hipMemcpyDeviceToHost, s_));
-  PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  ReadBack(hcaps, caps.get(), sizeof(hcaps), s_);
   int64_t run = 0;
   for (int b = 0; b < kBins; ++b) {
     c->layout.base[b] = run;
@@ -741,9 +736,7 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
     hte7.resize((size_t)subs);
     PCC_HIP_CHECK(hipMemcpyAsync(hcnt.data(), ws.sub_cnt.get(), sizeof(uint32_t) * K,
                                  hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipMemcpyAsync(hte7.data(), ws.sub_te.get(),
-                                 sizeof(unsigned long long) * (size_t)subs, hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    ReadBack(hte7.data(), ws.sub_te.get(), sizeof(unsigned long long) * (size_t)subs, s_);
     for (int b = 0; b < kBins; ++b) {
       uint32_t run = 0;
       for (int j = 0; j < subs; ++j) {
@@ -1049,8 +1042,7 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
     DeviceRoundStats hs;
     RoundCounters hrc;
     PCC_HIP_CHECK(hipMemcpyAsync(&hs, ws.dstats.get(), sizeof(hs), hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipMemcpyAsync(&hrc, ws.rc.get(), sizeof(hrc), hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    ReadBack(&hrc, ws.rc.get(), sizeof(hrc), s_);
     CollectTimers();
     rounds = (int64_t)hs.rounds;
     rounds_ += rounds;
@@ -1095,8 +1087,7 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
       moved[(size_t)p] = (rounds > 0 && selected && L.lay.base[(size_t)p + 1] > L.lay.base[(size_t)p]) ? 1 : 0;
     }
   } else {
-    PCC_HIP_CHECK(hipMemcpyAsync(moved.data(), gmoved.get(), (size_t)G, hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    ReadBack(moved.data(), gmoved.get(), (size_t)G, s_);
   }
   return moved;
 }
@@ -1247,8 +1238,7 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     PCC_HIP_CHECK(hipMemsetAsync(gmax.get(), 0, 4 * (size_t)Gp, s_));
     LaunchGraphMaxIds(L.g.n, L.ids.get(), L.lay.vgraph.get(), gmax.get(), s_);
     std::vector<uint32_t> hmax((size_t)Gp);
-    PCC_HIP_CHECK(hipMemcpyAsync(hmax.data(), gmax.get(), 4 * (size_t)Gp, hipMemcpyDeviceToHost, s_));
-    PCC_HIP_CHECK(hipStreamSynchronize(s_));
+    ReadBack(hmax.data(), gmax.get(), 4 * (size_t)Gp, s_);
     std::vector<int64_t> newbase((size_t)Gp, -1);
     auto next = std::make_unique<Level>();
     next->lay.batch = batch;

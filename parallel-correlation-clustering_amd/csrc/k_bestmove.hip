@@ -1115,22 +1115,23 @@ struct ItemLists {
   int64_t end[kHubBin - kWaveBin0];  // cumulative item counts
   int n;                             // lists in use
   __device__ __forceinline__ uint32_t At(int64_t i) const {
-    if (n == 1) return list[0][i];
     int b = 0;
     while (b < n - 1 && i >= end[b]) ++b;
     return list[b][i - (b ? end[b - 1] : 0)];
   }
 };
+// One bin's list.
+struct OneList {
+  const uint32_t* list;
+  __device__ __forceinline__ uint32_t At(int64_t i) const { return list[i]; }
+};
 
-template <int THREADS, int TABLE, bool kAsync>
-__global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
-                                                     ItemLists il,
-                                                     int64_t count, int64_t dlo, int64_t dhi,
-                                                     MoveOut out,
-                                                     uint32_t* __restrict__ fb_list,
-                                                     RoundCounters* __restrict__ rc,
-                                                     uint64_t salt,
-                                                     const unsigned long long* __restrict__ dcount) {
+template <int THREADS, int TABLE, bool kAsync, typename Items>
+__device__ __forceinline__ void HashBinBody(CsrView g, StateView st, MoveParams mp, Items il,
+                                            int64_t count, int64_t dlo, int64_t dhi, MoveOut out,
+                                            uint32_t* __restrict__ fb_list,
+                                            RoundCounters* __restrict__ rc, uint64_t salt,
+                                            const unsigned long long* __restrict__ dcount) {
   if (dcount) count = (int64_t)*dcount;
   constexpr int NW = THREADS / kWave;
   constexpr int E = TABLE / (2 * THREADS);
@@ -1375,6 +1376,29 @@ __global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st,
   }
   if (!kAsync && pend && threadIdx.x == 0)
     SyncEpilogue(pv, pt, pd, mp.new_id, st, out);
+}
+
+template <int THREADS, int TABLE, bool kAsync>
+__global__ __launch_bounds__(THREADS, 4) void k_bm_hash(CsrView g, StateView st, MoveParams mp,
+                                                     const uint32_t* __restrict__ list,
+                                                     int64_t count, int64_t dlo, int64_t dhi,
+                                                     MoveOut out,
+                                                     uint32_t* __restrict__ fb_list,
+                                                     RoundCounters* __restrict__ rc,
+                                                     uint64_t salt,
+                                                     const unsigned long long* __restrict__ dcount) {
+  HashBinBody<THREADS, TABLE, kAsync>(g, st, mp, OneList{list}, count, dlo, dhi, out, fb_list, rc,
+                                      salt, dcount);
+}
+
+// Sync rounds: the lists of several bins in one launch.
+template <int THREADS, int TABLE>
+__global__ __launch_bounds__(THREADS, 4) void k_bm_hash_lists(CsrView g, StateView st, MoveParams mp,
+                                                           ItemLists il, int64_t count,
+                                                           int64_t dlo, int64_t dhi, MoveOut out,
+                                                           uint32_t* __restrict__ fb_list,
+                                                           RoundCounters* __restrict__ rc) {
+  HashBinBody<THREADS, TABLE, false>(g, st, mp, il, count, dlo, dhi, out, fb_list, rc, 0, nullptr);
 }
 
 // Accumulate over v's whole adjacency (partition `part` of `parts`), the
@@ -2042,8 +2066,15 @@ static void LaunchHash(int64_t count, int64_t dlo, int64_t dhi, int64_t hash_wav
     max_grid = (int)std::min<int64_t>(8192, 256 * per_cu * hash_waves);
   }
   const int grid = GridFor(count, 1, max_grid);
+  if constexpr (!A) {
+    if (il.n > 1) {  // merged lists (sync rounds only)
+      hipLaunchKernelGGL((k_bm_hash_lists<THREADS, TABLE>), dim3(grid), dim3(THREADS), 0, s, g, st,
+                         mp, il, count, dlo, dhi, out, fb_list, rc);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_bm_hash<THREADS, TABLE, A>), dim3(grid), dim3(THREADS), 0, s, g, st, mp,
-                     il, count, dlo, dhi, out, fb_list, rc, salt, dcount);
+                     il.list[0], count, dlo, dhi, out, fb_list, rc, salt, dcount);
 }
 
 // The kernel of hash list k (0 = bin 5) over the items il, degrees (lo, hi].

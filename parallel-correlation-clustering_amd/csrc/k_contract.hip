@@ -598,9 +598,9 @@ __global__ __launch_bounds__(256) void k_cluster_weights(int64_t n, const uint32
 // Per-vertex intra-cluster edge weight (halved) in pbbslib's reduce order
 // (PCCI.cc:128-138): a left fold for degrees up to one reduce block of
 // max(1024, 4 ceil(sqrt(deg))) terms, block sums folded left otherwise.
-// Degrees up to 16 are folded by the vertex's own thread; larger ones by the
-// whole wave, which loads kObjChunk consecutive terms at once and folds them in
-// order through shuffles.
+// Degrees up to kObjThreadDeg are folded by the vertex's own thread; larger
+// ones by the whole wave, which loads kObjChunk consecutive terms at once and
+// folds them in order through shuffles.
 __device__ __forceinline__ double ObjTerm(const CsrView& g, const uint32_t* __restrict__ ids,
                                           uint32_t cv, double offset, int64_t p) {
   const float w = g.w[p];
@@ -608,6 +608,8 @@ __device__ __forceinline__ double ObjTerm(const CsrView& g, const uint32_t* __re
 }
 
 constexpr int kObjChunk = 4 * kWave;
+// Rows up to this degree are folded by their vertex's own thread.
+constexpr int64_t kObjThreadDeg = 16;
 
 // Ordered left fold of terms [s0, en) of vertex (vb, vc) onto acc (have: acc
 // holds a value); wave-uniform.
@@ -658,16 +660,33 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
       const double k = NwAt(nw, v);
       r[v] = k * (K[cv] - k);
       if (deg > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;  // k_objective_long
-      if (deg <= 16) {
+      if (deg <= kObjThreadDeg) {
+        // The left fold (the reference's order for rows of at most 1024
+        // terms), eight terms' loads in flight at a time.
         double tot = 0.0;
-        for (int64_t p = 0; p < deg; ++p) {
-          const double x = ObjTerm(g, ids, cv, offset, b + p);
-          tot = p == 0 ? x : tot + x;
+        for (int64_t p0 = 0; p0 < deg; p0 += 8) {
+          uint32_t u[8];
+          float w[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const bool ok = p0 + k < deg;
+            u[k] = ok ? g.nbr[b + p0 + k] : 0u;
+            w[k] = ok ? g.w[b + p0 + k] : 0.0f;
+          }
+          uint32_t c[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) c[k] = p0 + k < deg ? ids[u[k]] : ~cv;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if (p0 + k >= deg) break;
+            const double x = c[k] == cv ? ((double)w[k] - offset) / 2 : 0.0;
+            tot = p0 + k == 0 ? x : tot + x;
+          }
         }
         e[v] = tot;
       }
     }
-    uint64_t big = __ballot(in && deg > 16 && deg <= kLongRow);
+    uint64_t big = __ballot(in && deg > kObjThreadDeg && deg <= kLongRow);
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
@@ -809,7 +828,13 @@ __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint3
       deg = g.off[v + 1] - b;
       if (deg <= 64) {
         int64_t cnt = 0;
-        for (int64_t p = b; p < b + deg; ++p) cnt += ids[g.nbr[p]] == cv ? 1 : 0;
+        for (int64_t p0 = b; p0 < b + deg; p0 += 8) {
+          uint32_t u[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) u[k] = p0 + k < b + deg ? g.nbr[p0 + k] : 0u;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) cnt += (p0 + k < b + deg && ids[u[k]] == cv) ? 1 : 0;
+        }
         intra[v] = (double)cnt;
       }
     }
@@ -822,7 +847,13 @@ __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint3
       const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
       const uint32_t vc = __shfl(cv, l);
       unsigned long long cnt = 0;
-      for (int64_t p = lane; p < vd; p += kWave) cnt += ids[g.nbr[vb + p]] == vc ? 1ull : 0ull;
+      for (int64_t p0 = lane; p0 < vd; p0 += 4 * kWave) {
+        uint32_t u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[k] = p0 + k * kWave < vd ? g.nbr[vb + p0 + k * kWave] : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cnt += (p0 + k * kWave < vd && ids[u[k]] == vc) ? 1ull : 0ull;
+      }
       for (int s2 = 32; s2 > 0; s2 >>= 1) cnt += __shfl_xor(cnt, s2);
       if (lane == 0) intra[wave_v0 + l] = (double)cnt;
     }
@@ -954,8 +985,7 @@ void SegmentSums(DeviceAllocator* al, const V* vals, const int64_t* start, int64
   PCC_HIP_CHECK(hipGetLastError());
   if (nvals <= 1024) return;
   unsigned long long hl = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&hl, nlong.get(), sizeof(hl), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&hl, nlong.get(), sizeof(hl), s);
   const int64_t nl = (int64_t)hl;
   if (nl == 0) return;
   DevBuf<uint32_t> nblk(al, nl);
@@ -965,8 +995,7 @@ void SegmentSums(DeviceAllocator* al, const V* vals, const int64_t* start, int64
   PCC_HIP_CHECK(hipGetLastError());
   ExclusiveScanU32toI64(al, nblk.get(), boff.get(), nl, boff.get() + nl, s);
   int64_t total = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&total, boff.get() + nl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&total, boff.get() + nl, sizeof(int64_t), s);
   // Each run has len > 1024 terms and at most ceil(len / 1024) blocks.
   if (total <= 0 || total > nvals / 1024 + nl)
     throw HipError{hipErrorUnknown, "SegmentSums: inconsistent block counts", __FILE__, __LINE__};
@@ -996,8 +1025,7 @@ int64_t Segments(DeviceAllocator* al, const uint64_t* keys, int64_t n, DevBuf<in
   hipLaunchKernelGGL(k_heads, dim3(grid), dim3(256), 0, s, keys, n, head.get());
   ExclusiveScanU8toU32(al, head.get(), rank.get(), n, total.get(), s);
   uint32_t nseg = 0;
-  PCC_HIP_CHECK(hipMemcpyAsync(&nseg, total.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&nseg, total.get(), sizeof(uint32_t), s);
   start->Reset(al, (int64_t)nseg + 1);
   hipLaunchKernelGGL(k_seg_starts, dim3(grid), dim3(256), 0, s, head.get(), rank.get(), n,
                      start->get());
@@ -1015,8 +1043,7 @@ bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s) {
                      nchunks, acc.get());
   PCC_HIP_CHECK(hipGetLastError());
   unsigned long long h = 1;
-  PCC_HIP_CHECK(hipMemcpyAsync(&h, acc.get(), sizeof(h), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&h, acc.get(), sizeof(h), s);
   return h == 0;
 }
 
@@ -1067,9 +1094,7 @@ void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* 
                        ccnt.get());
     PCC_HIP_CHECK(hipGetLastError());
     ExclusiveScanU32toI64(al, ccnt.get(), coff.get(), nchunks, coff.get() + nchunks, s);
-    PCC_HIP_CHECK(hipMemcpyAsync(&mk, coff.get() + nchunks, sizeof(int64_t), hipMemcpyDeviceToHost,
-                                 s));
-    PCC_HIP_CHECK(hipStreamSynchronize(s));
+    ReadBack(&mk, coff.get() + nchunks, sizeof(int64_t), s);
     keys.Reset(al, mk);
     vals.Reset(al, mk);
     if (mk > 0) {
@@ -1098,9 +1123,7 @@ void CompressGraphSorted(DeviceAllocator* al, const CsrView& g, const uint32_t* 
   }
   ExclusiveScanU32toI64(al, cnt.get(), out->off.get(), n2, out->off.get() + n2, s);
   int64_t mend = -1;
-  PCC_HIP_CHECK(hipMemcpyAsync(&mend, out->off.get() + n2, sizeof(int64_t), hipMemcpyDeviceToHost,
-                               s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&mend, out->off.get() + n2, sizeof(int64_t), s);
   if (mend != m2)
     throw HipError{hipErrorUnknown, "CompressGraph: contracted offsets do not cover the edges",
                    __FILE__, __LINE__};
@@ -1202,9 +1225,7 @@ void GraphSums(DeviceAllocator* al, const double* a, const double* b, int G,
                const int64_t* gbase, double* sa, double* sb, hipStream_t s) {
   constexpr int64_t kChunk = 4096;
   std::vector<int64_t> hb((size_t)G + 1);
-  PCC_HIP_CHECK(hipMemcpyAsync(hb.data(), gbase, sizeof(int64_t) * hb.size(), hipMemcpyDeviceToHost,
-                               s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(hb.data(), gbase, sizeof(int64_t) * hb.size(), s);
   // Chunk starts; graphs are contiguous, so each chunk ends where the next
   // one (or the last graph) ends.
   std::vector<int64_t> cs, coff((size_t)G + 1);

@@ -819,6 +819,10 @@ __global__ __launch_bounds__(256) void k_gv_node_weights(const uint64_t* __restr
   }
 }
 
+// Up to this many items in classes 0-4 (or 5-11, or all classes for the
+// copies) go through one launch.
+constexpr int64_t kGvFewItems = 256;
+
 template <int G>
 void LaunchSmall(const CsrView& g, const uint32_t* ids, const GroupIndex& gi, const GvItems& L,
                  int64_t count, uint32_t* nbr2p, float* w2p, uint32_t* len, hipStream_t s) {
@@ -909,8 +913,7 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
                      bbase.get() + kGvClasses * nblk, dbounds.get());
   PCC_HIP_CHECK(hipGetLastError());
   unsigned long long hb[kGvClasses + 1] = {};
-  PCC_HIP_CHECK(hipMemcpyAsync(hb, dbounds.get(), sizeof(hb), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(hb, dbounds.get(), sizeof(hb), s);
   const int64_t nitems = (int64_t)hb[kGvClasses];
   const int64_t nbig = (int64_t)(hb[kGvClasses] - hb[kGvBig]);
   const int rbits = GvBitsFor((uint64_t)(nbig > 1 ? nbig - 1 : 1));
@@ -931,18 +934,42 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   DevBuf<float> w2p(al, mpad);
   uint32_t* np = nbr2p.get();
   float* wp = w2p.get();
-  LaunchSmall<4>(g, ids, gi, cls(0), cnt(0), np, wp, len.get(), s);
-  LaunchSmall<8>(g, ids, gi, cls(1), cnt(1), np, wp, len.get(), s);
-  LaunchSmall<16>(g, ids, gi, cls(2), cnt(2), np, wp, len.get(), s);
-  LaunchSmall<32>(g, ids, gi, cls(3), cnt(3), np, wp, len.get(), s);
-  LaunchSmall<64>(g, ids, gi, cls(4), cnt(4), np, wp, len.get(), s);
-  LaunchWave<2>(g, ids, gi, cls(5), cnt(5), np, wp, len.get(), s);
-  LaunchWave<4>(g, ids, gi, cls(6), cnt(6), np, wp, len.get(), s);
-  LaunchWave<8>(g, ids, gi, cls(7), cnt(7), np, wp, len.get(), s);
-  LaunchMulti<2>(g, ids, gi, cls(8), cnt(8), np, wp, len.get(), s);
-  LaunchMulti<4>(g, ids, gi, cls(9), cnt(9), np, wp, len.get(), s);
-  LaunchMulti<8>(g, ids, gi, cls(10), cnt(10), np, wp, len.get(), s);
-  LaunchMulti<16>(g, ids, gi, cls(11), cnt(11), np, wp, len.get(), s);
+  // Each class kernel sorts any capacity up to its own, and the sorted order
+  // (hence the row) does not depend on the network size, so a few clusters
+  // spread over several classes take one launch of the largest kernel they
+  // need: on small graphs the class launches are short and dependent.
+  const int64_t nsmall = (int64_t)(hb[5] - hb[0]), nmid = (int64_t)(hb[kGvBig] - hb[5]);
+  if (nsmall <= kGvFewItems) {
+    LaunchSmall<64>(g, ids, gi, cls(0), nsmall, np, wp, len.get(), s);
+  } else {
+    LaunchSmall<4>(g, ids, gi, cls(0), cnt(0), np, wp, len.get(), s);
+    LaunchSmall<8>(g, ids, gi, cls(1), cnt(1), np, wp, len.get(), s);
+    LaunchSmall<16>(g, ids, gi, cls(2), cnt(2), np, wp, len.get(), s);
+    LaunchSmall<32>(g, ids, gi, cls(3), cnt(3), np, wp, len.get(), s);
+    LaunchSmall<64>(g, ids, gi, cls(4), cnt(4), np, wp, len.get(), s);
+  }
+  if (nmid <= kGvFewItems) {
+    int top = 5;
+    for (int k = 5; k < kGvBig; ++k)
+      if (cnt(k) > 0) top = k;
+    switch (top) {
+      case 5: LaunchWave<2>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      case 6: LaunchWave<4>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      case 7: LaunchWave<8>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      case 8: LaunchMulti<2>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      case 9: LaunchMulti<4>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      case 10: LaunchMulti<8>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+      default: LaunchMulti<16>(g, ids, gi, cls(5), nmid, np, wp, len.get(), s); break;
+    }
+  } else {
+    LaunchWave<2>(g, ids, gi, cls(5), cnt(5), np, wp, len.get(), s);
+    LaunchWave<4>(g, ids, gi, cls(6), cnt(6), np, wp, len.get(), s);
+    LaunchWave<8>(g, ids, gi, cls(7), cnt(7), np, wp, len.get(), s);
+    LaunchMulti<2>(g, ids, gi, cls(8), cnt(8), np, wp, len.get(), s);
+    LaunchMulti<4>(g, ids, gi, cls(9), cnt(9), np, wp, len.get(), s);
+    LaunchMulti<8>(g, ids, gi, cls(10), cnt(10), np, wp, len.get(), s);
+    LaunchMulti<16>(g, ids, gi, cls(11), cnt(11), np, wp, len.get(), s);
+  }
   PCC_HIP_CHECK(hipGetLastError());
   if (nbig > 0) {
     // Large clusters through the radix sort, in chunks of consecutive ranks
@@ -956,9 +983,7 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
     PCC_HIP_CHECK(hipGetLastError());
     ExclusiveScanU32toI64(al, bcap.get(), boff.get(), nbig, boff.get() + nbig, s);
     std::vector<int64_t> hoff((size_t)nbig + 1);
-    PCC_HIP_CHECK(hipMemcpyAsync(hoff.data(), boff.get(), sizeof(int64_t) * (size_t)(nbig + 1),
-                                 hipMemcpyDeviceToHost, s));
-    PCC_HIP_CHECK(hipStreamSynchronize(s));
+    ReadBack(hoff.data(), boff.get(), sizeof(int64_t) * (size_t)(nbig + 1), s);
     const int64_t chunk = std::max<int64_t>(chunk_terms, 1);
     for (int64_t r0 = 0; r0 < nbig;) {
       int64_t r1 = r0 + 1;
@@ -991,8 +1016,7 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   out->off.Reset(al, n2 + 1);
   ExclusiveScanU32toI64(al, len.get(), out->off.get(), n2, out->off.get() + n2, s);
   int64_t m2 = -1;
-  PCC_HIP_CHECK(hipMemcpyAsync(&m2, out->off.get() + n2, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  ReadBack(&m2, out->off.get() + n2, sizeof(int64_t), s);
   if (m2 < 0 || m2 > g.m)
     throw HipError{hipErrorUnknown, "CompressGraphGrouped: contracted rows exceed the terms",
                    __FILE__, __LINE__};
@@ -1003,12 +1027,16 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
     const int64_t* o2 = out->off.get();
     uint32_t* onb = out->nbr.get();
     float* ow = out->w.get();
-    LaunchCopy<4>(cls(0), cnt(0), len.get(), o2, np, wp, onb, ow, s);
-    LaunchCopy<8>(cls(1), cnt(1), len.get(), o2, np, wp, onb, ow, s);
-    LaunchCopy<16>(cls(2), cnt(2), len.get(), o2, np, wp, onb, ow, s);
-    LaunchCopy<32>(cls(3), cnt(3), len.get(), o2, np, wp, onb, ow, s);
-    LaunchCopy<64>(cls(4), hb[9] - hb[4], len.get(), o2, np, wp, onb, ow, s);
-    LaunchCopy<256>(cls(9), hb[kGvClasses] - hb[9], len.get(), o2, np, wp, onb, ow, s);
+    if (nitems <= kGvFewItems) {
+      LaunchCopy<64>(cls(0), nitems, len.get(), o2, np, wp, onb, ow, s);
+    } else {
+      LaunchCopy<4>(cls(0), cnt(0), len.get(), o2, np, wp, onb, ow, s);
+      LaunchCopy<8>(cls(1), cnt(1), len.get(), o2, np, wp, onb, ow, s);
+      LaunchCopy<16>(cls(2), cnt(2), len.get(), o2, np, wp, onb, ow, s);
+      LaunchCopy<32>(cls(3), cnt(3), len.get(), o2, np, wp, onb, ow, s);
+      LaunchCopy<64>(cls(4), hb[9] - hb[4], len.get(), o2, np, wp, onb, ow, s);
+      LaunchCopy<256>(cls(9), hb[kGvClasses] - hb[9], len.get(), o2, np, wp, onb, ow, s);
+    }
   }
   PCC_HIP_CHECK(hipGetLastError());
   nbr2p.Release();

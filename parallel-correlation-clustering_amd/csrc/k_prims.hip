@@ -5,6 +5,9 @@
 // makes (parallel/parallel-sequence-ops.h:44-247).  Stability of the radix
 // sort reproduces the stable sample_sort order (PSO.h:227-232).
 
+#include <algorithm>
+#include <cstring>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -372,6 +375,29 @@ void RadixSortPairsF64(DeviceAllocator* al, uint64_t* keys, double* vals, int64_
 void RadixSortPairsU32(DeviceAllocator* al, uint64_t* keys, uint32_t* vals, int64_t n,
                        int begin_bit, int end_bit, hipStream_t s) {
   RadixSortImpl<uint32_t>(al, keys, vals, n, begin_bit, end_bit, s);
+}
+
+void ReadBack(void* host, const void* dev, size_t bytes, hipStream_t s) {
+  // A copy into pageable memory is staged by the runtime; one into pinned
+  // memory is a plain DMA (about 15 instead of 23 us for a few bytes after a
+  // kernel).  The staging buffer lives as long as its thread's process.
+  thread_local void* stage = nullptr;
+  thread_local size_t cap = 0;
+  if (bytes == 0) return;
+  if (bytes > cap) {
+    const size_t want = std::max<size_t>(bytes, 4096);
+    void* p = nullptr;
+    PCC_HIP_CHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
+    if (stage) {
+      PCC_HIP_CHECK(hipStreamSynchronize(s));
+      (void)hipHostFree(stage);
+    }
+    stage = p;
+    cap = want;
+  }
+  PCC_HIP_CHECK(hipMemcpyAsync(stage, dev, bytes, hipMemcpyDeviceToHost, s));
+  PCC_HIP_CHECK(hipStreamSynchronize(s));
+  std::memcpy(host, stage, bytes);
 }
 
 }  // namespace pcc

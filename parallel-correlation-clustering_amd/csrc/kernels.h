@@ -528,6 +528,10 @@ bool GraphIsSymmetric(DeviceAllocator* al, const CsrView& g, hipStream_t s);
 
 // ------------------------------------------------------------- primitives --
 
+// Copies bytes from device memory to host memory after the work queued on s
+// (synchronizes s), through a pinned staging buffer of the calling thread.
+void ReadBack(void* host, const void* dev, size_t bytes, hipStream_t s);
+
 // Exclusive scan; total (device pointer, may be null) receives the sum.  If
 // gate is non-null and *gate == 0 the kernels return immediately.
 // work (may be null): ScanWorkElems(n) preallocated elements, so the scan
