@@ -535,7 +535,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
   c->hs.vals = c->hvals.get();
   if (hashed) {
     c->gscale.Reset(al_, g.n);
-    LaunchGranThreshold(al_, g, cfg_.edge_weight_offset, c->gscale.get(), s_);
+    LaunchGranThreshold(al_, g, cfg_.edge_weight_offset, c->gscale.get(), s_, maxdeg);
   }
   c->hb = HubScratch();
   const int64_t c7 = c->caps[kBins - 1], t7 = c->caps_te[kBins - 1];

@@ -1993,20 +1993,23 @@ void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64
 }
 
 void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, int32_t* gscale,
-                         hipStream_t s) {
+                         hipStream_t s, int64_t max_deg) {
   if (g.n == 0) return;
-  const int64_t maxlong = g.m / kLongRow + 1;
+  // Long rows exist only above kLongRow edges (k_gran_threshold queues them).
+  const bool longs = g.m > kLongRow && (max_deg < 0 || max_deg > kLongRow);
+  const int64_t maxlong = longs ? g.m / kLongRow + 1 : 1;
   DevBuf<uint32_t> longq(al, maxlong);
   DevBuf<unsigned int> nlong(al, 1);
   DevBuf<double> acc_s(al, maxlong);
   DevBuf<unsigned long long> acc_m(al, maxlong);
   DevBuf<int> acc_g(al, maxlong);
-  hipLaunchKernelGGL(k_gran_clear, dim3(GridFor(maxlong, 256, 1024)), dim3(256), 0, s, maxlong,
-                     nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
+  if (longs)
+    hipLaunchKernelGGL(k_gran_clear, dim3(GridFor(maxlong, 256, 1024)), dim3(256), 0, s, maxlong,
+                       nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
   const int grid = GridFor(g.n, 256, 8192);
   hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset,
                      gscale, longq.get(), nlong.get());
-  if (g.m > kLongRow) {
+  if (longs) {
     hipLaunchKernelGGL(k_gran_long, dim3(1024), dim3(256), 0, s, g.off, g.w, offset, longq.get(),
                        nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
     hipLaunchKernelGGL(k_gran_long_final, dim3(16), dim3(256), 0, s, longq.get(), nlong.get(),

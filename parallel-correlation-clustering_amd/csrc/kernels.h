@@ -329,8 +329,10 @@ void LaunchSubScatter(const uint32_t* list, const BinLayout& layout, const int64
                       uint32_t S, uint32_t* cur, uint32_t* out, hipStream_t s);
 
 // gscale[v] for every vertex with deg > 64 (see MoveParams::gscale).
+// max_deg (-1: unknown) bounds the degrees; without rows longer than
+// kLongRow the long-row passes are skipped.
 void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, int32_t* gscale,
-                         hipStream_t s);
+                         hipStream_t s, int64_t max_deg = -1);
 
 // Bin-7 prefix sums (HubScratch::part_off / ent_off) of a list, with
 // partitions of part_edges edges, and the list item of every partition
