@@ -243,6 +243,43 @@ __device__ __forceinline__ void ForLongChunks(const uint32_t* __restrict__ rows,
   }
 }
 
+// A round's clears, over threads tid = 0, stride, ...: byte arrays a, b, c
+// (n bytes each, any may be null), the bin counts (counters & 1), the
+// fallback / new-cluster / repair counters folded into fb_total
+// (counters & 2), and one 64-bit accumulator.
+template <typename Counters>
+__device__ __forceinline__ void RoundResetSpan(int64_t tid, int64_t stride, int64_t n,
+                                               uint8_t* __restrict__ a, uint8_t* __restrict__ b,
+                                               uint8_t* __restrict__ c, Counters* __restrict__ rc,
+                                               int counters, unsigned long long* __restrict__ acc,
+                                               int bins) {
+  const int64_t n16 = n / 16;
+  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  for (int64_t i = tid; i < n16; i += stride) {
+    if (a) reinterpret_cast<uint4*>(a)[i] = zero;
+    if (b) reinterpret_cast<uint4*>(b)[i] = zero;
+    if (c) reinterpret_cast<uint4*>(c)[i] = zero;
+  }
+  for (int64_t i = n16 * 16 + tid; i < n; i += stride) {
+    if (a) a[i] = 0;
+    if (b) b[i] = 0;
+    if (c) c[i] = 0;
+  }
+  if (tid < bins && (counters & 1)) {
+    rc->cnt[tid] = 0;
+    rc->te[tid] = 0;
+  }
+  if (tid == 0) {
+    if (counters & 2) {
+      rc->fb_total += rc->fb_count + rc->repair_count;
+      rc->fb_count = 0;
+      rc->new_count = 0;
+      rc->repair_count = 0;
+    }
+    if (acc) *acc = 0;
+  }
+}
+
 // Exclusive block scan; `smem` needs blockDim.x / 64 slots.  *total receives
 // the block sum.
 template <typename T>

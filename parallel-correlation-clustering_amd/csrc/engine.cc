@@ -629,10 +629,12 @@ void Engine::BinFrontier(LevelCtx& c, Workspace& ws, const uint8_t* frontier, in
 void Engine::LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
                          const uint32_t* const lists[kBins], const int64_t counts[kBins],
                          const int64_t te[kBins], const MoveOut& out, uint64_t salt,
-                         bool scan_hubs) {
+                         bool scan_hubs, const RoundResetJob* reset) {
   if (!h_->timing) {
     LaunchBestMoveBins(counts, lists, async, h_->th, c.g, c.st, mp, out, ws.fb_list.get(),
-                       ws.rc.get(), salt, s_);
+                       ws.rc.get(), salt, s_, nullptr, reset);
+  } else if (reset) {
+    LaunchRoundReset(reset->n, reset->a, reset->b, nullptr, reset->rc, reset->counters, nullptr, s_);
   }
   // With timing, one launch per bin and one timer per reported group (the
   // three ranges of a hash group together).
@@ -695,9 +697,16 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   // apply), the next frontier, the movers when the pull expansion of
   // NBHR_MOVE reads every vertex's flag, the round counters (the host has
   // read this frontier's bin counts already) and the seed volume.
-  LaunchRoundReset(n, ws.flags.get(), cfg_.move_method != PCC_ALL_MOVE ? ws.next.get() : nullptr,
-                   (pull && cfg_.move_method == PCC_NBHR_MOVE) ? ws.moved.get() : nullptr,
-                   ws.rc.get(), 3, pull ? c.vol.get() : nullptr, s_);
+  // Without pull the clears touch nothing the best-move kernels write, so
+  // sync rounds do them in the bins-0-4 launch.
+  const RoundResetJob reset{n, ws.flags.get(),
+                            cfg_.move_method != PCC_ALL_MOVE ? ws.next.get() : nullptr,
+                            ws.rc.get(), 3};
+  const bool fuse_reset = !async && !pull && !h_->timing;
+  if (!fuse_reset)
+    LaunchRoundReset(n, ws.flags.get(), cfg_.move_method != PCC_ALL_MOVE ? ws.next.get() : nullptr,
+                     (pull && cfg_.move_method == PCC_NBHR_MOVE) ? ws.moved.get() : nullptr,
+                     ws.rc.get(), 3, pull ? c.vol.get() : nullptr, s_);
   const uint64_t salt =
       (async && cfg_.permute) ? (cfg_.permute_seed * 0x9E3779B97F4A7C15ull + (uint64_t)rounds_ + 1) : 0;
   Timer all = StartTimer();
@@ -767,7 +776,8 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
       }
       ste[kBins - 1] = (int64_t)hte7[(size_t)sub];
     }
-    LaunchMoves(c, ws, mp, async, lists, scount, ste, out, salt, subs > 1 || sub == 0);
+    LaunchMoves(c, ws, mp, async, lists, scount, ste, out, salt, subs > 1 || sub == 0,
+                fuse_reset ? &reset : nullptr);
   }
   if (subs > 1) {
     // The frontier expansion walks bin 7 of the whole list.
@@ -778,7 +788,10 @@ int64_t Engine::RoundStep(LevelCtx& c, Workspace& ws, const uint32_t* vgraph, in
   }
   int64_t hashed_count = 0;
   for (int b = kWaveBin0; b < kBins; ++b) hashed_count += counts[b];
-  if (!async && hashed_count > 0) {
+  // Sync rounds list fallback vertices only from the hub bin or when every
+  // hashed vertex is forced through the ordered kernel (each hash bin's
+  // degrees fit its table, so its overflow guard never fires).
+  if (!async && (c.mp.force_ordered ? hashed_count > 0 : counts[kBins - 1] > 0)) {
     Timer tf = StartTimer();
     LaunchBestMoveOrdered(g, st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s_);
     StopTimer(tf, 8);
@@ -930,7 +943,7 @@ void Engine::RoundStepDevice(LevelCtx& c, Workspace& ws, const uint32_t* vgraph,
   }
   int64_t hashed = 0;
   for (int b = kWaveBin0; b < kBins; ++b) hashed += c.caps[b];
-  if (hashed > 0) {
+  if (c.mp.force_ordered ? hashed > 0 : c.caps[kBins - 1] > 0) {  // as in RoundStep
     Timer tf = StartTimer();
     LaunchBestMoveOrdered(g, st, c.mp, ws.fb_list.get(), ws.rc.get(), out, c.hs, s_);
     StopTimer(tf, 8);

@@ -183,7 +183,8 @@ class Engine {
   // bin-7 prefix sums first.
   void LaunchMoves(LevelCtx& c, Workspace& ws, const MoveParams& mp, bool async,
                    const uint32_t* const lists[kBins], const int64_t counts[kBins],
-                   const int64_t te[kBins], const MoveOut& out, uint64_t salt, bool scan_hubs);
+                   const int64_t te[kBins], const MoveOut& out, uint64_t salt, bool scan_hubs,
+                   const RoundResetJob* reset = nullptr);
   // IterateBestMoves (parallel-correlation-clusterer.cc:340-365) over the
   // graphs selected by sel (present-graph flags; null = all).  Returns the
   // per-present-graph "moved" flags.

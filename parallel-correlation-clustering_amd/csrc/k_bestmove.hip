@@ -109,6 +109,7 @@ constexpr int kBinItems = 16;
 constexpr int kBinThreads = 256;
 constexpr int64_t kBinChunk = (int64_t)kBinItems * kBinThreads;
 static_assert(kBins <= 12 && kBinChunk < 65536, "three 64-bit words of 16-bit counts");
+constexpr int64_t kBinSmallN = 65536;  // one vertex per thread and chunk up to here
 
 __device__ __forceinline__ uint64_t PackedField(uint64_t w0, uint64_t w1, uint64_t w2, int b) {
   const uint64_t w = b < 4 ? w0 : (b < 8 ? w1 : w2);
@@ -121,22 +122,26 @@ __device__ __forceinline__ void PackedAdd(uint64_t& w0, uint64_t& w1, uint64_t& 
   w2 += b >= 8 ? inc : 0ull;
 }
 
+// ITEMS vertices per thread and chunk (kBinItems; 1 on small graphs, whose
+// few chunks would otherwise leave all but a few CUs idle).
+template <int ITEMS>
 __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
     int64_t n, const int64_t* __restrict__ off, const uint8_t* __restrict__ frontier,
     BinThresholds th, BinLayout layout, uint32_t* __restrict__ list,
     RoundCounters* __restrict__ rc) {
+  constexpr int64_t kChunk = (int64_t)ITEMS * kBinThreads;
   __shared__ unsigned long long s_scan[kBinThreads / kWave];
   __shared__ unsigned long long s_te[kBins], s_base[kBins];
-  for (int64_t c0 = (int64_t)blockIdx.x * kBinChunk; c0 < n;
-       c0 += (int64_t)gridDim.x * kBinChunk) {
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < n;
+       c0 += (int64_t)gridDim.x * kChunk) {
     if (threadIdx.x < kBins) s_te[threadIdx.x] = 0;
-    int8_t bin[kBinItems];
+    int8_t bin[ITEMS];
     uint64_t p0 = 0, p1 = 0, p2 = 0;
     unsigned long long te[kBins];
 #pragma unroll
     for (int j = 0; j < kBins; ++j) te[j] = 0;
 #pragma unroll
-    for (int k = 0; k < kBinItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
       const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
       int b = -1;
       unsigned long long deg = 0;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kBinItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
       const int b = bin[k];
       if (b < 0) continue;
       const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
@@ -746,16 +751,26 @@ __global__ __launch_bounds__(256) void k_bm_small(CsrView g, StateView st, MoveP
 // keeps the XCD it has in the grid).  In sync rounds the bins read the same
 // state and write disjoint vertices, so running them together changes no
 // result; it saves four dependent launches per round on small graphs.
+// Workgroups from end[4] on do the round's clears (RoundResetJob).
 struct SmallBins {
   const uint32_t* list[kWaveBin0];
   int64_t count[kWaveBin0];
   const unsigned long long* dcount[kWaveBin0];
   uint32_t end[kWaveBin0];
+  RoundResetJob reset;
+  uint32_t reset_blocks;
 };
 
 template <bool kAsync>
 __global__ __launch_bounds__(256) void k_bm_small_all(CsrView g, StateView st, MoveParams mp,
                                                       SmallBins sb, MoveOut out, uint64_t salt) {
+  if (blockIdx.x >= sb.end[kWaveBin0 - 1]) {
+    const RoundResetJob& r = sb.reset;
+    RoundResetSpan((int64_t)(blockIdx.x - sb.end[kWaveBin0 - 1]) * blockDim.x + threadIdx.x,
+                   (int64_t)sb.reset_blocks * blockDim.x, r.n, r.a, r.b, (uint8_t*)nullptr, r.rc,
+                   r.counters, (unsigned long long*)nullptr, kBins);
+    return;
+  }
   int b = 0;
   while (b < kWaveBin0 - 1 && blockIdx.x >= sb.end[b]) ++b;
   const uint32_t b0 = b ? sb.end[b - 1] : 0u;
@@ -1946,16 +1961,20 @@ int GridFor(int64_t items, int per_block, int max_blocks) {
 void LaunchBinFrontier(const CsrView& g, const uint8_t* frontier, const BinThresholds& th,
                        const BinLayout& layout, uint32_t* list, RoundCounters* rc, hipStream_t s) {
   if (g.n == 0) return;
-  const int grid = GridFor(g.n, (int)kBinChunk, 2048);
-  hipLaunchKernelGGL(k_bin_frontier, dim3(grid), dim3(kBinThreads), 0, s, g.n, g.off, frontier, th,
-                     layout, list, rc);
+  if (g.n <= kBinSmallN) {
+    hipLaunchKernelGGL(k_bin_frontier<1>, dim3(GridFor(g.n, kBinThreads, 2048)), dim3(kBinThreads),
+                       0, s, g.n, g.off, frontier, th, layout, list, rc);
+  } else {
+    hipLaunchKernelGGL(k_bin_frontier<kBinItems>, dim3(GridFor(g.n, (int)kBinChunk, 2048)),
+                       dim3(kBinThreads), 0, s, g.n, g.off, frontier, th, layout, list, rc);
+  }
   PCC_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchBinCapacity(const CsrView& g, const BinThresholds& th, const double* nw,
                        const double* gres, int64_t G, unsigned long long* caps, hipStream_t s) {
   if (g.n == 0 && !gres) return;
-  const int grid = GridFor(g.n, 256 * 16, 1024);
+  const int grid = GridFor(g.n, g.n <= kBinSmallN ? 256 : 256 * 16, 1024);
   hipLaunchKernelGGL(k_bin_capacity, dim3(grid), dim3(256), 0, s, g.n, g.off, th, nw, gres, G,
                      caps);
   PCC_HIP_CHECK(hipGetLastError());
@@ -2166,7 +2185,11 @@ void LaunchBestMoveBins(const int64_t counts[kHubBin], const uint32_t* const lis
                         bool async, const BinThresholds& th, const CsrView& g,
                         const StateView& st, const MoveParams& mp, const MoveOut& out,
                         uint32_t* fb_list, RoundCounters* rc, uint64_t salt, hipStream_t s,
-                        const DeviceCounts* dc) {
+                        const DeviceCounts* dc, const RoundResetJob* reset) {
+  if (reset && async) {
+    LaunchRoundReset(reset->n, reset->a, reset->b, nullptr, reset->rc, reset->counters, nullptr, s);
+    reset = nullptr;
+  }
   if (async) {
     // Async rounds keep one launch per bin, each seeing the moves of the
     // ones before: running the bins of a sub-round together lowered the
@@ -2190,9 +2213,14 @@ void LaunchBestMoveBins(const int64_t counts[kHubBin], const uint32_t* const lis
     }
     sb.end[b] = blocks;
   }
-  if (blocks > 0) {
-    hipLaunchKernelGGL(k_bm_small_all<false>, dim3(blocks), dim3(256), 0, s, g, st, mp, sb, out,
-                       salt);
+  if (reset) {
+    sb.reset = *reset;
+    const int64_t work = (reset->a || reset->b) ? (reset->n + 15) / 16 : 1;
+    sb.reset_blocks = (uint32_t)GridFor(work, 256, 2048);
+  }
+  if (blocks + sb.reset_blocks > 0) {
+    hipLaunchKernelGGL(k_bm_small_all<false>, dim3(blocks + sb.reset_blocks), dim3(256), 0, s, g,
+                       st, mp, sb, out, salt);
     PCC_HIP_CHECK(hipGetLastError());
   }
   int64_t hashed = 0;

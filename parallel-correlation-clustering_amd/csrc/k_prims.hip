@@ -87,6 +87,39 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const Tin* __restri
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
 
+// One workgroup walking the tiles in order with a running offset: a small
+// scan in one launch instead of three.
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kScanThreads) void k_scan_serial(const Tin* __restrict__ in,
+                                                              Tout* __restrict__ out, int64_t n,
+                                                              Tout* __restrict__ total,
+                                                              const unsigned long long* gate) {
+  if (gate && *gate == 0) return;
+  Tout carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * kScanItems;
+    Tout v[kScanItems];
+    Tout s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      v[j] = (base + j < n) ? (Tout)in[base + j] : (Tout)0;
+      s += v[j];
+    }
+    Tout tot;
+    Tout run = BlockExclusiveScan(s, &tot) + carry;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      if (base + j < n) out[base + j] = run;
+      run += v[j];
+    }
+    carry += tot;
+  }
+  if (total && threadIdx.x == 0) *total = carry;
+}
+
+// Scans of up to this many elements run as one k_scan_serial launch.
+constexpr int64_t kScanSerialMax = 8 * kScanTile;
+
 // work (may be null): ScanWorkElems(n) elements for the tile sums of every
 // level of the recursion, so the scan allocates nothing (and can run inside a
 // captured graph); null draws them from the allocator.
@@ -101,6 +134,12 @@ void ScanImpl(DeviceAllocator* al, const Tin* in, Tout* out, int64_t n, Tout* to
   if (tiles == 1) {
     hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3(1), dim3(kScanThreads), 0, s, in, out, n,
                        (const Tout*)nullptr, total, gate);
+    PCC_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (n <= kScanSerialMax) {
+    hipLaunchKernelGGL((k_scan_serial<Tin, Tout>), dim3(1), dim3(kScanThreads), 0, s, in, out, n,
+                       total, gate);
     PCC_HIP_CHECK(hipGetLastError());
     return;
   }

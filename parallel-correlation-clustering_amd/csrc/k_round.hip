@@ -192,33 +192,8 @@ __global__ __launch_bounds__(256) void k_round_reset(int64_t n, uint8_t* __restr
                                                      unsigned long long* __restrict__ acc,
                                                      const unsigned int* __restrict__ c_gate) {
   if (c_gate && *c_gate == 0) c = nullptr;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t n16 = n / 16;
-  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-  for (int64_t i = tid; i < n16; i += stride) {
-    if (a) reinterpret_cast<uint4*>(a)[i] = zero;
-    if (b) reinterpret_cast<uint4*>(b)[i] = zero;
-    if (c) reinterpret_cast<uint4*>(c)[i] = zero;
-  }
-  for (int64_t i = n16 * 16 + tid; i < n; i += stride) {
-    if (a) a[i] = 0;
-    if (b) b[i] = 0;
-    if (c) c[i] = 0;
-  }
-  if (tid < kBins && (counters & 1)) {
-    rc->cnt[tid] = 0;
-    rc->te[tid] = 0;
-  }
-  if (tid == 0) {
-    if (counters & 2) {
-      rc->fb_total += rc->fb_count + rc->repair_count;
-      rc->fb_count = 0;
-      rc->new_count = 0;
-      rc->repair_count = 0;
-    }
-    if (acc) *acc = 0;
-  }
+  RoundResetSpan((int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x, n,
+                 a, b, c, rc, counters, acc, kBins);
 }
 
 // 16 lanes per seed vertex mark its neighbours.

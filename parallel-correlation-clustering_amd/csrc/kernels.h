@@ -360,11 +360,22 @@ void LaunchBestMoveBin(int bin, int64_t count, bool async, const BinThresholds& 
 // rounds run bins 0-4 as one launch and, with at most 1024 hashed vertices
 // and host-known counts, bins 5-10 as one launch of the largest table
 // needed; async rounds launch bin by bin.  Results equal the per-bin launches.
+// reset (may be null): the round's clears (LaunchRoundReset without a third
+// array or accumulator), done by extra workgroups of the bins-0-4 launch in
+// sync rounds; it must not clear anything the best-move kernels write.
+struct RoundResetJob {
+  int64_t n;
+  uint8_t* a;
+  uint8_t* b;
+  RoundCounters* rc;
+  int counters;
+};
 void LaunchBestMoveBins(const int64_t counts[kHubBin], const uint32_t* const lists[kHubBin],
                         bool async, const BinThresholds& th, const CsrView& g,
                         const StateView& st, const MoveParams& mp, const MoveOut& out,
                         uint32_t* fb_list, RoundCounters* rc, uint64_t permute_salt,
-                        hipStream_t s, const DeviceCounts* dc = nullptr);
+                        hipStream_t s, const DeviceCounts* dc = nullptr,
+                        const RoundResetJob* reset = nullptr);
 
 // Bin 7: partition pass and selection pass over a list whose HubScratch
 // prefix sums are current; te = sum of the listed degrees.
