@@ -1085,11 +1085,17 @@ std::vector<uint8_t> Engine::IterateBestMoves(const Level& L, const std::vector<
       local_moved = fsize != 0;
       ++rounds;
     }
-    // Ordered-path and repaired vertices of every round of this call.
-    RoundCounters hrc;
-    int64_t c2[kBins], t2[kBins];
-    ReadCounters(ws.rc.get(), c2, t2, &hrc);
-    fallback_ += (int64_t)(hrc.fb_total + hrc.fb_count + hrc.repair_count);
+    // Ordered-path and repaired vertices of every round of this call, added
+    // up on device (read once per Run, no synchronization here).
+    if (!fb_acc_.get()) {
+      fb_acc_.Reset(al_, 1);
+      ZeroSpans z{};
+      z.p[0] = reinterpret_cast<uint8_t*>(fb_acc_.get());
+      z.bytes[0] = sizeof(unsigned long long);
+      z.n = 1;
+      LaunchZeroSpans(z, s_);
+    }
+    LaunchAddFallbacks(ws.rc.get(), fb_acc_.get(), s_);
   }
   std::vector<uint8_t> moved((size_t)G, 0);
   if (cfg_.move_method == PCC_ALL_MOVE) {
@@ -1196,8 +1202,11 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
     }
     const bool last = iter == num_iterations - 1;
     if (!cfg_.refine && any && (!subcl || last)) {  // PCC.cc:550
-      DevBuf<uint8_t> dsel = Upload(al_, sel_o, s_);
-      LaunchFlattenShift(n, cluster_ids.get(), vgraph0, dsel.get(), nullptr, L.ids.get(), s_);
+      const bool every = std::all_of(sel_o.begin(), sel_o.end(), [](uint8_t x) { return x != 0; });
+      DevBuf<uint8_t> dsel;
+      if (!every) dsel = Upload(al_, sel_o, s_);
+      LaunchFlattenShift(n, cluster_ids.get(), vgraph0, every ? nullptr : dsel.get(), nullptr,
+                         L.ids.get(), s_);
     }
     // With subclustering a level starts from its parent clustering, which is
     // the result when nothing moves there.
@@ -1375,6 +1384,11 @@ int Engine::Run(DevBuf<double> nw0, const double* gres, const uint32_t* init, ui
                                  hipMemcpyDeviceToHost, s_));
   }
   PCC_HIP_CHECK(hipStreamSynchronize(s_));
+  if (fb_acc_.get()) {
+    unsigned long long fb = 0;
+    ReadBack(&fb, fb_acc_.get(), sizeof(fb), s_);
+    fallback_ += (int64_t)fb;
+  }
   if (out_iter) *out_iter = iter_o;
   if (stats_) {
     int64_t lv = 0;

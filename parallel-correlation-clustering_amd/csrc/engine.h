@@ -227,6 +227,9 @@ class Engine {
   int64_t bin_edges_[kStatGroups] = {0};
   int64_t bin_vertices_[kStatGroups] = {0};
   int64_t rounds_ = 0, edges_scanned_ = 0, frontier_vertices_ = 0, fallback_ = 0;
+  // Fallback and repair counts of host-driven rounds, summed on device and
+  // read once at the end of Run.
+  DevBuf<unsigned long long> fb_acc_;
   int64_t cluster_moves_ = 0;
   double bestmove_bytes_ = 0;
   bool exact_nw_ = true;

@@ -738,6 +738,17 @@ __global__ __launch_bounds__(256) void k_fill_u32_gated(uint32_t* __restrict__ p
     p[i] = x;
 }
 
+// acc += the fallback and repair counts of a workspace's rounds (one thread).
+__global__ void k_add_fallbacks(const RoundCounters* __restrict__ rc,
+                                unsigned long long* __restrict__ acc) {
+  *acc += rc->fb_total + rc->fb_count + rc->repair_count;
+}
+
+void LaunchAddFallbacks(const RoundCounters* rc, unsigned long long* acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_fallbacks, dim3(1), dim3(1), 0, s, rc, acc);
+  PCC_HIP_CHECK(hipGetLastError());
+}
+
 // Word stores over each range's aligned middle, byte stores at its ends.
 __global__ __launch_bounds__(256) void k_zero_spans(ZeroSpans z) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

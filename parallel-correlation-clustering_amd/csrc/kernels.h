@@ -422,6 +422,9 @@ struct ZeroSpans {
 };
 void LaunchZeroSpans(const ZeroSpans& z, hipStream_t s);
 
+// *acc += rc->fb_total + rc->fb_count + rc->repair_count, on the stream.
+void LaunchAddFallbacks(const RoundCounters* rc, unsigned long long* acc, hipStream_t s);
+
 struct NewAssignWork {
   DevBuf<uint32_t> zrank, zero_list, nrank, scan;
   DevBuf<uint8_t> z;
