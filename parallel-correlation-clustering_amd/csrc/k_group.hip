@@ -113,15 +113,23 @@ struct GvMap {
 };
 
 // Sort keys of the vertices: new id, or n2 for vertices of graphs that leave
-// the batch, which sort last.
+// the batch, which sort last.  Also clears the n2 member counts and new node
+// weights.
 __global__ __launch_bounds__(256) void k_gv_keys(int64_t n, const uint32_t* __restrict__ ids,
                                                  GvMap gm, int64_t n2, uint64_t* __restrict__ vkey,
-                                                 uint32_t* __restrict__ vval) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
+                                                 uint32_t* __restrict__ vval,
+                                                 uint32_t* __restrict__ cnt,
+                                                 double* __restrict__ nw2) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = t0; v < n; v += step) {
     const int64_t shift = gm.Shift(v);
     vkey[v] = shift == INT64_MIN ? (uint64_t)n2 : (uint64_t)((int64_t)ids[v] + shift);
     vval[v] = (uint32_t)v;
+  }
+  for (int64_t c = t0; c < n2; c += step) {
+    cnt[c] = 0u;
+    nw2[c] = 0.0;
   }
 }
 
@@ -875,9 +883,9 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   int64_t ng = 0;
   {
     DevBuf<uint32_t> cnt(al, n2);
-    PCC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(uint32_t) * (size_t)n2, s));
+    out_nw->Reset(al, n2);
     hipLaunchKernelGGL(k_gv_keys, dim3(grid_n), dim3(256), 0, s, n, ids, gm, n2, vkey.get(),
-                       perm.get());
+                       perm.get(), cnt.get(), out_nw->get());
     PCC_HIP_CHECK(hipGetLastError());
     RadixSortPairsU32(al, vkey.get(), perm.get(), n, 0, GvBitsFor((uint64_t)n2), s);
     ng = Segments(al, vkey.get(), n, &nstart, s);
@@ -1049,8 +1057,6 @@ bool CompressGraphGrouped(DeviceAllocator* al, const CsrView& g, const uint32_t*
   // new vertex in pbbslib's order.
   DevBuf<double> gsum(al, ng > 0 ? ng : 1);
   SegmentSumsF64(al, nws.get(), nstart.get(), ng, n, gsum.get(), s);
-  out_nw->Reset(al, n2);
-  PCC_HIP_CHECK(hipMemsetAsync(out_nw->get(), 0, sizeof(double) * (size_t)n2, s));
   if (ng > 0) {
     hipLaunchKernelGGL(k_gv_node_weights, dim3(GvGrid(ng, 256, 8192)), dim3(256), 0, s,
                        vkey.get(), nstart.get(), gsum.get(), ng, n2, out_nw->get());

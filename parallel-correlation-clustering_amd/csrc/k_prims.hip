@@ -343,7 +343,7 @@ void RadixSortImpl(DeviceAllocator* al, uint64_t* keys, V* vals, int64_t n, int 
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
     const uint32_t dmask = (1u << bits) - 1u;
-    PCC_HIP_CHECK(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * 256 * (size_t)tiles, s));
+    // k_radix_hist writes all 256 counts of every tile.
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kRadixThreads), 0, s, ka, n,
                        shift, dmask, tiles, hist.get());
     PCC_HIP_CHECK(hipGetLastError());
