@@ -85,7 +85,7 @@ def main() -> int:
                     help="skip the random-gather ceiling (a child process using torch)")
     ap.add_argument("--workload", default="rmat22",
                     choices=["rmat22", "batched", "modularity-async", "serving", "copy", "gather",
-                             "cluster-moves", "subclustering"],
+                             "cluster-moves", "subclustering", "sbm-allmove"],
                     help="rmat22: the headline line; the others are secondary measurements")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of SBM graphs")
     ap.add_argument("--batch-edges", type=int, default=1 << 30,
@@ -135,6 +135,8 @@ def main() -> int:
         return bench_cluster_moves(args)
     if args.workload == "subclustering":
         return bench_subclustering(args)
+    if args.workload == "sbm-allmove":
+        return bench_sbm_allmove(args)
 
     import numpy as np
     import pcc_amd
@@ -354,6 +356,56 @@ def bench_subclustering(args):
         "bestmove_ms": round(st.bestmove_seconds * 1e3, 3),
         "rounds": st.rounds, "levels": st.levels, "edges_scanned": st.edges_scanned,
         "objective": st.objective, "num_clusters": st.num_clusters,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_sbm_allmove(args):
+    """The small planted-partition configuration: SBM with 10,000 vertices in
+    100 communities (p_in 0.1, p_out 0.001, unit weights), sync ALL_MOVE,
+    resolution 0.01, 10 x 10 iterations, no refinement.  Every level is a
+    chain of short dependent launches, so this measures launch and read-back
+    latency: value = ms per Cluster call (wall).  The CPU figure is the
+    oracle's single-threaded run of the same call, ids compared."""
+    import numpy as np
+    import pcc_amd
+
+    csr = pcc_amd.gen_sbm(10000, 100, 0.1, 0.001, seed=1)
+    cfg = pcc_amd.parse_config("correlation_clusterer_config { resolution: 0.01 move_method: ALL_MOVE "
+                               "louvain_config { num_iterations: 10 num_inner_iterations: 10 } }")
+    g = pcc_amd.DeviceGraph.from_csr(csr)
+    steps = max(args.steps, 1)
+    for _ in range(max(args.warmup, 1)):
+        ids, st = g.cluster(cfg)
+    walls = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ids, st = g.cluster(cfg)
+        walls.append(time.perf_counter() - t0)
+    g.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        oracle.set_threads(1)
+        r = oracle.cluster(oracle.G.from_csr(csr), oracle.config_from(cfg))
+        cpu = {"value": round(r.seconds * 1e3, 3), "unit": "ms per Cluster call", "cores": 1,
+               "kind": "port", "sample": "the same Cluster call", "ids_equal": bool(np.array_equal(r.ids, ids))}
+    line = {
+        "metric": "ms per Cluster call (ParallelCorrelationClusterer sync ALL_MOVE)",
+        "value": round(float(np.median(walls)) * 1e3, 3), "unit": "ms", "n_gpus": 1,
+        "steps": steps, "warmup": max(args.warmup, 1),
+        "ms_per_step": round(float(np.median(walls)) * 1e3, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (move deltas) / f32 (edge weights)",
+        "data": "synthetic (seeded SBM)",
+        "config": {"workload": "SBM n=10000, 100 blocks, p_in 0.1, p_out 0.001, unit weights; sync ALL_MOVE "
+                               "resolution 0.01, 10 x 10 iterations, no refinement",
+                   "n": csr.n, "m_directed": csr.m, "parallelism": "single GPU"},
+        "min_ms": round(min(walls) * 1e3, 3), "device_span_ms": round(st.cluster_seconds * 1e3, 3),
+        "rounds": st.rounds, "levels": st.levels, "num_clusters": st.num_clusters,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
