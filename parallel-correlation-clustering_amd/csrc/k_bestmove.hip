@@ -136,10 +136,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
        c0 += (int64_t)gridDim.x * kChunk) {
     if (threadIdx.x < kBins) s_te[threadIdx.x] = 0;
     int8_t bin[ITEMS];
+    unsigned long long dg[ITEMS];
     uint64_t p0 = 0, p1 = 0, p2 = 0;
-    unsigned long long te[kBins];
-#pragma unroll
-    for (int j = 0; j < kBins; ++j) te[j] = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
@@ -150,9 +148,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
         b = BinOf((int64_t)deg, th.wave_max, th.block_max);
       }
       bin[k] = (int8_t)b;
+      dg[k] = deg;
       if (b >= 0) PackedAdd(p0, p1, p2, b);
-#pragma unroll
-      for (int j = 0; j < kBins; ++j) te[j] += b == j ? deg : 0ull;
     }
     // Thread ranks within each bin (the scans synchronize first, so s_te is
     // clear before the sums below).
@@ -160,11 +157,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
     uint64_t r0 = BlockExclusiveScan<unsigned long long>(p0, s_scan, &t0);
     uint64_t r1 = BlockExclusiveScan<unsigned long long>(p1, s_scan, &t1);
     uint64_t r2 = BlockExclusiveScan<unsigned long long>(p2, s_scan, &t2);
+    // Degree sums per bin by LDS atomics (cheaper than twelve wave sums of
+    // per-thread registers).
 #pragma unroll
-    for (int j = 0; j < kBins; ++j) {
-      const unsigned long long x = WaveSumU64(te[j]);
-      if (LaneId() == 0 && x) atomicAdd(&s_te[j], x);
-    }
+    for (int k = 0; k < ITEMS; ++k)
+      if (bin[k] >= 0) atomicAdd(&s_te[bin[k]], dg[k]);
     __syncthreads();
     if (threadIdx.x < kBins) {
       const int j = threadIdx.x;

@@ -72,8 +72,17 @@ __global__ __launch_bounds__(256) void k_apply(Bins bins, const uint32_t* __rest
       }
     }
   }
+  // One global atomic per workgroup: atomics on one address from every wave
+  // of a large grid serialize.
+  __shared__ unsigned long long s_new[256 / kWave];
   for (int s = 32; s > 0; s >>= 1) new_movers += __shfl_xor(new_movers, s);
-  if (LaneId() == 0 && new_movers) atomicAdd(&rc->new_count, new_movers);
+  if (LaneId() == 0) s_new[threadIdx.x / kWave] = new_movers;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_new[w];
+    if (t) atomicAdd(&rc->new_count, t);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_zero_flags(uint32_t n, const uint32_t* __restrict__ sizes,
@@ -934,7 +943,8 @@ void LaunchSeedVolume(const CsrView& g, const uint32_t* ids, const uint8_t* modi
                       const uint8_t* moved, unsigned long long* vol, hipStream_t s,
                       const unsigned int* gate) {
   if (g.n == 0) return;
-  hipLaunchKernelGGL(k_seed_volume, dim3(Grid(g.n, 256, 2048)), dim3(256), 0, s, g.n, g.off, ids,
+  // 512 workgroups: one atomic each on the single accumulator.
+  hipLaunchKernelGGL(k_seed_volume, dim3(Grid(g.n, 256, 512)), dim3(256), 0, s, g.n, g.off, ids,
                      modified, moved, vol, gate);
   PCC_HIP_CHECK(hipGetLastError());
 }
