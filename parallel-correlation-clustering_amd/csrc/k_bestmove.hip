@@ -138,15 +138,26 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_frontier(
     int8_t bin[ITEMS];
     unsigned long long dg[ITEMS];
     uint64_t p0 = 0, p1 = 0, p2 = 0;
+    // All flags first, then all offsets of the listed ones, so that the
+    // loads of a thread's vertices are in flight together rather than one
+    // flag-then-offset chain after another.
+    bool in[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
-      int b = -1;
-      unsigned long long deg = 0;
-      if (v < n && (frontier == nullptr || frontier[v] != 0)) {
-        deg = (unsigned long long)(off[v + 1] - off[v]);
-        b = BinOf((int64_t)deg, th.wave_max, th.block_max);
-      }
+      in[k] = v < n && (frontier == nullptr || frontier[v] != 0);
+    }
+    int64_t o0[ITEMS], o1[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int64_t v = c0 + (int64_t)k * kBinThreads + threadIdx.x;
+      o0[k] = in[k] ? off[v] : 0;
+      o1[k] = in[k] ? off[v + 1] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const unsigned long long deg = (unsigned long long)(o1[k] - o0[k]);
+      const int b = in[k] ? BinOf((int64_t)deg, th.wave_max, th.block_max) : -1;
       bin[k] = (int8_t)b;
       dg[k] = deg;
       if (b >= 0) PackedAdd(p0, p1, p2, b);
