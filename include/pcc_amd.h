@@ -180,10 +180,11 @@ int pcc_compute_objective(pcc_graph* g, const pcc_config* cfg,
 
 /* No counterpart in the reference: pcc_compute_objective for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
- * loops, harnesses that forbid hipMalloc).  The same kernels in the same
- * order give the same value bit for bit (when every cluster weight is an
- * exact sum of node weights, e.g. integer weights; fractional weights are
- * summed by atomic adds in an unspecified order), but no device memory is allocated
+ * loops, harnesses that forbid hipMalloc).  The same per-vertex terms and
+ * the same fixed-order sums give the same value bit for bit (when every
+ * cluster weight is an exact sum of node weights, e.g. integer weights;
+ * fractional weights are summed by atomic adds in an unspecified order), but
+ * no device memory is allocated
  * and nothing synchronizes: cluster_ids_dev (n entries) and node_weights_dev
  * (n entries, NULL: unit weights) are device arrays, *out_objective_dev a
  * device double written on `stream` (a hipStream_t; NULL: the handle's

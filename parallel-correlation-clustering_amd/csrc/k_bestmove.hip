@@ -359,14 +359,62 @@ __device__ __forceinline__ double EdgeValue(float wf, bool hub, double offset) {
   return x;
 }
 
-// gscale[v] = PackScale(ThresholdFor(B), MagExpFor(M)) for deg > 64, with B
-// and M the sum and the maximum over v's edges of |w| + |offset|; one wave
-// per such vertex.
+// gscale value of a vertex with more than 64 and at most kLongRow edges
+// (rows [vs, vs + vd)), computed by one wave: PackScale(ThresholdFor(B),
+// MagExpFor(M)) with B and M the sum and the maximum over its edges of
+// |w| + |offset|.
+__device__ __forceinline__ int32_t WaveGranScale(const float* __restrict__ w, double offset,
+                                                 double ao, int64_t vs, int64_t vd) {
+  // Besides B and M, the smallest granularity of the values the kernels
+  // add (EdgeValue): with none below the threshold the vertex gets
+  // kNoFineThr.
+  const bool hub = vd > kHubDegree;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0, mx = 0;
+  int gmin = 1 << 20;
+  int64_t p = LaneId();
+  for (; p + 3 * kWave < vd; p += 4 * kWave) {
+    const float w0 = w[vs + p], w1 = w[vs + p + kWave];
+    const float w2 = w[vs + p + 2 * kWave], w3 = w[vs + p + 3 * kWave];
+    const double a0 = fabs((double)w0) + ao;
+    const double a1 = fabs((double)w1) + ao;
+    const double a2 = fabs((double)w2) + ao;
+    const double a3 = fabs((double)w3) + ao;
+    s0 += a0;
+    s1 += a1;
+    s2 += a2;
+    s3 += a3;
+    mx = fmax(mx, fmax(fmax(a0, a1), fmax(a2, a3)));
+    gmin = min(gmin, min(min(GranExp(EdgeValue(w0, hub, offset)), GranExp(EdgeValue(w1, hub, offset))),
+                         min(GranExp(EdgeValue(w2, hub, offset)), GranExp(EdgeValue(w3, hub, offset)))));
+  }
+  for (; p < vd; p += kWave) {
+    const float wq = w[vs + p];
+    const double a = fabs((double)wq) + ao;
+    s0 += a;
+    mx = fmax(mx, a);
+    gmin = min(gmin, GranExp(EdgeValue(wq, hub, offset)));
+  }
+  const double s = WaveSum((s0 + s1) + (s2 + s3));
+  for (int sh = 32; sh > 0; sh >>= 1) mx = fmax(mx, __shfl_xor(mx, sh));
+  gmin = WaveMinInt(gmin);
+  int thr = ThresholdFor(s);
+  if (!(gmin < thr)) thr = kNoFineThr;
+  return PackScale(thr, MagExpFor(mx));
+}
+
+// gscale[v] for deg > 64.  Rows longer than kLongRow are queued for
+// k_gran_long; the others are done by one wave each, here (midq null: the
+// wave of their 64 vertices in turn) or, queued on midq, by k_gran_mid, which
+// spreads them over all waves (vertices of similar id, and so of similar
+// degree on RMAT-like graphs, would otherwise load a few waves with all the
+// large rows).
 __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t* __restrict__ off,
                                                         const float* __restrict__ w, double offset,
                                                         int32_t* __restrict__ gscale,
                                                         uint32_t* __restrict__ longq,
-                                                        unsigned int* __restrict__ nlong) {
+                                                        unsigned int* __restrict__ nlong,
+                                                        uint32_t* __restrict__ midq,
+                                                        unsigned int* __restrict__ nmid) {
   const double ao = fabs(offset);
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -378,48 +426,41 @@ __global__ __launch_bounds__(256) void k_gran_threshold(int64_t n, const int64_t
       if (deg > kLongRow) longq[atomicAdd(nlong, 1u)] = (uint32_t)v;  // k_gran_long
     }
     uint64_t big = __ballot(v < n && deg > 64 && deg <= kLongRow);
+    if (midq) {
+      if (big) {
+        const int leader = __ffsll((unsigned long long)big) - 1;
+        unsigned int pos = 0;
+        if (LaneId() == leader) pos = atomicAdd(nmid, (unsigned int)__popcll(big));
+        pos = __shfl(pos, leader);
+        if ((big >> LaneId()) & 1ull)
+          midq[pos + __popcll(big & LaneMaskLt())] = (uint32_t)v;
+      }
+      continue;
+    }
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const int64_t vs = __shfl(st, l);
-      const int64_t vd = __shfl(deg, l);
-      // Besides B and M, the smallest granularity of the values the kernels
-      // add (EdgeValue): with none below the threshold the vertex gets
-      // kNoFineThr.
-      const bool hub = vd > kHubDegree;
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, mx = 0;
-      int gmin = 1 << 20;
-      int64_t p = LaneId();
-      for (; p + 3 * kWave < vd; p += 4 * kWave) {
-        const float w0 = w[vs + p], w1 = w[vs + p + kWave];
-        const float w2 = w[vs + p + 2 * kWave], w3 = w[vs + p + 3 * kWave];
-        const double a0 = fabs((double)w0) + ao;
-        const double a1 = fabs((double)w1) + ao;
-        const double a2 = fabs((double)w2) + ao;
-        const double a3 = fabs((double)w3) + ao;
-        s0 += a0;
-        s1 += a1;
-        s2 += a2;
-        s3 += a3;
-        mx = fmax(mx, fmax(fmax(a0, a1), fmax(a2, a3)));
-        gmin = min(gmin, min(min(GranExp(EdgeValue(w0, hub, offset)), GranExp(EdgeValue(w1, hub, offset))),
-                             min(GranExp(EdgeValue(w2, hub, offset)), GranExp(EdgeValue(w3, hub, offset)))));
-      }
-      for (; p < vd; p += kWave) {
-        const float wq = w[vs + p];
-        const double a = fabs((double)wq) + ao;
-        s0 += a;
-        mx = fmax(mx, a);
-        gmin = min(gmin, GranExp(EdgeValue(wq, hub, offset)));
-      }
-      const double s = WaveSum((s0 + s1) + (s2 + s3));
-      for (int sh = 32; sh > 0; sh >>= 1) mx = fmax(mx, __shfl_xor(mx, sh));
-      gmin = WaveMinInt(gmin);
-      int thr = ThresholdFor(s);
-      if (!(gmin < thr)) thr = kNoFineThr;
-      if (LaneId() == 0) gscale[wave_v0 + l] = PackScale(thr, MagExpFor(mx));
+      const int32_t sc = WaveGranScale(w, offset, ao, __shfl(st, l), __shfl(deg, l));
+      if (LaneId() == 0) gscale[wave_v0 + l] = sc;
     }
+  }
+}
+
+// The queued rows of k_gran_threshold, one wave each.
+__global__ __launch_bounds__(256) void k_gran_mid(const int64_t* __restrict__ off,
+                                                  const float* __restrict__ w, double offset,
+                                                  const uint32_t* __restrict__ midq,
+                                                  const unsigned int* __restrict__ nmid,
+                                                  int32_t* __restrict__ gscale) {
+  const double ao = fabs(offset);
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t count = (int64_t)*nmid;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < count; q += nwaves) {
+    const uint32_t v = midq[q];
+    const int64_t vs = off[v];
+    const int32_t sc = WaveGranScale(w, offset, ao, vs, off[v + 1] - vs);
+    if (LaneId() == 0) gscale[v] = sc;
   }
 }
 
@@ -2034,8 +2075,25 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
     hipLaunchKernelGGL(k_gran_clear, dim3(GridFor(maxlong, 256, 1024)), dim3(256), 0, s, maxlong,
                        nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());
   const int grid = GridFor(g.n, 256, 8192);
+  // Large graphs queue their rows of 65..kLongRow edges for k_gran_mid.
+  const bool mid = g.n > kBinSmallN;
+  DevBuf<uint32_t> midq;
+  DevBuf<unsigned int> nmid;
+  if (mid) {
+    midq.Reset(al, g.n);
+    nmid.Reset(al, 1);
+    ZeroSpans z{};
+    z.p[0] = reinterpret_cast<uint8_t*>(nmid.get());
+    z.bytes[0] = sizeof(unsigned int);
+    z.n = 1;
+    LaunchZeroSpans(z, s);
+  }
   hipLaunchKernelGGL(k_gran_threshold, dim3(grid), dim3(256), 0, s, g.n, g.off, g.w, offset,
-                     gscale, longq.get(), nlong.get());
+                     gscale, longq.get(), nlong.get(), mid ? midq.get() : nullptr,
+                     mid ? nmid.get() : nullptr);
+  if (mid)
+    hipLaunchKernelGGL(k_gran_mid, dim3(2048), dim3(256), 0, s, g.off, g.w, offset, midq.get(),
+                       nmid.get(), gscale);
   if (longs) {
     hipLaunchKernelGGL(k_gran_long, dim3(1024), dim3(256), 0, s, g.off, g.w, offset, longq.get(),
                        nlong.get(), acc_s.get(), acc_m.get(), acc_g.get());

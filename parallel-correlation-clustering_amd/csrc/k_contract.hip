@@ -608,6 +608,8 @@ __device__ __forceinline__ double ObjTerm(const CsrView& g, const uint32_t* __re
 }
 
 constexpr int kObjChunk = 4 * kWave;
+// Graphs with more vertices than this queue their mid-size rows.
+constexpr int64_t kObjQueueN = 65536;
 // Rows up to this degree are folded by their vertex's own thread.
 constexpr int64_t kObjThreadDeg = 16;
 
@@ -637,6 +639,50 @@ __device__ __forceinline__ double ObjFold(const CsrView& g, const uint32_t* __re
   return acc;
 }
 
+// The intra-cluster half weight of one vertex with more than kObjThreadDeg
+// and at most kLongRow edges (row [vb, vb + vd), cluster vc), by one wave.
+__device__ __forceinline__ double WaveObjRow(const CsrView& g, const uint32_t* __restrict__ ids,
+                                             uint32_t vc, double offset, int64_t vb, int64_t vd) {
+  const int lane = LaneId();
+  // First an any-order sum with the order-free test: at most two nonzero
+  // terms, or all terms multiples of 2^gmin with sum of magnitudes below
+  // 2^(gmin + 52), make every partial sum exact, so the fold order of the
+  // reference cannot change the result.
+  double sum = 0.0, sabs = 0.0;
+  int gmin = 1 << 20, nz = 0;
+  for (int64_t c0 = 0; c0 < vd; c0 += kObjChunk) {
+    double x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t p = c0 + k * kWave + lane;
+      x[k] = p < vd ? ObjTerm(g, ids, vc, offset, vb + p) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sum += x[k];
+      sabs += fabs(x[k]);
+      gmin = min(gmin, GranExp(x[k]));
+      nz += x[k] != 0.0;
+    }
+  }
+  sum = WaveSum(sum);
+  sabs = WaveSum(sabs);
+  gmin = WaveMinInt(gmin);
+  for (int sh = 32; sh > 0; sh >>= 1) nz += __shfl_xor(nz, sh);
+  double tot = sum;
+  if (nz > 2 && !(gmin >= (1 << 19) || sabs < ldexp(1.0, gmin + 52))) {
+    int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
+    if (bs < 1024) bs = 1024;
+    bool have = false;
+    for (int64_t s0 = 0; s0 < vd; s0 += bs) {
+      const double blk = ObjFold(g, ids, vc, offset, vb, s0, min(s0 + bs, vd), 0.0, false);
+      tot = have ? tot + blk : blk;
+      have = true;
+    }
+  }
+  return tot;
+}
+
 __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32_t* __restrict__ ids,
                                                          const double* __restrict__ nw,
                                                          const double* __restrict__ K,
@@ -644,7 +690,9 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
                                                          double* __restrict__ r,
                                                          uint32_t* __restrict__ longq,
                                                          unsigned int* __restrict__ nlong,
-                                                         const unsigned int* __restrict__ gate) {
+                                                         const unsigned int* __restrict__ gate,
+                                                         uint32_t* __restrict__ midq,
+                                                         unsigned int* __restrict__ nmid) {
   if (gate && *gate) return;
   const int lane = LaneId();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n;
@@ -687,50 +735,43 @@ __global__ __launch_bounds__(256) void k_objective_terms(CsrView g, const uint32
       }
     }
     uint64_t big = __ballot(in && deg > kObjThreadDeg && deg <= kLongRow);
+    if (midq) {  // queued for k_objective_mid
+      if (big) {
+        const int leader = __ffsll((unsigned long long)big) - 1;
+        unsigned int pos = 0;
+        if (lane == leader) pos = atomicAdd(nmid, (unsigned int)__popcll(big));
+        pos = __shfl(pos, leader);
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if ((big >> lane) & 1ull) midq[pos + __popcll(big & lt)] = (uint32_t)v;
+      }
+      continue;
+    }
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
-      const uint32_t vc = __shfl(cv, l);
-      // First an any-order sum with the order-free test: at most two nonzero
-      // terms, or all terms multiples of 2^gmin with sum of magnitudes below
-      // 2^(gmin + 52), make every partial sum exact, so the fold order of the
-      // reference cannot change the result.
-      double sum = 0.0, sabs = 0.0;
-      int gmin = 1 << 20, nz = 0;
-      for (int64_t c0 = 0; c0 < vd; c0 += kObjChunk) {
-        double x[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int64_t p = c0 + k * kWave + lane;
-          x[k] = p < vd ? ObjTerm(g, ids, vc, offset, vb + p) : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          sum += x[k];
-          sabs += fabs(x[k]);
-          gmin = min(gmin, GranExp(x[k]));
-          nz += x[k] != 0.0;
-        }
-      }
-      sum = WaveSum(sum);
-      sabs = WaveSum(sabs);
-      gmin = WaveMinInt(gmin);
-      for (int sh = 32; sh > 0; sh >>= 1) nz += __shfl_xor(nz, sh);
-      double tot = sum;
-      if (nz > 2 && !(gmin >= (1 << 19) || sabs < ldexp(1.0, gmin + 52))) {
-        int64_t bs = (int64_t)ceil(sqrt((double)vd)) * 4;
-        if (bs < 1024) bs = 1024;
-        bool have = false;
-        for (int64_t s0 = 0; s0 < vd; s0 += bs) {
-          const double blk = ObjFold(g, ids, vc, offset, vb, s0, min(s0 + bs, vd), 0.0, false);
-          tot = have ? tot + blk : blk;
-          have = true;
-        }
-      }
+      const double tot = WaveObjRow(g, ids, __shfl(cv, l), offset, __shfl(b, l), __shfl(deg, l));
       if (lane == 0) e[wave_v0 + l] = tot;
     }
+  }
+}
+
+// The rows k_objective_terms queued (more than kObjThreadDeg and at most
+// kLongRow edges), one wave each over all waves of the grid, so that rows of
+// similar id (and, on RMAT-like graphs, similar degree) do not pile up on a
+// few waves.
+__global__ __launch_bounds__(256) void k_objective_mid(CsrView g, const uint32_t* __restrict__ ids,
+                                                       double offset,
+                                                       const uint32_t* __restrict__ midq,
+                                                       const unsigned int* __restrict__ nmid,
+                                                       double* __restrict__ e) {
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t count = (int64_t)*nmid;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < count; q += nwaves) {
+    const uint32_t v = midq[q];
+    const int64_t vb = g.off[v];
+    const double tot = WaveObjRow(g, ids, ids[v], offset, vb, g.off[v + 1] - vb);
+    if (LaneId() == 0) e[v] = tot;
   }
 }
 
@@ -810,9 +851,26 @@ __global__ __launch_bounds__(256) void k_objective_long_final(CsrView g,
 
 // Per-vertex intra-cluster edge count and per-cluster degree sums
 // (PMC.cc:43-61).
+__device__ __forceinline__ double WaveIntraCount(const CsrView& g, const uint32_t* __restrict__ ids,
+                                                 uint32_t vc, int64_t vb, int64_t vd) {
+  const int lane = LaneId();
+  unsigned long long cnt = 0;
+  for (int64_t p0 = lane; p0 < vd; p0 += 4 * kWave) {
+    uint32_t u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = p0 + k * kWave < vd ? g.nbr[vb + p0 + k * kWave] : 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cnt += (p0 + k * kWave < vd && ids[u[k]] == vc) ? 1ull : 0ull;
+  }
+  for (int s2 = 32; s2 > 0; s2 >>= 1) cnt += __shfl_xor(cnt, s2);
+  return (double)cnt;
+}
+
 __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint32_t* __restrict__ ids,
                                                           double* __restrict__ intra,
-                                                          double* __restrict__ dsum) {
+                                                          double* __restrict__ dsum,
+                                                          uint32_t* __restrict__ midq,
+                                                          unsigned int* __restrict__ nmid) {
   // Integer counts: any order is exact.  Degrees above 64 are counted by the
   // whole wave, rows above kLongRow by k_modularity_long.
   const int lane = LaneId();
@@ -840,23 +898,39 @@ __global__ __launch_bounds__(256) void k_modularity_terms(CsrView g, const uint3
     }
     WaveKeyedAdd(in, cv, (double)deg, dsum);
     uint64_t big = __ballot(in && deg > 64 && deg <= kLongRow);
+    if (midq) {  // queued for k_modularity_mid
+      if (big) {
+        const int leader = __ffsll((unsigned long long)big) - 1;
+        unsigned int pos = 0;
+        if (lane == leader) pos = atomicAdd(nmid, (unsigned int)__popcll(big));
+        pos = __shfl(pos, leader);
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if ((big >> lane) & 1ull) midq[pos + __popcll(big & lt)] = (uint32_t)v;
+      }
+      continue;
+    }
     const int64_t wave_v0 = base + (threadIdx.x & ~(kWave - 1));
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const int64_t vb = __shfl(b, l), vd = __shfl(deg, l);
-      const uint32_t vc = __shfl(cv, l);
-      unsigned long long cnt = 0;
-      for (int64_t p0 = lane; p0 < vd; p0 += 4 * kWave) {
-        uint32_t u[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) u[k] = p0 + k * kWave < vd ? g.nbr[vb + p0 + k * kWave] : 0u;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cnt += (p0 + k * kWave < vd && ids[u[k]] == vc) ? 1ull : 0ull;
-      }
-      for (int s2 = 32; s2 > 0; s2 >>= 1) cnt += __shfl_xor(cnt, s2);
-      if (lane == 0) intra[wave_v0 + l] = (double)cnt;
+      const double cnt = WaveIntraCount(g, ids, __shfl(cv, l), __shfl(b, l), __shfl(deg, l));
+      if (lane == 0) intra[wave_v0 + l] = cnt;
     }
+  }
+}
+
+// The rows k_modularity_terms queued, one wave each over the whole grid.
+__global__ __launch_bounds__(256) void k_modularity_mid(CsrView g, const uint32_t* __restrict__ ids,
+                                                        const uint32_t* __restrict__ midq,
+                                                        const unsigned int* __restrict__ nmid,
+                                                        double* __restrict__ intra) {
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t count = (int64_t)*nmid;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < count; q += nwaves) {
+    const uint32_t v = midq[q];
+    const int64_t vb = g.off[v];
+    const double cnt = WaveIntraCount(g, ids, ids[v], vb, g.off[v + 1] - vb);
+    if (LaneId() == 0) intra[v] = cnt;
   }
 }
 
@@ -1274,8 +1348,22 @@ void ComputeObjective(DeviceAllocator* al, const CsrView& g, const uint32_t* ids
     const int grid = Grid(n, 256, 8192);
     hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, K.get(),
                        (const unsigned int*)nullptr);
+    // Large graphs queue their rows of kObjThreadDeg..kLongRow edges for
+    // k_objective_mid (the same per-row computation, spread over the grid).
+    const bool mid = n > kObjQueueN;
+    DevBuf<uint32_t> midq;
+    DevBuf<unsigned int> nmid;
+    if (mid) {
+      midq.Reset(al, n);
+      nmid.Reset(al, 1);
+      PCC_HIP_CHECK(hipMemsetAsync(nmid.get(), 0, sizeof(unsigned int), s));
+    }
     hipLaunchKernelGGL(k_objective_terms, dim3(grid), dim3(256), 0, s, g, ids, nw, K.get(), offset,
-                       e.get(), r.get(), longq.get(), nlong.get(), (const unsigned int*)nullptr);
+                       e.get(), r.get(), longq.get(), nlong.get(), (const unsigned int*)nullptr,
+                       midq.get(), nmid.get());
+    if (mid)
+      hipLaunchKernelGGL(k_objective_mid, dim3(2048), dim3(256), 0, s, g, ids, offset, midq.get(),
+                         nmid.get(), e.get());
     if (g.m > kLongRow) {
       hipLaunchKernelGGL(k_objective_long, dim3(1024), dim3(256), 0, s, g, ids, offset, longq.get(),
                          nlong.get(), acc.get(), gmin.get(), nz.get());
@@ -1300,8 +1388,19 @@ void ComputeModularity(DeviceAllocator* al, const CsrView& g, const uint32_t* id
   PCC_HIP_CHECK(hipMemsetAsync(intra.get(), 0, sizeof(double) * (size_t)nn, s));
   if (n > 0) {
     const int grid = Grid(n, 256, 8192);
+    const bool mid = n > kObjQueueN;
+    DevBuf<uint32_t> midq;
+    DevBuf<unsigned int> nmid;
+    if (mid) {
+      midq.Reset(al, n);
+      nmid.Reset(al, 1);
+      PCC_HIP_CHECK(hipMemsetAsync(nmid.get(), 0, sizeof(unsigned int), s));
+    }
     hipLaunchKernelGGL(k_modularity_terms, dim3(grid), dim3(256), 0, s, g, ids, intra.get(),
-                       dsum.get());
+                       dsum.get(), midq.get(), nmid.get());
+    if (mid)
+      hipLaunchKernelGGL(k_modularity_mid, dim3(2048), dim3(256), 0, s, g, ids, midq.get(),
+                         nmid.get(), intra.get());
     if (g.m > kLongRow) {
       const int64_t maxlong = g.m / kLongRow + 1;
       DevBuf<uint32_t> longq(al, maxlong);
@@ -1487,7 +1586,8 @@ void ComputeObjectiveNoAlloc(void* scratch_dev, const CsrView& g, const uint32_t
     hipLaunchKernelGGL(k_cluster_weights, dim3(grid), dim3(256), 0, s, n, ids, nw, w.K,
                        (const unsigned int*)w.bad);
     hipLaunchKernelGGL(k_objective_terms, dim3(grid), dim3(256), 0, s, g, ids, nw, w.K, offset,
-                       w.e, w.r, w.longq, w.nlong, (const unsigned int*)w.bad);
+                       w.e, w.r, w.longq, w.nlong, (const unsigned int*)w.bad, (uint32_t*)nullptr,
+                       (unsigned int*)nullptr);  // each wave does its own rows
     if (g.m > kLongRow) {
       hipLaunchKernelGGL(k_objective_long, dim3(1024), dim3(256), 0, s, g, ids, offset, w.longq,
                          w.nlong, w.acc, w.gmin, w.nz);
