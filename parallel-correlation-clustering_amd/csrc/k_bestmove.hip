@@ -528,6 +528,7 @@ __global__ __launch_bounds__(256) void k_gran_long_final(const uint32_t* __restr
 
 // Bin-7 prefix sums of a list, one workgroup.
 constexpr int kScanThreads = 1024;
+constexpr int kHubScanItems = 8;
 __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ list,
                                                            int64_t count, int64_t part_edges,
@@ -537,39 +538,38 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            const unsigned long long* __restrict__ dcount) {
   if (dcount) count = (int64_t)*dcount;
   __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
-  __shared__ uint32_t s_xp[kScanThreads];  // the chunk's first partition per item
+  // kHubScanItems consecutive items per thread and pass, their list entries
+  // and then their row offsets loaded together.
+  constexpr int IT = kHubScanItems;
   int64_t carry_p = 0, carry_e = 0;
-  for (int64_t b0 = 0; b0 < count; b0 += kScanThreads) {
-    const int64_t i = b0 + threadIdx.x;
-    int64_t parts = 0, deg = 0;
-    if (i < count) {
-      const uint32_t v = list[i];
-      deg = off[v + 1] - off[v];
-      parts = deg > 0 ? (deg + part_edges - 1) / part_edges : 1;
+  for (int64_t b0 = 0; b0 < count; b0 += (int64_t)kScanThreads * IT) {
+    const int64_t i0 = b0 + (int64_t)threadIdx.x * IT;
+    uint32_t v[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) v[k] = i0 + k < count ? list[i0 + k] : 0u;
+    int64_t deg[IT], parts[IT];
+    int64_t lp = 0, le = 0;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      deg[k] = i0 + k < count ? off[v[k] + 1] - off[v[k]] : 0;
+      parts[k] = i0 + k < count ? (deg[k] > 0 ? (deg[k] + part_edges - 1) / part_edges : 1) : 0;
+      lp += parts[k];
+      le += deg[k];
     }
-    int64_t tp = 0, tdeg = 0;
-    const int64_t xp = BlockExclusiveScan(parts, sp, &tp);
-    const int64_t xe = BlockExclusiveScan(deg, se, &tdeg);
-    if (i < count) {
-      part_off[i] = carry_p + xp;
-      ent_off[i] = carry_e + xe;
+    int64_t tp = 0, te = 0;
+    int64_t rp = carry_p + BlockExclusiveScan(lp, sp, &tp);
+    int64_t re = carry_e + BlockExclusiveScan(le, se, &te);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      if (i0 + k >= count) break;
+      part_off[i0 + k] = rp;
+      ent_off[i0 + k] = re;
+      for (int64_t q = 0; q < parts[k]; ++q) item_of[rp + q] = (uint32_t)(i0 + k);
+      rp += parts[k];
+      re += deg[k];
     }
-    // Partition q of the chunk belongs to the last item whose first
-    // partition is <= q (every item has at least one); the workgroup writes
-    // the chunk's owners together.
-    s_xp[threadIdx.x] = i < count ? (uint32_t)xp : ~0u;
-    __syncthreads();
-    for (int64_t q = threadIdx.x; q < tp; q += kScanThreads) {
-      int lo = 0, hi = kScanThreads - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_xp[mid] <= (uint32_t)q) lo = mid; else hi = mid - 1;
-      }
-      item_of[carry_p + q] = (uint32_t)(b0 + lo);
-    }
-    __syncthreads();
     carry_p += tp;
-    carry_e += tdeg;
+    carry_e += te;
   }
   if (threadIdx.x == 0) {
     part_off[count] = carry_p;
