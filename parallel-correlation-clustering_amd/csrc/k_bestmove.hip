@@ -529,7 +529,9 @@ __global__ __launch_bounds__(256) void k_gran_long_final(const uint32_t* __restr
 // Bin-7 prefix sums of a list, one workgroup.
 constexpr int kScanThreads = 1024;
 constexpr int kHubScanItems = 8;
-__global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __restrict__ off,
+// THREADS = kScanThreads, or one wave for short lists (no barriers to wait on).
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_hub_scan(const int64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ list,
                                                            int64_t count, int64_t part_edges,
                                                            int64_t* __restrict__ part_off,
@@ -537,12 +539,12 @@ __global__ __launch_bounds__(kScanThreads) void k_hub_scan(const int64_t* __rest
                                                            uint32_t* __restrict__ item_of,
                                                            const unsigned long long* __restrict__ dcount) {
   if (dcount) count = (int64_t)*dcount;
-  __shared__ int64_t sp[kScanThreads / kWave], se[kScanThreads / kWave];
+  __shared__ int64_t sp[THREADS / kWave], se[THREADS / kWave];
   // kHubScanItems consecutive items per thread and pass, their list entries
   // and then their row offsets loaded together.
   constexpr int IT = kHubScanItems;
   int64_t carry_p = 0, carry_e = 0;
-  for (int64_t b0 = 0; b0 < count; b0 += (int64_t)kScanThreads * IT) {
+  for (int64_t b0 = 0; b0 < count; b0 += (int64_t)THREADS * IT) {
     const int64_t i0 = b0 + (int64_t)threadIdx.x * IT;
     uint32_t v[IT];
 #pragma unroll
@@ -2106,8 +2108,12 @@ void LaunchGranThreshold(DeviceAllocator* al, const CsrView& g, double offset, i
 void LaunchHubScan(const CsrView& g, const uint32_t* list, int64_t count, int64_t part_edges,
                    int64_t* part_off, int64_t* ent_off, uint32_t* item_of, hipStream_t s,
                    const unsigned long long* dcount) {
-  hipLaunchKernelGGL(k_hub_scan, dim3(1), dim3(kScanThreads), 0, s, g.off, list, count,
-                     part_edges, part_off, ent_off, item_of, dcount);
+  if (count <= (int64_t)kWave * kHubScanItems)
+    hipLaunchKernelGGL(k_hub_scan<kWave>, dim3(1), dim3(kWave), 0, s, g.off, list, count,
+                       part_edges, part_off, ent_off, item_of, dcount);
+  else
+    hipLaunchKernelGGL(k_hub_scan<kScanThreads>, dim3(1), dim3(kScanThreads), 0, s, g.off, list,
+                       count, part_edges, part_off, ent_off, item_of, dcount);
   PCC_HIP_CHECK(hipGetLastError());
 }
 
