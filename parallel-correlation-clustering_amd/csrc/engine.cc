@@ -551,7 +551,20 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
     c->hb_px2.Reset(al_, items);
     c->hb_item_of.Reset(al_, items);
     c->hb_hpart.Reset(al_, items);
-    const int64_t bcap = HubBucketCap(h_->hub_part_edges);
+    int64_t bcap = HubBucketCap(h_->hub_part_edges);
+    if (cfg_.async) {
+      // Async rows concentrate on a few heavy clusters (with unit weights the
+      // clusters of hubs grow fast), and a partition whose bucket overflows
+      // streams its vertex's whole row.  Buckets of 8 partitions' edges keep
+      // most of those pairs in the bucket (RMAT scale 24 async: bin 7 from
+      // 0.60 to 0.53 s per Cluster call); they take 64 bytes per bin-7 edge,
+      // so only when that fits in a quarter of the free device memory.
+      const int64_t big = HubBucketCapAsync(h_->hub_part_edges);
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+          (double)items * (double)big * (double)sizeof(uint2) < 0.25 * (double)free_b)
+        bcap = big;
+    }
     c->hb_bkt.Reset(al_, items * bcap);
     c->hb_bcur.Reset(al_, items);
     c->hb_pfine.Reset(al_, items);

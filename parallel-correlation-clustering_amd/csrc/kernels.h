@@ -291,6 +291,8 @@ struct HubScratch {
 // edges is at most part_edges when clusters hash evenly; heavy clusters can
 // overflow it.
 inline int64_t HubBucketCap(int64_t part_edges) { return part_edges + part_edges / 2 + 64; }
+// The larger capacity async rounds take when memory allows.
+inline int64_t HubBucketCapAsync(int64_t part_edges) { return 8 * part_edges + 64; }
 
 // Scratch for the ordered fallback path.
 struct HashScratch {
