@@ -557,7 +557,7 @@ void Engine::PrepareLevel(const CsrView& g, const StateView& st, const Layout& l
       // clusters of hubs grow fast), and a partition whose bucket overflows
       // streams its vertex's whole row.  Buckets of 8 partitions' edges keep
       // most of those pairs in the bucket (RMAT scale 24 async: bin 7 from
-      // 0.60 to 0.53 s per Cluster call); they take 64 bytes per bin-7 edge,
+      // 0.60 to 0.53-0.57 s per Cluster call); they take 64 bytes per bin-7 edge,
       // so only when that fits in a quarter of the free device memory.
       const int64_t big = HubBucketCapAsync(h_->hub_part_edges);
       size_t free_b = 0, total_b = 0;
